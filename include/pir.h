@@ -1,0 +1,146 @@
+/*
+ * pir.h -- C ABI of the MI355X-native secret-share compute path
+ * (libpir_hip.so).  Plain pointers and sizes only; every call returns 0 on
+ * success or a nonzero status, with a message available from pir_last_error().
+ * All device pointers are HIP device memory; `stream` is a hipStream_t (NULL =
+ * the default stream).  Calls are asynchronous on `stream` unless stated.
+ *
+ * Share layouts
+ *   RSS tensor (all three parties): [3 parties][2 shares][n] uint64, party p
+ *   holds (shareA, shareB) = (x_p, x_{p+1}) as in src/mpc/RSS.h:97-100.
+ *   Party-local tensor (one DeviceData per party): [3][n].
+ *   Bit / derivative shares use uint8 (the layers' Share<uint8_t>) or uint64.
+ *
+ * The reference interface each entry point replaces is cited as file:line of
+ * the reference repository (ucbrise/piranha).
+ */
+#ifndef PIR_H
+#define PIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status / plumbing ------------------------------------------------- */
+int pir_version(void);
+const char *pir_last_error(void);
+int pir_malloc(void **ptr, size_t bytes);                   /* thrust::device_vector storage, DeviceData.h:231-277 */
+int pir_free(void *ptr);
+int pir_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream); /* synchronous w.r.t. the host */
+int pir_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream); /* synchronous w.r.t. the host */
+int pir_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int pir_fill_u64(uint64_t *dst, uint64_t value, size_t n, void *stream); /* DeviceData::fill, DeviceData.h:83-85 */
+int pir_fill_u8(uint8_t *dst, uint8_t value, size_t n, void *stream);
+int pir_stream_create(void **stream);
+int pir_stream_destroy(void *stream);
+int pir_stream_synchronize(void *stream);
+int pir_device_synchronize(void);
+
+/* ---- ring GEMM / convolution (gpu:: layer) ------------------------------- */
+/* algo: 0 = plain 64-bit VALU, 1 = int8-limb MFMA (exact, same result). */
+enum { PIR_ALGO_PLAIN = 0, PIR_ALGO_LIMB = 1 };
+
+/* gpu::gemm, src/gpu/gemm.cuh:64-112: C = A.B mod 2^64, transpose_x != 0 means
+ * row-major storage of x (lda = K, ldb = N, ldc = N), else column-major. */
+int pir_gemm(int M, int N, int K, const uint64_t *A, int transpose_a, const uint64_t *B, int transpose_b,
+             uint64_t *C, int transpose_c, int algo, void *stream);
+
+/* gpu::conv_fprop, src/gpu/conv.cuh:380-401: NHWC x[b][H][W][Din] (*) KRSC
+ * w[Dout][fh][fw][Din] -> NHWC y[b][OH][OW][Dout], cross-correlation. */
+int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int H, int W, int Din, int Dout,
+                   int fh, int fw, int ph, int pw, int stride, int dilation, int algo, void *stream);
+
+/* The RSS local product of one party, localMatMul (src/mpc/RSS.inl:1096-1110):
+ * C = A0.B0 + A0.B1 + A1.B0 mod 2^64, all row-major (A: MxK, B: KxN, C: MxN).
+ * algo 0 runs the reference's form (three plain GEMMs, one launch each, plus
+ * the two accumulations); algo 1 the fused K-concatenated limb-MFMA form
+ * A0.(B0+B1) + A1.B0; algo 2 the same K-concatenated form on the VALU. */
+int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint64_t *A1, const uint64_t *B0,
+                   const uint64_t *B1, uint64_t *C, void *stream);
+
+/* ---- bit / layout kernels (src/gpu/bitwise.cuh, matrix.cuh, convolution.cuh) */
+int pir_bitexpand(const uint64_t *a, size_t n, void *bits, int bits_elem_bytes, void *stream); /* bitwise.cuh:98-124 */
+int pir_set_carry_out_msb(void *rbits, void *abits, void *msb, size_t n, int bit_width, int xor_msb,
+                          int elem_bytes, void *stream);                                   /* bitwise.cuh:184-213 */
+int pir_expand_compare(const void *b, const void *inv_b, void *expanded, size_t n, size_t expanded_n,
+                       int elem_bytes, void *stream);                                      /* bitwise.cuh:215-246 */
+int pir_element_vector_add(uint64_t *a, const uint64_t *b, int rowwise, int rows, int cols, void *stream); /* matrix.cuh:253-277 */
+int pir_truncate_cols(const void *a, size_t a_n, void *b, int start_cols, int end_cols, int elem_bytes,
+                      void *stream);                                                       /* matrix.cuh:382-400 */
+int pir_stride_pad(const uint64_t *in, uint64_t *out, size_t n_in, int stride, int pad, uint64_t pad_value,
+                   void *stream);                                                          /* convolution.cuh:306-337 */
+/* maxpool_im2row, convolution.cuh:252-304; out must hold pir_maxpool_im2row_size() values */
+size_t pir_maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);
+int pir_maxpool_im2row(const uint64_t *im, uint64_t *out, int H, int W, int f, int Din, int batch, int stride,
+                       int padding, uint64_t pad_value, void *stream);
+
+/* ---- DeviceData element ops (src/gpu/DeviceData.h:131-209) ----------------- */
+enum { PIR_OP_ADD = 0, PIR_OP_SUB, PIR_OP_MUL, PIR_OP_XOR, PIR_OP_AND, PIR_OP_SHR, PIR_OP_SHL, PIR_OP_DIV };
+/* x[i*sx] = x[i*sx] op y[i*sy] (y == NULL: op scalar).  elem_bytes 8 or 1.  DIV
+ * is the signed division of functors.cuh:33-45, restricted to power-of-two y. */
+int pir_dd_op(int op, int elem_bytes, void *x, long long x_stride, const void *y, long long y_stride,
+              uint64_t scalar, size_t n, void *stream);
+
+/* ---- RSS protocol, all three parties (src/mpc/RSS.h:103-145) -------------- */
+int pir_rss_reconstruct(const void *x, size_t n, void *out, int elem_bytes, void *stream); /* RSS.inl:457-479 */
+int pir_rss_reshare(const uint64_t *c, size_t n, uint64_t *out, void *stream);             /* RSS.inl:1052-1074 */
+int pir_rss_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream);                 /* RSS.inl:283-320 */
+int pir_rss_and(void *x, const void *y, size_t n, int elem_bytes, void *stream);          /* RSS.inl:330-345 */
+/* dividePublic, RSS.inl:422-455: d (scalar) or d_local ([3][n]) power-of-two denominators */
+int pir_rss_divide_public(uint64_t *a, size_t n, uint64_t d, const uint64_t *d_local, void *stream);
+/* dividePublic3out3, RSS.inl:1076-1094: a = 3-of-3 shares [3][n], denominator 2^k */
+int pir_rss_divide_public_3out3(const uint64_t *a, size_t n, int k, uint64_t *out, void *stream);
+/* matmul, RSS.inl:481-540 (transpose flags as in pir_gemm); truncation by 2^trunc */
+int pir_rss_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
+                   int trunc, int algo, void *stream);
+/* convolution(kFprop), RSS.inl:705-803 */
+int pir_rss_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
+                        int Dout, int stride, int pad, int trunc, int algo, void *stream);
+int pir_rss_drelu(const uint64_t *x, size_t n, void *result, int elem_bytes, void *stream);  /* RSS.inl:806-852 */
+int pir_rss_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int elem_bytes, void *stream); /* :854-865 */
+int pir_rss_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int elem_bytes,
+                         void *stream);                                                    /* RSS.inl:542-569 */
+/* maxpool over gathered pools [n/k windows][k] (modified in place like the reference), RSS.inl:867-1017 */
+int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
+
+/* ---- layers and network (src/nn/*Layer.cu, NeuralNetwork.cu) -------------- */
+typedef struct pir_net pir_net;
+enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3 };
+enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
+/* spec: per layer 8 ints {type, a0..a6}:
+ *   FC      {0, inputDim, outputDim}                                (FCConfig.h:16)
+ *   CNN     {1, H, W, Cin, Cout, filter, stride, padding}           (CNNConfig.h:27)
+ *   ReLU    {2, inputDim}                                           (ReLUConfig.h:12)
+ *   Maxpool {3, H, W, C, pool, stride}                              (MaxpoolConfig.h:16)
+ * Weights are initialised as in FCLayer.cu:31-48 / CNNLayer.cu:36-50 with
+ * std::default_random_engine(seed + layer) unless set with pir_net_set_weights. */
+int pir_net_create(pir_net **net, const int *spec, int num_layers, int batch, int frac_bits, int seed, int mode,
+                   void *stream);
+int pir_net_destroy(pir_net *net);
+int pir_net_num_layers(const pir_net *net);
+/* setPublic of weights (and FC biases) from doubles, RSS.inl:122-142 */
+int pir_net_set_weights(pir_net *net, int layer, const double *weights, size_t n_weights, const double *biases,
+                        size_t n_biases);
+/* NeuralNetwork::forward (NeuralNetwork.cu:108-133): input setPublic from doubles */
+int pir_net_forward(pir_net *net, const double *input, size_t n);
+/* forward from device-resident input shares [3][2][batch*input_size] */
+int pir_net_forward_shares(pir_net *net, const uint64_t *input_shares);
+/* device pointer to a layer's activation shares [3][2][n] and its size */
+int pir_net_activation(pir_net *net, int layer, const uint64_t **shares, size_t *n);
+/* device pointer to a ReLU / Maxpool layer's derivative shares (uint8) */
+int pir_net_derivative(pir_net *net, int layer, const uint8_t **shares, size_t *n);
+int pir_net_weights(pir_net *net, int layer, const uint64_t **shares, size_t *n, const uint64_t **bias_shares,
+                    size_t *n_bias);
+/* hipGraph capture of the whole forward (input from pir_net_forward_shares' buffer) */
+int pir_net_capture(pir_net *net, const uint64_t *input_shares);
+int pir_net_replay(pir_net *net);
+int pir_net_set_mode(pir_net *net, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIR_H */

@@ -1,0 +1,400 @@
+"""CPU oracle for the 3-party RSS secure-inference path (test infrastructure).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package.  It wraps ``oracle/build/libpir_ref.so``,
+a literal per-party C++ restatement of the reference's RSS protocol
+(src/mpc/RSS.inl, src/gpu/*.cuh, src/nn/*Layer.cu), and is the thing the HIP
+path is checked against -- never the thing measured or shipped.
+
+Array conventions: an RSS value for all three parties is a numpy array of
+shape (3, 2, n) (party, share, element); a party-local value is (3, n).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libpir_ref.so")
+
+ADD, SUB, MUL, XOR, AND, SHR, SHL, DIV = range(8)
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_dp = ctypes.POINTER(ctypes.c_double)
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+
+_SIGS = {
+    "ref_gemm": [_i, _i, _i, _u64p, _i, _u64p, _i, _u64p, _i],
+    "ref_conv_fprop": [_u64p, _u64p, _u64p] + [_i] * 11,
+    "ref_bitexpand_u8": [_u64p, _sz, _u8p],
+    "ref_bitexpand_u64": [_u64p, _sz, _u64p],
+    "ref_set_carry_out_msb_u8": [_u8p, _u8p, _u8p, _sz, _i, _i],
+    "ref_expand_compare_u8": [_u8p, _u8p, _u8p, _sz, _sz],
+    "ref_element_vector_add": [_u64p, _u64p, _i, _i, _i],
+    "ref_truncate_cols_u8": [_u8p, _sz, _u8p, _i, _i],
+    "ref_stride_pad": [_u64p, _u64p, _sz, _i, _i, _u64],
+    "ref_expanded_pool_size": [_i],
+    "ref_maxpool_im2row": [_u64p, _u64p, _i, _i, _i, _i, _i, _i, _i, _u64],
+    "ref_rss_set_public": [_u64p, _sz, _dp, _i],
+    "ref_rss_init_list": [_u64p, _sz, _dp, _i, _i],
+    "ref_rss_fill_u64": [_u64p, _sz, _u64],
+    "ref_rss_fill_u8": [_u8p, _sz, ctypes.c_uint8],
+    "ref_reconstruct_u64": [_u64p, _sz, _u64p],
+    "ref_reconstruct_u8": [_u8p, _sz, _u8p],
+    "ref_reshare_u64": [_u64p, _sz, _u64p],
+    "ref_reconstruct3out3": [_u64p, _sz, _u64p],
+    "ref_rss_mult": [_u64p, _u64p, _sz],
+    "ref_rss_and_u8": [_u8p, _u8p, _sz],
+    "ref_rss_and_u64": [_u64p, _u64p, _sz],
+    "ref_rss_op_rss": [_u64p, _u64p, _sz, _i],
+    "ref_rss_op_local": [_u64p, _u64p, _sz, _i],
+    "ref_rss_op_scalar": [_u64p, _sz, _u64, _i],
+    "ref_divide_public": [_u64p, _sz, _u64],
+    "ref_divide_public_vec": [_u64p, _sz, _u64p],
+    "ref_divide_public_3out3": [_u64p, _sz, _u64, _u64p],
+    "ref_scalar_divide": [_u64, _u64],
+    "ref_matmul": [_u64p, _sz, _u64p, _sz, _u64p] + [_i] * 7,
+    "ref_local_matmul": [_u64p, _sz, _u64p, _sz, _u64p] + [_i] * 6,
+    "ref_convolution": [_u64p, _sz, _u64p, _sz, _u64p, _sz] + [_i] * 9,
+    "ref_drelu_u8": [_u64p, _sz, _u8p],
+    "ref_drelu_u64": [_u64p, _sz, _u64p],
+    "ref_relu_u8": [_u64p, _sz, _u64p, _u8p],
+    "ref_relu_u64": [_u64p, _sz, _u64p, _u64p],
+    "ref_select_share_u8": [_u64p, _u64p, _u8p, _u64p, _sz],
+    "ref_select_share_u64": [_u64p, _u64p, _u64p, _u64p, _sz],
+    "ref_maxpool_u8": [_u64p, _sz, _u64p, _u8p, _i],
+    "ref_maxpool_u64": [_u64p, _sz, _u64p, _u64p, _i],
+    "ref_fc_forward": [_u64p, _u64p, _u64p, _u64p, _i, _i, _i, _i],
+    "ref_cnn_forward": [_u64p, _u64p, _u64p] + [_i] * 9,
+    "ref_relu_forward": [_u64p, _sz, _u64p, _u8p],
+    "ref_maxpool_forward": [_u64p, _u64p, _u8p] + [_i] * 7,
+}
+_RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_scalar_divide": _u64, "ref_version": _i}
+
+_lib = None
+
+
+def build():
+    """Compile the oracle library with its Makefile (g++, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RET.get(name, None)
+        L.ref_version.restype = _i
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    """ctypes pointer to a C-contiguous numpy array (uint64, uint8 or float64)."""
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
+    if a.dtype == np.uint64:
+        return a.ctypes.data_as(_u64p)
+    if a.dtype == np.uint8:
+        return a.ctypes.data_as(_u8p)
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    raise TypeError(a.dtype)
+
+
+def _u(a, dtype=np.uint64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# --- gpu:: level ------------------------------------------------------------
+
+def gemm(M, N, K, A, ta, B, tb, tc):
+    A, B = _u(A), _u(B)
+    C = np.zeros(M * N, np.uint64)
+    lib().ref_gemm(M, N, K, _p(A), int(ta), _p(B), int(tb), _p(C), int(tc))
+    return C
+
+
+def conv_fprop(x, w, b, H, W, Din, Dout, fh, fw, ph, pw, stride, dil=1):
+    x, w = _u(x), _u(w)
+    OH = (H + 2 * ph - ((fh - 1) * dil + 1)) // stride + 1
+    OW = (W + 2 * pw - ((fw - 1) * dil + 1)) // stride + 1
+    y = np.zeros(b * OH * OW * Dout, np.uint64)
+    lib().ref_conv_fprop(_p(x), _p(w), _p(y), b, H, W, Din, Dout, fh, fw, ph, pw, stride, dil)
+    return y
+
+
+def bitexpand(a, out_dtype=np.uint8):
+    a = _u(a)
+    b = np.zeros(a.size * 64, out_dtype)
+    if out_dtype == np.uint8:
+        lib().ref_bitexpand_u8(_p(a), a.size, _p(b))
+    else:
+        lib().ref_bitexpand_u64(_p(a), a.size, _p(b))
+    return b
+
+
+def expanded_pool_size(f):
+    return lib().ref_expanded_pool_size(f)
+
+
+def maxpool_im2row(im, H, W, f, Din, batch, stride, padding, pad_val):
+    im = _u(im)
+    n = lib().ref_maxpool_im2row(_p(im), ctypes.cast(None, _u64p), H, W, f, Din, batch, stride, padding,
+                                 int(pad_val) & 0xFFFFFFFFFFFFFFFF)
+    out = np.zeros(n, np.uint64)
+    lib().ref_maxpool_im2row(_p(im), _p(out), H, W, f, Din, batch, stride, padding,
+                             int(pad_val) & 0xFFFFFFFFFFFFFFFF)
+    return out
+
+
+def stride_pad(a, stride, pad, pad_val):
+    a = _u(a)
+    out = np.zeros(a.size // stride * (stride + pad), np.uint64)
+    lib().ref_stride_pad(_p(a), _p(out), a.size, stride, pad, int(pad_val) & 0xFFFFFFFFFFFFFFFF)
+    return out
+
+
+def truncate_cols(a, start_cols, end_cols):
+    a = _u(a, np.uint8)
+    b = np.zeros(a.size // start_cols * end_cols, np.uint8)
+    lib().ref_truncate_cols_u8(_p(a), a.size, _p(b), start_cols, end_cols)
+    return b
+
+
+def element_vector_add(a, b, rowwise, rows, cols):
+    a = _u(a).copy()
+    b = _u(b)
+    lib().ref_element_vector_add(_p(a), _p(b), int(rowwise), rows, cols)
+    return a
+
+
+def expand_compare(b, inv, out_n):
+    b, inv = _u(b, np.uint8), _u(inv, np.uint8)
+    out = np.zeros(out_n, np.uint8)
+    lib().ref_expand_compare_u8(_p(b), _p(inv), _p(out), b.size, out_n)
+    return out
+
+
+def set_carry_out_msb(rbits, abits, n, bit_width, xor_msb):
+    rbits, abits = _u(rbits, np.uint8).copy(), _u(abits, np.uint8).copy()
+    msb = np.zeros(n, np.uint8)
+    lib().ref_set_carry_out_msb_u8(_p(rbits), _p(abits), _p(msb), n, bit_width, int(xor_msb))
+    return rbits, abits, msb
+
+
+# --- shares ---------------------------------------------------------------
+
+def set_public(vals, fp):
+    """RSS setPublic for all parties (RSS.inl:122-142), returns (3, 2, n) uint64."""
+    vals = np.ascontiguousarray(vals, np.float64).ravel()
+    x = np.zeros((3, 2, vals.size), np.uint64)
+    lib().ref_rss_set_public(_p(x), vals.size, _p(vals), fp)
+    return x
+
+
+def init_list(vals, fp, convert=True):
+    """RSS initializer-list constructor (RSS.inl:388-414)."""
+    vals = np.ascontiguousarray(vals, np.float64).ravel()
+    x = np.zeros((3, 2, vals.size), np.uint64)
+    lib().ref_rss_init_list(_p(x), vals.size, _p(vals), int(convert), fp)
+    return x
+
+
+def fill(n, v, dtype=np.uint64):
+    x = np.zeros((3, 2, n), dtype)
+    if dtype == np.uint8:
+        lib().ref_rss_fill_u8(_p(x), n, v)
+    else:
+        lib().ref_rss_fill_u64(_p(x), n, int(v) & 0xFFFFFFFFFFFFFFFF)
+    return x
+
+
+def reconstruct(x):
+    x = np.ascontiguousarray(x)
+    n = x.shape[-1]
+    out = np.zeros((3, n), x.dtype)
+    if x.dtype == np.uint8:
+        lib().ref_reconstruct_u8(_p(x), n, _p(out))
+    else:
+        lib().ref_reconstruct_u64(_p(x), n, _p(out))
+    return out
+
+
+def reshare(c):
+    c = _u(c)
+    n = c.shape[-1]
+    out = np.zeros((3, 2, n), np.uint64)
+    lib().ref_reshare_u64(_p(c), n, _p(out))
+    return out
+
+
+def reconstruct3out3(a):
+    a = _u(a)
+    out = np.zeros_like(a)
+    lib().ref_reconstruct3out3(_p(a), a.shape[-1], _p(out))
+    return out
+
+
+def mult(x, y):
+    x, y = _u(x).copy(), _u(y)
+    lib().ref_rss_mult(_p(x), _p(y), x.shape[-1])
+    return x
+
+
+def rss_and(x, y):
+    x = np.ascontiguousarray(x).copy()
+    y = np.ascontiguousarray(y, x.dtype)
+    if x.dtype == np.uint8:
+        lib().ref_rss_and_u8(_p(x), _p(y), x.shape[-1])
+    else:
+        lib().ref_rss_and_u64(_p(x), _p(y), x.shape[-1])
+    return x
+
+
+def op_rss(x, y, op):
+    x, y = _u(x).copy(), _u(y)
+    lib().ref_rss_op_rss(_p(x), _p(y), x.shape[-1], op)
+    return x
+
+
+def op_local(x, y, op):
+    x, y = _u(x).copy(), _u(y)
+    lib().ref_rss_op_local(_p(x), _p(y), x.shape[-1], op)
+    return x
+
+
+def op_scalar(x, v, op):
+    x = _u(x).copy()
+    lib().ref_rss_op_scalar(_p(x), x.shape[-1], int(v) & 0xFFFFFFFFFFFFFFFF, op)
+    return x
+
+
+def divide_public(a, d):
+    a = _u(a).copy()
+    lib().ref_divide_public(_p(a), a.shape[-1], int(d))
+    return a
+
+
+def divide_public_vec(a, d):
+    a, d = _u(a).copy(), _u(d)
+    lib().ref_divide_public_vec(_p(a), a.shape[-1], _p(d))
+    return a
+
+
+def divide_public_3out3(a, d):
+    a = _u(a).copy()
+    out = np.zeros((3, 2, a.shape[-1]), np.uint64)
+    lib().ref_divide_public_3out3(_p(a), a.shape[-1], int(d), _p(out))
+    return out
+
+
+def scalar_divide(x, d):
+    return lib().ref_scalar_divide(int(x) & 0xFFFFFFFFFFFFFFFF, int(d) & 0xFFFFFFFFFFFFFFFF)
+
+
+def matmul(a, b, M, N, K, ta, tb, tc, trunc):
+    a, b = _u(a), _u(b)
+    c = np.zeros((3, 2, M * N), np.uint64)
+    lib().ref_matmul(_p(a), a.shape[-1], _p(b), b.shape[-1], _p(c), M, N, K, int(ta), int(tb), int(tc), trunc)
+    return c
+
+
+def local_matmul(a, b, M, N, K, ta, tb, tc):
+    a, b = _u(a), _u(b)
+    c = np.zeros((3, M * N), np.uint64)
+    lib().ref_local_matmul(_p(a), a.shape[-1], _p(b), b.shape[-1], _p(c), M, N, K, int(ta), int(tb), int(tc))
+    return c
+
+
+def convolution(A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
+    A, B = _u(A), _u(B)
+    OH = (H + 2 * pad - f) // stride + 1
+    OW = (W + 2 * pad - f) // stride + 1
+    n = batch * OH * OW * Dout
+    C = np.zeros((3, 2, n), np.uint64)
+    lib().ref_convolution(_p(A), A.shape[-1], _p(B), B.shape[-1], _p(C), n, batch, H, W, f, Din, Dout, stride, pad,
+                          trunc)
+    return C
+
+
+def drelu(x, out_dtype=np.uint8):
+    x = _u(x)
+    n = x.shape[-1]
+    out = np.zeros((3, 2, n), out_dtype)
+    (lib().ref_drelu_u8 if out_dtype == np.uint8 else lib().ref_drelu_u64)(_p(x), n, _p(out))
+    return out
+
+
+def relu(x, out_dtype=np.uint8):
+    x = _u(x)
+    n = x.shape[-1]
+    out = np.zeros((3, 2, n), np.uint64)
+    dout = np.zeros((3, 2, n), out_dtype)
+    (lib().ref_relu_u8 if out_dtype == np.uint8 else lib().ref_relu_u64)(_p(x), n, _p(out), _p(dout))
+    return out, dout
+
+
+def select_share(x, y, b):
+    x, y = _u(x), _u(y)
+    b = np.ascontiguousarray(b)
+    n = x.shape[-1]
+    z = np.zeros((3, 2, n), np.uint64)
+    (lib().ref_select_share_u8 if b.dtype == np.uint8 else lib().ref_select_share_u64)(_p(x), _p(y), _p(b), _p(z), n)
+    return z
+
+
+def maxpool(pools, k, out_dtype=np.uint8):
+    """maxpool over gathered pools (RSS.inl:867-1017); returns (result, dresult, pools_after)."""
+    pools = _u(pools).copy()
+    n = pools.shape[-1]
+    result = np.zeros((3, 2, n // k), np.uint64)
+    dres = np.zeros((3, 2, n), out_dtype)
+    (lib().ref_maxpool_u8 if out_dtype == np.uint8 else lib().ref_maxpool_u64)(_p(pools), n, _p(result), _p(dres), k)
+    return result, dres, pools
+
+
+# --- layers -----------------------------------------------------------------
+
+def fc_forward(x, W, b, batch, d_in, d_out, fp):
+    x, W, b = _u(x), _u(W), _u(b)
+    act = np.zeros((3, 2, batch * d_out), np.uint64)
+    lib().ref_fc_forward(_p(x), _p(W), _p(b), _p(act), batch, d_in, d_out, fp)
+    return act
+
+
+def cnn_forward(x, W, batch, H, Wd, Din, Dout, f, stride, pad, fp):
+    x, W = _u(x), _u(W)
+    OH = (H - f + 2 * pad) // stride + 1
+    OW = (Wd - f + 2 * pad) // stride + 1
+    act = np.zeros((3, 2, batch * OH * OW * Dout), np.uint64)
+    lib().ref_cnn_forward(_p(x), _p(W), _p(act), batch, H, Wd, Din, Dout, f, stride, pad, fp)
+    return act
+
+
+def relu_forward(x):
+    x = _u(x)
+    n = x.shape[-1]
+    act = np.zeros((3, 2, n), np.uint64)
+    rp = np.zeros((3, 2, n), np.uint8)
+    lib().ref_relu_forward(_p(x), n, _p(act), _p(rp))
+    return act, rp
+
+
+def maxpool_forward(x, batch, H, W, C, pool, stride, fp):
+    x = _u(x)
+    OH = (H - pool) // stride + 1
+    OW = (W - pool) // stride + 1
+    n = batch * C * OH * OW
+    act = np.zeros((3, 2, n), np.uint64)
+    mp = np.zeros((3, 2, n * pool * pool), np.uint8)
+    lib().ref_maxpool_forward(_p(x), _p(act), _p(mp), batch, H, W, C, pool, stride, fp)
+    return act, mp

@@ -1,0 +1,915 @@
+// pir_ref.cpp -- CPU restatement of Piranha's 3-party RSS secure-inference path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle that the HIP product path
+// is checked against, bit for bit.  Only tests/, the smoke check and the CPU
+// baseline leg of bench.py may load it; nothing in piranha_amd/ links or calls it.
+//
+// It restates, per party and line by line, the semantics of the reference files
+// it cites (paths relative to the reference repository root).  All three parties
+// A, B, C are simulated in lockstep: every share buffer of the reference
+// (one DeviceData per party) becomes three host arrays, and every message the
+// reference sends over a socket becomes a read of the peer party's array.
+//
+// Conventions
+//   * An RSS value for all parties is 6 arrays, party p / share i  ("Rs").
+//     Party p holds (sA, sB) = (x_p, x_{p+1}) in a consistent sharing.
+//   * A party-local DeviceData (one per party) is 3 arrays ("Loc").
+//   * Views carry (base, offset, stride, count) to restate the reference's
+//     thrust StridedRange views (src/gpu/StridedRange.cuh:12-52).
+//   * The FP64 divisions of the reference (src/gpu/functors.cuh:33-45,94-105)
+//     are done here in IEEE double on the CPU, exactly as written there.  The
+//     GPU path emulates them in integer arithmetic and is tested against this.
+//
+// Pinning (how this file itself is checked, see tests/test_oracle.py):
+//   * reference KATs of src/test/{GPU,Func,Layer}.cu and the files/test fixtures;
+//   * closed forms computed independently in numpy / exact Python integers
+//     (mod-2^64 products, [v>=0], IEEE truncation, plaintext fixed-point twin);
+//   * share invariants of the placeholder precompute (x1 = x2 = 0 except the
+//     party-B setPublic quirk).
+// Functions with no such check would be marked "parity unpinned" here; every
+// exported function below is covered by at least one of those checks.
+
+#include <cstdint>
+#include <cstddef>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <type_traits>
+#include <algorithm>
+
+namespace {
+
+typedef uint64_t u64;
+typedef int64_t s64;
+
+enum { PA = 0, PB = 1, PC = 2 };
+// RSS.inl:347-369
+inline int nextParty(int p) { return p == PA ? PB : (p == PB ? PC : PA); }
+inline int prevParty(int p) { return p == PA ? PC : (p == PB ? PA : PB); }
+
+template <typename T>
+struct View {
+    T *base = nullptr;
+    size_t off = 0, stride = 1, n = 0;
+    T &operator[](size_t k) const { return base[off + k * stride]; }
+    size_t size() const { return n; }
+};
+
+// StridedRange(first, last, stride): count = ((last - first) + stride - 1) / stride
+// (StridedRange.cuh:36-38); first/last are offsets into the same contiguous array.
+template <typename T>
+View<T> strided(const View<T> &whole, size_t first, size_t stride) {
+    View<T> v;
+    v.base = whole.base;
+    v.off = whole.off + first * whole.stride;
+    v.stride = whole.stride * stride;
+    size_t last = whole.n;
+    v.n = last > first ? ((last - first) + stride - 1) / stride : 0;
+    return v;
+}
+
+template <typename T>
+struct Loc {
+    View<T> p[3];
+    size_t size() const { return p[0].n; }
+};
+
+template <typename T>
+struct Rs {
+    View<T> s[3][2];
+    size_t size() const { return s[0][0].n; }
+};
+
+template <typename T>
+struct LocBuf {
+    std::vector<T> d[3];
+    explicit LocBuf(size_t n) {
+        for (auto &v : d) v.assign(n, 0);
+    }
+    Loc<T> v() {
+        Loc<T> r;
+        for (int q = 0; q < 3; q++) r.p[q] = View<T>{d[q].data(), 0, 1, d[q].size()};
+        return r;
+    }
+    size_t size() const { return d[0].size(); }
+};
+
+template <typename T>
+struct RsBuf {
+    std::vector<T> d[3][2];
+    explicit RsBuf(size_t n) { resize(n); }
+    void resize(size_t n) {
+        for (auto &pp : d)
+            for (auto &v : pp) v.assign(n, 0);
+    }
+    Rs<T> v() {
+        Rs<T> r;
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 2; i++) r.s[q][i] = View<T>{d[q][i].data(), 0, 1, d[q][i].size()};
+        return r;
+    }
+    size_t size() const { return d[0][0].size(); }
+};
+
+template <typename T>
+Rs<T> wrapRs(T *a, size_t n) {
+    Rs<T> r;
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) r.s[q][i] = View<T>{a + (2 * q + i) * n, 0, 1, n};
+    return r;
+}
+
+template <typename T>
+Loc<T> wrapLoc(T *a, size_t n) {
+    Loc<T> r;
+    for (int q = 0; q < 3; q++) r.p[q] = View<T>{a + q * n, 0, 1, n};
+    return r;
+}
+
+template <typename T>
+Rs<T> stridedRs(const Rs<T> &x, size_t first, size_t stride) {
+    Rs<T> r;
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) r.s[q][i] = strided(x.s[q][i], first, stride);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// DeviceData element ops (DeviceData.h:131-209, functors.cuh:6-116).  Each one
+// is a thrust::transform over the left operand's range.
+
+enum Op { ADD, SUB, MUL, XOR, AND, SHR, SHL, DIV };
+
+template <typename T>
+inline T sdiv(T x, T y) {
+    // functors.cuh:33-45 / 94-105: (S)((double)(S)x / (double)(S)y)
+    typedef typename std::make_signed<T>::type S;
+    double q = (double)(S)x / (double)(S)y;
+    // out-of-range conversion is undefined in C++; x86 yields the "indefinite"
+    // integer 0x80..0.  Only reachable for |y| == 1, never on the hot path.
+    const double lim = std::ldexp(1.0, (int)(sizeof(S) * 8 - 1));
+    if (!(q < lim) || !(q >= -lim)) return (T)((T)1 << (sizeof(T) * 8 - 1));
+    return (T)(S)q;
+}
+
+template <typename T>
+inline T apply_op(Op op, T x, T y) {
+    typedef typename std::make_signed<T>::type S;
+    switch (op) {
+        case ADD: return (T)(x + y);
+        case SUB: return (T)(x - y);
+        case MUL: return (T)(x * y);
+        case XOR: return (T)(x ^ y);
+        case AND: return (T)(x & y);
+        case SHR: return (T)((S)x >> y);  // arithmetic shift, functors.cuh:46-59
+        case SHL: return (T)(x << y);
+        case DIV: return sdiv<T>(x, y);
+    }
+    return x;
+}
+
+template <typename T>
+void dd_op(View<T> x, View<T> y, Op op) {
+    for (size_t k = 0; k < x.n; k++) x[k] = apply_op<T>(op, x[k], y[k]);
+}
+template <typename T>
+void dd_op_s(View<T> x, T y, Op op) {
+    for (size_t k = 0; k < x.n; k++) x[k] = apply_op<T>(op, x[k], y);
+}
+template <typename T>
+void dd_fill(View<T> x, T v) {
+    for (size_t k = 0; k < x.n; k++) x[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// RSS share algebra (RSS.inl:110-345).
+
+template <typename T>
+void r_zero(Rs<T> x) {
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) dd_fill(x.s[q][i], (T)0);
+}
+
+// RSS.inl:116-120
+template <typename T>
+void r_fill(Rs<T> x, T v) {
+    for (int q = 0; q < 3; q++) {
+        dd_fill(x.s[q][0], q == PA ? v : (T)0);
+        dd_fill(x.s[q][1], q == PC ? v : (T)0);
+    }
+}
+
+template <typename T>
+inline T to_fixed(double f, int fp) {
+    // (T)(f * (1 << FLOAT_PRECISION)) on x86-64 (RSS.inl:126, util.cuh:28-33):
+    // a double below 2^63 converts through a signed 64-bit truncation.
+    double v = f * (double)(1 << fp);
+    return (T)(u64)(s64)v;
+}
+
+// RSS.inl:122-142, including the missing `break` after PARTY_B (B ends as (0, f)).
+template <typename T>
+void r_set_public(Rs<T> x, const double *vals, int fp) {
+    for (int q = 0; q < 3; q++) {
+        View<T> a = x.s[q][0], b = x.s[q][1];
+        switch (q) {
+            case PA:
+                for (size_t k = 0; k < a.n; k++) a[k] = to_fixed<T>(vals[k], fp);
+                dd_fill(b, (T)0);
+                break;
+            case PB:
+                dd_fill(a, (T)0);
+                dd_fill(b, (T)0);
+                // fall through (RSS.inl:134-140)
+            case PC:
+                dd_fill(a, (T)0);
+                for (size_t k = 0; k < b.n; k++) b[k] = to_fixed<T>(vals[k], fp);
+                break;
+        }
+    }
+}
+
+// Initializer-list constructor, RSS.inl:388-414 (buffer starts zeroed).
+template <typename T>
+void r_init_list(Rs<T> x, const double *vals, bool convert, int fp) {
+    r_zero(x);
+    for (size_t k = 0; k < x.size(); k++) {
+        T v = convert ? to_fixed<T>(vals[k], fp) : (T)(u64)(s64)vals[k];
+        x.s[PA][0][k] = v;
+        x.s[PC][1][k] = v;
+    }
+}
+
+// scalar +=, -= touch A.sA and C.sB only (RSS.inl:168-186); *=, >>= both shares.
+template <typename T>
+void r_op_s(Rs<T> x, T v, Op op) {
+    if (op == ADD || op == SUB) {
+        dd_op_s(x.s[PA][0], v, op);
+        dd_op_s(x.s[PC][1], v, op);
+    } else {
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 2; i++) dd_op_s(x.s[q][i], v, op);
+    }
+}
+
+// DeviceData rhs: + - ^ on A.sA and C.sB (RSS.inl:202-241); * & >> << on both.
+template <typename T>
+void r_op_l(Rs<T> x, Loc<T> y, Op op) {
+    bool both = (op == MUL || op == AND || op == SHR || op == SHL);
+    for (int q = 0; q < 3; q++) {
+        if (both) {
+            dd_op(x.s[q][0], y.p[q], op);
+            dd_op(x.s[q][1], y.p[q], op);
+        } else {
+            if (q == PA) dd_op(x.s[q][0], y.p[q], op);
+            if (q == PC) dd_op(x.s[q][1], y.p[q], op);
+        }
+    }
+}
+
+// RSS rhs + - ^: share-wise on every party (RSS.inl:267-328).
+template <typename T>
+void r_op_r(Rs<T> x, Rs<T> y, Op op) {
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) dd_op(x.s[q][i], y.s[q][i], op);
+}
+
+// reshare, RSS.inl:1052-1074: out_p = (c_p, c_{next(p)}), no mask (placeholder).
+template <typename T>
+void reshare(Loc<T> c, Rs<T> out) {
+    LocBuf<T> mask(c.size());
+    for (int q = 0; q < 3; q++) dd_op(mask.v().p[q], c.p[q], ADD);
+    r_zero(out);
+    for (int q = 0; q < 3; q++) dd_op(out.s[q][0], mask.v().p[q], ADD);
+    // party q transmits shareA to prev, receives shareB from next
+    for (int q = 0; q < 3; q++) {
+        View<T> dst = out.s[q][1], src = out.s[nextParty(q)][0];
+        for (size_t k = 0; k < dst.n; k++) dst[k] = src[k];
+    }
+}
+
+// reconstruct, RSS.inl:457-479: out_p = sA + sB + (prev party's sA).
+template <typename T>
+void reconstruct(Rs<T> in, Loc<T> out) {
+    for (int q = 0; q < 3; q++) {
+        View<T> o = out.p[q];
+        View<T> a = in.s[q][0], b = in.s[q][1], rx = in.s[prevParty(q)][0];
+        for (size_t k = 0; k < o.n; k++) o[k] = (T)((T)((T)(a[k] + b[k])) + rx[k]);
+    }
+}
+
+// reconstruct3out3, RSS.inl:1019-1050: every party sums all three 3-of-3 shares.
+template <typename T>
+void reconstruct3out3(Loc<T> in, Loc<T> out) {
+    for (int q = 0; q < 3; q++) {
+        View<T> o = out.p[q];
+        View<T> x = in.p[q], r1 = in.p[prevParty(q)], r2 = in.p[nextParty(q)];
+        for (size_t k = 0; k < o.n; k++) o[k] = (T)(x[k] + r1[k] + r2[k]);
+    }
+}
+
+// RSS *=, RSS.inl:283-320
+template <typename T>
+void r_mult(Rs<T> x, Rs<T> y) {
+    LocBuf<T> summed(y.size());
+    for (int q = 0; q < 3; q++) {
+        dd_op(summed.v().p[q], y.s[q][0], ADD);
+        dd_op(summed.v().p[q], y.s[q][1], ADD);
+    }
+    for (int q = 0; q < 3; q++) {
+        dd_op(x.s[q][0], summed.v().p[q], MUL);
+        dd_op(x.s[q][1], y.s[q][0], MUL);
+        dd_op(x.s[q][0], x.s[q][1], ADD);
+    }
+    Loc<T> c;
+    for (int q = 0; q < 3; q++) c.p[q] = x.s[q][0];
+    reshare(c, x);
+}
+
+// RSS &=, RSS.inl:330-345
+template <typename T>
+void r_and(Rs<T> x, Rs<T> y) {
+    LocBuf<T> summed(y.size());
+    for (int q = 0; q < 3; q++) {
+        dd_op(summed.v().p[q], y.s[q][0], XOR);
+        dd_op(summed.v().p[q], y.s[q][1], XOR);
+    }
+    for (int q = 0; q < 3; q++) {
+        dd_op(x.s[q][0], summed.v().p[q], AND);
+        dd_op(x.s[q][1], y.s[q][0], AND);
+        dd_op(x.s[q][0], x.s[q][1], XOR);
+    }
+    Loc<T> c;
+    for (int q = 0; q < 3; q++) c.p[q] = x.s[q][0];
+    reshare(c, x);
+}
+
+// dividePublic with a scalar denominator, RSS.inl:422-436, Precompute.h:107-119
+template <typename T>
+void divide_public(Rs<T> a, T d) {
+    size_t n = a.size();
+    RsBuf<T> r(n), rPrime(n);
+    r_fill(rPrime.v(), d);
+    r_fill(r.v(), (T)1);
+    r_op_r(a, rPrime.v(), SUB);
+    LocBuf<T> rec(n);
+    reconstruct(a, rec.v());
+    for (int q = 0; q < 3; q++) dd_op_s(rec.v().p[q], d, DIV);
+    r_zero(a);
+    r_op_r(a, r.v(), ADD);
+    r_op_l(a, rec.v(), ADD);
+}
+
+// dividePublic with per-element denominators, RSS.inl:438-455, Precompute.h:121-134
+template <typename T>
+void divide_public_vec(Rs<T> a, Loc<T> d) {
+    size_t n = a.size();
+    RsBuf<T> r(n), rPrime(n);
+    r_zero(rPrime.v());
+    r_op_l(rPrime.v(), d, ADD);
+    r_fill(r.v(), (T)1);
+    r_op_r(a, rPrime.v(), SUB);
+    LocBuf<T> rec(n);
+    reconstruct(a, rec.v());
+    for (int q = 0; q < 3; q++) dd_op(rec.v().p[q], d.p[q], DIV);
+    r_zero(a);
+    r_op_r(a, r.v(), ADD);
+    r_op_l(a, rec.v(), ADD);
+}
+
+// dividePublic3out3, RSS.inl:1076-1094.  `a` (3-of-3 shares) is modified.
+template <typename T>
+void divide_public_3out3(Loc<T> a, T d, Rs<T> out) {
+    size_t n = a.size();
+    RsBuf<T> r(n), rPrime(n);
+    r_fill(rPrime.v(), d);
+    r_fill(r.v(), (T)1);
+    for (int q = 0; q < 3; q++) dd_op(a.p[q], rPrime.v().s[q][0], SUB);
+    LocBuf<T> rec(n);
+    reconstruct3out3(a, rec.v());
+    for (int q = 0; q < 3; q++) dd_op_s(rec.v().p[q], d, DIV);
+    r_zero(out);
+    r_op_r(out, r.v(), ADD);
+    r_op_l(out, rec.v(), ADD);
+}
+
+// ---------------------------------------------------------------------------
+// Ring GEMM and convolution (gemm.cuh:64-112, conv.cuh:359-401).
+
+// gpu::gemm: transpose_x == true means row-major storage of that operand
+// (gemm.cuh:67-98 maps it to cutlass RowMajor with ld = K / N / N).
+void gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc) {
+    std::vector<u64> At((size_t)M * K), Bt((size_t)N * K);
+    for (int m = 0; m < M; m++)
+        for (int k = 0; k < K; k++) At[(size_t)m * K + k] = ta ? A[(size_t)m * K + k] : A[(size_t)k * M + m];
+    for (int n = 0; n < N; n++)
+        for (int k = 0; k < K; k++) Bt[(size_t)n * K + k] = tb ? B[(size_t)k * N + n] : B[(size_t)n * K + k];
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < M; m++) {
+        const u64 *a = &At[(size_t)m * K];
+        for (int n = 0; n < N; n++) {
+            const u64 *b = &Bt[(size_t)n * K];
+            u64 acc = 0;
+            for (int k = 0; k < K; k++) acc += a[k] * b[k];
+            if (tc) C[(size_t)m * N + n] = acc;
+            else C[(size_t)n * M + m] = acc;
+        }
+    }
+}
+
+// NHWC x [b][H][W][Din] (*) KRSC w [Dout][fh][fw][Din] -> NHWC y [b][OH][OW][Dout],
+// cross-correlation with symmetric zero padding (conv.cuh:73-91,101).
+void conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh,
+                int fw, int ph, int pw, int stride, int dil) {
+    int OH = (H + 2 * ph - ((fh - 1) * dil + 1)) / stride + 1;
+    int OW = (W + 2 * pw - ((fw - 1) * dil + 1)) / stride + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int n = 0; n < b; n++)
+        for (int oh = 0; oh < OH; oh++)
+            for (int ow = 0; ow < OW; ow++)
+                for (int co = 0; co < Dout; co++) {
+                    u64 acc = 0;
+                    for (int r = 0; r < fh; r++) {
+                        int ih = oh * stride - ph + r * dil;
+                        if (ih < 0 || ih >= H) continue;
+                        for (int s = 0; s < fw; s++) {
+                            int iw = ow * stride - pw + s * dil;
+                            if (iw < 0 || iw >= W) continue;
+                            const u64 *xp = x + (((size_t)n * H + ih) * W + iw) * Din;
+                            const u64 *wp = w + (((size_t)co * fh + r) * fw + s) * Din;
+                            for (int c = 0; c < Din; c++) acc += xp[c] * wp[c];
+                        }
+                    }
+                    y[(((size_t)n * OH + oh) * OW + ow) * Dout + co] = acc;
+                }
+}
+
+// localMatMul, RSS.inl:1096-1110: c_p += A_p0 B_p0 + A_p0 B_p1 + A_p1 B_p0
+void local_matmul(Rs<u64> a, Rs<u64> b, Loc<u64> c, int M, int N, int K, bool ta, bool tb, bool tc) {
+    std::vector<u64> acc((size_t)M * N);
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int q = 0; q < 3; q++)
+        for (auto &t : terms) {
+            gemm(M, N, K, a.s[q][t[0]].base + a.s[q][t[0]].off, ta, b.s[q][t[1]].base + b.s[q][t[1]].off, tb,
+                 acc.data(), tc);
+            for (size_t k = 0; k < acc.size(); k++) c.p[q][k] += acc[k];
+        }
+}
+
+// matmul, RSS.inl:481-540 (without the debug_overflow printouts)
+void matmul(Rs<u64> a, Rs<u64> b, Rs<u64> c, int M, int N, int K, bool ta, bool tb, bool tc, int trunc) {
+    LocBuf<u64> raw((size_t)M * N);
+    local_matmul(a, b, raw.v(), M, N, K, ta, tb, tc);
+    divide_public_3out3(raw.v(), (u64)1 << trunc, c);
+}
+
+// localFprop + convolution(kFprop), RSS.inl:613-639, 705-803
+void convolution(Rs<u64> A, Rs<u64> B, Rs<u64> C, int batch, int H, int W, int f, int Din, int Dout,
+                 int stride, int pad, int trunc) {
+    size_t outN = C.size();
+    LocBuf<u64> local(outN);
+    std::vector<u64> acc(outN);
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int q = 0; q < 3; q++)
+        for (auto &t : terms) {
+            conv_fprop(A.s[q][t[0]].base + A.s[q][t[0]].off, B.s[q][t[1]].base + B.s[q][t[1]].off, acc.data(), batch,
+                       H, W, Din, Dout, f, f, pad, pad, stride, 1);
+            for (size_t k = 0; k < outN; k++) local.d[q][k] += acc[k];
+        }
+    divide_public_3out3(local.v(), (u64)1 << trunc, C);
+}
+
+// ---------------------------------------------------------------------------
+// Bit kernels (bitwise.cuh) and layout kernels (matrix.cuh, convolution.cuh).
+
+// bitexpand, bitwise.cuh:11-21 (the `<=` read/write one element past the end is
+// not restated: it only touches memory outside the buffers).
+template <typename T, typename U>
+void bitexpand(View<T> a, View<U> b) {
+    const int nb = sizeof(T) * 8;
+    for (size_t i = 0; i < a.n; i++)
+        for (int j = 0; j < nb; j++) b[i * nb + j] = (U)((a[i] >> j) & 1);
+}
+
+// setCarryOutMSB, bitwise.cuh:41-64
+template <typename U>
+void set_carry_out_msb(View<U> rbits, View<U> abits, View<U> msb, int bitWidth, bool xor_msb) {
+    for (size_t i = 0; i < msb.n; i++) {
+        size_t bi = i * bitWidth + (bitWidth - 1);
+        msb[i] = rbits[bi];
+        if (xor_msb) msb[i] ^= abits[bi];
+        abits[bi] = 1;
+        rbits[bi] = 0;
+    }
+}
+
+// expandCompare, bitwise.cuh:66-92 and host wrapper :215-246
+template <typename U>
+void expand_compare(View<U> b, View<U> inv, View<U> expanded) {
+    size_t f = (expanded.n / b.n) / 2;
+    for (size_t i = 0; i < b.n; i++)
+        for (size_t e = 0; e < 2 * f; e++) expanded[i * 2 * f + e] = (e < f ? b : inv)[i];
+}
+
+// elementVectorAdd, matrix.cuh:76-84
+template <typename T>
+void element_vector_add(View<T> a, View<T> b, bool rowwise, int rows, int cols) {
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) a[(size_t)c * rows + r] += b[rowwise ? c : r];
+}
+
+// truncate_cols, matrix.cuh:142-155, host :383-400 (rows = a.size / start_cols)
+template <typename T>
+void truncate_cols(View<T> a, View<T> b, int start_cols, int end_cols) {
+    size_t rows = a.n / start_cols;
+    for (size_t r = 0; r < rows; r++)
+        for (int i = 0; i < end_cols; i++) b[r * end_cols + i] = a[r * start_cols + i];
+}
+
+// stride_pad kernel, convolution.cuh:175-191: every block of `stride` values is
+// followed by `pad` copies of pad_value.
+template <typename T>
+void stride_pad(const T *in, T *out, size_t blocks, int stride, int pad, T pad_value) {
+    for (size_t j = 0; j < blocks; j++) {
+        for (int i = 0; i < stride; i++) out[j * (stride + pad) + i] = in[j * stride + i];
+        for (int i = 0; i < pad; i++) out[j * (stride + pad) + stride + i] = pad_value;
+    }
+}
+
+inline int expanded_pool_size(int f) {
+    // pow(ceil(log2(f*f)), 2), convolution.cuh:288 and MaxpoolLayer.cu:69
+    return (int)std::pow(std::ceil(std::log2((double)(f * f))), 2);
+}
+
+// maxpool_im2row, convolution.cuh:57-108 and host :252-304.  The host passes
+// (imageHeight, imageWidth) into the kernel's (imageWidth, imageHeight) slots,
+// which makes the gather plain NHWC: x runs over rows, y over columns.
+// Out-of-image taps are skipped without advancing the output index.
+template <typename T>
+void maxpool_im2row(const T *im, std::vector<T> &out, int H, int W, int f, int Din, int batch, int stride,
+                    int padding, T pad_val) {
+    int kW = H, kH = W;  // kernel-side names after the swap
+    int widthKernels = ((kW - f + 2 * padding) / stride) + 1;
+    int heightKernels = ((kH - f + 2 * padding) / stride) + 1;
+    size_t windows = (size_t)widthKernels * heightKernels;
+    size_t pool = (size_t)f * f;
+    std::vector<T> unpadded(pool * windows * batch * Din, 0);
+    for (int ic = 0; ic < batch * Din; ic++)
+        for (size_t wi = 0; wi < windows; wi++) {
+            int bb = ic / Din, ch = ic % Din;
+            int baseCol = (int)(wi / heightKernels) * stride - padding;
+            int baseRow = (int)(wi % heightKernels) * stride - padding;
+            size_t o = (size_t)bb * (windows * Din * pool) + wi * (Din * pool) + (size_t)ch * pool;
+            for (int c = 0; c < f; c++)
+                for (int r = 0; r < f; r++) {
+                    int y = baseRow + r, x = baseCol + c;
+                    if (y < 0 || y >= kH || x < 0 || x >= kW) continue;
+                    size_t src = (size_t)bb * kH * kW * Din + (size_t)x * kH * Din + (size_t)y * Din + ch;
+                    unpadded[o++] = im[src];
+                }
+        }
+    int padded = expanded_pool_size(f);
+    out.assign((size_t)padded * windows * batch * Din, 0);
+    if ((size_t)padded == pool) {
+        std::copy(unpadded.begin(), unpadded.end(), out.begin());
+    } else {
+        stride_pad(unpadded.data(), out.data(), windows * batch * Din, (int)pool, padded - (int)pool, pad_val);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Comparison protocols.
+
+// carryOut, RSS.inl:1112-1209 (strided views over contiguous p and g)
+template <typename U>
+void carry_out(Rs<U> p, Rs<U> g, int k, Rs<U> out) {
+    int stride = 2, offset = 1;
+    Rs<U> pEven = stridedRs(p, 0, stride), pOdd = stridedRs(p, offset, stride);
+    Rs<U> gEven = stridedRs(g, 0, stride), gOdd = stridedRs(g, offset, stride);
+    while (k > 1) {
+        r_and(gEven, pOdd);
+        r_and(pEven, pOdd);
+        r_op_r(gOdd, gEven, XOR);
+        gEven = stridedRs(g, offset, stride * 2);
+        offset += stride;
+        gOdd = stridedRs(g, offset, stride * 2);
+        stride *= 2;
+        pEven = stridedRs(p, 0, stride);
+        pOdd = stridedRs(p, stride / 2, stride);
+        k /= 2;
+    }
+    Rs<U> outEven = stridedRs(out, 0, 2), outOdd = stridedRs(out, 1, 2);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) {
+            for (size_t k2 = 0; k2 < gEven.s[q][i].n; k2++) outEven.s[q][i][k2] = gEven.s[q][i][k2];
+            for (size_t k2 = 0; k2 < gOdd.s[q][i].n; k2++) outOdd.s[q][i][k2] = gOdd.s[q][i][k2];
+        }
+}
+
+// dReLU, RSS.inl:806-852
+template <typename U>
+void drelu(Rs<u64> input, Rs<U> result) {
+    const int bw = 64;
+    size_t n = input.size();
+    RsBuf<u64> r(n);
+    RsBuf<U> rbits(n * bw);
+    r_fill(rbits.v(), (U)1);
+    LocBuf<u64> a(n);
+    r_op_r(r.v(), input, ADD);
+    reconstruct(r.v(), a.v());
+    for (int q = 0; q < 3; q++) dd_op_s(a.v().p[q], (u64)1, ADD);
+    LocBuf<U> abits(n * bw);
+    for (int q = 0; q < 3; q++) bitexpand(a.v().p[q], abits.v().p[q]);
+    RsBuf<U> msb(n);
+    Rs<U> rb = rbits.v(), mb = msb.v();
+    Loc<U> ab = abits.v();
+    for (int q = 0; q < 3; q++) {
+        if (q != PC) {
+            set_carry_out_msb(rb.s[q][0], ab.p[q], mb.s[q][0], bw, q == PA);
+            set_carry_out_msb(rb.s[q][1], ab.p[q], mb.s[q][1], bw, q == PC);
+        } else {
+            set_carry_out_msb(rb.s[q][1], ab.p[q], mb.s[q][1], bw, q == PC);
+            set_carry_out_msb(rb.s[q][0], ab.p[q], mb.s[q][0], bw, q == PA);
+        }
+    }
+    RsBuf<U> g(n * bw);
+    r_zero(g.v());
+    r_op_r(g.v(), rb, ADD);
+    r_op_l(g.v(), ab, AND);
+    RsBuf<U> pp(n * bw);
+    r_zero(pp.v());
+    r_op_r(pp.v(), rb, ADD);
+    r_op_l(pp.v(), ab, XOR);
+    RsBuf<U> pre(result.size());
+    carry_out(pp.v(), g.v(), bw, pre.v());
+    r_op_r(pre.v(), mb, XOR);
+    r_fill(result, (U)1);
+    r_op_r(result, pre.v(), SUB);
+}
+
+// convex_comb + rss_convex_comb_functor, RSS.inl:33-61, 1321-1333
+template <typename U>
+void convex_comb(Rs<u64> a, Rs<u64> c, Loc<U> b) {
+    for (int q = 0; q < 3; q++)
+        for (size_t k = 0; k < b.size(); k++) {
+            u64 c0 = c.s[q][0][k], c1 = c.s[q][1][k];
+            if (b.p[q][k] == 1) {
+                switch (q) {
+                    case PA: a.s[q][0][k] = 1 - c0; a.s[q][1][k] = (u64)0 - c1; break;
+                    case PB: a.s[q][0][k] = (u64)0 - c0; a.s[q][1][k] = (u64)0 - c1; break;
+                    case PC: a.s[q][0][k] = (u64)0 - c0; a.s[q][1][k] = 1 - c1; break;
+                }
+            } else {
+                a.s[q][0][k] = c0;
+                a.s[q][1][k] = c1;
+            }
+        }
+}
+
+// selectShare, RSS.inl:542-569: z = b ? y : x
+template <typename U>
+void select_share(Rs<u64> x, Rs<u64> y, Rs<U> b, Rs<u64> z) {
+    size_t n = x.size();
+    RsBuf<u64> c(n);
+    RsBuf<U> cbits(b.size());
+    r_op_r(cbits.v(), b, XOR);
+    LocBuf<U> e(cbits.size());
+    reconstruct(cbits.v(), e.v());
+    RsBuf<u64> d(e.size());
+    convex_comb(d.v(), c.v(), e.v());
+    RsBuf<u64> result(n);
+    r_op_r(result.v(), y, ADD);
+    r_op_r(result.v(), x, SUB);
+    r_mult(result.v(), d.v());
+    r_op_r(result.v(), x, ADD);
+    r_zero(z);
+    r_op_r(z, result.v(), ADD);
+}
+
+// ReLU, RSS.inl:854-865
+template <typename U>
+void relu(Rs<u64> input, Rs<u64> result, Rs<U> dresult) {
+    drelu(input, dresult);
+    RsBuf<u64> zeros(input.size());
+    select_share(zeros.v(), input, dresult, result);
+}
+
+// maxpool, RSS.inl:867-1017 (selectShare writes into the even view of `input`)
+template <typename U>
+void maxpool(Rs<u64> input, Rs<u64> result, Rs<U> dresult, int k) {
+    r_fill(dresult, (U)1);
+    int stride = 2, offset = 1;
+    Rs<u64> even = stridedRs(input, 0, stride), odd = stridedRs(input, offset, stride);
+    auto round = [&](bool last) {
+        RsBuf<u64> diff(even.size());
+        r_zero(diff.v());
+        r_op_r(diff.v(), even, ADD);
+        r_op_r(diff.v(), odd, SUB);
+        RsBuf<U> b(even.size());
+        drelu(diff.v(), b.v());
+        select_share(odd, even, b.v(), even);
+        if (!last) {
+            stride *= 2;
+            even = stridedRs(input, 0, stride);
+            odd = stridedRs(input, stride / 2, stride);
+        } else {
+            r_zero(result);
+            r_op_r(result, even, ADD);
+        }
+        RsBuf<U> negated(b.size());
+        r_fill(negated.v(), (U)1);
+        r_op_r(negated.v(), b.v(), SUB);
+        RsBuf<U> expandedB(input.size());
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 2; i++)
+                expand_compare(b.v().s[q][i], negated.v().s[q][i], expandedB.v().s[q][i]);
+        r_and(dresult, expandedB.v());
+    };
+    while (k > 2) {
+        round(false);
+        k /= 2;
+    }
+    round(true);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C interface used by the Python test harness (oracle/__init__.py).
+// Rss arguments are [3 parties][2 shares][n] contiguous arrays; Loc arguments
+// are [3 parties][n].
+
+extern "C" {
+
+int ref_version() { return 1; }
+
+void ref_gemm(int M, int N, int K, const u64 *A, int ta, const u64 *B, int tb, u64 *C, int tc) {
+    gemm(M, N, K, A, ta != 0, B, tb != 0, C, tc != 0);
+}
+
+void ref_conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
+                    int ph, int pw, int stride, int dil) {
+    conv_fprop(x, w, y, b, H, W, Din, Dout, fh, fw, ph, pw, stride, dil);
+}
+
+void ref_bitexpand_u8(const u64 *a, size_t n, uint8_t *b) {
+    bitexpand(View<u64>{(u64 *)a, 0, 1, n}, View<uint8_t>{b, 0, 1, n * 64});
+}
+void ref_bitexpand_u64(const u64 *a, size_t n, u64 *b) {
+    bitexpand(View<u64>{(u64 *)a, 0, 1, n}, View<u64>{b, 0, 1, n * 64});
+}
+
+void ref_set_carry_out_msb_u8(uint8_t *rbits, uint8_t *abits, uint8_t *msb, size_t n, int bitWidth, int xor_msb) {
+    set_carry_out_msb(View<uint8_t>{rbits, 0, 1, n * bitWidth}, View<uint8_t>{abits, 0, 1, n * bitWidth},
+                      View<uint8_t>{msb, 0, 1, n}, bitWidth, xor_msb != 0);
+}
+
+void ref_expand_compare_u8(const uint8_t *b, const uint8_t *inv, uint8_t *out, size_t n, size_t out_n) {
+    expand_compare(View<uint8_t>{(uint8_t *)b, 0, 1, n}, View<uint8_t>{(uint8_t *)inv, 0, 1, n},
+                   View<uint8_t>{out, 0, 1, out_n});
+}
+
+void ref_element_vector_add(u64 *a, const u64 *b, int rowwise, int rows, int cols) {
+    element_vector_add(View<u64>{a, 0, 1, (size_t)rows * cols}, View<u64>{(u64 *)b, 0, 1, (size_t)(rowwise ? cols : rows)},
+                       rowwise != 0, rows, cols);
+}
+
+void ref_truncate_cols_u8(const uint8_t *a, size_t a_n, uint8_t *b, int start_cols, int end_cols) {
+    truncate_cols(View<uint8_t>{(uint8_t *)a, 0, 1, a_n}, View<uint8_t>{b, 0, 1, a_n / start_cols * end_cols},
+                  start_cols, end_cols);
+}
+
+void ref_stride_pad(const u64 *in, u64 *out, size_t n_in, int stride, int pad, u64 pad_value) {
+    stride_pad(in, out, n_in / stride, stride, pad, pad_value);
+}
+
+int ref_expanded_pool_size(int f) { return expanded_pool_size(f); }
+
+size_t ref_maxpool_im2row(const u64 *im, u64 *out, int H, int W, int f, int Din, int batch, int stride, int padding,
+                          u64 pad_val) {
+    std::vector<u64> o;
+    maxpool_im2row(im, o, H, W, f, Din, batch, stride, padding, pad_val);
+    if (out) std::copy(o.begin(), o.end(), out);
+    return o.size();
+}
+
+// --- share constructors -----------------------------------------------------
+void ref_rss_set_public(u64 *x, size_t n, const double *vals, int fp) { r_set_public(wrapRs(x, n), vals, fp); }
+void ref_rss_init_list(u64 *x, size_t n, const double *vals, int convert, int fp) {
+    r_init_list(wrapRs(x, n), vals, convert != 0, fp);
+}
+void ref_rss_fill_u64(u64 *x, size_t n, u64 v) { r_fill(wrapRs(x, n), v); }
+void ref_rss_fill_u8(uint8_t *x, size_t n, uint8_t v) { r_fill(wrapRs(x, n), v); }
+
+// --- protocol ops -------------------------------------------------------------
+void ref_reconstruct_u64(u64 *in, size_t n, u64 *out) { reconstruct(wrapRs(in, n), wrapLoc(out, n)); }
+void ref_reconstruct_u8(uint8_t *in, size_t n, uint8_t *out) { reconstruct(wrapRs(in, n), wrapLoc(out, n)); }
+void ref_reshare_u64(u64 *c, size_t n, u64 *out) { reshare(wrapLoc(c, n), wrapRs(out, n)); }
+void ref_reconstruct3out3(u64 *in, size_t n, u64 *out) { reconstruct3out3(wrapLoc(in, n), wrapLoc(out, n)); }
+void ref_rss_mult(u64 *x, const u64 *y, size_t n) { r_mult(wrapRs(x, n), wrapRs((u64 *)y, n)); }
+void ref_rss_and_u8(uint8_t *x, const uint8_t *y, size_t n) { r_and(wrapRs(x, n), wrapRs((uint8_t *)y, n)); }
+void ref_rss_and_u64(u64 *x, const u64 *y, size_t n) { r_and(wrapRs(x, n), wrapRs((u64 *)y, n)); }
+void ref_rss_op_rss(u64 *x, const u64 *y, size_t n, int op) { r_op_r(wrapRs(x, n), wrapRs((u64 *)y, n), (Op)op); }
+void ref_rss_op_local(u64 *x, const u64 *y, size_t n, int op) { r_op_l(wrapRs(x, n), wrapLoc((u64 *)y, n), (Op)op); }
+void ref_rss_op_scalar(u64 *x, size_t n, u64 v, int op) { r_op_s(wrapRs(x, n), v, (Op)op); }
+void ref_divide_public(u64 *a, size_t n, u64 d) { divide_public(wrapRs(a, n), d); }
+void ref_divide_public_vec(u64 *a, size_t n, const u64 *d) { divide_public_vec(wrapRs(a, n), wrapLoc((u64 *)d, n)); }
+void ref_divide_public_3out3(u64 *a, size_t n, u64 d, u64 *out) {
+    divide_public_3out3(wrapLoc(a, n), d, wrapRs(out, n));
+}
+u64 ref_scalar_divide(u64 x, u64 d) { return sdiv<u64>(x, d); }
+
+void ref_matmul(const u64 *a, size_t a_n, const u64 *b, size_t b_n, u64 *c, int M, int N, int K, int ta, int tb,
+                int tc, int trunc) {
+    matmul(wrapRs((u64 *)a, a_n), wrapRs((u64 *)b, b_n), wrapRs(c, (size_t)M * N), M, N, K, ta != 0, tb != 0,
+           tc != 0, trunc);
+}
+void ref_local_matmul(const u64 *a, size_t a_n, const u64 *b, size_t b_n, u64 *c, int M, int N, int K, int ta, int tb,
+                      int tc) {
+    local_matmul(wrapRs((u64 *)a, a_n), wrapRs((u64 *)b, b_n), wrapLoc(c, (size_t)M * N), M, N, K, ta != 0,
+                 tb != 0, tc != 0);
+}
+
+void ref_convolution(const u64 *A, size_t a_n, const u64 *B, size_t b_n, u64 *C, size_t c_n, int batch, int H, int W,
+                     int f, int Din, int Dout, int stride, int pad, int trunc) {
+    convolution(wrapRs((u64 *)A, a_n), wrapRs((u64 *)B, b_n), wrapRs(C, c_n), batch, H, W, f, Din, Dout, stride, pad,
+                trunc);
+}
+
+void ref_drelu_u8(const u64 *in, size_t n, uint8_t *out) { drelu(wrapRs((u64 *)in, n), wrapRs(out, n)); }
+void ref_drelu_u64(const u64 *in, size_t n, u64 *out) { drelu(wrapRs((u64 *)in, n), wrapRs(out, n)); }
+void ref_relu_u8(const u64 *in, size_t n, u64 *out, uint8_t *dout) {
+    relu(wrapRs((u64 *)in, n), wrapRs(out, n), wrapRs(dout, n));
+}
+void ref_relu_u64(const u64 *in, size_t n, u64 *out, u64 *dout) {
+    relu(wrapRs((u64 *)in, n), wrapRs(out, n), wrapRs(dout, n));
+}
+void ref_select_share_u8(const u64 *x, const u64 *y, const uint8_t *b, u64 *z, size_t n) {
+    select_share(wrapRs((u64 *)x, n), wrapRs((u64 *)y, n), wrapRs((uint8_t *)b, n), wrapRs(z, n));
+}
+void ref_select_share_u64(const u64 *x, const u64 *y, const u64 *b, u64 *z, size_t n) {
+    select_share(wrapRs((u64 *)x, n), wrapRs((u64 *)y, n), wrapRs((u64 *)b, n), wrapRs(z, n));
+}
+// maxpool over an already gathered pools buffer (modified in place, as in the reference)
+void ref_maxpool_u8(u64 *pools, size_t n, u64 *result, uint8_t *dresult, int k) {
+    maxpool(wrapRs(pools, n), wrapRs(result, n / k), wrapRs(dresult, n), k);
+}
+void ref_maxpool_u64(u64 *pools, size_t n, u64 *result, u64 *dresult, int k) {
+    maxpool(wrapRs(pools, n), wrapRs(result, n / k), wrapRs(dresult, n), k);
+}
+
+// --- layers (forward only) ------------------------------------------------------
+// FCLayer::forward, FCLayer.cu:78-130.  W is [in][out] row-major, bias [out].
+void ref_fc_forward(const u64 *input, const u64 *W, const u64 *bias, u64 *act, int batch, int in, int out, int fp) {
+    size_t actN = (size_t)batch * out;
+    Rs<u64> A = wrapRs(act, actN);
+    r_zero(A);
+    matmul(wrapRs((u64 *)input, (size_t)batch * in), wrapRs((u64 *)W, (size_t)in * out), A, batch, out, in, true,
+           true, true, fp);
+    Rs<u64> Bv = wrapRs((u64 *)bias, (size_t)out);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) element_vector_add(A.s[q][i], Bv.s[q][i], false, out, batch);
+}
+
+// CNNLayer::forward, CNNLayer.cu:81-115.  W is [Dout][f][f][Din].
+void ref_cnn_forward(const u64 *input, const u64 *W, u64 *act, int batch, int H, int Wd, int Din, int Dout, int f,
+                     int stride, int pad, int fp) {
+    int OH = (H - f + 2 * pad) / stride + 1, OW = (Wd - f + 2 * pad) / stride + 1;
+    size_t actN = (size_t)batch * OH * OW * Dout;
+    Rs<u64> A = wrapRs(act, actN);
+    r_zero(A);
+    convolution(wrapRs((u64 *)input, (size_t)batch * H * Wd * Din), wrapRs((u64 *)W, (size_t)Dout * f * f * Din), A,
+                batch, H, Wd, f, Din, Dout, stride, pad, fp);
+}
+
+// ReLULayer::forward, ReLULayer.cu:51-88
+void ref_relu_forward(const u64 *input, size_t n, u64 *act, uint8_t *relu_prime) {
+    Rs<u64> A = wrapRs(act, n);
+    r_zero(A);
+    relu(wrapRs((u64 *)input, n), A, wrapRs(relu_prime, n));
+}
+
+// MaxpoolLayer::forward, MaxpoolLayer.cu:52-131.  max_prime has batch*C*OH*OW*pool*pool entries.
+void ref_maxpool_forward(const u64 *input, u64 *act, uint8_t *max_prime, int batch, int H, int W, int C, int pool,
+                         int stride, int fp) {
+    int OH = (H - pool) / stride + 1, OW = (W - pool) / stride + 1;
+    size_t actN = (size_t)batch * C * OH * OW;
+    size_t mpN = actN * pool * pool;
+    Rs<u64> A = wrapRs(act, actN);
+    Rs<uint8_t> MP = wrapRs(max_prime, mpN);
+    r_zero(A);
+    r_zero(MP);
+    int eps = expanded_pool_size(pool);
+    u64 pad_value = (u64)(s64)(-10 * (1 << fp));
+    size_t inN = (size_t)batch * H * W * C;
+    RsBuf<u64> pools(0);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            maxpool_im2row(input + (2 * q + i) * inN, pools.d[q][i], H, W, pool, C, batch, stride, 0, pad_value);
+    RsBuf<uint8_t> emp(pools.size());
+    maxpool(pools.v(), A, emp.v(), eps);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) truncate_cols(emp.v().s[q][i], MP.s[q][i], eps, pool * pool);
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// common.h -- shared types, error handling and small device helpers for the
+// MI355X (gfx950) secret-share compute path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace pir {
+
+typedef uint64_t u64;
+typedef int64_t s64;
+typedef uint32_t u32;
+typedef uint8_t u8;
+
+// Party indices of the 3-party replicated sharing (RSS.h:24, enum Party).
+enum { PARTY_A = 0, PARTY_B = 1, PARTY_C = 2, NUM_PARTIES = 3 };
+
+__host__ __device__ inline int next_party(int p) { return p == 2 ? 0 : p + 1; }
+__host__ __device__ inline int prev_party(int p) { return p == 0 ? 2 : p - 1; }
+
+// Status codes returned through the C ABI.
+enum Status { OK = 0, ERR_ARG = 1, ERR_HIP = 2, ERR_UNSUPPORTED = 3, ERR_STATE = 4 };
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+#define PIR_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            ::pir::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));               \
+            return ::pir::ERR_HIP;                                                             \
+        }                                                                                      \
+    } while (0)
+
+#define PIR_REQUIRE(cond, msg)                                                                 \
+    do {                                                                                       \
+        if (!(cond)) {                                                                         \
+            ::pir::set_error(msg);                                                             \
+            return ::pir::ERR_ARG;                                                             \
+        }                                                                                      \
+    } while (0)
+
+#define PIR_TRY(expr)                                                                          \
+    do {                                                                                       \
+        int _s = (expr);                                                                       \
+        if (_s != 0) return _s;                                                                \
+    } while (0)
+
+// Launch check: kernel launches do not return a status, query it right away.
+#define PIR_LAUNCH_CHECK() PIR_HIP(hipGetLastError())
+
+inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// Emulates the reference truncation q = (S)((double)(S)t / (double)(S)2^k)
+// (functors.cuh:33-45) in integer arithmetic: the int64 -> double conversion
+// rounds |t| to 53 significant bits (IEEE round-to-nearest-even, sign
+// symmetric), the division by 2^k is exact, and the conversion back truncates
+// toward zero.  Valid for 1 <= k <= 62.
+__host__ __device__ inline u64 trunc_div_pow2(u64 t, int k) {
+    bool neg = (s64)t < 0;
+    u64 mag = neg ? (u64)0 - t : t;
+    if (mag >> 53) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        int lz = __clzll((long long)mag);
+#else
+        int lz = __builtin_clzll(mag);
+#endif
+        int shift = 11 - lz;  // bit length - 53
+        u64 rem = mag & ((1ull << shift) - 1);
+        u64 half = 1ull << (shift - 1);
+        u64 base = mag >> shift;
+        if (rem > half || (rem == half && (base & 1))) base++;
+        mag = base << shift;
+    }
+    u64 q = mag >> k;
+    return neg ? (u64)0 - q : q;
+}
+
+}  // namespace pir

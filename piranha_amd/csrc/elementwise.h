@@ -1,0 +1,38 @@
+// elementwise.h -- DeviceData element ops and reference bit / layout kernels.
+#pragma once
+
+#include "common.h"
+
+namespace pir {
+
+enum EwOp { OP_ADD = 0, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV, OP_SET };
+
+// Up to six strided element-op pairs in one launch: x[a][i*xs] op= y[a][i*ys]
+// (or op= scalar[a] when y[a] is null).
+struct MultiOp {
+    void *x[6];
+    long long xs[6];
+    const void *y[6];
+    long long ys[6];
+    u64 scalar[6];
+    int count;
+    int op;
+    size_t n;
+};
+
+int multi_op(const MultiOp &m, int elem_bytes, hipStream_t s);
+int dd_op(int op, int elem_bytes, void *x, long long xs, const void *y, long long ys, u64 scalar, size_t n,
+          hipStream_t s);
+int fill_u64(u64 *p, u64 v, size_t n, hipStream_t s);
+int fill_u8(u8 *p, u8 v, size_t n, hipStream_t s);
+int bitexpand(const u64 *a, size_t n, void *bits, int elem_bytes, hipStream_t s);
+int set_carry_out_msb(void *rbits, void *abits, void *msb, size_t n, int bw, int xr, int eb, hipStream_t s);
+int expand_compare(const void *b, const void *inv, void *out, size_t n, size_t out_n, int eb, hipStream_t s);
+int element_vector_add(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s);
+int truncate_cols(const void *a, size_t a_n, void *b, int sc, int ec, int eb, hipStream_t s);
+int stride_pad(const u64 *in, u64 *out, size_t n_in, int stride, int pad, u64 pv, hipStream_t s);
+size_t maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);
+int maxpool_im2row(const u64 *im, u64 *out, int H, int W, int f, int Din, int batch, int stride, int padding, u64 pv,
+                   hipStream_t s);
+
+}  // namespace pir

@@ -1,0 +1,36 @@
+// fused_rss.h -- fused 3-party RSS protocol kernels (host launchers).
+// All share tensors are [3 parties][2 shares][n] arrays; party-local tensors
+// are [3][n].  Every launcher is asynchronous on the given stream.
+#pragma once
+
+#include "common.h"
+
+namespace pir {
+
+struct MaxpoolGeom {
+    int batch, H, W, C, f, stride, OH, OW;
+    u64 pad_value;
+};
+
+int expanded_pool_size(int f);
+
+template <typename U>
+int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s);
+template <typename U>
+int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s);
+template <typename U>
+int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s);
+template <typename U>
+int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s);
+template <typename U>
+int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s);
+template <typename T>
+int fused_reconstruct(const T *x, size_t n, T *out, hipStream_t s);
+int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s);
+int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s);
+template <typename T>
+int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s);
+int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, int *bad_flag, hipStream_t s);
+int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s);
+
+}  // namespace pir

@@ -1,0 +1,390 @@
+// fused_rss.hip -- fused, wavefront-level 3-party RSS protocol kernels.
+//
+// Each kernel reads every party's shares of an element once, runs the whole
+// protocol step of the reference for all three parties in registers (the
+// inter-party messages are register moves) and writes each output share once.
+// Reference functions restated: dReLU / carryOut (RSS.inl:806-852,1112-1209),
+// selectShare (RSS.inl:542-569), ReLU (RSS.inl:854-865), maxpool
+// (RSS.inl:867-1017) with maxpool_im2row / stride_pad / expandCompare /
+// truncate_cols (convolution.cuh:57-108,176-191; bitwise.cuh:66-92;
+// matrix.cuh:142-155), reconstruct (RSS.inl:457-479), reshare
+// (RSS.inl:1052-1074), RSS *= and &= (RSS.inl:283-345), dividePublic
+// (RSS.inl:422-455) and dividePublic3out3 (RSS.inl:1076-1094).
+#include "fused_rss.h"
+#include "rss_device.h"
+
+namespace pir {
+
+constexpr int EW_BLOCK = 256;
+
+static unsigned ew_grid(size_t n) {
+    size_t g = (n + EW_BLOCK - 1) / EW_BLOCK;
+    return (unsigned)(g == 0 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------
+// dReLU / ReLU
+
+template <typename U, bool RELU>
+__global__ __launch_bounds__(EW_BLOCK) void relu_kernel(const u64 *__restrict__ x, size_t n, U *__restrict__ dres,
+                                                        u64 *__restrict__ y) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<u64> in = load3(x, n, i);
+    Sh3<U> b = drelu_<U>(in);
+    store3(dres, n, i, b);
+    if (RELU) {
+        Sh3<u64> zero;
+#pragma unroll
+        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+        store3(y, n, i, select_<U>(zero, in, b));
+    }
+}
+
+template <typename U>
+int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((relu_kernel<U, false>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, (u64 *)nullptr);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename U>
+int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((relu_kernel<U, true>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, y);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// selectShare
+
+template <typename U>
+__global__ __launch_bounds__(EW_BLOCK) void select_kernel(const u64 *__restrict__ x, const u64 *__restrict__ y,
+                                                          const U *__restrict__ b, u64 *__restrict__ z, size_t n) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    store3(z, n, i, select_<U>(load3(x, n, i), load3(y, n, i), load3(b, n, i)));
+}
+
+template <typename U>
+int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((select_kernel<U>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, b, z, n);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// maxpool tournament over K (a power of two) gathered values of one window.
+
+template <typename U>
+__device__ __forceinline__ Sh3<U> fill1_() {
+    Sh3<U> v;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        v.s[p][0] = (U)(p == PARTY_A ? 1 : 0);
+        v.s[p][1] = (U)(p == PARTY_C ? 1 : 0);
+    }
+    return v;
+}
+
+template <typename U>
+__device__ __forceinline__ void rss_and_elem(Sh3<U> &x, const Sh3<U> &y) {
+    U z[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) z[p] = (U)((x.s[p][0] & (U)(y.s[p][0] ^ y.s[p][1])) ^ (x.s[p][1] & y.s[p][0]));
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        x.s[p][0] = z[p];
+        x.s[p][1] = z[next_party(p)];
+    }
+}
+
+// RSS.inl:867-1017 for one window: even/odd views at stride 2, 4, ..., K.
+template <typename U, int K>
+__device__ __forceinline__ void tournament(Sh3<u64> (&e)[K], Sh3<U> (&d)[K]) {
+#pragma unroll
+    for (int j = 0; j < K; j++) d[j] = fill1_<U>();
+#pragma unroll
+    for (int stride = 2; stride <= K; stride *= 2) {
+        const int cnt = K / stride;
+        Sh3<U> b[K / 2];
+#pragma unroll
+        for (int j = 0; j < cnt; j++) {
+            const Sh3<u64> &ev = e[j * stride];
+            const Sh3<u64> &od = e[j * stride + stride / 2];
+            Sh3<u64> diff;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int s = 0; s < 2; s++) diff.s[p][s] = ev.s[p][s] - od.s[p][s];
+            b[j] = drelu_<U>(diff);
+            e[j * stride] = select_<U>(od, ev, b[j]);
+        }
+        // expandCompare(b, 1 - b) then dresult &= expanded
+#pragma unroll
+        for (int j = 0; j < cnt; j++) {
+            Sh3<U> neg;
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                neg.s[p][0] = (U)((U)(p == PARTY_A ? 1 : 0) - b[j].s[p][0]);
+                neg.s[p][1] = (U)((U)(p == PARTY_C ? 1 : 0) - b[j].s[p][1]);
+            }
+#pragma unroll
+            for (int t = 0; t < stride; t++) rss_and_elem<U>(d[j * stride + t], t < stride / 2 ? b[j] : neg);
+        }
+    }
+}
+
+// Free-function form over an already gathered pools buffer [windows][K]; the
+// even positions are written back, as the reference's strided views do.
+template <typename U, int K>
+__global__ __launch_bounds__(EW_BLOCK) void maxpool_pools_kernel(u64 *__restrict__ pools, size_t windows,
+                                                                 u64 *__restrict__ result, U *__restrict__ dres) {
+    size_t w = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (w >= windows) return;
+    const size_t n = windows * K;
+    Sh3<u64> e[K];
+    Sh3<U> d[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) e[j] = load3(pools, n, w * K + j);
+    tournament<U, K>(e, d);
+    store3(result, windows, w, e[0]);
+#pragma unroll
+    for (int j = 0; j < K; j += 2) store3(pools, n, w * K + j, e[j]);
+#pragma unroll
+    for (int j = 0; j < K; j++) store3(dres, n, w * K + j, d[j]);
+}
+
+template <typename U>
+int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s) {
+    PIR_REQUIRE(k > 0 && n % k == 0, "maxpool: pools size must be a multiple of k");
+    size_t windows = n / k;
+    if (windows == 0) return OK;
+    switch (k) {
+        case 2: hipLaunchKernelGGL((maxpool_pools_kernel<U, 2>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
+        case 4: hipLaunchKernelGGL((maxpool_pools_kernel<U, 4>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
+        case 8: hipLaunchKernelGGL((maxpool_pools_kernel<U, 8>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
+        case 16: hipLaunchKernelGGL((maxpool_pools_kernel<U, 16>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
+        default: set_error("maxpool: k must be 2, 4, 8 or 16"); return ERR_UNSUPPORTED;
+    }
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// Layer form (MaxpoolLayer.cu:52-131): gathers each f x f window straight
+// from the NHWC activation (maxpool_im2row order: row offset outer, column
+// offset inner), pads to K with pad_value in every share (stride_pad; the RSS
+// layer uses the same pad value for both shares of every party), runs the
+// tournament and keeps the first f*f derivative entries (truncate_cols).
+template <typename U, int K>
+__global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
+                                                                 u64 *__restrict__ act, U *__restrict__ mprime) {
+    const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    size_t o = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (o >= outN) return;
+    const size_t inN = (size_t)g.batch * g.H * g.W * g.C;
+    int c = (int)(o % g.C);
+    size_t r = o / g.C;
+    int ow = (int)(r % g.OW);
+    r /= g.OW;
+    int oh = (int)(r % g.OH);
+    int b = (int)(r / g.OH);
+    Sh3<u64> e[K];
+    Sh3<U> d[K];
+    const int f2 = g.f * g.f;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        if (j < f2) {
+            int dh = j / g.f, dw = j - dh * g.f;
+            int ih = oh * g.stride + dh, iw = ow * g.stride + dw;
+            e[j] = load3(x, inN, (((size_t)b * g.H + ih) * g.W + iw) * g.C + c);
+        } else {
+#pragma unroll
+            for (int p = 0; p < 3; p++) e[j].s[p][0] = e[j].s[p][1] = g.pad_value;
+        }
+    }
+    tournament<U, K>(e, d);
+    store3(act, outN, o, e[0]);
+    const size_t mpN = outN * f2;
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (j < f2) store3(mprime, mpN, o * f2 + j, d[j]);
+}
+
+template <typename U>
+int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s) {
+    PIR_REQUIRE(g.f >= 2 && g.stride >= 1 && g.H >= g.f && g.W >= g.f, "maxpool layer: invalid geometry");
+    PIR_REQUIRE(g.OH == (g.H - g.f) / g.stride + 1 && g.OW == (g.W - g.f) / g.stride + 1,
+                "maxpool layer: inconsistent output size");
+    size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    if (outN == 0) return OK;
+    int k = expanded_pool_size(g.f);
+    switch (k) {
+        case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime); break;
+        case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime); break;
+        default: set_error("maxpool layer: only 2x2 and 3x3 pools are supported"); return ERR_UNSUPPORTED;
+    }
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+int expanded_pool_size(int f) {
+    // pow(ceil(log2(f*f)), 2) (convolution.cuh:288, MaxpoolLayer.cu:69)
+    int f2 = f * f, c = 0;
+    while ((1 << c) < f2) c++;
+    return c * c;
+}
+
+// ---------------------------------------------------------------------------
+// Share algebra with communication.
+
+template <typename T>
+__global__ __launch_bounds__(EW_BLOCK) void reconstruct_kernel(const T *__restrict__ x, size_t n, T *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<T> v = load3(x, n, i);
+#pragma unroll
+    for (int p = 0; p < 3; p++) out[(size_t)p * n + i] = (T)(v.s[p][0] + v.s[p][1] + v.s[prev_party(p)][0]);
+}
+
+template <typename T>
+int fused_reconstruct(const T *x, size_t n, T *out, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((reconstruct_kernel<T>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(EW_BLOCK) void reshare_kernel(const T *__restrict__ c, size_t n, T *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    T z[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) z[p] = c[(size_t)p * n + i];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        out[(size_t)(2 * p) * n + i] = z[p];
+        out[(size_t)(2 * p + 1) * n + i] = z[next_party(p)];
+    }
+}
+
+int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((reshare_kernel<u64>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, c, n, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename T, bool AND>
+__global__ __launch_bounds__(EW_BLOCK) void rss_mul_kernel(T *__restrict__ x, const T *__restrict__ y, size_t n) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<T> a = load3(x, n, i), b = load3(y, n, i);
+    if (AND) rss_and_elem<T>(a, b);
+    else rss_mult_<T>(a, b);
+    store3(x, n, i, a);
+}
+
+int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((rss_mul_kernel<u64, false>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, n);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename T>
+int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((rss_mul_kernel<T, true>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, n);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// dividePublic with power-of-two denominators (RSS.inl:422-455 with the
+// placeholder r = 1, r' = d of Precompute.h:107-134).  Per party p:
+// a -= r'; t_p = reconstruct(a)_p; a = r + (A.sA, C.sB += t_p / d).
+// `dv` (per-party local denominators [3][n]) overrides the scalar `d` when set;
+// a denominator that is not a power of two sets *bad.
+__global__ __launch_bounds__(EW_BLOCK) void divide_public_kernel(u64 *__restrict__ a, size_t n, u64 d,
+                                                                 const u64 *__restrict__ dv, int *__restrict__ bad) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<u64> v = load3(a, n, i);
+    u64 dd[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) dd[p] = dv ? dv[(size_t)p * n + i] : d;
+    // rPrime: scalar -> fill(d) (A.sA = C.sB = d); vector -> 0 + d on A.sA and C.sB
+    v.s[PARTY_A][0] -= dd[PARTY_A];
+    v.s[PARTY_C][1] -= dd[PARTY_C];
+    u64 t[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) t[p] = v.s[p][0] + v.s[p][1] + v.s[prev_party(p)][0];
+    Sh3<u64> o;
+#pragma unroll
+    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
+#pragma unroll
+    for (int p = 0; p < 3; p += 2) {  // only A (shareA) and C (shareB) add the opened quotient
+        u64 den = dd[p];
+        if (den == 0 || (den & (den - 1)) != 0) {
+            *bad = 1;
+            continue;
+        }
+        int k = __ffsll((unsigned long long)den) - 1;
+        u64 q = trunc_div_pow2(t[p], k);
+        if (p == PARTY_A) o.s[p][0] = 1 + q;
+        else o.s[p][1] = 1 + q;
+    }
+    store3(a, n, i, o);
+}
+
+int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, int *bad_flag, hipStream_t s) {
+    if (!dv) PIR_REQUIRE(d != 0 && (d & (d - 1)) == 0 && d < (1ull << 63), "dividePublic: denominator must be a power of two");
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(divide_public_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, d, dv, bad_flag);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// dividePublic3out3 over 3-of-3 shares a [3][n] -> out [3][2][n]
+__global__ __launch_bounds__(EW_BLOCK) void divide_3out3_kernel(const u64 *__restrict__ a, size_t n, int k,
+                                                                u64 *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    u64 t = a[i] + a[n + i] + a[2 * n + i] - ((u64)1 << k);
+    u64 base = 1 + trunc_div_pow2(t, k);
+    Sh3<u64> o;
+#pragma unroll
+    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
+    o.s[PARTY_A][0] = base;
+    o.s[PARTY_C][1] = base;
+    store3(out, n, i, o);
+}
+
+int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s) {
+    PIR_REQUIRE(k >= 0 && k <= 62, "dividePublic3out3: denominator must be 2^k, 0 <= k <= 62");
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(divide_3out3_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, k, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// Explicit instantiations
+template int fused_drelu<u8>(const u64 *, size_t, u8 *, hipStream_t);
+template int fused_drelu<u64>(const u64 *, size_t, u64 *, hipStream_t);
+template int fused_relu<u8>(const u64 *, size_t, u64 *, u8 *, hipStream_t);
+template int fused_relu<u64>(const u64 *, size_t, u64 *, u64 *, hipStream_t);
+template int fused_select<u8>(const u64 *, const u64 *, const u8 *, u64 *, size_t, hipStream_t);
+template int fused_select<u64>(const u64 *, const u64 *, const u64 *, u64 *, size_t, hipStream_t);
+template int fused_maxpool_pools<u8>(u64 *, size_t, u64 *, u8 *, int, hipStream_t);
+template int fused_maxpool_pools<u64>(u64 *, size_t, u64 *, u64 *, int, hipStream_t);
+template int fused_maxpool_layer<u8>(const u64 *, const MaxpoolGeom &, u64 *, u8 *, hipStream_t);
+template int fused_reconstruct<u64>(const u64 *, size_t, u64 *, hipStream_t);
+template int fused_reconstruct<u8>(const u8 *, size_t, u8 *, hipStream_t);
+template int fused_rss_and<u8>(u8 *, const u8 *, size_t, hipStream_t);
+template int fused_rss_and<u64>(u64 *, const u64 *, size_t, hipStream_t);
+
+}  // namespace pir

@@ -1,0 +1,88 @@
+// ring_gemm.h -- Z_2^64 ring GEMM / implicit-GEMM convolution on gfx950.
+//
+// Two exact forms of C = A.B mod 2^64:
+//   * plain:  64-bit VALU multiply-add, LDS-tiled (the reference's CUTLASS
+//     SIMT uint64 GEMM restated for CDNA4 wave64);
+//   * limb:   every 64-bit operand is recoded into 8 signed 8-bit digits
+//     a = sum_i d_i 2^(8i), d_i in [-128, 127] (digits are the bytes of
+//     (a + 0x8080..80) ^ 0x8080..80), and C = sum_{i+j<8} 2^(8(i+j)) A_i.B_j
+//     is evaluated as 36 int8 MFMA products with int32 accumulation, grouped
+//     by s = i+j into 8 accumulators, flushed into 64-bit sums every 8192 k
+//     (|partial| <= 8192 * 8 * 2^14 = 2^30, so no int32 overflow can occur).
+#pragma once
+
+#include "common.h"
+
+namespace pir {
+
+// Operand A description for the limb GEMM: up to 6 K-segments (one per RSS
+// share array of the three parties), each either a dense matrix or the
+// implicit im2col view of an NHWC activation tensor.
+struct LimbA {
+    const u64 *seg[6];
+    int nseg;
+    int M, K;        // per-segment logical shape
+    int Kp;          // K rounded up to the K-step (32)
+    int lda;         // dense: leading dimension
+    int row_major;   // dense: A(m,k) = row_major ? p[m*lda+k] : p[k*lda+m]
+    int conv;        // 1: implicit im2col (NHWC input, k = (r*fw + s)*Cin + c)
+    int H, W, Cin, fh, fw, ph, pw, stride, OH, OW;
+};
+
+// Operand B (weights) description for the packing kernel: per segment the
+// value is b0[idx] + (b1 ? b1[idx] : 0); B(k,n) = row_major ? p[k*N+n] : p[n*K+k].
+struct LimbBSrc {
+    const u64 *b0[6];
+    const u64 *b1[6];
+    int nseg;
+    int K, N, Kp;
+    int row_major;
+};
+
+// Epilogue modes.
+enum EpiMode { EPI_RAW = 0, EPI_RSS_TRUNC = 1 };
+
+struct Epilogue {
+    int mode;
+    // EPI_RAW: C(m,n) stored row-major (ldc = N) if tc, else column-major (ldc = M)
+    u64 *C;
+    int tc;
+    // EPI_RSS_TRUNC: out is [3][2][M*N]; share (p,s) = base(p,s) + bias(p,s)[n], with
+    // base A=(1+q,0) B=(0,0) C=(0,1+q), q = trunc((raw - 2^k) / 2^k)
+    // (dividePublic3out3 with placeholder r = 1, r' = 2^k, RSS.inl:1076-1094).
+    u64 *out;
+    int trunc_bits;
+    const u64 *bias;  // [3][2][N] or null
+};
+
+constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;
+constexpr int LIMB_FLUSH_STEPS = 256;  // 256 * 32 = 8192 k per int32 accumulation chunk
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Size in bytes of the packed B operand.
+inline size_t limb_b_bytes(int N, int nseg, int Kp) {
+    size_t nt = (size_t)(N + LIMB_BN - 1) / LIMB_BN;
+    size_t ks = (size_t)nseg * Kp / LIMB_BK;
+    return nt * ks * 8 * LIMB_BN * LIMB_BK;
+}
+
+int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s);
+
+// Runs the limb GEMM.  `split` > 1 partitions the K-steps across workgroups and
+// reduces the 64-bit partials (exact: addition mod 2^64 commutes); it needs a
+// workspace of split*M*N*8 bytes.
+int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split, u64 *workspace,
+              hipStream_t s);
+
+// Plain VALU forms with the reference's layout conventions (gemm.cuh:64-112,
+// conv.cuh:359-401).
+int plain_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc, hipStream_t s);
+int plain_conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
+                     int ph, int pw, int stride, int dil, hipStream_t s);
+
+// Standalone truncation epilogue over a raw (3-of-3 summed) product, used by
+// the split-K path and by callers that already hold Sum_p raw_p.
+int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &epi, hipStream_t s);
+
+}  // namespace pir

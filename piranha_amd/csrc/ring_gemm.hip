@@ -1,0 +1,456 @@
+// ring_gemm.hip -- Z_2^64 ring GEMM and implicit-GEMM convolution for gfx950.
+//
+// Reference interface replaced: gpu::gemm (src/gpu/gemm.cuh:64-112, CUTLASS
+// SIMT uint64) and gpu::conv_fprop (src/gpu/conv.cuh:359-401, CUTLASS implicit
+// GEMM, NHWC x KRSC -> NHWC, cross-correlation).  Both are restated here as two
+// exact CDNA4 forms (see ring_gemm.h): a 64-bit VALU kernel and the signed-digit
+// int8-limb MFMA kernel.
+#include "ring_gemm.h"
+
+namespace pir {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// Shared helpers
+
+// Signed-digit recoding: the 8 bytes of the result, read as int8, are digits
+// d_i in [-128, 127] with sum_i d_i 2^(8i) == v (mod 2^64).
+__device__ __forceinline__ u64 recode_digits(u64 v) {
+    const u64 k80 = 0x8080808080808080ull;
+    return (v + k80) ^ k80;
+}
+
+__device__ __forceinline__ u32 perm(u32 s0, u32 s1, u32 sel) { return __builtin_amdgcn_perm(s0, s1, sel); }
+
+// 4x4 byte transpose: rows a,b,c,d (bytes = columns) -> o[i] = (a_i, b_i, c_i, d_i).
+__device__ __forceinline__ void transpose4(u32 a, u32 b, u32 c, u32 d, u32 &o0, u32 &o1, u32 &o2, u32 &o3) {
+    u32 t0 = perm(b, a, 0x06020400u);  // a0 b0 a2 b2
+    u32 t1 = perm(b, a, 0x07030501u);  // a1 b1 a3 b3
+    u32 t2 = perm(d, c, 0x06020400u);  // c0 d0 c2 d2
+    u32 t3 = perm(d, c, 0x07030501u);  // c1 d1 c3 d3
+    o0 = perm(t2, t0, 0x05040100u);
+    o2 = perm(t2, t0, 0x07060302u);
+    o1 = perm(t3, t1, 0x05040100u);
+    o3 = perm(t3, t1, 0x07060302u);
+}
+
+// 8x8 byte transpose: w[k] holds digit bytes (limb 0..7) of element k;
+// out[i] holds limb i of elements k = 0..7 (byte k).
+__device__ __forceinline__ void transpose8x8(const u64 w[8], u64 out[8]) {
+    u32 lo[8], hi[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        lo[k] = (u32)w[k];
+        hi[k] = (u32)(w[k] >> 32);
+    }
+    u32 a0, a1, a2, a3, b0, b1, b2, b3, c0, c1, c2, c3, d0, d1, d2, d3;
+    transpose4(lo[0], lo[1], lo[2], lo[3], a0, a1, a2, a3);  // limbs 0-3, k 0-3
+    transpose4(lo[4], lo[5], lo[6], lo[7], b0, b1, b2, b3);  // limbs 0-3, k 4-7
+    transpose4(hi[0], hi[1], hi[2], hi[3], c0, c1, c2, c3);  // limbs 4-7, k 0-3
+    transpose4(hi[4], hi[5], hi[6], hi[7], d0, d1, d2, d3);  // limbs 4-7, k 4-7
+    out[0] = (u64)a0 | ((u64)b0 << 32);
+    out[1] = (u64)a1 | ((u64)b1 << 32);
+    out[2] = (u64)a2 | ((u64)b2 << 32);
+    out[3] = (u64)a3 | ((u64)b3 << 32);
+    out[4] = (u64)c0 | ((u64)d0 << 32);
+    out[5] = (u64)c1 | ((u64)d1 << 32);
+    out[6] = (u64)c2 | ((u64)d2 << 32);
+    out[7] = (u64)c3 | ((u64)d3 << 32);
+}
+
+// RSS truncation + bias epilogue for one output element (see Epilogue).
+__device__ __forceinline__ void rss_trunc_store(const Epilogue &e, size_t MN, int N, int m, int n, u64 raw) {
+    const int k = e.trunc_bits;
+    u64 q = trunc_div_pow2(raw - ((u64)1 << k), k);
+    u64 base = (u64)1 + q;
+    size_t idx = (size_t)m * N + n;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            u64 v = ((p == PARTY_A && s == 0) || (p == PARTY_C && s == 1)) ? base : (u64)0;
+            if (e.bias) v += e.bias[(size_t)(2 * p + s) * N + n];
+            e.out[(size_t)(2 * p + s) * MN + idx] = v;
+        }
+}
+
+__device__ __forceinline__ void epilogue_store(const Epilogue &e, int M, int N, int m, int n, u64 raw) {
+    if (e.mode == EPI_RAW) {
+        if (e.tc) e.C[(size_t)m * N + n] = raw;
+        else e.C[(size_t)n * M + m] = raw;
+    } else {
+        rss_trunc_store(e, (size_t)M * N, N, m, n, raw);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// B packing: Bp[nt][ks][limb][64 n][32 k] int8, zero padded in n and k.
+
+__global__ __launch_bounds__(256) void limb_pack_b_kernel(LimbBSrc src, int8_t *__restrict__ out, int nks) {
+    // one thread per (n, group of 8 k'' within a K-step)
+    const size_t total = (size_t)((src.N + LIMB_BN - 1) / LIMB_BN) * LIMB_BN * nks * (LIMB_BK / 8);
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int kq = (int)(t % (LIMB_BK / 8));
+    size_t r = t / (LIMB_BK / 8);
+    int ks = (int)(r % nks);
+    int n = (int)(r / nks);
+    const int kps = src.Kp / LIMB_BK;  // K-steps per segment
+    int seg = ks / kps;
+    int kbase = (ks % kps) * LIMB_BK + kq * 8;
+    u64 w[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        int k = kbase + j;
+        u64 v = 0;
+        if (n < src.N && k < src.K) {
+            size_t idx = src.row_major ? (size_t)k * src.N + n : (size_t)n * src.K + k;
+            v = src.b0[seg][idx];
+            if (src.b1[seg]) v += src.b1[seg][idx];
+        }
+        w[j] = recode_digits(v);
+    }
+    u64 limbs[8];
+    transpose8x8(w, limbs);
+    int nt = n / LIMB_BN, nl = n % LIMB_BN;
+    int8_t *blk = out + ((size_t)nt * nks + ks) * (8 * LIMB_BN * LIMB_BK);
+#pragma unroll
+    for (int i = 0; i < 8; i++) *(u64 *)(blk + (size_t)i * LIMB_BN * LIMB_BK + nl * LIMB_BK + kq * 8) = limbs[i];
+}
+
+int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
+    PIR_REQUIRE(src.nseg >= 1 && src.nseg <= 6, "limb_pack_b: nseg must be 1..6");
+    PIR_REQUIRE(src.Kp % LIMB_BK == 0 && src.Kp >= src.K, "limb_pack_b: Kp must be a multiple of 32 >= K");
+    int nks = src.nseg * src.Kp / LIMB_BK;
+    size_t total = (size_t)((src.N + LIMB_BN - 1) / LIMB_BN) * LIMB_BN * nks * (LIMB_BK / 8);
+    if (total == 0) return OK;
+    hipLaunchKernelGGL(limb_pack_b_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, src, packed, nks);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// Limb MFMA GEMM.  Workgroup tile 64x64, 4 waves each owning a 32x32 output
+// block computed with v_mfma_i32_32x32x32_i8; K-step 32.
+
+template <int AMODE>  // 0: dense row-major A, 1: dense column-major A, 2: implicit im2col
+__global__ __launch_bounds__(256) void limb_gemm_kernel(LimbA a, const int8_t *__restrict__ Bp, int N, int nks,
+                                                        int steps_per_split, Epilogue epi, u64 *__restrict__ part) {
+    __shared__ __attribute__((aligned(16))) int8_t As[8][LIMB_BM][LIMB_BK];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[8][LIMB_BN][LIMB_BK];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.x * LIMB_BM, nt = blockIdx.y;
+    const int M = a.M;
+    const int ks_begin = blockIdx.z * steps_per_split;
+    const int ks_end = min(nks, ks_begin + steps_per_split);
+    const int kps = a.Kp / LIMB_BK;
+
+    // A-loader coordinates: each thread stages 8 consecutive k of one row.
+    int arow, akq;
+    if (AMODE == 1) {
+        arow = tid & 63;
+        akq = tid >> 6;
+    } else {
+        arow = tid >> 2;
+        akq = tid & 3;
+    }
+    const int am = m0 + arow;
+    // conv: decode the output pixel of this row once
+    int cb = 0, coh = 0, cow = 0;
+    if (AMODE == 2 && am < M) {
+        int ohw = a.OH * a.OW;
+        cb = am / ohw;
+        int rem = am - cb * ohw;
+        coh = rem / a.OW;
+        cow = rem - coh * a.OW;
+    }
+
+    v16i acc[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc[s] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    u64 c64[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) c64[r] = 0;
+
+    int since_flush = 0;
+    for (int ks = ks_begin; ks < ks_end; ++ks) {
+        // ---- global -> registers
+        const int seg = ks / kps;
+        const int kbase = (ks - seg * kps) * LIMB_BK + akq * 8;
+        const u64 *__restrict__ src = a.seg[seg];
+        u64 w[8];
+        if (AMODE == 2) {
+            const int fwc = a.fw * a.Cin;
+            if ((a.Cin & 7) == 0 && kbase < a.K && am < M) {
+                int rr = kbase / fwc;
+                int rem = kbase - rr * fwc;
+                int ss = rem / a.Cin;
+                int c = rem - ss * a.Cin;
+                int ih = coh * a.stride - a.ph + rr, iw = cow * a.stride - a.pw + ss;
+                if (ih >= 0 && ih < a.H && iw >= 0 && iw < a.W) {
+                    const u64 *p = src + (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + c;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) w[j] = recode_digits(p[j]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) w[j] = recode_digits(0);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    int k = kbase + j;
+                    u64 v = 0;
+                    if (k < a.K && am < M) {
+                        int rr = k / fwc;
+                        int rem = k - rr * fwc;
+                        int ss = rem / a.Cin;
+                        int c = rem - ss * a.Cin;
+                        int ih = coh * a.stride - a.ph + rr, iw = cow * a.stride - a.pw + ss;
+                        if (ih >= 0 && ih < a.H && iw >= 0 && iw < a.W)
+                            v = src[(((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + c];
+                    }
+                    w[j] = recode_digits(v);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                int k = kbase + j;
+                u64 v = 0;
+                if (k < a.K && am < M) v = AMODE == 0 ? src[(size_t)am * a.lda + k] : src[(size_t)k * a.lda + am];
+                w[j] = recode_digits(v);
+            }
+        }
+        u64 limbs[8];
+        transpose8x8(w, limbs);
+        const int4 *bsrc = (const int4 *)(Bp + ((size_t)nt * nks + ks) * (8 * LIMB_BN * LIMB_BK));
+        int4 bq[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) bq[c] = bsrc[tid + c * 256];
+
+        // ---- registers -> LDS
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; i++) *(u64 *)&As[i][arow][akq * 8] = limbs[i];
+#pragma unroll
+        for (int c = 0; c < 4; c++) ((int4 *)&Bs[0][0][0])[tid + c * 256] = bq[c];
+        __syncthreads();
+
+        // ---- MFMA: acc[i+j] += A_i . B_j for i + j < 8
+        v4i av[8], bv[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) av[i] = *(const v4i *)&As[i][wm * 32 + (lane & 31)][(lane >> 5) * 16];
+#pragma unroll
+        for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)&Bs[j][wn * 32 + (lane & 31)][(lane >> 5) * 16];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 8 - i; j++)
+                acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[i], bv[j], acc[i + j], 0, 0, 0);
+
+        if (++since_flush == LIMB_FLUSH_STEPS) {
+            since_flush = 0;
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) c64[r] += ((u64)(s64)acc[s][r]) << (8 * s);
+                acc[s] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) c64[r] += ((u64)(s64)acc[s][r]) << (8 * s);
+
+    // ---- epilogue: 32x32 MFMA output layout, lane -> column, register -> row
+    const int n = nt * LIMB_BN + wn * 32 + (lane & 31);
+    if (n >= N) return;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m >= M) continue;
+        if (part) part[((size_t)blockIdx.z * M + m) * N + n] = c64[r];
+        else epilogue_store(epi, M, N, m, n, c64[r]);
+    }
+}
+
+__global__ __launch_bounds__(256) void split_reduce_kernel(const u64 *__restrict__ part, int split, int M, int N,
+                                                           Epilogue epi) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t MN = (size_t)M * N;
+    if (t >= MN) return;
+    u64 raw = 0;
+    for (int z = 0; z < split; z++) raw += part[(size_t)z * MN + t];
+    epilogue_store(epi, M, N, (int)(t / N), (int)(t % N), raw);
+}
+
+int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &epi, hipStream_t s) {
+    size_t MN = (size_t)M * N;
+    if (MN == 0) return OK;
+    hipLaunchKernelGGL(split_reduce_kernel, dim3(grid_for(MN, 256)), dim3(256), 0, s, raw, nsum, M, N, epi);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split, u64 *workspace,
+              hipStream_t s) {
+    PIR_REQUIRE(a.nseg >= 1 && a.nseg <= 6, "limb_gemm: nseg must be 1..6");
+    PIR_REQUIRE(a.Kp % LIMB_BK == 0 && a.Kp >= a.K, "limb_gemm: Kp must be a multiple of 32 >= K");
+    PIR_REQUIRE(epi.mode != EPI_RSS_TRUNC || (epi.trunc_bits >= 1 && epi.trunc_bits <= 62),
+                "limb_gemm: truncation bits must be in [1, 62]");
+    if (a.M == 0 || N == 0) return OK;
+    const int nks = a.nseg * a.Kp / LIMB_BK;
+    if (split < 1) split = 1;
+    if (split > nks) split = nks;
+    const int per = (nks + split - 1) / split;
+    split = (nks + per - 1) / per;
+    PIR_REQUIRE(split == 1 || workspace, "limb_gemm: split-K needs a workspace");
+    dim3 grid((a.M + LIMB_BM - 1) / LIMB_BM, (N + LIMB_BN - 1) / LIMB_BN, split);
+    u64 *part = split > 1 ? workspace : nullptr;
+    switch (a.conv ? 2 : (a.row_major ? 0 : 1)) {
+        case 0: hipLaunchKernelGGL(limb_gemm_kernel<0>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
+        case 1: hipLaunchKernelGGL(limb_gemm_kernel<1>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
+        default: hipLaunchKernelGGL(limb_gemm_kernel<2>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
+    }
+    PIR_LAUNCH_CHECK();
+    if (split > 1) PIR_TRY(rss_trunc_epilogue(workspace, a.M, N, split, epi, s));
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain 64-bit VALU GEMM: 64x64 tile, K-step 16, 4x4 outputs per thread.
+
+struct PlainConv {
+    int H, W, Cin, fh, fw, ph, pw, stride, dil, OH, OW;
+};
+
+template <int AMODE, bool TB, bool TC>  // AMODE 0: A column-major, 1: row-major, 2: im2col
+__global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, const u64 *__restrict__ A,
+                                                         const u64 *__restrict__ B, u64 *__restrict__ C,
+                                                         PlainConv cv) {
+    __shared__ u64 As[16][64 + 1];
+    __shared__ u64 Bs[16][64 + 1];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    u64 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            int idx = tid + e * 256;
+            int mm, kk;
+            if (AMODE == 0) {
+                kk = idx >> 6;
+                mm = idx & 63;
+            } else {
+                mm = idx >> 4;
+                kk = idx & 15;
+            }
+            int m = m0 + mm, k = k0 + kk;
+            u64 v = 0;
+            if (m < M && k < K) {
+                if (AMODE == 0) v = A[(size_t)k * M + m];
+                else if (AMODE == 1) v = A[(size_t)m * K + k];
+                else {
+                    int ohw = cv.OH * cv.OW;
+                    int b = m / ohw, rem = m - b * ohw;
+                    int oh = rem / cv.OW, ow = rem - oh * cv.OW;
+                    int fwc = cv.fw * cv.Cin;
+                    int r = k / fwc, rm = k - r * fwc;
+                    int s = rm / cv.Cin, c = rm - s * cv.Cin;
+                    int ih = oh * cv.stride - cv.ph + r * cv.dil, iw = ow * cv.stride - cv.pw + s * cv.dil;
+                    if (ih >= 0 && ih < cv.H && iw >= 0 && iw < cv.W)
+                        v = A[(((size_t)b * cv.H + ih) * cv.W + iw) * cv.Cin + c];
+                }
+            }
+            As[kk][mm] = v;
+            int nn;
+            if (TB) {
+                kk = idx >> 6;
+                nn = idx & 63;
+            } else {
+                nn = idx >> 4;
+                kk = idx & 15;
+            }
+            int n = n0 + nn;
+            k = k0 + kk;
+            u64 bvv = 0;
+            if (n < N && k < K) bvv = TB ? B[(size_t)k * N + n] : B[(size_t)n * K + k];
+            Bs[kk][nn] = bvv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            u64 av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int m = m0 + ty + 16 * i, n = n0 + tx + 16 * j;
+            if (m < M && n < N) {
+                if (TC) C[(size_t)m * N + n] = acc[i][j];
+                else C[(size_t)n * M + m] = acc[i][j];
+            }
+        }
+}
+
+template <int AMODE>
+static void launch_plain(dim3 g, hipStream_t s, int M, int N, int K, const u64 *A, const u64 *B, u64 *C, bool tb,
+                         bool tc, PlainConv cv) {
+    if (tb) {
+        if (tc) hipLaunchKernelGGL((plain_gemm_kernel<AMODE, true, true>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
+        else hipLaunchKernelGGL((plain_gemm_kernel<AMODE, true, false>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
+    } else {
+        if (tc) hipLaunchKernelGGL((plain_gemm_kernel<AMODE, false, true>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
+        else hipLaunchKernelGGL((plain_gemm_kernel<AMODE, false, false>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
+    }
+}
+
+int plain_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc, hipStream_t s) {
+    PIR_REQUIRE(M >= 0 && N >= 0 && K >= 0, "gemm: negative dimension");
+    if (M == 0 || N == 0) return OK;
+    dim3 g((M + 63) / 64, (N + 63) / 64);
+    PlainConv cv{};
+    if (ta) launch_plain<1>(g, s, M, N, K, A, B, C, tb, tc, cv);
+    else launch_plain<0>(g, s, M, N, K, A, B, C, tb, tc, cv);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+int plain_conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
+                     int ph, int pw, int stride, int dil, hipStream_t s) {
+    PIR_REQUIRE(b > 0 && H > 0 && W > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 && stride > 0 && dil > 0,
+                "conv_fprop: invalid geometry");
+    PlainConv cv{H, W, Din, fh, fw, ph, pw, stride, dil, 0, 0};
+    cv.OH = (H + 2 * ph - ((fh - 1) * dil + 1)) / stride + 1;
+    cv.OW = (W + 2 * pw - ((fw - 1) * dil + 1)) / stride + 1;
+    PIR_REQUIRE(cv.OH > 0 && cv.OW > 0, "conv_fprop: empty output");
+    int M = b * cv.OH * cv.OW, N = Dout, K = fh * fw * Din;
+    dim3 g((M + 63) / 64, (N + 63) / 64);
+    // filter [Dout][fh][fw][Din] is B(k, n) = w[n*K + k]: column-major K x N
+    launch_plain<2>(g, s, M, N, K, x, w, y, false, true, cv);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+}  // namespace pir

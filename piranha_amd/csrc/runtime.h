@@ -1,0 +1,99 @@
+// runtime.h -- execution context: stream, caching device allocator, mode.
+//
+// The reference allocates a fresh thrust::device_vector for every temporary
+// and synchronises after every kernel (DeviceData.h:250-254, gemm.cuh:99).
+// Here every temporary comes from a per-context caching allocator (exact-size
+// free lists, never returned to HIP until the context dies), so a forward pass
+// that ran once allocates nothing afterwards and can be captured in a hipGraph;
+// work is only ordered on the context's stream, with no host synchronisation.
+#pragma once
+
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "common.h"
+
+namespace pir {
+
+enum ExecMode { MODE_FUSED = 0, MODE_PLAIN = 1 };
+enum GemmAlgo { ALGO_PLAIN = 0, ALGO_LIMB = 1 };
+
+class Allocator {
+  public:
+    ~Allocator();
+    void *alloc(size_t bytes);
+    void release(void *p);
+    size_t bytes_reserved() const { return reserved_; }
+    bool capturing = false;  // set while a hipGraph capture is open: no new hipMalloc allowed
+
+  private:
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> used_;
+    size_t reserved_ = 0;
+};
+
+struct Context {
+    hipStream_t stream = nullptr;
+    Allocator alloc;
+    int mode = MODE_FUSED;
+    int algo = ALGO_LIMB;
+    int error = 0;
+};
+
+// The context used by the C++ protocol API (the reference uses globals:
+// partyNum, PrecomputeObject, piranha_config; main.cu:23-40).
+Context &ctx();
+void set_ctx(Context *c);
+
+// RAII device buffer from the context allocator.
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) : bytes_(bytes) {
+        if (bytes) p_ = ctx().alloc.alloc(bytes);
+    }
+    ~DevBuf() {
+        if (p_) ctx().alloc.release(p_);
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            if (p_) ctx().alloc.release(p_);
+            p_ = o.p_;
+            bytes_ = o.bytes_;
+            o.p_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    void *get() const { return p_; }
+    size_t bytes() const { return bytes_; }
+
+  private:
+    void *p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Host-side failure inside the C++ protocol API (which mirrors the reference's
+// void-returning functions): recorded in the context, reported by the C ABI.
+class PirError : public std::exception {
+  public:
+    explicit PirError(int code) : code_(code) {}
+    int code() const { return code_; }
+    const char *what() const noexcept override { return last_error(); }
+
+  private:
+    int code_;
+};
+
+inline void check(int status) {
+    if (status != 0) throw PirError(status);
+}
+
+}  // namespace pir

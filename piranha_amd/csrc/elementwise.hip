@@ -66,9 +66,9 @@ int multi_op(const MultiOp &m, int elem_bytes, hipStream_t s) {
 
 int dd_op(int op, int elem_bytes, void *x, long long xs, const void *y, long long ys, u64 scalar, size_t n,
           hipStream_t s) {
-    if (op == OP_DIV) {
-        PIR_REQUIRE(y == nullptr && scalar != 0 && (scalar & (scalar - 1)) == 0,
-                    "DeviceData /=: only power-of-two scalar divisors are supported");
+    if (op == OP_DIV && y == nullptr) {
+        PIR_REQUIRE(scalar != 0 && (scalar & (scalar - 1)) == 0,
+                    "DeviceData /=: only power-of-two divisors are supported");
     }
     MultiOp m{};
     m.count = 1;

@@ -372,7 +372,38 @@ int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s) {
     return OK;
 }
 
+// convex_comb (RSS.inl:33-61, 1321-1333) per party: d = e ? (A: 1-c0, -c1;
+// B: -c0, -c1; C: -c0, 1-c1) : c, with e the party's opened bit.
+template <typename U>
+__global__ __launch_bounds__(EW_BLOCK) void convex_comb_kernel(u64 *__restrict__ d, const u64 *__restrict__ c,
+                                                               const U *__restrict__ e, size_t n) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<u64> cv = load3(c, n, i), dv;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        if (e[(size_t)p * n + i] == (U)1) {
+            dv.s[p][0] = (p == PARTY_A ? 1ull : 0ull) - cv.s[p][0];
+            dv.s[p][1] = (p == PARTY_C ? 1ull : 0ull) - cv.s[p][1];
+        } else {
+            dv.s[p][0] = cv.s[p][0];
+            dv.s[p][1] = cv.s[p][1];
+        }
+    }
+    store3(d, n, i, dv);
+}
+
+template <typename U>
+int fused_convex_comb(u64 *d, const u64 *c, const U *e, size_t n, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((convex_comb_kernel<U>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, d, c, e, n);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 // Explicit instantiations
+template int fused_convex_comb<u8>(u64 *, const u64 *, const u8 *, size_t, hipStream_t);
+template int fused_convex_comb<u64>(u64 *, const u64 *, const u64 *, size_t, hipStream_t);
 template int fused_drelu<u8>(const u64 *, size_t, u8 *, hipStream_t);
 template int fused_drelu<u64>(const u64 *, size_t, u64 *, hipStream_t);
 template int fused_relu<u8>(const u64 *, size_t, u64 *, u8 *, hipStream_t);

@@ -1,0 +1,214 @@
+// layers.cpp -- FC / CNN / ReLU / Maxpool forward layers and the network.
+#include "layers.h"
+
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "fused_rss.h"
+
+namespace pir {
+
+static bool fused_limb() { return ctx().mode == MODE_FUSED && ctx().algo == ALGO_LIMB; }
+
+// ---------------------------------------------------------------------------
+// FC (FCLayer.cu:20-130)
+
+FCLayer::FCLayer(const FCConfig &c, int n, int seed)
+    : Layer(n), conf(c), activations(c.batchSize * c.outputDim), weights(c.inputDim * c.outputDim),
+      biases(c.outputDim) {
+    initialize(seed);
+}
+
+// Kaiming initialisation with std::default_random_engine(seed) and bias 0.01
+// (FCLayer.cu:31-48).
+void FCLayer::initialize(int seed) {
+    std::default_random_engine generator(seed);
+    double std_dev = std::sqrt((double)2 / (double)conf.inputDim);
+    std::normal_distribution<double> distribution(0.0, std_dev);
+    std::vector<double> w(weights.size());
+    for (size_t i = 0; i < w.size(); i++) w[i] = distribution(generator);
+    weights.setPublic(w, ctx().frac_bits);
+    std::vector<double> b(biases.size(), 0.01);
+    biases.setPublic(b, ctx().frac_bits);
+    packed_valid = false;
+}
+
+void FCLayer::printLayer() {
+    printf("----------------------------------------------\n(%d) FC Layer\t\t  %zu x %zu\n\t\t\t  %zu\t\t (Batch Size)\n",
+           layerNum + 1, conf.inputDim, conf.outputDim, conf.batchSize);
+}
+
+void FCLayer::forward(const RSS64 &input) {
+    const int M = (int)conf.batchSize, N = (int)conf.outputDim, K = (int)conf.inputDim;
+    const int fp = ctx().frac_bits;
+    if (fused_limb() && input.base()) {
+        if (!packed_valid) {
+            packed = DevBuf(packed_rss_weights_bytes(K, N));
+            check(pack_rss_weights(weights, K, N, true, (int8_t *)packed.get()));
+            packed_valid = true;
+        }
+        // matmul + dividePublic3out3 + elementVectorAdd of both bias shares, one kernel
+        check(rss_limb_product(input, true, nullptr, M, N, K, (const int8_t *)packed.get(), fp, &biases, activations,
+                               true));
+        return;
+    }
+    activations.zero();
+    matmul(input, weights, activations, M, N, K, true, true, true, (u64)fp);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(element_vector_add(activations.getShare(q, i)->data(), biases.getShare(q, i)->data(), 0, N, M,
+                                     ctx().stream));
+}
+
+// ---------------------------------------------------------------------------
+// CNN (CNNLayer.cu:23-115); weights are [Dout][f][f][Din], no bias.
+
+CNNLayer::CNNLayer(const CNNConfig &c, int n, int seed)
+    : Layer(n), conf(c), weights(c.filterSize * c.filterSize * c.inputFeatures * c.outputFeatures),
+      activations(c.batchSize * c.outputFeatures *
+                  ((c.imageWidth - c.filterSize + 2 * c.padding) / c.stride + 1) *
+                  ((c.imageHeight - c.filterSize + 2 * c.padding) / c.stride + 1)) {
+    initialize(seed);
+}
+
+void CNNLayer::initialize(int seed) {
+    std::default_random_engine generator(seed);
+    double std_dev = std::sqrt((double)2 / (double)(conf.filterSize * conf.filterSize * conf.inputFeatures));
+    std::normal_distribution<double> distribution(0.0, std_dev);
+    std::vector<double> w(weights.size());
+    for (size_t i = 0; i < w.size(); i++) w[i] = distribution(generator);
+    weights.setPublic(w, ctx().frac_bits);
+    packed_valid = false;
+}
+
+void CNNLayer::printLayer() {
+    printf("----------------------------------------------\n(%d) CNN Layer\t\t  %zu x %zu x %zu\n\t\t\t  %zu x %zu  \t(Filter Size)\n"
+           "\t\t\t  %zu , %zu \t(Stride, padding)\n\t\t\t  %zu\t\t(Batch Size)\n\t\t\t  %d x %d x %zu \t(Output)\n",
+           layerNum + 1, conf.imageHeight, conf.imageWidth, conf.inputFeatures, conf.filterSize, conf.filterSize,
+           conf.stride, conf.padding, conf.batchSize, outW(), outH(), conf.outputFeatures);
+}
+
+void CNNLayer::forward(const RSS64 &input) {
+    const int fp = ctx().frac_bits;
+    const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
+    if (fused_limb() && input.base()) {
+        const int K = f * f * Din;
+        if (!packed_valid) {
+            packed = DevBuf(packed_rss_weights_bytes(K, Dout));
+            check(pack_rss_weights(weights, K, Dout, false, (int8_t *)packed.get()));
+            packed_valid = true;
+        }
+        ConvShape cs{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, Din, f, (int)conf.stride,
+                     (int)conf.padding, outH(), outW()};
+        check(rss_limb_product(input, true, &cs, cs.batch * cs.OH * cs.OW, Dout, K, (const int8_t *)packed.get(), fp,
+                               nullptr, activations, true));
+        return;
+    }
+    activations.zero();
+    convolution(input, weights, activations, kFprop, (int)conf.batchSize, (int)conf.imageHeight,
+                (int)conf.imageWidth, f, Din, Dout, (int)conf.stride, (int)conf.padding, fp);
+}
+
+// ---------------------------------------------------------------------------
+// ReLU (ReLULayer.cu:21-88)
+
+ReLULayer::ReLULayer(const ReLUConfig &c, int n, int)
+    : Layer(n), conf(c), activations(c.batchSize * c.inputDim), reluPrime(c.batchSize * c.inputDim) {}
+
+void ReLULayer::printLayer() {
+    printf("----------------------------------------------\n(%d) ReLU Layer\t\t  %zu x %zu\n", layerNum + 1,
+           conf.batchSize, conf.inputDim);
+}
+
+void ReLULayer::forward(const RSS64 &input) {
+    if (ctx().mode != MODE_FUSED) activations.zero();
+    ReLU(input, activations, reluPrime);
+}
+
+// ---------------------------------------------------------------------------
+// Maxpool (MaxpoolLayer.cu:17-131)
+
+MaxpoolLayer::MaxpoolLayer(const MaxpoolConfig &c, int n, int)
+    : Layer(n), conf(c),
+      activations(c.batchSize * c.features * ((c.imageWidth - c.poolSize) / c.stride + 1) *
+                  ((c.imageHeight - c.poolSize) / c.stride + 1)),
+      maxPrime(((c.imageWidth - c.poolSize) / c.stride + 1) * ((c.imageHeight - c.poolSize) / c.stride + 1) *
+               c.features * c.batchSize * c.poolSize * c.poolSize) {}
+
+void MaxpoolLayer::printLayer() {
+    printf("----------------------------------------------\n(%d) Maxpool Layer\t  %zu x %zu x %zu\n\t\t\t  %zu  \t\t(Pooling Size)\n"
+           "\t\t\t  %zu \t\t(Stride)\n\t\t\t  %zu\t\t(Batch Size)\n",
+           layerNum + 1, conf.imageHeight, conf.imageWidth, conf.features, conf.poolSize, conf.stride, conf.batchSize);
+}
+
+void MaxpoolLayer::forward(const RSS64 &input) {
+    const int fp = ctx().frac_bits;
+    // (T)(-10 * (1 << FLOAT_PRECISION)) in every share (RSS has two shares, so the
+    // 4-share special case of MaxpoolLayer.cu:77-92 never applies)
+    const u64 pad_value = (u64)(s64)(-10LL * (1LL << fp));
+    const int f = (int)conf.poolSize;
+    if (ctx().mode == MODE_FUSED && input.base()) {
+        MaxpoolGeom g{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, (int)conf.features, f,
+                      (int)conf.stride, outH(), outW(), pad_value};
+        check(fused_maxpool_layer<u8>(input.base(), g, activations.base(), maxPrime.base(), ctx().stream));
+        return;
+    }
+    activations.zero();
+    maxPrime.zero();
+    int eps = expanded_pool_size(f);
+    size_t poolsN = maxpool_im2row_size((int)conf.imageHeight, (int)conf.imageWidth, f, (int)conf.features,
+                                        (int)conf.batchSize, (int)conf.stride, 0);
+    RSS64 pools(poolsN);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(maxpool_im2row(input.getShare(q, i)->data(), pools.getShare(q, i)->data(), (int)conf.imageHeight,
+                                 (int)conf.imageWidth, f, (int)conf.features, (int)conf.batchSize, (int)conf.stride, 0,
+                                 pad_value, ctx().stream));
+    RSS<u8> emp(pools.size());
+    maxpool(pools, activations, emp, eps);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(truncate_cols(emp.getShare(q, i)->data(), emp.size(), maxPrime.getShare(q, i)->data(), eps, f * f, 1,
+                                ctx().stream));
+}
+
+// ---------------------------------------------------------------------------
+// Network (NeuralNetwork.cu:32-133)
+
+NeuralNetwork::NeuralNetwork(const std::vector<LayerSpec> &spec, int b, int seed) : batch(b) {
+    for (size_t i = 0; i < spec.size(); i++) {
+        const int *v = spec[i].v;
+        int s = seed + (int)i;
+        switch (v[0]) {
+            case 0: layers.emplace_back(new FCLayer(FCConfig(v[1], b, v[2]), (int)i, s)); break;
+            case 1: layers.emplace_back(new CNNLayer(CNNConfig(v[1], v[2], v[3], v[4], v[5], v[6], v[7], b), (int)i, s)); break;
+            case 2: layers.emplace_back(new ReLULayer(ReLUConfig(v[1], b), (int)i, s)); break;
+            case 3: layers.emplace_back(new MaxpoolLayer(MaxpoolConfig(v[1], v[2], v[3], v[4], v[5], b), (int)i, s)); break;
+            default:
+                set_error("network: only FC, CNN, ReLU and Maxpool layers are supported");
+                throw PirError(ERR_UNSUPPORTED);
+        }
+    }
+    if (!layers.empty()) input.resize(layers[0]->inputSize());
+}
+
+NeuralNetwork::~NeuralNetwork() = default;
+
+void NeuralNetwork::forward(const std::vector<double> &data) {
+    input.zero();
+    input.setPublic(data, ctx().frac_bits);
+    run();
+}
+
+void NeuralNetwork::run() {
+    if (layers.empty()) return;
+    layers[0]->forward(input);
+    for (size_t i = 1; i < layers.size(); i++) layers[i]->forward(*layers[i - 1]->getActivation());
+}
+
+void NeuralNetwork::printNetwork() {
+    for (auto &l : layers) l->printLayer();
+}
+
+}  // namespace pir

@@ -1,0 +1,522 @@
+// protocol.cpp -- RSS protocol functions (src/mpc/RSS.inl) for all three
+// parties, in the fused and the op-by-op execution modes (see protocol.h).
+#include "protocol.h"
+
+#include "fused_rss.h"
+
+namespace pir {
+
+static bool fused_mode() { return ctx().mode == MODE_FUSED; }
+static hipStream_t S() { return ctx().stream; }
+
+static int log2_exact(u64 d) {
+    if (d == 0 || (d & (d - 1)) != 0) {
+        set_error("denominator must be a power of two");
+        throw PirError(ERR_UNSUPPORTED);
+    }
+    int k = 0;
+    while (((u64)1 << k) != d) k++;
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// reconstruct / reshare / reconstruct3out3
+
+// RSS.inl:457-479: out_p = 0 + sA + sB + (prev party's sA, received)
+template <typename T>
+void reconstruct(const RSS<T> &in, PartyData<T> &out) {
+    if (fused_mode() && in.base() && out.base()) {
+        check(fused_reconstruct<T>(in.base(), in.size(), out.base(), S()));
+        return;
+    }
+    out.zero();
+    MultiLaunch l0(OP_ADD), l1(OP_ADD), l2(OP_ADD);
+    for (int q = 0; q < 3; q++) {
+        l0.add(out[q], in.getShare(q, 0));
+        l1.add(out[q], in.getShare(q, 1));
+        l2.add(out[q], in.getShare(prev_party(q), 0));
+    }
+    l0.run(sizeof(T));
+    l1.run(sizeof(T));
+    l2.run(sizeof(T));
+}
+
+// RSS.inl:1052-1074: out_p = (c_p, c_{next(p)}), no mask (placeholder precompute)
+template <typename T>
+void reshare(const PartyData<T> &c, RSS<T> &out) {
+    if (sizeof(T) == 8 && fused_mode() && c.base() && out.base()) {
+        check(fused_reshare((const u64 *)c.base(), c.size(), (u64 *)out.base(), S()));
+        return;
+    }
+    PartyData<T> mask(c.size());
+    mask += c;
+    out.zero();
+    MultiLaunch add(OP_ADD);
+    for (int q = 0; q < 3; q++) add.add(*out.getShare(q, 0), &mask[q]);
+    add.run(sizeof(T));
+    MultiLaunch rx(OP_SET);  // party q receives shareB from next(q)'s transmitted shareA
+    for (int q = 0; q < 3; q++) rx.add(*out.getShare(q, 1), out.getShare(next_party(q), 0));
+    rx.run(sizeof(T));
+}
+
+// RSS.inl:1019-1050
+void reconstruct3out3(const PartyData<u64> &in, PartyData<u64> &out) {
+    out.zero();
+    MultiLaunch l0(OP_ADD), l1(OP_ADD), l2(OP_ADD);
+    for (int q = 0; q < 3; q++) {
+        l0.add(out[q], &in[q]);
+        l1.add(out[q], &in[prev_party(q)]);
+        l2.add(out[q], &in[next_party(q)]);
+    }
+    l0.run(8);
+    l1.run(8);
+    l2.run(8);
+}
+
+// ---------------------------------------------------------------------------
+// RSS *= and &= (RSS.inl:283-345)
+
+template <typename T>
+void rss_mult_impl(RSS<T> &x, const RSS<T> &y) {
+    if (sizeof(T) == 8 && fused_mode() && x.base() && y.base()) {
+        check(fused_rss_mult((u64 *)x.base(), (const u64 *)y.base(), x.size(), S()));
+        return;
+    }
+    PartyData<T> summed(y.size());
+    MultiLaunch s0(OP_ADD), s1(OP_ADD), m0(OP_MUL), m1(OP_MUL), a0(OP_ADD);
+    for (int q = 0; q < 3; q++) {
+        s0.add(summed[q], y.getShare(q, 0));
+        s1.add(summed[q], y.getShare(q, 1));
+        m0.add(*x.getShare(q, 0), &summed[q]);
+        m1.add(*x.getShare(q, 1), y.getShare(q, 0));
+        a0.add(*x.getShare(q, 0), x.getShare(q, 1));
+    }
+    s0.run(sizeof(T));
+    s1.run(sizeof(T));
+    m0.run(sizeof(T));
+    m1.run(sizeof(T));
+    a0.run(sizeof(T));
+    PartyData<T> c(*x.getShare(0, 0), *x.getShare(1, 0), *x.getShare(2, 0));
+    reshare(c, x);
+}
+
+template <typename T>
+void rss_and_impl(RSS<T> &x, const RSS<T> &y) {
+    if (fused_mode() && x.base() && y.base()) {
+        check(fused_rss_and<T>(x.base(), y.base(), x.size(), S()));
+        return;
+    }
+    PartyData<T> summed(y.size());
+    MultiLaunch s0(OP_XOR), s1(OP_XOR), m0(OP_AND), m1(OP_AND), a0(OP_XOR);
+    for (int q = 0; q < 3; q++) {
+        s0.add(summed[q], y.getShare(q, 0));
+        s1.add(summed[q], y.getShare(q, 1));
+        m0.add(*x.getShare(q, 0), &summed[q]);
+        m1.add(*x.getShare(q, 1), y.getShare(q, 0));
+        a0.add(*x.getShare(q, 0), x.getShare(q, 1));
+    }
+    s0.run(sizeof(T));
+    s1.run(sizeof(T));
+    m0.run(sizeof(T));
+    m1.run(sizeof(T));
+    a0.run(sizeof(T));
+    PartyData<T> c(*x.getShare(0, 0), *x.getShare(1, 0), *x.getShare(2, 0));
+    reshare(c, x);
+}
+
+// ---------------------------------------------------------------------------
+// Truncation
+
+// RSS.inl:422-436
+void dividePublic(RSS64 &a, u64 d) {
+    log2_exact(d);
+    if (fused_mode() && a.base()) {
+        check(fused_divide_public(a.base(), a.size(), d, nullptr, nullptr, S()));
+        return;
+    }
+    size_t n = a.size();
+    RSS64 r(n), rPrime(n);
+    rPrime.fill(d);
+    r.fill(1);
+    a -= rPrime;
+    PartyData<u64> rec(n);
+    reconstruct(a, rec);
+    rec /= d;
+    a.zero();
+    a += r;
+    a += rec;
+}
+
+// RSS.inl:438-455 (power-of-two denominators, as produced by taylorSeries)
+void dividePublic(RSS64 &a, PartyData<u64> &den) {
+    size_t n = a.size();
+    if (fused_mode() && a.base() && den.base()) {
+        DevBuf flag(sizeof(int));
+        check(fill_u8((u8 *)flag.get(), 0, sizeof(int), S()));
+        check(fused_divide_public(a.base(), n, 0, den.base(), (int *)flag.get(), S()));
+        int bad = 0;
+        if (hipMemcpyAsync(&bad, flag.get(), sizeof(int), hipMemcpyDeviceToHost, S()) != hipSuccess ||
+            hipStreamSynchronize(S()) != hipSuccess) {
+            set_error("dividePublic: flag readback failed");
+            throw PirError(ERR_HIP);
+        }
+        if (bad) {
+            set_error("dividePublic: denominators must be powers of two");
+            throw PirError(ERR_UNSUPPORTED);
+        }
+        return;
+    }
+    RSS64 r(n), rPrime(n);
+    rPrime.zero();
+    rPrime += den;
+    r.fill(1);
+    a -= rPrime;
+    PartyData<u64> rec(n);
+    reconstruct(a, rec);
+    MultiLaunch dv(OP_DIV);
+    for (int q = 0; q < 3; q++) dv.add(rec[q], &den[q]);
+    dv.run(8);
+    a.zero();
+    a += r;
+    a += rec;
+}
+
+// RSS.inl:1076-1094 (`a` is left modified as in the reference in plain mode)
+void dividePublic3out3(PartyData<u64> &a, u64 d, RSS64 &out) {
+    int k = log2_exact(d);
+    if (fused_mode() && a.base() && out.base()) {
+        check(fused_divide_3out3(a.base(), a.size(), k, out.base(), S()));
+        return;
+    }
+    size_t n = a.size();
+    RSS64 r(n), rPrime(n);
+    rPrime.fill(d);
+    r.fill(1);
+    MultiLaunch sub(OP_SUB);
+    for (int q = 0; q < 3; q++) sub.add(a[q], rPrime.getShare(q, 0));
+    sub.run(8);
+    PartyData<u64> rec(n);
+    reconstruct3out3(a, rec);
+    rec /= d;
+    out.zero();
+    out += r;
+    out += rec;
+}
+
+// ---------------------------------------------------------------------------
+// Ring products
+
+size_t packed_rss_weights_bytes(int K, int N) { return limb_b_bytes(N, 6, round_up(K, LIMB_BK)); }
+
+int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed) {
+    LimbBSrc s{};
+    for (int p = 0; p < 3; p++) {
+        s.b0[2 * p] = B.getShare(p, 0)->data();
+        s.b1[2 * p] = B.getShare(p, 1)->data();
+        s.b0[2 * p + 1] = B.getShare(p, 0)->data();
+        s.b1[2 * p + 1] = nullptr;
+    }
+    s.nseg = 6;
+    s.K = K;
+    s.N = N;
+    s.Kp = round_up(K, LIMB_BK);
+    s.row_major = b_row_major ? 1 : 0;
+    return limb_pack_b(s, packed, S());
+}
+
+int choose_split(int M, int N, int nks) {
+    int tiles = ((M + LIMB_BM - 1) / LIMB_BM) * ((N + LIMB_BN - 1) / LIMB_BN);
+    if (tiles >= 256) return 1;
+    int want = (512 + tiles - 1) / tiles;
+    int cap = nks / 8;  // keep at least 8 K-steps per split
+    if (cap < 1) cap = 1;
+    if (want > cap) want = cap;
+    if (want > 64) want = 64;
+    return want < 1 ? 1 : want;
+}
+
+int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
+                     int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split) {
+    LimbA a{};
+    for (int p = 0; p < 3; p++)
+        for (int i = 0; i < 2; i++) a.seg[2 * p + i] = A.getShare(p, i)->data();
+    a.nseg = 6;
+    a.M = M;
+    a.K = K;
+    a.Kp = round_up(K, LIMB_BK);
+    if (conv) {
+        a.conv = 1;
+        a.row_major = 1;
+        a.H = conv->H;
+        a.W = conv->W;
+        a.Cin = conv->Cin;
+        a.fh = a.fw = conv->f;
+        a.ph = a.pw = conv->pad;
+        a.stride = conv->stride;
+        a.OH = conv->OH;
+        a.OW = conv->OW;
+    } else {
+        a.lda = ta ? K : M;
+        a.row_major = ta ? 1 : 0;
+    }
+    Epilogue e{};
+    e.mode = EPI_RSS_TRUNC;
+    e.out = out.base();
+    e.tc = tc ? 1 : 0;
+    e.trunc_bits = trunc_bits;
+    e.bias = bias ? bias->base() : nullptr;
+    int nks = a.nseg * a.Kp / LIMB_BK;
+    if (split <= 0) split = choose_split(M, N, nks);
+    DevBuf ws(split > 1 ? (size_t)split * M * N * sizeof(u64) : 0);
+    return limb_gemm(a, packed, N, e, split, (u64 *)ws.get(), S());
+}
+
+// RSS.inl:1096-1110: three gemms per party into a scratch accumulator
+void localMatMul(const RSS64 &a, const RSS64 &b, PartyData<u64> &c, int M, int N, int K, bool ta, bool tb, bool tc) {
+    PartyData<u64> acc(c.size());
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int t = 0; t < 3; t++) {
+        for (int q = 0; q < 3; q++)
+            check(plain_gemm(M, N, K, a.getShare(q, terms[t][0])->data(), ta, b.getShare(q, terms[t][1])->data(), tb,
+                             acc[q].data(), tc, S()));
+        c += acc;
+        if (t < 2) acc.zero();
+    }
+}
+
+// RSS.inl:481-540
+void matmul(const RSS64 &a, const RSS64 &b, RSS64 &c, int M, int N, int K, bool ta, bool tb, bool tc, u64 truncation) {
+    if (fused_mode() && a.base() && b.base() && c.base()) {
+        if (ctx().algo == ALGO_LIMB) {
+            DevBuf packed(packed_rss_weights_bytes(K, N));
+            check(pack_rss_weights(b, K, N, tb, (int8_t *)packed.get()));
+            check(rss_limb_product(a, ta, nullptr, M, N, K, (const int8_t *)packed.get(), (int)truncation, nullptr, c,
+                                   tc));
+            return;
+        }
+        PartyData<u64> raw((size_t)M * N);
+        localMatMul(a, b, raw, M, N, K, ta, tb, tc);
+        check(fused_divide_3out3(raw.base(), raw.size(), (int)truncation, c.base(), S()));
+        return;
+    }
+    PartyData<u64> raw((size_t)M * N);
+    localMatMul(a, b, raw, M, N, K, ta, tb, tc);
+    dividePublic3out3(raw, (u64)1 << truncation, c);
+}
+
+// RSS.inl:613-639
+void localFprop(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batch, int H, int W, int Din, int Dout,
+                int fh, int fw, int ph, int pw, int stride, int dil) {
+    PartyData<u64> acc(C.size());
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int t = 0; t < 3; t++) {
+        for (int q = 0; q < 3; q++)
+            check(plain_conv_fprop(A.getShare(q, terms[t][0])->data(), B.getShare(q, terms[t][1])->data(),
+                                   acc[q].data(), batch, H, W, Din, Dout, fh, fw, ph, pw, stride, dil, S()));
+        C += acc;
+    }
+}
+
+// RSS.inl:705-803 (kFprop)
+void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int batch, int H, int W, int f, int Din,
+                 int Dout, int stride, int pad, int trunc) {
+    if (op != kFprop) {
+        set_error("convolution: only kFprop is implemented");
+        throw PirError(ERR_UNSUPPORTED);
+    }
+    int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
+    if (fused_mode() && A.base() && B.base() && C.base()) {
+        if (ctx().algo == ALGO_LIMB) {
+            int K = f * f * Din;
+            DevBuf packed(packed_rss_weights_bytes(K, Dout));
+            check(pack_rss_weights(B, K, Dout, false, (int8_t *)packed.get()));
+            ConvShape cs{batch, H, W, Din, f, stride, pad, OH, OW};
+            check(rss_limb_product(A, true, &cs, batch * OH * OW, Dout, K, (const int8_t *)packed.get(), trunc,
+                                   nullptr, C, true));
+            return;
+        }
+        PartyData<u64> local(C.size());
+        localFprop(A, B, local, batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+        check(fused_divide_3out3(local.base(), local.size(), trunc, C.base(), S()));
+        return;
+    }
+    PartyData<u64> local(C.size());
+    localFprop(A, B, local, batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+    dividePublic3out3(local, (u64)1 << trunc, C);
+}
+
+// ---------------------------------------------------------------------------
+// Comparison protocols
+
+// RSS.inl:1112-1209 on strided views
+template <typename T>
+void carryOut(RSS<T> &p, RSS<T> &g, int k, RSS<T> &out) {
+    int stride = 2, offset = 1;
+    RSS<T> pEven = p.strided(0, stride), pOdd = p.strided(offset, stride);
+    RSS<T> gEven = g.strided(0, stride), gOdd = g.strided(offset, stride);
+    while (k > 1) {
+        gEven &= pOdd;
+        pEven &= pOdd;
+        gOdd ^= gEven;
+        gEven = g.strided(offset, stride * 2);
+        offset += stride;
+        gOdd = g.strided(offset, stride * 2);
+        stride *= 2;
+        pEven = p.strided(0, stride);
+        pOdd = p.strided(stride / 2, stride);
+        k /= 2;
+    }
+    RSS<T> oe = out.strided(0, 2), oo = out.strided(1, 2);
+    MultiLaunch ce(OP_SET), co(OP_SET);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) {
+            ce.add(*oe.getShare(q, i), gEven.getShare(q, i));
+            co.add(*oo.getShare(q, i), gOdd.getShare(q, i));
+        }
+    ce.run(sizeof(T));
+    co.run(sizeof(T));
+}
+
+// RSS.inl:806-852
+template <typename U>
+void dReLU(const RSS64 &input, RSS<U> &result) {
+    if (fused_mode() && input.base() && result.base()) {
+        check(fused_drelu<U>(input.base(), input.size(), result.base(), S()));
+        return;
+    }
+    const int bw = 64;
+    size_t n = input.size();
+    RSS64 r(n);
+    RSS<U> rbits(n * bw);
+    rbits.fill(1);
+    PartyData<u64> a(n);
+    r += input;
+    reconstruct(r, a);
+    a += 1;
+    PartyData<U> abits(n * bw);
+    for (int q = 0; q < 3; q++) check(bitexpand(a[q].data(), n, abits[q].data(), sizeof(U), S()));
+    RSS<U> msb(n);
+    for (int q = 0; q < 3; q++) {
+        auto cm = [&](int share, bool xr) {
+            check(set_carry_out_msb(rbits.getShare(q, share)->data(), abits[q].data(), msb.getShare(q, share)->data(),
+                                    n, bw, xr, sizeof(U), S()));
+        };
+        if (q != PARTY_C) {
+            cm(0, q == PARTY_A);
+            cm(1, q == PARTY_C);
+        } else {
+            cm(1, q == PARTY_C);
+            cm(0, q == PARTY_A);
+        }
+    }
+    RSS<U> g(n * bw);
+    g += rbits;
+    g &= abits;
+    RSS<U> p(n * bw);
+    p += rbits;
+    p ^= abits;
+    RSS<U> pre(result.size());
+    carryOut(p, g, bw, pre);
+    pre ^= msb;
+    result.fill(1);
+    result -= pre;
+}
+
+// RSS.inl:542-569
+template <typename U>
+void selectShare(const RSS64 &x, const RSS64 &y, const RSS<U> &b, RSS64 &z) {
+    if (fused_mode() && x.base() && y.base() && b.base() && z.base()) {
+        check(fused_select<U>(x.base(), y.base(), b.base(), z.base(), x.size(), S()));
+        return;
+    }
+    size_t n = x.size();
+    RSS64 c(n);
+    RSS<U> cbits(b.size());
+    cbits ^= b;
+    PartyData<U> e(cbits.size());
+    reconstruct(cbits, e);
+    RSS64 d(e.size());
+    check(fused_convex_comb<U>(d.base(), c.base(), e.base(), n, S()));
+    RSS64 result(n);
+    result += y;
+    result -= x;
+    result *= d;
+    result += x;
+    z.zero();
+    z += result;
+}
+
+// RSS.inl:854-865
+template <typename U>
+void ReLU(const RSS64 &input, RSS64 &result, RSS<U> &dresult) {
+    if (fused_mode() && input.base() && result.base() && dresult.base()) {
+        check(fused_relu<U>(input.base(), input.size(), result.base(), dresult.base(), S()));
+        return;
+    }
+    dReLU(input, dresult);
+    RSS64 zeros(input.size());
+    selectShare(zeros, input, dresult, result);
+}
+
+// RSS.inl:867-1017 (selectShare writes into the even view of `input`)
+template <typename U>
+void maxpool(RSS64 &input, RSS64 &result, RSS<U> &dresult, int k) {
+    if (fused_mode() && input.base() && result.base() && dresult.base()) {
+        check(fused_maxpool_pools<U>(input.base(), input.size(), result.base(), dresult.base(), k, S()));
+        return;
+    }
+    dresult.fill(1);
+    int stride = 2, offset = 1;
+    RSS64 even = input.strided(0, stride), odd = input.strided(offset, stride);
+    auto round = [&](bool last) {
+        RSS64 diff(even.size());
+        diff += even;
+        diff -= odd;
+        RSS<U> b(even.size());
+        dReLU(diff, b);
+        selectShare(odd, even, b, even);
+        if (!last) {
+            stride *= 2;
+            even = input.strided(0, stride);
+            odd = input.strided(stride / 2, stride);
+        } else {
+            result.zero();
+            result += even;
+        }
+        RSS<U> negated(b.size());
+        negated.fill(1);
+        negated -= b;
+        RSS<U> expandedB(input.size());
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 2; i++)
+                check(expand_compare(b.getShare(q, i)->data(), negated.getShare(q, i)->data(),
+                                     expandedB.getShare(q, i)->data(), b.size(), expandedB.size(), sizeof(U), S()));
+        dresult &= expandedB;
+    };
+    while (k > 2) {
+        round(false);
+        k /= 2;
+    }
+    round(true);
+}
+
+// Explicit instantiations
+template void reconstruct<u64>(const RSS<u64> &, PartyData<u64> &);
+template void reconstruct<u8>(const RSS<u8> &, PartyData<u8> &);
+template void reshare<u64>(const PartyData<u64> &, RSS<u64> &);
+template void reshare<u8>(const PartyData<u8> &, RSS<u8> &);
+template void rss_mult_impl<u64>(RSS<u64> &, const RSS<u64> &);
+template void rss_and_impl<u64>(RSS<u64> &, const RSS<u64> &);
+template void rss_and_impl<u8>(RSS<u8> &, const RSS<u8> &);
+template void carryOut<u8>(RSS<u8> &, RSS<u8> &, int, RSS<u8> &);
+template void carryOut<u64>(RSS<u64> &, RSS<u64> &, int, RSS<u64> &);
+template void dReLU<u8>(const RSS64 &, RSS<u8> &);
+template void dReLU<u64>(const RSS64 &, RSS<u64> &);
+template void selectShare<u8>(const RSS64 &, const RSS64 &, const RSS<u8> &, RSS64 &);
+template void selectShare<u64>(const RSS64 &, const RSS64 &, const RSS<u64> &, RSS64 &);
+template void ReLU<u8>(const RSS64 &, RSS64 &, RSS<u8> &);
+template void ReLU<u64>(const RSS64 &, RSS64 &, RSS<u64> &);
+template void maxpool<u8>(RSS64 &, RSS64 &, RSS<u8> &, int);
+template void maxpool<u64>(RSS64 &, RSS64 &, RSS<u64> &, int);
+
+}  // namespace pir

@@ -1,0 +1,82 @@
+// protocol.h -- 3-party RSS protocol functions with the reference's names and
+// argument order (src/mpc/RSS.h:103-145), operating on all parties at once.
+//
+// Each function runs in one of two execution modes (Context::mode):
+//   MODE_FUSED  one fused kernel per protocol step (fused_rss.hip) and the
+//               int8-limb MFMA ring GEMM with the truncation fused into its
+//               epilogue; needs contiguous share tensors.
+//   MODE_PLAIN  the reference's op-by-op structure, one launch per DeviceData /
+//               RSS operation, with uint8-per-bit dReLU shares and the plain
+//               64-bit VALU GEMM (the baseline form).
+// Both produce bit-identical shares.
+#pragma once
+
+#include "device_data.h"
+#include "ring_gemm.h"
+
+namespace pir {
+
+typedef RSS<u64> RSS64;
+
+enum ConvOperator { kFprop = 0 };
+
+template <typename T>
+void reconstruct(const RSS<T> &in, PartyData<T> &out);
+template <typename T>
+void reshare(const PartyData<T> &c, RSS<T> &out);
+void reconstruct3out3(const PartyData<u64> &in, PartyData<u64> &out);
+void dividePublic(RSS64 &a, u64 denominator);
+void dividePublic(RSS64 &a, PartyData<u64> &denominators);
+void dividePublic3out3(PartyData<u64> &a, u64 denominator, RSS64 &out);
+void localMatMul(const RSS64 &a, const RSS64 &b, PartyData<u64> &c, int M, int N, int K, bool transpose_a,
+                 bool transpose_b, bool transpose_c);
+void matmul(const RSS64 &a, const RSS64 &b, RSS64 &c, int M, int N, int K, bool transpose_a, bool transpose_b,
+            bool transpose_c, u64 truncation);
+void localFprop(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batchSize, int imageHeight, int imageWidth,
+                int Din, int Dout, int filterHeight, int filterWidth, int paddingHeight, int paddingWidth, int stride,
+                int dilation);
+void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int batchSize, int imageHeight,
+                 int imageWidth, int filterSize, int Din, int Dout, int stride, int padding, int truncation);
+template <typename U>
+void dReLU(const RSS64 &input, RSS<U> &result);
+template <typename U>
+void ReLU(const RSS64 &input, RSS64 &result, RSS<U> &dresult);
+template <typename U>
+void selectShare(const RSS64 &x, const RSS64 &y, const RSS<U> &b, RSS64 &z);
+template <typename U>
+void maxpool(RSS64 &input, RSS64 &result, RSS<U> &dresult, int k);
+template <typename T>
+void carryOut(RSS<T> &p, RSS<T> &g, int k, RSS<T> &out);
+
+template <typename T>
+void rss_mult_impl(RSS<T> &x, const RSS<T> &y);
+template <typename T>
+void rss_and_impl(RSS<T> &x, const RSS<T> &y);
+
+template <typename T>
+RSS<T> &RSS<T>::operator*=(const RSS<T> &r) {
+    rss_mult_impl(*this, r);
+    return *this;
+}
+template <typename T>
+RSS<T> &RSS<T>::operator&=(const RSS<T> &r) {
+    rss_and_impl(*this, r);
+    return *this;
+}
+
+// Limb-GEMM helpers shared by the protocol and the layers.
+// Packs the RSS weight operand of all parties into the 6-segment K-concatenated
+// limb form [B_p0 + B_p1 ; B_p0] for p = A, B, C.
+int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed);
+size_t packed_rss_weights_bytes(int K, int N);
+// Choose a split-K factor for an M x N output with `nks` K-steps.
+int choose_split(int M, int N, int nks);
+// Runs the fused 3-party product + truncation (+ bias) on packed weights.
+// conv == nullptr: dense A (row-major if ta).  out layout row-major if tc.
+struct ConvShape {
+    int batch, H, W, Cin, f, stride, pad, OH, OW;
+};
+int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
+                     int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split = 0);
+
+}  // namespace pir

@@ -44,7 +44,7 @@ enum EpiMode { EPI_RAW = 0, EPI_RSS_TRUNC = 1 };
 
 struct Epilogue {
     int mode;
-    // EPI_RAW: C(m,n) stored row-major (ldc = N) if tc, else column-major (ldc = M)
+    // C(m,n) / out(m,n) stored row-major (ldc = N) if tc, else column-major (ldc = M)
     u64 *C;
     int tc;
     // EPI_RSS_TRUNC: out is [3][2][M*N]; share (p,s) = base(p,s) + bias(p,s)[n], with

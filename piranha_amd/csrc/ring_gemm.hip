@@ -65,7 +65,7 @@ __device__ __forceinline__ void rss_trunc_store(const Epilogue &e, size_t MN, in
     const int k = e.trunc_bits;
     u64 q = trunc_div_pow2(raw - ((u64)1 << k), k);
     u64 base = (u64)1 + q;
-    size_t idx = (size_t)m * N + n;
+    size_t idx = e.tc ? (size_t)m * N + n : (size_t)n * (MN / N) + m;
 #pragma unroll
     for (int p = 0; p < 3; p++)
 #pragma unroll

@@ -38,6 +38,7 @@ struct Context {
     Allocator alloc;
     int mode = MODE_FUSED;
     int algo = ALGO_LIMB;
+    int frac_bits = 26;  // FLOAT_PRECISION (globals.h:19-21), a runtime value here
     int error = 0;
 };
 

@@ -56,10 +56,21 @@ int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int
 /* The RSS local product of one party, localMatMul (src/mpc/RSS.inl:1096-1110):
  * C = A0.B0 + A0.B1 + A1.B0 mod 2^64, all row-major (A: MxK, B: KxN, C: MxN).
  * algo 0 runs the reference's form (three plain GEMMs, one launch each, plus
- * the two accumulations); algo 1 the fused K-concatenated limb-MFMA form
- * A0.(B0+B1) + A1.B0; algo 2 the same K-concatenated form on the VALU. */
+ * the two accumulations); algo 1 the K-concatenated limb-MFMA form
+ * A0.(B0+B1) + A1.B0 (equal mod 2^64), including the operand packing. */
 int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint64_t *A1, const uint64_t *B0,
                    const uint64_t *B1, uint64_t *C, void *stream);
+
+/* The limb form with explicit operands, no counterpart in the reference (it is
+ * how the CUTLASS gemm above is re-expressed): C = sum_s A_s . B_s over nseg
+ * K-segments, B_s(k,n) = b0[s] + (b1 ? b1[s] : 0) packed once (weights) by
+ * pir_limb_pack into pir_limb_packed_bytes() bytes.  A_s is M x K, row-major if
+ * a_row_major; C is row-major if tc.  split <= 0 picks a split-K factor. */
+size_t pir_limb_packed_bytes(int K, int N, int nseg);
+int pir_limb_pack(const uint64_t *const *b0, const uint64_t *const *b1, int nseg, int K, int N, int b_row_major,
+                  int8_t *packed, void *stream);
+int pir_limb_gemm(const uint64_t *const *a, int nseg, int M, int K, int a_row_major, const int8_t *packed, int N,
+                  uint64_t *C, int tc, int split, void *stream);
 
 /* ---- bit / layout kernels (src/gpu/bitwise.cuh, matrix.cuh, convolution.cuh) */
 int pir_bitexpand(const uint64_t *a, size_t n, void *bits, int bits_elem_bytes, void *stream); /* bitwise.cuh:98-124 */
@@ -85,6 +96,10 @@ int pir_dd_op(int op, int elem_bytes, void *x, long long x_stride, const void *y
               uint64_t scalar, size_t n, void *stream);
 
 /* ---- RSS protocol, all three parties (src/mpc/RSS.h:103-145) -------------- */
+/* Execution form of the pir_rss_* calls: PIR_MODE_FUSED (one fused kernel per
+ * protocol step, default) or PIR_MODE_PLAIN (the reference's op-by-op launches).
+ * Both give bit-identical shares. */
+int pir_set_mode(int mode);
 int pir_rss_reconstruct(const void *x, size_t n, void *out, int elem_bytes, void *stream); /* RSS.inl:457-479 */
 int pir_rss_reshare(const uint64_t *c, size_t n, uint64_t *out, void *stream);             /* RSS.inl:1052-1074 */
 int pir_rss_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream);                 /* RSS.inl:283-320 */
@@ -106,7 +121,7 @@ int pir_rss_select_share(const uint64_t *x, const uint64_t *y, const void *b, ui
 /* maxpool over gathered pools [n/k windows][k] (modified in place like the reference), RSS.inl:867-1017 */
 int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
 
-/* ---- layers and network (src/nn/*Layer.cu, NeuralNetwork.cu) -------------- */
+/* ---- layers and network (src/nn/FCLayer.cu etc., NeuralNetwork.cu) -------------- */
 typedef struct pir_net pir_net;
 enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3 };
 enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
@@ -126,8 +141,13 @@ int pir_net_set_weights(pir_net *net, int layer, const double *weights, size_t n
                         size_t n_biases);
 /* NeuralNetwork::forward (NeuralNetwork.cu:108-133): input setPublic from doubles */
 int pir_net_forward(pir_net *net, const double *input, size_t n);
-/* forward from device-resident input shares [3][2][batch*input_size] */
+/* forward from device-resident input shares [3][2][batch*input_size]
+ * (NULL: the shares already in the network's input buffer) */
 int pir_net_forward_shares(pir_net *net, const uint64_t *input_shares);
+/* the network's own input share buffer and its per-share element count */
+int pir_net_input(pir_net *net, uint64_t **shares, size_t *n);
+/* wait for all work queued by the network */
+int pir_net_synchronize(pir_net *net);
 /* device pointer to a layer's activation shares [3][2][n] and its size */
 int pir_net_activation(pir_net *net, int layer, const uint64_t **shares, size_t *n);
 /* device pointer to a ReLU / Maxpool layer's derivative shares (uint8) */

@@ -398,3 +398,35 @@ def maxpool_forward(x, batch, H, W, C, pool, stride, fp):
     mp = np.zeros((3, 2, n * pool * pool), np.uint8)
     lib().ref_maxpool_forward(_p(x), _p(act), _p(mp), batch, H, W, C, pool, stride, fp)
     return act, mp
+
+
+def net_forward(layers, batch, fp, params, x_shares):
+    """NeuralNetwork::forward (NeuralNetwork.cu:108-133) over the oracle layers.
+
+    layers: symbolic specs (piranha_amd.configs form); params[i] = (W, b) share
+    tensors for fc / cnn layers (b None for cnn); x_shares: input share tensor.
+    Returns the list of (activation, derivative-or-None) per layer.
+    """
+    out = []
+    h = x_shares
+    for i, l in enumerate(layers):
+        kind = l[0]
+        if kind == "fc":
+            W, b = params[i]
+            h = fc_forward(h, W, b, batch, l[1], l[2], fp)
+            out.append((h, None))
+        elif kind == "cnn":
+            W, _ = params[i]
+            H, Wd, Cin, Cout, f, s, p = l[1:]
+            h = cnn_forward(h, W, batch, H, Wd, Cin, Cout, f, s, p, fp)
+            out.append((h, None))
+        elif kind == "relu":
+            h, d = relu_forward(h)
+            out.append((h, d))
+        elif kind == "maxpool":
+            H, Wd, C, pool, s = l[1:]
+            h, d = maxpool_forward(h, batch, H, Wd, C, pool, s, fp)
+            out.append((h, d))
+        else:
+            raise ValueError(kind)
+    return out

@@ -1,0 +1,256 @@
+"""MI355X-native 3-party RSS secure-inference path (Piranha's per-party
+secret-share compute path) -- Python handle on libpir_hip.so.
+
+The product is the C ABI in include/pir.h (HIP kernels + C++ host runtime);
+this module only loads it with ctypes and moves numpy arrays to and from
+device memory, for tests and benchmarks.  It fails loudly if the library is
+missing: there is no CPU fallback.
+
+Share arrays: an RSS tensor for all parties is (3, 2, n) uint64 (party,
+share, element); derivative / bit shares are (3, 2, n) uint8 or uint64.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpir_hip.so")
+
+ALGO_PLAIN, ALGO_LIMB = 0, 1
+MODE_FUSED, MODE_PLAIN = 0, 1
+LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL = 0, 1, 2, 3
+OP_ADD, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV = range(8)
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_sz = ctypes.c_size_t
+_u64 = ctypes.c_uint64
+_ll = ctypes.c_longlong
+
+_SIGS = {
+    "pir_version": ([], _i),
+    "pir_last_error": ([], ctypes.c_char_p),
+    "pir_malloc": ([ctypes.POINTER(_vp), _sz], _i),
+    "pir_free": ([_vp], _i),
+    "pir_memcpy_h2d": ([_vp, _vp, _sz, _vp], _i),
+    "pir_memcpy_d2h": ([_vp, _vp, _sz, _vp], _i),
+    "pir_memcpy_d2d": ([_vp, _vp, _sz, _vp], _i),
+    "pir_fill_u64": ([_vp, _u64, _sz, _vp], _i),
+    "pir_fill_u8": ([_vp, ctypes.c_uint8, _sz, _vp], _i),
+    "pir_stream_create": ([ctypes.POINTER(_vp)], _i),
+    "pir_stream_destroy": ([_vp], _i),
+    "pir_stream_synchronize": ([_vp], _i),
+    "pir_device_synchronize": ([], _i),
+    "pir_set_mode": ([_i], _i),
+    "pir_gemm": ([_i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp], _i),
+    "pir_conv_fprop": ([_vp, _vp, _vp] + [_i] * 12 + [_vp], _i),
+    "pir_ring_gemm3": ([_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "pir_limb_packed_bytes": ([_i, _i, _i], _sz),
+    "pir_limb_pack": ([_vp, _vp, _i, _i, _i, _i, _vp, _vp], _i),
+    "pir_limb_gemm": ([_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _i, _vp], _i),
+    "pir_bitexpand": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_set_carry_out_msb": ([_vp, _vp, _vp, _sz, _i, _i, _i, _vp], _i),
+    "pir_expand_compare": ([_vp, _vp, _vp, _sz, _sz, _i, _vp], _i),
+    "pir_element_vector_add": ([_vp, _vp, _i, _i, _i, _vp], _i),
+    "pir_truncate_cols": ([_vp, _sz, _vp, _i, _i, _i, _vp], _i),
+    "pir_stride_pad": ([_vp, _vp, _sz, _i, _i, _u64, _vp], _i),
+    "pir_maxpool_im2row_size": ([_i] * 7, _sz),
+    "pir_maxpool_im2row": ([_vp, _vp] + [_i] * 7 + [_u64, _vp], _i),
+    "pir_dd_op": ([_i, _i, _vp, _ll, _vp, _ll, _u64, _sz, _vp], _i),
+    "pir_rss_reconstruct": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_rss_reshare": ([_vp, _sz, _vp, _vp], _i),
+    "pir_rss_mult": ([_vp, _vp, _sz, _vp], _i),
+    "pir_rss_and": ([_vp, _vp, _sz, _i, _vp], _i),
+    "pir_rss_divide_public": ([_vp, _sz, _u64, _vp, _vp], _i),
+    "pir_rss_divide_public_3out3": ([_vp, _sz, _i, _vp, _vp], _i),
+    "pir_rss_matmul": ([_vp, _vp, _vp] + [_i] * 9 + [_vp], _i),
+    "pir_rss_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
+    "pir_rss_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_rss_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),
+    "pir_rss_select_share": ([_vp, _vp, _vp, _vp, _sz, _i, _vp], _i),
+    "pir_rss_maxpool": ([_vp, _sz, _vp, _vp, _i, _i, _vp], _i),
+    "pir_net_create": ([ctypes.POINTER(_vp), _vp, _i, _i, _i, _i, _i, _vp], _i),
+    "pir_net_destroy": ([_vp], _i),
+    "pir_net_num_layers": ([_vp], _i),
+    "pir_net_set_weights": ([_vp, _i, _vp, _sz, _vp, _sz], _i),
+    "pir_net_forward": ([_vp, _vp, _sz], _i),
+    "pir_net_forward_shares": ([_vp, _vp], _i),
+    "pir_net_input": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
+    "pir_net_synchronize": ([_vp], _i),
+    "pir_net_activation": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
+    "pir_net_derivative": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
+    "pir_net_weights": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_vp),
+                         ctypes.POINTER(_sz)], _i),
+    "pir_net_capture": ([_vp, _vp], _i),
+    "pir_net_replay": ([_vp], _i),
+    "pir_net_set_mode": ([_vp, _i], _i),
+}
+
+_lib = None
+
+
+class PirError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpir_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PirError(f"{LIB_PATH} is missing: build it with `python -m piranha_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, ret) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ret
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise PirError(f"status {rc}: {lib().pir_last_error().decode()}")
+
+
+class Buffer:
+    """Device buffer holding a numpy-shaped array (freed on deletion)."""
+
+    def __init__(self, shape, dtype, stream=None):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.stream = stream
+        p = _vp()
+        check(lib().pir_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        b = cls(a.shape, a.dtype, stream)
+        if b.nbytes:
+            check(lib().pir_memcpy_h2d(b.ptr, a.ctypes.data, b.nbytes, stream))
+        return b
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().pir_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, self.stream))
+        return out
+
+    def zero(self):
+        if self.dtype.itemsize == 8:
+            check(lib().pir_fill_u64(self.ptr, 0, self.nbytes // 8, self.stream))
+        else:
+            check(lib().pir_fill_u8(self.ptr, 0, self.nbytes, self.stream))
+        return self
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.pir_free(self.ptr)
+            self.ptr = None
+
+
+def to_dev(a, stream=None):
+    return Buffer.from_numpy(a, stream)
+
+
+def synchronize():
+    check(lib().pir_device_synchronize())
+
+
+def set_mode(mode):
+    check(lib().pir_set_mode(mode))
+
+
+def _ptr_array(ptrs):
+    arr = (_vp * len(ptrs))(*[p for p in ptrs])
+    return ctypes.cast(arr, _vp), arr
+
+
+class Network:
+    """Handle on a pir_net (NeuralNetwork of the reference, all three parties)."""
+
+    def __init__(self, spec, batch, frac_bits, seed=0, mode=MODE_FUSED):
+        flat = []
+        for layer in spec:
+            row = list(layer) + [0] * (8 - len(layer))
+            flat.extend(row[:8])
+        self._spec = np.asarray(flat, np.int32)
+        h = _vp()
+        check(lib().pir_net_create(ctypes.byref(h), self._spec.ctypes.data, len(spec), batch, frac_bits, seed, mode,
+                                   None))
+        self.h = h.value
+        self.batch = batch
+        self.frac_bits = frac_bits
+        self.spec = [tuple(s) for s in spec]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pir_net_destroy(self.h)
+            self.h = None
+
+    @property
+    def num_layers(self):
+        return lib().pir_net_num_layers(self.h)
+
+    def set_mode(self, mode):
+        check(lib().pir_net_set_mode(self.h, mode))
+
+    def set_weights(self, layer, w, b=None):
+        w = np.ascontiguousarray(w, np.float64).ravel()
+        bp, bn = (None, 0)
+        if b is not None:
+            b = np.ascontiguousarray(b, np.float64).ravel()
+            bp, bn = b.ctypes.data, b.size
+        check(lib().pir_net_set_weights(self.h, layer, w.ctypes.data, w.size, bp, bn))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float64).ravel()
+        check(lib().pir_net_forward(self.h, x.ctypes.data, x.size))
+
+    def forward_shares(self, dev_ptr=None):
+        check(lib().pir_net_forward_shares(self.h, dev_ptr))
+
+    def input_ptr(self):
+        p, n = _vp(), _sz()
+        check(lib().pir_net_input(self.h, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def synchronize(self):
+        check(lib().pir_net_synchronize(self.h))
+
+    def capture(self, dev_ptr=None):
+        check(lib().pir_net_capture(self.h, dev_ptr))
+
+    def replay(self):
+        check(lib().pir_net_replay(self.h))
+
+    def _read(self, fn, layer, dtype):
+        p, n = _vp(), _sz()
+        check(fn(self.h, layer, ctypes.byref(p), ctypes.byref(n)))
+        self.synchronize()
+        out = np.empty((3, 2, n.value), dtype)
+        check(lib().pir_memcpy_d2h(out.ctypes.data, p.value, out.nbytes, None))
+        return out
+
+    def activation(self, layer):
+        return self._read(lib().pir_net_activation, layer, np.uint64)
+
+    def derivative(self, layer):
+        return self._read(lib().pir_net_derivative, layer, np.uint8)
+
+    def weights(self, layer):
+        p, n, bp, bn = _vp(), _sz(), _vp(), _sz()
+        check(lib().pir_net_weights(self.h, layer, ctypes.byref(p), ctypes.byref(n), ctypes.byref(bp),
+                                    ctypes.byref(bn)))
+        self.synchronize()
+        w = np.empty((3, 2, n.value), np.uint64)
+        check(lib().pir_memcpy_d2h(w.ctypes.data, p.value, w.nbytes, None))
+        b = None
+        if bp.value:
+            b = np.empty((3, 2, bn.value), np.uint64)
+            check(lib().pir_memcpy_d2h(b.ctypes.data, bp.value, b.nbytes, None))
+        return w, b

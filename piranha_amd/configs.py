@@ -1,0 +1,107 @@
+"""Network layer specs for the measured configurations, in the layer-spec form
+of pir_net_create (include/pir.h), plus a reader for the reference's model JSON
+format (src/util/model.cu:24-115, files/models/*.json).
+
+Layer specs: ("fc", in, out) | ("cnn", H, W, Cin, Cout, f, stride, pad) |
+("relu", n_per_image) | ("maxpool", H, W, C, pool, stride).
+"""
+import json
+
+from . import LAYER_CNN, LAYER_FC, LAYER_MAXPOOL, LAYER_RELU
+
+_TYPES = {"fc": LAYER_FC, "cnn": LAYER_CNN, "relu": LAYER_RELU, "maxpool": LAYER_MAXPOOL}
+
+
+def to_spec(layers):
+    """Symbolic layers -> integer rows for pir_net_create."""
+    return [(_TYPES[l[0]],) + tuple(int(v) for v in l[1:]) for l in layers]
+
+
+def input_size(layers):
+    """Per-image input element count of the first layer."""
+    l = layers[0]
+    if l[0] in ("fc", "relu"):
+        return l[1]
+    return l[1] * l[2] * l[3]
+
+
+def output_size(layers):
+    """Per-image output element count of the last layer."""
+    l = layers[-1]
+    if l[0] == "fc":
+        return l[2]
+    if l[0] == "relu":
+        return l[1]
+    if l[0] == "cnn":
+        H, W, _, Cout, f, s, p = l[1:]
+        return ((H + 2 * p - f) // s + 1) * ((W + 2 * p - f) // s + 1) * Cout
+    H, W, C, pool, s = l[1:]
+    return ((H - pool) // s + 1) * ((W - pool) // s + 1) * C
+
+
+def from_model_json(doc, avgpool_as_maxpool=True):
+    """Reads the reference's model JSON (model.cu:24-115).  Average-pool layers
+    become max-pool layers of the same geometry when `avgpool_as_maxpool`."""
+    if isinstance(doc, str):
+        with open(doc) as f:
+            doc = json.load(f)
+    out = []
+    for l in doc["model"]:
+        t = l["layer"]
+        if t == "fc":
+            out.append(("fc", l["input_dim"], l["output_dim"]))
+        elif t == "relu":
+            out.append(("relu", l["input_dim"]))
+        elif t == "cnn":
+            out.append(("cnn", l["input_hw"][0], l["input_hw"][1], l["in_channels"], l["out_channels"],
+                        l["filter_hw"][0], l["stride"], l["padding"]))
+        elif t == "maxpool" or (t == "averagepool" and avgpool_as_maxpool):
+            out.append(("maxpool", l["input_hw"][0], l["input_hw"][1], l["in_channels"], l["pool_hw"][0],
+                        l["stride"]))
+        else:
+            raise ValueError(f"layer type {t!r} is not on the inference path")
+    return out
+
+
+# files/models/secureml.json: FC 784-128-128-10 with ReLU after each FC
+SECUREML = [("fc", 784, 128), ("relu", 128), ("fc", 128, 128), ("relu", 128), ("fc", 128, 10), ("relu", 10)]
+
+# files/models/lenet.json
+LENET = [
+    ("cnn", 28, 28, 1, 20, 5, 1, 0), ("maxpool", 24, 24, 20, 2, 2), ("relu", 2880),
+    ("cnn", 12, 12, 20, 50, 5, 1, 0), ("maxpool", 8, 8, 50, 2, 2), ("relu", 800),
+    ("fc", 800, 500), ("relu", 500), ("fc", 500, 10), ("relu", 10),
+]
+
+
+def _vgg16():
+    # files/models/vgg16-cifar10.json with its 2x2 stride-2 pools as max-pools
+    L = []
+    hw, cin = 32, 3
+    blocks = [(64, 2), (128, 2), (256, 3), (512, 3), (512, 3)]
+    for cout, nconv in blocks:
+        for i in range(nconv):
+            L.append(("cnn", hw, hw, cin, cout, 3, 1, 1))
+            cin = cout
+            if i < nconv - 1:
+                L.append(("relu", hw * hw * cout))
+        L.append(("maxpool", hw, hw, cout, 2, 2))
+        hw //= 2
+        L.append(("relu", hw * hw * cout))
+    L += [("fc", 512, 256), ("relu", 256), ("fc", 256, 256), ("relu", 256), ("fc", 256, 10), ("relu", 10)]
+    return L
+
+
+VGG16_CIFAR = _vgg16()
+
+# files/models/alexnet-cifar10.json with its pools as max-pools
+ALEXNET_CIFAR = [
+    ("cnn", 32, 32, 3, 96, 11, 4, 9), ("maxpool", 10, 10, 96, 3, 2), ("relu", 1536),
+    ("cnn", 4, 4, 96, 256, 5, 1, 1), ("maxpool", 2, 2, 256, 2, 1), ("relu", 256),
+    ("cnn", 1, 1, 256, 384, 3, 1, 1), ("relu", 384),
+    ("cnn", 1, 1, 384, 384, 3, 1, 1), ("relu", 384),
+    ("cnn", 1, 1, 384, 256, 3, 1, 1), ("relu", 256),
+    ("fc", 256, 256), ("relu", 256), ("fc", 256, 256), ("relu", 256), ("fc", 256, 10), ("relu", 10),
+]
+
+MODELS = {"secureml": SECUREML, "lenet": LENET, "vgg16_cifar": VGG16_CIFAR, "alexnet_cifar": ALEXNET_CIFAR}

@@ -1,0 +1,653 @@
+// capi.cpp -- extern "C" entry points of libpir_hip.so (declared in include/pir.h).
+#include "../../include/pir.h"
+
+#include <memory>
+
+#include "elementwise.h"
+#include "fused_rss.h"
+#include "layers.h"
+#include "protocol.h"
+#include "ring_gemm.h"
+
+using namespace pir;
+
+namespace {
+
+// Settings of the default context used by the free-standing pir_rss_* calls.
+int g_mode = MODE_FUSED;
+
+struct Scope {
+    explicit Scope(void *stream, int algo = ALGO_LIMB) {
+        Context &c = ctx();
+        c.stream = (hipStream_t)stream;
+        c.mode = g_mode;
+        c.algo = algo;
+    }
+};
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const PirError &e) {
+        return e.code();
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return ERR_STATE;
+    }
+}
+
+}  // namespace
+
+struct pir_net {
+    Context ctx;
+    std::unique_ptr<NeuralNetwork> net;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool own_stream = false;
+};
+
+namespace {
+struct NetScope {
+    explicit NetScope(pir_net *n) { set_ctx(&n->ctx); }
+    ~NetScope() { set_ctx(nullptr); }
+};
+}  // namespace
+
+extern "C" {
+
+int pir_version(void) { return 1; }
+const char *pir_last_error(void) { return last_error(); }
+
+int pir_malloc(void **ptr, size_t bytes) {
+    PIR_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return OK;
+}
+int pir_free(void *ptr) {
+    PIR_HIP(hipFree(ptr));
+    return OK;
+}
+int pir_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    PIR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return OK;
+}
+int pir_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    PIR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return OK;
+}
+int pir_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    PIR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return OK;
+}
+int pir_fill_u64(uint64_t *dst, uint64_t value, size_t n, void *stream) {
+    return fill_u64(dst, value, n, (hipStream_t)stream);
+}
+int pir_fill_u8(uint8_t *dst, uint8_t value, size_t n, void *stream) { return fill_u8(dst, value, n, (hipStream_t)stream); }
+int pir_stream_create(void **stream) {
+    hipStream_t s;
+    PIR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return OK;
+}
+int pir_stream_destroy(void *stream) {
+    PIR_HIP(hipStreamDestroy((hipStream_t)stream));
+    return OK;
+}
+int pir_stream_synchronize(void *stream) {
+    PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return OK;
+}
+int pir_device_synchronize(void) {
+    PIR_HIP(hipDeviceSynchronize());
+    return OK;
+}
+int pir_set_mode(int mode) {
+    PIR_REQUIRE(mode == MODE_FUSED || mode == MODE_PLAIN, "pir_set_mode: unknown mode");
+    g_mode = mode;
+    return OK;
+}
+
+// ---- ring GEMM -------------------------------------------------------------
+
+size_t pir_limb_packed_bytes(int K, int N, int nseg) { return limb_b_bytes(N, nseg, round_up(K, LIMB_BK)); }
+
+int pir_limb_pack(const uint64_t *const *b0, const uint64_t *const *b1, int nseg, int K, int N, int row_major,
+                  int8_t *packed, void *stream) {
+    PIR_REQUIRE(nseg >= 1 && nseg <= 6 && K > 0 && N > 0, "pir_limb_pack: invalid shape");
+    LimbBSrc s{};
+    for (int i = 0; i < nseg; i++) {
+        s.b0[i] = b0[i];
+        s.b1[i] = b1 ? b1[i] : nullptr;
+    }
+    s.nseg = nseg;
+    s.K = K;
+    s.N = N;
+    s.Kp = round_up(K, LIMB_BK);
+    s.row_major = row_major;
+    return limb_pack_b(s, packed, (hipStream_t)stream);
+}
+
+int pir_limb_gemm(const uint64_t *const *a, int nseg, int M, int K, int a_row_major, const int8_t *packed, int N,
+                  uint64_t *C, int tc, int split, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(nseg >= 1 && nseg <= 6 && M >= 0 && K > 0 && N >= 0, "pir_limb_gemm: invalid shape");
+        LimbA la{};
+        for (int i = 0; i < nseg; i++) la.seg[i] = a[i];
+        la.nseg = nseg;
+        la.M = M;
+        la.K = K;
+        la.Kp = round_up(K, LIMB_BK);
+        la.lda = a_row_major ? K : M;
+        la.row_major = a_row_major;
+        Epilogue e{};
+        e.mode = EPI_RAW;
+        e.C = C;
+        e.tc = tc;
+        int nks = nseg * la.Kp / LIMB_BK;
+        if (split <= 0) split = choose_split(M, N, nks);
+        DevBuf ws(split > 1 ? (size_t)split * M * N * 8 : 0);
+        return limb_gemm(la, packed, N, e, split, (u64 *)ws.get(), (hipStream_t)stream);
+    });
+}
+
+int pir_gemm(int M, int N, int K, const uint64_t *A, int ta, const uint64_t *B, int tb, uint64_t *C, int tc, int algo,
+             void *stream) {
+    if (algo == PIR_ALGO_PLAIN) return plain_gemm(M, N, K, A, ta, B, tb, C, tc, (hipStream_t)stream);
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(M >= 0 && N >= 0 && K > 0, "pir_gemm: invalid shape");
+        DevBuf packed(pir_limb_packed_bytes(K, N, 1));
+        const uint64_t *b0[1] = {B};
+        PIR_TRY(pir_limb_pack(b0, nullptr, 1, K, N, tb, (int8_t *)packed.get(), stream));
+        const uint64_t *a0[1] = {A};
+        return pir_limb_gemm(a0, 1, M, K, ta, (const int8_t *)packed.get(), N, C, tc, 0, stream);
+    });
+}
+
+int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int H, int W, int Din, int Dout, int fh,
+                   int fw, int ph, int pw, int stride, int dilation, int algo, void *stream) {
+    if (algo == PIR_ALGO_PLAIN)
+        return plain_conv_fprop(x, w, y, b, H, W, Din, Dout, fh, fw, ph, pw, stride, dilation, (hipStream_t)stream);
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(dilation == 1, "pir_conv_fprop: the limb form supports dilation 1");
+        PIR_REQUIRE(b > 0 && H > 0 && W > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 && stride > 0,
+                    "pir_conv_fprop: invalid geometry");
+        int OH = (H + 2 * ph - fh) / stride + 1, OW = (W + 2 * pw - fw) / stride + 1;
+        PIR_REQUIRE(OH > 0 && OW > 0, "pir_conv_fprop: empty output");
+        int K = fh * fw * Din, M = b * OH * OW;
+        DevBuf packed(pir_limb_packed_bytes(K, Dout, 1));
+        const uint64_t *b0[1] = {w};
+        PIR_TRY(pir_limb_pack(b0, nullptr, 1, K, Dout, 0, (int8_t *)packed.get(), stream));
+        LimbA la{};
+        la.seg[0] = x;
+        la.nseg = 1;
+        la.M = M;
+        la.K = K;
+        la.Kp = round_up(K, LIMB_BK);
+        la.conv = 1;
+        la.row_major = 1;
+        la.H = H;
+        la.W = W;
+        la.Cin = Din;
+        la.fh = fh;
+        la.fw = fw;
+        la.ph = ph;
+        la.pw = pw;
+        la.stride = stride;
+        la.OH = OH;
+        la.OW = OW;
+        Epilogue e{};
+        e.mode = EPI_RAW;
+        e.C = y;
+        e.tc = 1;
+        int nks = la.Kp / LIMB_BK;
+        int split = choose_split(M, Dout, nks);
+        DevBuf ws(split > 1 ? (size_t)split * M * Dout * 8 : 0);
+        return limb_gemm(la, (const int8_t *)packed.get(), Dout, e, split, (u64 *)ws.get(), (hipStream_t)stream);
+    });
+}
+
+int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint64_t *A1, const uint64_t *B0,
+                   const uint64_t *B1, uint64_t *C, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(M > 0 && N > 0 && K > 0, "pir_ring_gemm3: invalid shape");
+        hipStream_t s = (hipStream_t)stream;
+        if (algo == PIR_ALGO_PLAIN) {
+            // localMatMul's form (RSS.inl:1096-1110): three gemms and two accumulations
+            DevBuf acc((size_t)M * N * 8);
+            PIR_TRY(plain_gemm(M, N, K, A0, true, B0, true, C, true, s));
+            PIR_TRY(plain_gemm(M, N, K, A0, true, B1, true, (u64 *)acc.get(), true, s));
+            PIR_TRY(dd_op(OP_ADD, 8, C, 1, acc.get(), 1, 0, (size_t)M * N, s));
+            PIR_TRY(plain_gemm(M, N, K, A1, true, B0, true, (u64 *)acc.get(), true, s));
+            return dd_op(OP_ADD, 8, C, 1, acc.get(), 1, 0, (size_t)M * N, s);
+        }
+        PIR_REQUIRE(algo == PIR_ALGO_LIMB, "pir_ring_gemm3: unknown algorithm");
+        DevBuf packed(pir_limb_packed_bytes(K, N, 2));
+        const uint64_t *b0[2] = {B0, B0}, *b1[2] = {B1, nullptr};
+        PIR_TRY(pir_limb_pack(b0, b1, 2, K, N, 1, (int8_t *)packed.get(), stream));
+        const uint64_t *a[2] = {A0, A1};
+        return pir_limb_gemm(a, 2, M, K, 1, (const int8_t *)packed.get(), N, C, 1, 0, stream);
+    });
+}
+
+// ---- bit / layout kernels ----------------------------------------------------
+
+int pir_bitexpand(const uint64_t *a, size_t n, void *bits, int eb, void *stream) {
+    return bitexpand(a, n, bits, eb, (hipStream_t)stream);
+}
+int pir_set_carry_out_msb(void *rbits, void *abits, void *msb, size_t n, int bw, int xr, int eb, void *stream) {
+    return set_carry_out_msb(rbits, abits, msb, n, bw, xr, eb, (hipStream_t)stream);
+}
+int pir_expand_compare(const void *b, const void *inv, void *out, size_t n, size_t out_n, int eb, void *stream) {
+    return expand_compare(b, inv, out, n, out_n, eb, (hipStream_t)stream);
+}
+int pir_element_vector_add(uint64_t *a, const uint64_t *b, int rowwise, int rows, int cols, void *stream) {
+    return element_vector_add(a, b, rowwise, rows, cols, (hipStream_t)stream);
+}
+int pir_truncate_cols(const void *a, size_t a_n, void *b, int sc, int ec, int eb, void *stream) {
+    return truncate_cols(a, a_n, b, sc, ec, eb, (hipStream_t)stream);
+}
+int pir_stride_pad(const uint64_t *in, uint64_t *out, size_t n_in, int stride, int pad, uint64_t pv, void *stream) {
+    return stride_pad(in, out, n_in, stride, pad, pv, (hipStream_t)stream);
+}
+size_t pir_maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding) {
+    return maxpool_im2row_size(H, W, f, Din, batch, stride, padding);
+}
+int pir_maxpool_im2row(const uint64_t *im, uint64_t *out, int H, int W, int f, int Din, int batch, int stride,
+                       int padding, uint64_t pv, void *stream) {
+    return maxpool_im2row(im, out, H, W, f, Din, batch, stride, padding, pv, (hipStream_t)stream);
+}
+int pir_dd_op(int op, int eb, void *x, long long xs, const void *y, long long ys, uint64_t scalar, size_t n,
+              void *stream) {
+    PIR_REQUIRE(op >= 0 && op <= PIR_OP_DIV, "pir_dd_op: unknown op");
+    return dd_op(op, eb, x, xs, y, ys, scalar, n, (hipStream_t)stream);
+}
+
+// ---- RSS protocol ------------------------------------------------------------
+
+int pir_rss_reconstruct(const void *x, size_t n, void *out, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        if (eb == 8) {
+            PartyData<u64> o;
+            RSS<u64> in = RSS<u64>::wrap((u64 *)x, n);
+            PartyData<u64> ov(DeviceData<u64>((u64 *)out, n), DeviceData<u64>((u64 *)out + n, n),
+                              DeviceData<u64>((u64 *)out + 2 * n, n));
+            if (g_mode == MODE_FUSED) return fused_reconstruct<u64>((const u64 *)x, n, (u64 *)out, (hipStream_t)stream);
+            reconstruct(in, ov);
+            return (int)OK;
+        }
+        PIR_REQUIRE(eb == 1, "pir_rss_reconstruct: element size must be 1 or 8");
+        if (g_mode == MODE_FUSED) return fused_reconstruct<u8>((const u8 *)x, n, (u8 *)out, (hipStream_t)stream);
+        RSS<u8> in = RSS<u8>::wrap((u8 *)x, n);
+        PartyData<u8> ov(DeviceData<u8>((u8 *)out, n), DeviceData<u8>((u8 *)out + n, n),
+                         DeviceData<u8>((u8 *)out + 2 * n, n));
+        reconstruct(in, ov);
+        return (int)OK;
+    });
+}
+
+int pir_rss_reshare(const uint64_t *c, size_t n, uint64_t *out, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        if (g_mode == MODE_FUSED) return fused_reshare(c, n, out, (hipStream_t)stream);
+        PartyData<u64> cv(DeviceData<u64>((u64 *)c, n), DeviceData<u64>((u64 *)c + n, n),
+                          DeviceData<u64>((u64 *)c + 2 * n, n));
+        RSS<u64> o = RSS<u64>::wrap(out, n);
+        reshare(cv, o);
+        return (int)OK;
+    });
+}
+
+int pir_rss_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        RSS<u64> a = RSS<u64>::wrap(x, n), b = RSS<u64>::wrap((u64 *)y, n);
+        a *= b;
+        return (int)OK;
+    });
+}
+
+int pir_rss_and(void *x, const void *y, size_t n, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        if (eb == 8) {
+            RSS<u64> a = RSS<u64>::wrap((u64 *)x, n), b = RSS<u64>::wrap((u64 *)y, n);
+            a &= b;
+        } else {
+            PIR_REQUIRE(eb == 1, "pir_rss_and: element size must be 1 or 8");
+            RSS<u8> a = RSS<u8>::wrap((u8 *)x, n), b = RSS<u8>::wrap((u8 *)y, n);
+            a &= b;
+        }
+        return (int)OK;
+    });
+}
+
+int pir_rss_divide_public(uint64_t *a, size_t n, uint64_t d, const uint64_t *d_local, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        RSS<u64> x = RSS<u64>::wrap(a, n);
+        if (d_local) {
+            PartyData<u64> dv(DeviceData<u64>((u64 *)d_local, n), DeviceData<u64>((u64 *)d_local + n, n),
+                              DeviceData<u64>((u64 *)d_local + 2 * n, n));
+            if (g_mode == MODE_FUSED) {
+                // the fused kernel needs a contiguous [3][n] denominator block
+                DevBuf flag(sizeof(int));
+                PIR_TRY(fill_u8((u8 *)flag.get(), 0, sizeof(int), (hipStream_t)stream));
+                PIR_TRY(fused_divide_public(a, n, 0, d_local, (int *)flag.get(), (hipStream_t)stream));
+                int bad = 0;
+                PIR_HIP(hipMemcpyAsync(&bad, flag.get(), sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream));
+                PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
+                PIR_REQUIRE(!bad, "dividePublic: denominators must be powers of two");
+                return (int)OK;
+            }
+            dividePublic(x, dv);
+        } else {
+            dividePublic(x, d);
+        }
+        return (int)OK;
+    });
+}
+
+int pir_rss_divide_public_3out3(const uint64_t *a, size_t n, int k, uint64_t *out, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(k >= 0 && k <= 62, "dividePublic3out3: k must be in [0, 62]");
+        if (g_mode == MODE_FUSED) return fused_divide_3out3(a, n, k, out, (hipStream_t)stream);
+        // the op-by-op form modifies its input (RSS.inl:1085): work on a copy
+        PartyData<u64> av(n);
+        PIR_HIP(hipMemcpyAsync(av.base(), a, 3 * n * 8, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+        RSS<u64> o = RSS<u64>::wrap(out, n);
+        dividePublic3out3(av, (u64)1 << k, o);
+        return (int)OK;
+    });
+}
+
+int pir_rss_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
+                   int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(M > 0 && N > 0 && K > 0 && trunc >= 0 && trunc <= 62, "pir_rss_matmul: invalid arguments");
+        RSS<u64> A = RSS<u64>::wrap((u64 *)a, (size_t)M * K), B = RSS<u64>::wrap((u64 *)b, (size_t)K * N);
+        RSS<u64> C = RSS<u64>::wrap(c, (size_t)M * N);
+        matmul(A, B, C, M, N, K, ta != 0, tb != 0, tc != 0, (u64)trunc);
+        return (int)OK;
+    });
+}
+
+int pir_rss_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
+                        int Dout, int stride, int pad, int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(batch > 0 && H > 0 && W > 0 && f > 0 && Din > 0 && Dout > 0 && stride > 0 && pad >= 0,
+                    "pir_rss_convolution: invalid geometry");
+        int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
+        PIR_REQUIRE(OH > 0 && OW > 0, "pir_rss_convolution: empty output");
+        RSS<u64> A = RSS<u64>::wrap((u64 *)a, (size_t)batch * H * W * Din);
+        RSS<u64> B = RSS<u64>::wrap((u64 *)w, (size_t)Dout * f * f * Din);
+        RSS<u64> C = RSS<u64>::wrap(c, (size_t)batch * OH * OW * Dout);
+        convolution(A, B, C, kFprop, batch, H, W, f, Din, Dout, stride, pad, trunc);
+        return (int)OK;
+    });
+}
+
+int pir_rss_drelu(const uint64_t *x, size_t n, void *result, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        RSS<u64> in = RSS<u64>::wrap((u64 *)x, n);
+        if (eb == 1) {
+            RSS<u8> r = RSS<u8>::wrap((u8 *)result, n);
+            dReLU(in, r);
+        } else {
+            PIR_REQUIRE(eb == 8, "pir_rss_drelu: element size must be 1 or 8");
+            RSS<u64> r = RSS<u64>::wrap((u64 *)result, n);
+            dReLU(in, r);
+        }
+        return (int)OK;
+    });
+}
+
+int pir_rss_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        RSS<u64> in = RSS<u64>::wrap((u64 *)x, n), out = RSS<u64>::wrap(y, n);
+        if (eb == 1) {
+            RSS<u8> d = RSS<u8>::wrap((u8 *)dresult, n);
+            ReLU(in, out, d);
+        } else {
+            PIR_REQUIRE(eb == 8, "pir_rss_relu: element size must be 1 or 8");
+            RSS<u64> d = RSS<u64>::wrap((u64 *)dresult, n);
+            ReLU(in, out, d);
+        }
+        return (int)OK;
+    });
+}
+
+int pir_rss_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int eb,
+                         void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        RSS<u64> X = RSS<u64>::wrap((u64 *)x, n), Y = RSS<u64>::wrap((u64 *)y, n), Z = RSS<u64>::wrap(z, n);
+        if (eb == 1) {
+            RSS<u8> Bv = RSS<u8>::wrap((u8 *)b, n);
+            selectShare(X, Y, Bv, Z);
+        } else {
+            PIR_REQUIRE(eb == 8, "pir_rss_select_share: element size must be 1 or 8");
+            RSS<u64> Bv = RSS<u64>::wrap((u64 *)b, n);
+            selectShare(X, Y, Bv, Z);
+        }
+        return (int)OK;
+    });
+}
+
+int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(k >= 2 && (k & (k - 1)) == 0 && n % k == 0, "pir_rss_maxpool: k must be a power of two dividing n");
+        RSS<u64> P = RSS<u64>::wrap(pools, n), R = RSS<u64>::wrap(result, n / k);
+        if (eb == 1) {
+            RSS<u8> D = RSS<u8>::wrap((u8 *)dresult, n);
+            maxpool(P, R, D, k);
+        } else {
+            PIR_REQUIRE(eb == 8, "pir_rss_maxpool: element size must be 1 or 8");
+            RSS<u64> D = RSS<u64>::wrap((u64 *)dresult, n);
+            maxpool(P, R, D, k);
+        }
+        return (int)OK;
+    });
+}
+
+// ---- network -------------------------------------------------------------------
+
+int pir_net_create(pir_net **out, const int *spec, int num_layers, int batch, int frac_bits, int seed, int mode,
+                   void *stream) {
+    PIR_REQUIRE(out && spec && num_layers > 0 && batch > 0, "pir_net_create: invalid arguments");
+    PIR_REQUIRE(frac_bits >= 1 && frac_bits <= 30, "pir_net_create: frac_bits must be in [1, 30]");
+    PIR_REQUIRE(mode == MODE_FUSED || mode == MODE_PLAIN, "pir_net_create: unknown mode");
+    pir_net *n = new pir_net();
+    n->ctx.frac_bits = frac_bits;
+    n->ctx.mode = mode;
+    n->ctx.algo = mode == MODE_FUSED ? ALGO_LIMB : ALGO_PLAIN;
+    if (stream) {
+        n->ctx.stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&n->ctx.stream, hipStreamNonBlocking) != hipSuccess) {
+            delete n;
+            set_error("pir_net_create: stream creation failed");
+            return ERR_HIP;
+        }
+        n->own_stream = true;
+    }
+    int rc = guarded([&]() -> int {
+        NetScope ns(n);
+        std::vector<LayerSpec> sp(num_layers);
+        for (int i = 0; i < num_layers; i++)
+            for (int j = 0; j < 8; j++) sp[i].v[j] = spec[i * 8 + j];
+        n->net.reset(new NeuralNetwork(sp, batch, seed));
+        return (int)OK;
+    });
+    if (rc) {
+        pir_net_destroy(n);
+        return rc;
+    }
+    *out = n;
+    return OK;
+}
+
+int pir_net_destroy(pir_net *n) {
+    if (!n) return OK;
+    {
+        NetScope ns(n);
+        (void)hipStreamSynchronize(n->ctx.stream);
+        if (n->exec) (void)hipGraphExecDestroy(n->exec);
+        if (n->graph) (void)hipGraphDestroy(n->graph);
+        n->net.reset();
+    }
+    hipStream_t s = n->ctx.stream;
+    bool own = n->own_stream;
+    delete n;
+    if (own) (void)hipStreamDestroy(s);
+    return OK;
+}
+
+int pir_net_num_layers(const pir_net *n) { return n && n->net ? (int)n->net->layers.size() : 0; }
+
+int pir_net_set_mode(pir_net *n, int mode) {
+    PIR_REQUIRE(n && (mode == MODE_FUSED || mode == MODE_PLAIN), "pir_net_set_mode: invalid arguments");
+    n->ctx.mode = mode;
+    n->ctx.algo = mode == MODE_FUSED ? ALGO_LIMB : ALGO_PLAIN;
+    return OK;
+}
+
+int pir_net_set_weights(pir_net *n, int layer, const double *w, size_t nw, const double *b, size_t nb) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && layer >= 0 && layer < pir_net_num_layers(n), "pir_net_set_weights: bad layer");
+        NetScope ns(n);
+        Layer *L = n->net->layers[layer].get();
+        PIR_REQUIRE(L->getWeights(), "pir_net_set_weights: layer has no weights");
+        PIR_REQUIRE(nw == L->getWeights()->size(), "pir_net_set_weights: weight count mismatch");
+        L->getWeights()->setPublic(std::vector<double>(w, w + nw), n->ctx.frac_bits);
+        if (b) {
+            PIR_REQUIRE(L->getBiases() && nb == L->getBiases()->size(), "pir_net_set_weights: bias count mismatch");
+            L->getBiases()->setPublic(std::vector<double>(b, b + nb), n->ctx.frac_bits);
+        }
+        L->weightsChanged();
+        return (int)OK;
+    });
+}
+
+int pir_net_forward(pir_net *n, const double *input, size_t count) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && input, "pir_net_forward: invalid arguments");
+        NetScope ns(n);
+        PIR_REQUIRE(count == n->net->input.size(), "pir_net_forward: input size mismatch");
+        n->net->forward(std::vector<double>(input, input + count));
+        return (int)OK;
+    });
+}
+
+int pir_net_forward_shares(pir_net *n, const uint64_t *shares) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_forward_shares: invalid arguments");
+        NetScope ns(n);
+        if (shares && shares != n->net->input.base())
+            PIR_HIP(hipMemcpyAsync(n->net->input.base(), shares, 6 * n->net->input.size() * 8,
+                                   hipMemcpyDeviceToDevice, n->ctx.stream));
+        n->net->run();
+        return (int)OK;
+    });
+}
+
+int pir_net_input(pir_net *n, uint64_t **shares, size_t *count) {
+    PIR_REQUIRE(n, "pir_net_input: invalid arguments");
+    *shares = n->net->input.base();
+    *count = n->net->input.size();
+    return OK;
+}
+
+int pir_net_synchronize(pir_net *n) {
+    PIR_REQUIRE(n, "pir_net_synchronize: invalid arguments");
+    PIR_HIP(hipStreamSynchronize(n->ctx.stream));
+    return OK;
+}
+
+int pir_net_activation(pir_net *n, int layer, const uint64_t **shares, size_t *count) {
+    PIR_REQUIRE(n && layer >= 0 && layer < pir_net_num_layers(n), "pir_net_activation: bad layer");
+    RSS64 *a = n->net->layers[layer]->getActivation();
+    *shares = a->base();
+    *count = a->size();
+    return OK;
+}
+
+int pir_net_derivative(pir_net *n, int layer, const uint8_t **shares, size_t *count) {
+    PIR_REQUIRE(n && layer >= 0 && layer < pir_net_num_layers(n), "pir_net_derivative: bad layer");
+    RSS<u8> *d = n->net->layers[layer]->getDerivative();
+    PIR_REQUIRE(d, "pir_net_derivative: layer has no derivative shares");
+    *shares = d->base();
+    *count = d->size();
+    return OK;
+}
+
+int pir_net_weights(pir_net *n, int layer, const uint64_t **shares, size_t *count, const uint64_t **bias,
+                    size_t *n_bias) {
+    PIR_REQUIRE(n && layer >= 0 && layer < pir_net_num_layers(n), "pir_net_weights: bad layer");
+    Layer *L = n->net->layers[layer].get();
+    PIR_REQUIRE(L->getWeights(), "pir_net_weights: layer has no weights");
+    *shares = L->getWeights()->base();
+    *count = L->getWeights()->size();
+    if (bias) *bias = L->getBiases() ? L->getBiases()->base() : nullptr;
+    if (n_bias) *n_bias = L->getBiases() ? L->getBiases()->size() : 0;
+    return OK;
+}
+
+int pir_net_capture(pir_net *n, const uint64_t *shares) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_capture: invalid arguments");
+        NetScope ns(n);
+        hipStream_t s = n->ctx.stream;
+        if (shares && shares != n->net->input.base())
+            PIR_HIP(hipMemcpyAsync(n->net->input.base(), shares, 6 * n->net->input.size() * 8,
+                                   hipMemcpyDeviceToDevice, s));
+        // one eager pass packs the weights and warms the allocator cache
+        n->net->run();
+        PIR_HIP(hipStreamSynchronize(s));
+        if (n->exec) {
+            (void)hipGraphExecDestroy(n->exec);
+            n->exec = nullptr;
+        }
+        if (n->graph) {
+            (void)hipGraphDestroy(n->graph);
+            n->graph = nullptr;
+        }
+        PIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        n->ctx.alloc.capturing = true;
+        int rc = guarded([&]() -> int {
+            n->net->run();
+            return (int)OK;
+        });
+        n->ctx.alloc.capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PIR_HIP(e);
+        n->graph = g;
+        PIR_HIP(hipGraphInstantiate(&n->exec, g, nullptr, nullptr, 0));
+        return (int)OK;
+    });
+}
+
+int pir_net_replay(pir_net *n) {
+    PIR_REQUIRE(n && n->exec, "pir_net_replay: no captured graph");
+    PIR_HIP(hipGraphLaunch(n->exec, n->ctx.stream));
+    return OK;
+}
+
+}  // extern "C"

@@ -127,6 +127,11 @@ class DeviceData {
     long long stride_ = 1;
 };
 
+template <typename T>
+struct NoDeduce {
+    typedef T type;
+};
+
 // Builds one multi-array launch.
 class MultiLaunch {
   public:
@@ -135,7 +140,7 @@ class MultiLaunch {
         m_.op = op;
     }
     template <typename T>
-    void add(DeviceData<T> &x, const DeviceData<T> *y, u64 scalar = 0) {
+    void add(DeviceData<T> &x, const typename NoDeduce<DeviceData<T>>::type *y, u64 scalar = 0) {
         int a = m_.count++;
         m_.x[a] = x.data();
         m_.xs[a] = x.stride();

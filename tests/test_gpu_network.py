@@ -1,0 +1,86 @@
+"""Whole-network forward on the GPU (fused and op-by-op modes, eager and
+hipGraph replay) against the oracle's layer-by-layer restatement
+(FCLayer.cu, CNNLayer.cu, ReLULayer.cu, MaxpoolLayer.cu, NeuralNetwork.cu):
+every layer's activation shares and derivative shares, bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+import piranha_amd as P
+from piranha_amd import configs as C
+
+pytestmark = pytest.mark.gpu
+
+
+def build(layers, batch, fp, seed, mode):
+    net = P.Network(C.to_spec(layers), batch, fp, seed=seed, mode=mode)
+    params = {}
+    for i, l in enumerate(layers):
+        if l[0] in ("fc", "cnn"):
+            params[i] = net.weights(i)
+    return net, params
+
+
+def check_against_oracle(net, layers, batch, fp, params, x):
+    want = O.net_forward(layers, batch, fp, params, O.set_public(x, fp))
+    for i, (act, der) in enumerate(want):
+        got = net.activation(i)
+        assert np.array_equal(got, act), f"layer {i} ({layers[i][0]}) activation shares differ"
+        if der is not None:
+            assert np.array_equal(net.derivative(i), der), f"layer {i} derivative shares differ"
+
+
+CASES = [
+    ("secureml", C.SECUREML, 128, 13),
+    ("lenet", C.LENET, 4, 13),
+    ("alexnet", C.ALEXNET_CIFAR, 3, 20),
+    ("vgg16", C.VGG16_CIFAR, 1, 26),
+]
+
+
+@pytest.mark.parametrize("mode", [P.MODE_FUSED, P.MODE_PLAIN], ids=["fused", "plain"])
+@pytest.mark.parametrize("name,layers,batch,fp", CASES, ids=[c[0] for c in CASES])
+def test_network_forward(name, layers, batch, fp, mode):
+    if mode == P.MODE_PLAIN and name == "vgg16":
+        pytest.skip("op-by-op VGG16 parity is covered through the fused/plain equality test")
+    rng = np.random.default_rng(7)
+    net, params = build(layers, batch, fp, seed=11, mode=mode)
+    x = rng.uniform(0.0, 1.0, batch * C.input_size(layers))
+    net.forward(x)
+    check_against_oracle(net, layers, batch, fp, params, x)
+
+
+def test_fused_equals_plain_vgg16():
+    layers, batch, fp = C.VGG16_CIFAR, 2, 26
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.0, 1.0, batch * C.input_size(layers))
+    outs = []
+    for mode in (P.MODE_FUSED, P.MODE_PLAIN):
+        net, _ = build(layers, batch, fp, seed=5, mode=mode)
+        net.forward(x)
+        outs.append([net.activation(i) for i in range(len(layers))])
+    for i, (a, b) in enumerate(zip(*outs)):
+        assert np.array_equal(a, b), f"layer {i} differs between fused and plain forms"
+
+
+def test_graph_replay_matches_eager():
+    layers, batch, fp = C.LENET, 8, 13
+    rng = np.random.default_rng(9)
+    net, params = build(layers, batch, fp, seed=2, mode=P.MODE_FUSED)
+    x1 = rng.uniform(0, 1, batch * C.input_size(layers))
+    net.forward(x1)
+    eager = net.activation(len(layers) - 1)
+    ptr, n = net.input_ptr()
+    net.capture()
+    for _ in range(3):
+        net.replay()
+        net.synchronize()
+        assert np.array_equal(net.activation(len(layers) - 1), eager)
+    # new input through the same graph
+    x2 = rng.uniform(0, 1, batch * C.input_size(layers))
+    sh = P.to_dev(O.set_public(x2, fp))
+    P.check(P.lib().pir_memcpy_d2d(ptr, sh.ptr, 6 * n * 8, None))
+    P.synchronize()
+    net.replay()
+    net.synchronize()
+    check_against_oracle(net, layers, batch, fp, params, x2)

@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3-party RSS secure-inference path on one MI355X.
+
+Headline (`value`): 3PC VGG16-CIFAR-shape inference, batch 64, Z_2^64 with 26
+fractional bits, all three parties on one GPU, int8-limb MFMA ring products and
+fused protocol kernels replayed as one hipGraph per forward.  One step = one
+forward over the batch, inputs resident in HBM.
+
+Also reported on the same line:
+  * plain_images_per_s: the same forward in the reference's form (plain 64-bit
+    VALU GEMMs, one launch per DeviceData / RSS op, uint8-per-bit dReLU);
+  * ring_gemm: the 3-term RSS local product A0B0+A0B1+A1B0 at M=N=K in
+    {512..8192}, Gop/s = 6MNK/t, plain VALU vs int8-limb MFMA;
+  * other_configs: SecureML MLP, LeNet and the uneven-batch AlexNet workload;
+  * roofline of the dominant kernel (the limb implicit-GEMM convolution);
+  * cpu_baseline: the CPU oracle on VGG16 at a small batch, host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POP/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def conv_limb_ops(layer, batch):
+    """Algorithmic int8 ops of one fused 3-party limb convolution launch:
+    2 ops x 36 limb pairs x M x N x (6 K-segments x K)."""
+    _, H, W, Cin, Cout, f, s, p = layer
+    OH, OW = (H + 2 * p - f) // s + 1, (W + 2 * p - f) // s + 1
+    M, N, K = batch * OH * OW, Cout, f * f * Cin
+    return 2 * 36 * M * N * 6 * K
+
+
+def time_graph(P, net, steps, warmup):
+    for _ in range(warmup):
+        net.replay()
+    net.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.replay()
+    net.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def time_eager(net, steps, warmup):
+    for _ in range(warmup):
+        net.forward_shares(None)
+    net.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward_shares(None)
+    net.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True):
+    net = P.Network(C.to_spec(layers), batch, fp, seed=seed, mode=mode)
+    rng = np.random.default_rng(seed)
+    net.forward(rng.uniform(0.0, 1.0, batch * C.input_size(layers)))  # setPublic of U[0,1) inputs
+    net.synchronize()
+    if capture:
+        net.capture()
+    return net
+
+
+def bench_ring_gemm(P, sizes, reps):
+    L = P.lib()
+    out = []
+    rng = np.random.default_rng(0)
+    for n in sizes:
+        A0, A1, B0, B1 = (rng.integers(0, 2**64, n * n, dtype=np.uint64) for _ in range(4))
+        d = [P.to_dev(x) for x in (A0, A1, B0, B1)]
+        Cp, Cl = P.Buffer((n * n,), np.uint64), P.Buffer((n * n,), np.uint64)
+        # plain: localMatMul's three gemms + two accumulations
+        P.check(L.pir_ring_gemm3(P.ALGO_PLAIN, n, n, n, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, Cp.ptr, None))
+        P.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            P.check(L.pir_ring_gemm3(P.ALGO_PLAIN, n, n, n, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, Cp.ptr, None))
+        P.synchronize()
+        tp = (time.perf_counter() - t0) / reps
+        # limb: B packed once (as for weights), then the K-concatenated product
+        packed = P.Buffer((L.pir_limb_packed_bytes(n, n, 2),), np.uint8)
+        b0, k0 = P._ptr_array([d[2].ptr, d[2].ptr])
+        b1, k1 = P._ptr_array([d[3].ptr, None])
+        a, k2 = P._ptr_array([d[0].ptr, d[1].ptr])
+        P.synchronize()
+        t0 = time.perf_counter()
+        P.check(L.pir_limb_pack(b0, b1, 2, n, n, 1, packed.ptr, None))
+        P.synchronize()
+        tpack = time.perf_counter() - t0
+        P.check(L.pir_limb_gemm(a, 2, n, n, 1, packed.ptr, n, Cl.ptr, 1, 0, None))
+        P.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            P.check(L.pir_limb_gemm(a, 2, n, n, 1, packed.ptr, n, Cl.ptr, 1, 0, None))
+        P.synchronize()
+        tl = (time.perf_counter() - t0) / reps
+        same = bool(np.array_equal(Cp.numpy(), Cl.numpy()))
+        ops = 6.0 * n ** 3
+        rec = {"n": n, "plain_gops": ops / tp / 1e9, "limb_gops": ops / tl / 1e9, "limb_pack_ms": tpack * 1e3,
+               "limb_int8_tops": 144.0 * n ** 3 / tl / 1e12, "bit_exact_plain_vs_limb": same}
+        rec["limb_mfma_frac"] = rec["limb_int8_tops"] / INT8_PEAK_TOPS
+        log(f"ring gemm n={n}: plain {rec['plain_gops']:.1f} Gop/s, limb {rec['limb_gops']:.1f} Gop/s "
+            f"({rec['limb_int8_tops']:.0f} int8 TOP/s), equal={same}")
+        out.append(rec)
+        del d, Cp, Cl, packed
+    return out
+
+
+def cpu_baseline(C, batch):
+    import oracle as O
+    layers, fp = C.VGG16_CIFAR, 26
+    rng = np.random.default_rng(1)
+    params = {}
+    for i, l in enumerate(layers):
+        if l[0] == "fc":
+            params[i] = (O.set_public(rng.normal(0, np.sqrt(2 / l[1]), l[1] * l[2]), fp),
+                         O.set_public(np.full(l[2], 0.01), fp))
+        elif l[0] == "cnn":
+            fan = l[5] * l[5] * l[3]
+            params[i] = (O.set_public(rng.normal(0, np.sqrt(2 / fan), l[3] * l[4] * l[5] * l[5]), fp), None)
+    x = O.set_public(rng.uniform(0, 1, batch * C.input_size(layers)), fp)
+    O.net_forward(layers, 1, fp, {k: v for k, v in params.items()}, O.set_public(rng.uniform(0, 1, 3072), fp))
+    t0 = time.perf_counter()
+    O.net_forward(layers, batch, fp, params, x)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": batch / dt, "unit": "images/s", "cores": cores, "kind": "port",
+            "sample": f"VGG16-CIFAR 3PC forward, batch {batch}, FP26, CPU oracle (g++ -O3 OpenMP, "
+                      f"plain uint64 GEMM loops, uint8-per-bit dReLU), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--gemm-sizes", default="512,1024,2048,4096,8192")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-batch", type=int, default=4)
+    ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (all three parties share it); --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import piranha_amd as P
+    from piranha_amd import configs as C
+
+    layers, batch, fp = C.VGG16_CIFAR, args.batch, 26
+    log(f"VGG16-CIFAR b{batch}: building (fused limb-MFMA, hipGraph)")
+    net = make_net(P, C, layers, batch, fp, P.MODE_FUSED)
+    t = time_graph(P, net, args.steps, args.warmup)
+    images_s = batch / t
+    log(f"VGG16 fused: {t * 1e3:.3f} ms/forward, {images_s:.1f} images/s")
+
+    # dominant kernel: limb implicit-GEMM convolution, timed per layer with HIP events
+    ms = net.profile_layers()
+    for _ in range(2):
+        ms = np.minimum(ms, net.profile_layers())
+    conv_idx = [i for i, l in enumerate(layers) if l[0] == "cnn"]
+    conv_ms = float(sum(ms[i] for i in conv_idx))
+    conv_ops = float(sum(conv_limb_ops(layers[i], batch) for i in conv_idx))
+    avg_ms = conv_ms / len(conv_idx)
+    achieved = conv_ops / len(conv_idx) / (avg_ms * 1e-3) / 1e12
+    share = conv_ms / float(ms.sum())
+    log("per-layer ms: " + ", ".join(f"{layers[i][0]}{i}:{m:.3f}" for i, m in enumerate(ms)))
+    roofline = {"bound": "mfma", "kernel": "limb_gemm_kernel<2> (3-party implicit-GEMM conv, int8 limbs)",
+                "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                "traffic": None, "avg_launch_ms": avg_ms, "launches_per_forward": len(conv_idx),
+                "share_of_forward": share}
+    del net
+
+    result = {
+        "metric": "3PC secure inference images/s (1 GPU) + ring-GEMM Gop/s, limb-MFMA vs plain",
+        "value": images_s, "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"3PC VGG16-CIFAR inference, batch {batch}, 32x32x3 inputs U[0,1), "
+                               "13 conv3x3 + 5 maxpool2x2 + FC 512-256-256-10, Z_2^64 FP26, "
+                               "He-init weights, all 3 parties on one GPU, hipGraph forward",
+                   "global_batch": batch, "parallelism": "none (1 GPU, 3 simulated parties)"},
+        "roofline": roofline,
+    }
+
+    if not args.quick:
+        log("VGG16 plain one-launch-per-op form")
+        pnet = make_net(P, C, layers, batch, fp, P.MODE_PLAIN, capture=False)
+        tp = time_eager(pnet, args.plain_steps, 1)
+        del pnet
+        result["plain_images_per_s"] = batch / tp
+        result["speedup_vs_plain"] = images_s / (batch / tp)
+        log(f"VGG16 plain: {tp * 1e3:.1f} ms/forward, {batch / tp:.1f} images/s, speedup {result['speedup_vs_plain']:.1f}x")
+
+        sizes = [int(s) for s in args.gemm_sizes.split(",") if s]
+        result["ring_gemm"] = bench_ring_gemm(P, sizes, reps=2)
+
+        other = {}
+        for name, L, b, f in [("secureml_b128_fp13", C.SECUREML, 128, 13), ("lenet_b128_fp13", C.LENET, 128, 13)]:
+            n = make_net(P, C, L, b, f, P.MODE_FUSED)
+            tt = time_graph(P, n, max(5, args.steps // 2), 2)
+            other[name] = {"images_per_s": b / tt, "ms_per_forward": tt * 1e3}
+            log(f"{name}: {b / tt:.1f} images/s")
+            del n
+        qrng = np.random.default_rng(20)
+        qb = qrng.integers(1, 33, 64)
+        total = int(qb.sum())
+        n = make_net(P, C, C.ALEXNET_CIFAR, total, 20, P.MODE_FUSED)
+        tt = time_graph(P, n, max(5, args.steps // 2), 2)
+        other["alexnet_uneven_64q_fp20"] = {"images_per_s": total / tt, "ms_per_forward": tt * 1e3,
+                                            "queries": 64, "images": total,
+                                            "grouping": "queries concatenated along the batch axis into one graph"}
+        log(f"alexnet uneven: {total} images, {total / tt:.1f} images/s")
+        del n
+        result["other_configs"] = other
+
+        log("CPU baseline (oracle)")
+        result["cpu_baseline"] = cpu_baseline(C, args.cpu_batch)
+        log(f"CPU: {result['cpu_baseline']['value']:.3f} images/s on {result['cpu_baseline']['cores']} threads")
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -157,6 +157,9 @@ int pir_net_weights(pir_net *net, int layer, const uint64_t **shares, size_t *n,
 /* hipGraph capture of the whole forward (input from pir_net_forward_shares' buffer) */
 int pir_net_capture(pir_net *net, const uint64_t *input_shares);
 int pir_net_replay(pir_net *net);
+/* one eager forward with a HIP event after each layer on the network's stream;
+ * ms[i] = device time of layer i (count = number of layers) */
+int pir_net_profile_layers(pir_net *net, float *ms, int count);
 int pir_net_set_mode(pir_net *net, int mode);
 
 #ifdef __cplusplus

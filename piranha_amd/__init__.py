@@ -64,7 +64,7 @@ _SIGS = {
     "pir_rss_and": ([_vp, _vp, _sz, _i, _vp], _i),
     "pir_rss_divide_public": ([_vp, _sz, _u64, _vp, _vp], _i),
     "pir_rss_divide_public_3out3": ([_vp, _sz, _i, _vp, _vp], _i),
-    "pir_rss_matmul": ([_vp, _vp, _vp] + [_i] * 9 + [_vp], _i),
+    "pir_rss_matmul": ([_vp, _vp, _vp] + [_i] * 8 + [_vp], _i),
     "pir_rss_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
     "pir_rss_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_rss_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),
@@ -85,6 +85,7 @@ _SIGS = {
     "pir_net_capture": ([_vp, _vp], _i),
     "pir_net_replay": ([_vp], _i),
     "pir_net_set_mode": ([_vp, _i], _i),
+    "pir_net_profile_layers": ([_vp, _vp, _i], _i),
 }
 
 _lib = None
@@ -227,6 +228,12 @@ class Network:
 
     def replay(self):
         check(lib().pir_net_replay(self.h))
+
+    def profile_layers(self):
+        """Device milliseconds of each layer over one eager forward (HIP events)."""
+        ms = np.zeros(self.num_layers, np.float32)
+        check(lib().pir_net_profile_layers(self.h, ms.ctypes.data, ms.size))
+        return ms
 
     def _read(self, fn, layer, dtype):
         p, n = _vp(), _sz()

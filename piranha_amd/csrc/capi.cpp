@@ -644,6 +644,26 @@ int pir_net_capture(pir_net *n, const uint64_t *shares) {
     });
 }
 
+int pir_net_profile_layers(pir_net *n, float *ms, int count) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && ms && count == pir_net_num_layers(n), "pir_net_profile_layers: invalid arguments");
+        NetScope ns(n);
+        hipStream_t s = n->ctx.stream;
+        std::vector<hipEvent_t> ev(count + 1);
+        for (auto &e : ev) PIR_HIP(hipEventCreate(&e));
+        PIR_HIP(hipEventRecord(ev[0], s));
+        auto &L = n->net->layers;
+        for (int i = 0; i < count; i++) {
+            L[i]->forward(i == 0 ? n->net->input : *L[i - 1]->getActivation());
+            PIR_HIP(hipEventRecord(ev[i + 1], s));
+        }
+        PIR_HIP(hipEventSynchronize(ev[count]));
+        for (int i = 0; i < count; i++) PIR_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        return (int)OK;
+    });
+}
+
 int pir_net_replay(pir_net *n) {
     PIR_REQUIRE(n && n->exec, "pir_net_replay: no captured graph");
     PIR_HIP(hipGraphLaunch(n->exec, n->ctx.stream));

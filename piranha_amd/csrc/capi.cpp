@@ -148,6 +148,7 @@ int pir_limb_gemm(const uint64_t *const *a, int nseg, int M, int K, int a_row_ma
         e.tc = tc;
         int nks = nseg * la.Kp / LIMB_BK;
         if (split <= 0) split = choose_split(M, N, nks);
+        split = limb_effective_split(nks, split);
         DevBuf ws(split > 1 ? (size_t)split * M * N * 8 : 0);
         return limb_gemm(la, packed, N, e, split, (u64 *)ws.get(), (hipStream_t)stream);
     });
@@ -205,7 +206,7 @@ int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int
         e.C = y;
         e.tc = 1;
         int nks = la.Kp / LIMB_BK;
-        int split = choose_split(M, Dout, nks);
+        int split = limb_effective_split(nks, choose_split(M, Dout, nks));
         DevBuf ws(split > 1 ? (size_t)split * M * Dout * 8 : 0);
         return limb_gemm(la, (const int8_t *)packed.get(), Dout, e, split, (u64 *)ws.get(), (hipStream_t)stream);
     });

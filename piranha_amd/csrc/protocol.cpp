@@ -267,6 +267,7 @@ int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int 
     e.bias = bias ? bias->base() : nullptr;
     int nks = a.nseg * a.Kp / LIMB_BK;
     if (split <= 0) split = choose_split(M, N, nks);
+    split = limb_effective_split(nks, split);
     DevBuf ws(split > 1 ? (size_t)split * M * N * sizeof(u64) : 0);
     return limb_gemm(a, packed, N, e, split, (u64 *)ws.get(), S());
 }

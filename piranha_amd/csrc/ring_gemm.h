@@ -60,6 +60,19 @@ constexpr int LIMB_FLUSH_STEPS = 256;  // 256 * 32 = 8192 k per int32 accumulati
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Split-K factor actually used by limb_gemm for `nks` K-steps: at least
+// ceil(nks / LIMB_FLUSH_STEPS) (each workgroup covers <= 8192 k), at most nks,
+// and rounded so that every split is non-empty.  Callers size the workspace
+// (split * M * N * 8 bytes when > 1) with this value.
+inline int limb_effective_split(int nks, int split) {
+    int lo = (nks + LIMB_FLUSH_STEPS - 1) / LIMB_FLUSH_STEPS;
+    if (split < lo) split = lo;
+    if (split > nks) split = nks;
+    if (split < 1) split = 1;
+    int per = (nks + split - 1) / split;
+    return (nks + per - 1) / per;
+}
+
 // Size in bytes of the packed B operand.
 inline size_t limb_b_bytes(int N, int nseg, int Kp) {
     size_t nt = (size_t)(N + LIMB_BN - 1) / LIMB_BN;

@@ -133,13 +133,148 @@ int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Limb MFMA GEMM.  Workgroup tile 64x64, 4 waves each owning a 32x32 output
-// block computed with v_mfma_i32_32x32x32_i8; K-step 32.
+// block computed with v_mfma_i32_32x32x32_i8; K-step 32.  The next K-step's
+// global loads are issued before the current step's MFMAs and staged into LDS
+// after them, so HBM/L2 latency hides behind the matrix work.  LDS rows are
+// padded from 32 to 48 bytes: the 16-byte operand reads of 16 consecutive rows
+// then fall on distinct banks.
 
-template <int AMODE>  // 0: dense row-major A, 1: dense column-major A, 2: implicit im2col
-__global__ __launch_bounds__(256) void limb_gemm_kernel(LimbA a, const int8_t *__restrict__ Bp, int N, int nks,
-                                                        int steps_per_split, Epilogue epi, u64 *__restrict__ part) {
-    __shared__ __attribute__((aligned(16))) int8_t As[8][LIMB_BM][LIMB_BK];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[8][LIMB_BN][LIMB_BK];
+constexpr int LDS_ROW = 48;
+constexpr int LDS_PLANE = LIMB_BM * LDS_ROW;
+
+// A-operand loader: each thread owns 8 consecutive k of one row of the tile.
+// Loads are branch-free (indices are clamped into the operand and the value is
+// masked to zero), so all eight are in flight together.  The K-step position
+// (segment, step within segment) is uniform across the workgroup.
+//   AMODE 0: dense row-major A, 1: dense column-major A,
+//   AMODE 2: implicit im2col, any Cin (per-element decode),
+//   AMODE 3: implicit im2col with Cin % 32 == 0 (one (r, s) per K-step, decoded
+//            once per workgroup and advanced incrementally).
+template <int AMODE>
+struct ALoader {
+    const LimbA &a;
+    int am, akq, kps;
+    bool row_ok;
+    int seg, kk;          // uniform: segment and K-step within the segment
+    int cb, ih0, iw0;     // conv: image and top-left input coordinate of this row's pixel
+    int r, s, c0;         // AMODE 3: k = kk*32 = (r*fw + s)*Cin + c0
+
+    __device__ ALoader(const LimbA &a_, int am_, int akq_) : a(a_), am(am_), akq(akq_) {
+        kps = a.Kp / LIMB_BK;
+        row_ok = am < a.M;
+        cb = ih0 = iw0 = 0;
+        r = s = c0 = 0;
+        if (AMODE >= 2) {
+            int m = row_ok ? am : 0;
+            int ohw = a.OH * a.OW;
+            cb = m / ohw;
+            int rem = m - cb * ohw;
+            int oh = rem / a.OW, ow = rem - (rem / a.OW) * a.OW;
+            ih0 = oh * a.stride - a.ph;
+            iw0 = ow * a.stride - a.pw;
+        }
+    }
+    __device__ void decode() {
+        if (AMODE == 3) {
+            int k = kk * LIMB_BK, fwc = a.fw * a.Cin;
+            r = k / fwc;
+            int rem = k - r * fwc;
+            s = rem / a.Cin;
+            c0 = rem - s * a.Cin;
+        }
+    }
+    __device__ void init(int ks) {
+        seg = ks / kps;
+        kk = ks - seg * kps;
+        decode();
+    }
+    __device__ void advance() {
+        if (++kk == kps) {
+            kk = 0;
+            seg++;
+            decode();
+        } else if (AMODE == 3) {
+            c0 += LIMB_BK;
+            if (c0 == a.Cin) {
+                c0 = 0;
+                if (++s == a.fw) {
+                    s = 0;
+                    ++r;
+                }
+            }
+        }
+    }
+    __device__ void load(u64 w[8]) const {
+        const u64 *__restrict__ src = a.seg[seg];
+        const int kb = kk * LIMB_BK + akq * 8;
+        if (AMODE == 3) {
+            int ih = ih0 + r, iw = iw0 + s;
+            bool ok = row_ok && kb < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
+            const u64 *p = src + (((size_t)cb * a.H + (ok ? ih : 0)) * a.W + (ok ? iw : 0)) * a.Cin +
+                           (ok ? c0 + akq * 8 : 0);
+            const u64 msk = ok ? ~0ull : 0ull;
+#pragma unroll
+            for (int j = 0; j < 8; j++) w[j] = p[j] & msk;
+        } else if (AMODE == 2) {
+            const int fwc = a.fw * a.Cin;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                int k = kb + j;
+                int rr = k / fwc;
+                int rem = k - rr * fwc;
+                int ss = rem / a.Cin;
+                int cc = rem - ss * a.Cin;
+                int ih = ih0 + rr, iw = iw0 + ss;
+                bool ok = row_ok && k < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
+                size_t idx = ok ? (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + cc : 0;
+                w[j] = src[idx] & (ok ? ~0ull : 0ull);
+            }
+        } else {
+            const int m = row_ok ? am : 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                int k = kb + j;
+                bool ok = row_ok && k < a.K;
+                int kc = ok ? k : 0;
+                size_t idx = AMODE == 0 ? (size_t)m * a.lda + kc : (size_t)kc * a.lda + m;
+                w[j] = src[idx] & (ok ? ~0ull : 0ull);
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ void stage_a(int8_t *As, int arow, int akq, const u64 w[8]) {
+    u64 d[8], limbs[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) d[j] = recode_digits(w[j]);
+    transpose8x8(d, limbs);
+#pragma unroll
+    for (int i = 0; i < 8; i++) *(u64 *)(As + i * LDS_PLANE + arow * LDS_ROW + akq * 8) = limbs[i];
+}
+
+__device__ __forceinline__ void stage_b(int8_t *Bs, int tid, int4 b0, int4 b1, int4 b2, int4 b3) {
+    // 16-byte chunk ch of the packed [limb][64 n][32 k] block
+    int ch = tid;
+    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b0;
+    ch += 256;
+    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b1;
+    ch += 256;
+    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b2;
+    ch += 256;
+    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b3;
+}
+
+// One workgroup covers at most LIMB_FLUSH_STEPS K-steps (8192 k), so the eight
+// int32 group accumulators never overflow and no 64-bit running sum is needed
+// inside the loop; longer K is split across workgroups (split-K) and the 64-bit
+// partials are summed by split_reduce_kernel.  With <= 256 registers per lane
+// (128 accumulators + VGPRs) two workgroups share a CU, so one's staging
+// overlaps the other's MFMAs.
+template <int AMODE>
+__global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t *__restrict__ Bp, int N, int nks,
+                                                           int steps_per_split, Epilogue epi, u64 *__restrict__ part) {
+    __shared__ __attribute__((aligned(16))) int8_t As[8 * LDS_PLANE];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[8 * LDS_PLANE];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -147,9 +282,7 @@ __global__ __launch_bounds__(256) void limb_gemm_kernel(LimbA a, const int8_t *_
     const int M = a.M;
     const int ks_begin = blockIdx.z * steps_per_split;
     const int ks_end = min(nks, ks_begin + steps_per_split);
-    const int kps = a.Kp / LIMB_BK;
 
-    // A-loader coordinates: each thread stages 8 consecutive k of one row.
     int arow, akq;
     if (AMODE == 1) {
         arow = tid & 63;
@@ -158,124 +291,72 @@ __global__ __launch_bounds__(256) void limb_gemm_kernel(LimbA a, const int8_t *_
         arow = tid >> 2;
         akq = tid & 3;
     }
-    const int am = m0 + arow;
-    // conv: decode the output pixel of this row once
-    int cb = 0, coh = 0, cow = 0;
-    if (AMODE == 2 && am < M) {
-        int ohw = a.OH * a.OW;
-        cb = am / ohw;
-        int rem = am - cb * ohw;
-        coh = rem / a.OW;
-        cow = rem - coh * a.OW;
-    }
+    ALoader<AMODE> L(a, m0 + arow, akq);
 
+    const int4 *bblk = (const int4 *)(Bp + (size_t)nt * nks * (8 * LIMB_BN * LIMB_BK)) + tid;
+    u64 w[8];
+    int4 b0, b1, b2, b3;
+    if (ks_begin < ks_end) {
+        L.init(ks_begin);
+        L.load(w);
+        const int4 *bp = bblk + (size_t)ks_begin * 1024;
+        b0 = bp[0];
+        b1 = bp[256];
+        b2 = bp[512];
+        b3 = bp[768];
+        stage_a(As, arow, akq, w);
+        stage_b(Bs, tid, b0, b1, b2, b3);
+    }
+    __syncthreads();
+
+    const int a_off = (wm * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 16;
+    const int b_off = (wn * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 16;
     v16i acc[8];
 #pragma unroll
-    for (int s = 0; s < 8; s++) acc[s] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    u64 c64[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) c64[r] = 0;
-
-    int since_flush = 0;
+    for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int ks = ks_begin; ks < ks_end; ++ks) {
-        // ---- global -> registers
-        const int seg = ks / kps;
-        const int kbase = (ks - seg * kps) * LIMB_BK + akq * 8;
-        const u64 *__restrict__ src = a.seg[seg];
-        u64 w[8];
-        if (AMODE == 2) {
-            const int fwc = a.fw * a.Cin;
-            if ((a.Cin & 7) == 0 && kbase < a.K && am < M) {
-                int rr = kbase / fwc;
-                int rem = kbase - rr * fwc;
-                int ss = rem / a.Cin;
-                int c = rem - ss * a.Cin;
-                int ih = coh * a.stride - a.ph + rr, iw = cow * a.stride - a.pw + ss;
-                if (ih >= 0 && ih < a.H && iw >= 0 && iw < a.W) {
-                    const u64 *p = src + (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + c;
-#pragma unroll
-                    for (int j = 0; j < 8; j++) w[j] = recode_digits(p[j]);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 8; j++) w[j] = recode_digits(0);
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    int k = kbase + j;
-                    u64 v = 0;
-                    if (k < a.K && am < M) {
-                        int rr = k / fwc;
-                        int rem = k - rr * fwc;
-                        int ss = rem / a.Cin;
-                        int c = rem - ss * a.Cin;
-                        int ih = coh * a.stride - a.ph + rr, iw = cow * a.stride - a.pw + ss;
-                        if (ih >= 0 && ih < a.H && iw >= 0 && iw < a.W)
-                            v = src[(((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + c];
-                    }
-                    w[j] = recode_digits(v);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                int k = kbase + j;
-                u64 v = 0;
-                if (k < a.K && am < M) v = AMODE == 0 ? src[(size_t)am * a.lda + k] : src[(size_t)k * a.lda + am];
-                w[j] = recode_digits(v);
-            }
-        }
-        u64 limbs[8];
-        transpose8x8(w, limbs);
-        const int4 *bsrc = (const int4 *)(Bp + ((size_t)nt * nks + ks) * (8 * LIMB_BN * LIMB_BK));
-        int4 bq[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) bq[c] = bsrc[tid + c * 256];
+        // prefetch the next K-step into registers (the last step re-reads itself)
+        const bool more = ks + 1 < ks_end;
+        if (more) L.advance();
+        L.load(w);
+        const int4 *bp = bblk + (size_t)(more ? ks + 1 : ks) * 1024;
+        b0 = bp[0];
+        b1 = bp[256];
+        b2 = bp[512];
+        b3 = bp[768];
 
-        // ---- registers -> LDS
-        __syncthreads();
+        v4i bv[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) *(u64 *)&As[i][arow][akq * 8] = limbs[i];
+        for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(Bs + j * LDS_PLANE + b_off);
+        v4i av = *(const v4i *)(As + a_off);
+        // acc[i+j] += A_i . B_j for i + j < 8, A planes streamed one at a time
 #pragma unroll
-        for (int c = 0; c < 4; c++) ((int4 *)&Bs[0][0][0])[tid + c * 256] = bq[c];
-        __syncthreads();
-
-        // ---- MFMA: acc[i+j] += A_i . B_j for i + j < 8
-        v4i av[8], bv[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) av[i] = *(const v4i *)&As[i][wm * 32 + (lane & 31)][(lane >> 5) * 16];
-#pragma unroll
-        for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)&Bs[j][wn * 32 + (lane & 31)][(lane >> 5) * 16];
-#pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < 8; i++) {
+            v4i an = av;
+            if (i < 7) an = *(const v4i *)(As + (i + 1) * LDS_PLANE + a_off);
 #pragma unroll
             for (int j = 0; j < 8 - i; j++)
-                acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[i], bv[j], acc[i + j], 0, 0, 0);
-
-        if (++since_flush == LIMB_FLUSH_STEPS) {
-            since_flush = 0;
-#pragma unroll
-            for (int s = 0; s < 8; s++) {
-#pragma unroll
-                for (int r = 0; r < 16; r++) c64[r] += ((u64)(s64)acc[s][r]) << (8 * s);
-                acc[s] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            }
+                acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv[j], acc[i + j], 0, 0, 0);
+            av = an;
         }
+        __syncthreads();  // every wave has consumed this step's LDS operands
+        stage_a(As, arow, akq, w);
+        stage_b(Bs, tid, b0, b1, b2, b3);
+        __syncthreads();
     }
-#pragma unroll
-    for (int s = 0; s < 8; s++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) c64[r] += ((u64)(s64)acc[s][r]) << (8 * s);
 
-    // ---- epilogue: 32x32 MFMA output layout, lane -> column, register -> row
+    // ---- epilogue: C = sum_s acc_s << 8s; 32x32 MFMA layout, lane -> column, register -> row
     const int n = nt * LIMB_BN + wn * 32 + (lane & 31);
     if (n >= N) return;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         if (m >= M) continue;
-        if (part) part[((size_t)blockIdx.z * M + m) * N + n] = c64[r];
-        else epilogue_store(epi, M, N, m, n, c64[r]);
+        u64 raw = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) raw += ((u64)(s64)acc[q][r]) << (8 * q);
+        if (part) part[((size_t)blockIdx.z * M + m) * N + n] = raw;
+        else epilogue_store(epi, M, N, m, n, raw);
     }
 }
 
@@ -305,17 +386,16 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
                 "limb_gemm: truncation bits must be in [1, 62]");
     if (a.M == 0 || N == 0) return OK;
     const int nks = a.nseg * a.Kp / LIMB_BK;
-    if (split < 1) split = 1;
-    if (split > nks) split = nks;
+    split = limb_effective_split(nks, split);
     const int per = (nks + split - 1) / split;
-    split = (nks + per - 1) / per;
     PIR_REQUIRE(split == 1 || workspace, "limb_gemm: split-K needs a workspace");
     dim3 grid((a.M + LIMB_BM - 1) / LIMB_BM, (N + LIMB_BN - 1) / LIMB_BN, split);
     u64 *part = split > 1 ? workspace : nullptr;
-    switch (a.conv ? 2 : (a.row_major ? 0 : 1)) {
+    switch (a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1)) {
         case 0: hipLaunchKernelGGL(limb_gemm_kernel<0>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
         case 1: hipLaunchKernelGGL(limb_gemm_kernel<1>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
-        default: hipLaunchKernelGGL(limb_gemm_kernel<2>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
+        case 2: hipLaunchKernelGGL(limb_gemm_kernel<2>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
+        default: hipLaunchKernelGGL(limb_gemm_kernel<3>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
     }
     PIR_LAUNCH_CHECK();
     if (split > 1) PIR_TRY(rss_trunc_epilogue(workspace, a.M, N, split, epi, s));

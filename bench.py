@@ -12,7 +12,9 @@ Also reported on the same line:
   * ring_gemm: the 3-term RSS local product A0B0+A0B1+A1B0 at M=N=K in
     {512..8192}, Gop/s = 6MNK/t, plain VALU vs int8-limb MFMA;
   * other_configs: SecureML MLP, LeNet and the uneven-batch AlexNet workload;
-  * roofline of the dominant kernel (the limb implicit-GEMM convolution);
+  * roofline of the dominant kernel (the limb implicit-GEMM convolution),
+    timed per launch with HIP events; its HBM traffic per launch comes from
+    the committed rocprofv3 counter summary under profiles/;
   * cpu_baseline: the CPU oracle on VGG16 at a small batch, host cores.
 """
 import argparse
@@ -34,13 +36,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def conv_limb_ops(layer, batch):
-    """Algorithmic int8 ops of one fused 3-party limb convolution launch:
-    2 ops x 36 limb pairs x M x N x (6 K-segments x K)."""
-    _, H, W, Cin, Cout, f, s, p = layer
-    OH, OW = (H + 2 * p - f) // s + 1, (W + 2 * p - f) // s + 1
-    M, N, K = batch * OH * OW, Cout, f * f * Cin
-    return 2 * 36 * M * N * 6 * K
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "limb_gemm_conv_vgg16_b64.md")
+
+
+def read_traffic(variant):
+    """HBM bytes per launch of limb_gemm_kernel<variant> in this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a line
+    `traffic_bytes_per_launch[<v>] = N`), or None when it is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                key = f"traffic_bytes_per_launch[<{variant}>] ="
+                if line.strip().startswith(key):
+                    return float(line.split("=", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(P, n):
+    """One x += y launch of the element-op kernel over n u64 (reads 16n bytes,
+    writes 8n): the known-size kernel that scales the HBM byte counters."""
+    L = P.lib()
+    x, y = P.Buffer((n,), np.uint64), P.Buffer((n,), np.uint64)
+    P.synchronize()
+    P.check(L.pir_dd_op(P.OP_ADD, 8, x.ptr, 1, y.ptr, 1, 0, n, None))
+    P.synchronize()
+    log(f"copy calibration: multi_op_kernel<unsigned long> over {n} u64, reads {16 * n} B, writes {8 * n} B")
+    del x, y
 
 
 def time_graph(P, net, steps, warmup):
@@ -151,7 +174,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--gemm-sizes", default="512,1024,2048,4096,8192")
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-batch", type=int, default=4)
+    ap.add_argument("--cpu-batch", type=int, default=32)
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one x += y element-op launch over this many u64 (counter calibration)")
     ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
     args = ap.parse_args()
 
@@ -163,6 +188,9 @@ def main():
     import piranha_amd as P
     from piranha_amd import configs as C
 
+    if args.copy_calib:
+        copy_calibration(P, args.copy_calib)
+
     layers, batch, fp = C.VGG16_CIFAR, args.batch, 26
     log(f"VGG16-CIFAR b{batch}: building (fused limb-MFMA, hipGraph)")
     net = make_net(P, C, layers, batch, fp, P.MODE_FUSED)
@@ -170,21 +198,25 @@ def main():
     images_s = batch / t
     log(f"VGG16 fused: {t * 1e3:.3f} ms/forward, {images_s:.1f} images/s")
 
-    # dominant kernel: limb implicit-GEMM convolution, timed per layer with HIP events
-    ms = net.profile_layers()
+    # dominant kernel: the limb implicit-GEMM convolution (Cin % 32 == 0 form),
+    # every launch bracketed by HIP events on the network's stream
+    prof = net.profile_gemm()
     for _ in range(2):
-        ms = np.minimum(ms, net.profile_layers())
-    conv_idx = [i for i, l in enumerate(layers) if l[0] == "cnn"]
-    conv_ms = float(sum(ms[i] for i in conv_idx))
-    conv_ops = float(sum(conv_limb_ops(layers[i], batch) for i in conv_idx))
-    avg_ms = conv_ms / len(conv_idx)
-    achieved = conv_ops / len(conv_idx) / (avg_ms * 1e-3) / 1e12
-    share = conv_ms / float(ms.sum())
-    log("per-layer ms: " + ", ".join(f"{layers[i][0]}{i}:{m:.3f}" for i, m in enumerate(ms)))
-    roofline = {"bound": "mfma", "kernel": "limb_gemm_kernel<2> (3-party implicit-GEMM conv, int8 limbs)",
+        again = net.profile_gemm()
+        prof = {v: (min(prof[v][0], again[v][0]),) + prof[v][1:] for v in prof}
+    layer_ms = net.profile_layers()
+    dom = max(prof, key=lambda v: prof[v][0])
+    k_ms, k_ops, k_n = prof[dom]
+    avg_ms = k_ms / k_n
+    achieved = k_ops / k_n / (avg_ms * 1e-3) / 1e12
+    log("limb GEMM kernels: " + ", ".join(f"<{v}> {prof[v][2]} launches {prof[v][0]:.3f} ms "
+                                          f"{prof[v][1] / prof[v][0] / 1e9:.0f} TOP/s" for v in sorted(prof)))
+    log("per-layer ms: " + ", ".join(f"{layers[i][0]}{i}:{m:.3f}" for i, m in enumerate(layer_ms)))
+    roofline = {"bound": "mfma", "kernel": f"limb_gemm_kernel<{dom}> (3-party implicit-GEMM conv, int8 limbs)",
                 "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
-                "traffic": None, "avg_launch_ms": avg_ms, "launches_per_forward": len(conv_idx),
-                "share_of_forward": share}
+                "traffic": read_traffic(dom), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
+                "ops_per_launch": k_ops / k_n,
+                "share_of_forward": k_ms / float(layer_ms.sum())}
     del net
 
     result = {

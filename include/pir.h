@@ -160,6 +160,12 @@ int pir_net_replay(pir_net *net);
 /* one eager forward with a HIP event after each layer on the network's stream;
  * ms[i] = device time of layer i (count = number of layers) */
 int pir_net_profile_layers(pir_net *net, float *ms, int count);
+/* one eager forward with HIP events around every limb GEMM kernel launch on
+ * the network's stream; per A-operand variant v (PIR_LIMB_* below): ms[v] =
+ * summed kernel time, ops[v] = summed algorithmic int8 ops (2*36*M*N*nseg*K),
+ * launches[v] = launch count.  Arrays hold PIR_LIMB_VARIANTS entries. */
+enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_C32, PIR_LIMB_VARIANTS };
+int pir_net_profile_gemm(pir_net *net, double *ms, double *ops, int *launches);
 int pir_net_set_mode(pir_net *net, int mode);
 
 #ifdef __cplusplus

@@ -86,6 +86,7 @@ _SIGS = {
     "pir_net_replay": ([_vp], _i),
     "pir_net_set_mode": ([_vp, _i], _i),
     "pir_net_profile_layers": ([_vp, _vp, _i], _i),
+    "pir_net_profile_gemm": ([_vp, _vp, _vp, _vp], _i),
 }
 
 _lib = None
@@ -234,6 +235,17 @@ class Network:
         ms = np.zeros(self.num_layers, np.float32)
         check(lib().pir_net_profile_layers(self.h, ms.ctypes.data, ms.size))
         return ms
+
+    def profile_gemm(self):
+        """One eager forward with HIP events around every limb GEMM launch.
+
+        Returns {variant: (kernel_ms, algorithmic_int8_ops, launches)} for the
+        variants launched (0 dense row-major A, 1 dense column-major A, 2 conv,
+        3 conv with Cin % 32 == 0)."""
+        ms, ops = np.zeros(4, np.float64), np.zeros(4, np.float64)
+        n = np.zeros(4, np.int32)
+        check(lib().pir_net_profile_gemm(self.h, ms.ctypes.data, ops.ctypes.data, n.ctypes.data))
+        return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(4) if n[v]}
 
     def _read(self, fn, layer, dtype):
         p, n = _vp(), _sz()

@@ -665,6 +665,41 @@ int pir_net_profile_layers(pir_net *n, float *ms, int count) {
     });
 }
 
+int pir_net_profile_gemm(pir_net *n, double *ms, double *ops, int *launches) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && ms && ops && launches, "pir_net_profile_gemm: invalid arguments");
+        NetScope ns(n);
+        GemmTimer &t = n->ctx.timer;
+        t.recs.clear();
+        t.on = true;
+        int rc = guarded([&]() -> int {
+            n->net->run();
+            return (int)OK;
+        });
+        t.on = false;
+        hipError_t e = hipStreamSynchronize(n->ctx.stream);
+        for (int v = 0; v < PIR_LIMB_VARIANTS; v++) {
+            ms[v] = 0;
+            ops[v] = 0;
+            launches[v] = 0;
+        }
+        for (auto &r : t.recs) {
+            float dt = 0;
+            if (rc == OK && e == hipSuccess && hipEventElapsedTime(&dt, r.start, r.stop) == hipSuccess) {
+                ms[r.variant] += dt;
+                ops[r.variant] += r.ops;
+                launches[r.variant]++;
+            }
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
+        }
+        t.recs.clear();
+        if (rc) return rc;
+        PIR_HIP(e);
+        return (int)OK;
+    });
+}
+
 int pir_net_replay(pir_net *n) {
     PIR_REQUIRE(n && n->exec, "pir_net_replay: no captured graph");
     PIR_HIP(hipGraphLaunch(n->exec, n->ctx.stream));

@@ -6,6 +6,7 @@
 // exact CDNA4 forms (see ring_gemm.h): a 64-bit VALU kernel and the signed-digit
 // int8-limb MFMA kernel.
 #include "ring_gemm.h"
+#include "runtime.h"
 
 namespace pir {
 
@@ -391,13 +392,25 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     PIR_REQUIRE(split == 1 || workspace, "limb_gemm: split-K needs a workspace");
     dim3 grid((a.M + LIMB_BM - 1) / LIMB_BM, (N + LIMB_BN - 1) / LIMB_BN, split);
     u64 *part = split > 1 ? workspace : nullptr;
-    switch (a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1)) {
+    const int variant = a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1);
+    GemmTimer &timer = ctx().timer;
+    GemmTimer::Rec rec{nullptr, nullptr, 2.0 * 36 * a.M * (double)N * a.nseg * a.K, variant};
+    if (timer.on) {
+        PIR_HIP(hipEventCreate(&rec.start));
+        PIR_HIP(hipEventCreate(&rec.stop));
+        PIR_HIP(hipEventRecord(rec.start, s));
+    }
+    switch (variant) {
         case 0: hipLaunchKernelGGL(limb_gemm_kernel<0>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
         case 1: hipLaunchKernelGGL(limb_gemm_kernel<1>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
         case 2: hipLaunchKernelGGL(limb_gemm_kernel<2>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
         default: hipLaunchKernelGGL(limb_gemm_kernel<3>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
     }
     PIR_LAUNCH_CHECK();
+    if (timer.on) {
+        PIR_HIP(hipEventRecord(rec.stop, s));
+        timer.recs.push_back(rec);
+    }
     if (split > 1) PIR_TRY(rss_trunc_epilogue(workspace, a.M, N, split, epi, s));
     return OK;
 }

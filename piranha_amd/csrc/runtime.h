@@ -33,8 +33,21 @@ class Allocator {
     size_t reserved_ = 0;
 };
 
+// Per-launch HIP events around the limb GEMM kernel, recorded only while `on`
+// (an eager profiling forward; never during graph capture).
+struct GemmTimer {
+    struct Rec {
+        hipEvent_t start, stop;
+        double ops;   // algorithmic int8 ops of the launch: 2 * 36 * M * N * nseg * K
+        int variant;  // A-operand form (0 dense row-major, 1 dense col-major, 2 conv, 3 conv Cin%32==0)
+    };
+    bool on = false;
+    std::vector<Rec> recs;
+};
+
 struct Context {
     hipStream_t stream = nullptr;
+    GemmTimer timer;
     Allocator alloc;
     int mode = MODE_FUSED;
     int algo = ALGO_LIMB;

@@ -63,7 +63,7 @@ int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint
 
 /* The limb form with explicit operands, no counterpart in the reference (it is
  * how the CUTLASS gemm above is re-expressed): C = sum_s A_s . B_s over nseg
- * K-segments, B_s(k,n) = b0[s] + (b1 ? b1[s] : 0) packed once (weights) by
+ * (1..12) K-segments, B_s(k,n) = b0[s] + (b1 ? b1[s] : 0) packed once (weights) by
  * pir_limb_pack into pir_limb_packed_bytes() bytes.  A_s is M x K, row-major if
  * a_row_major; C is row-major if tc.  split <= 0 picks a split-K factor. */
 size_t pir_limb_packed_bytes(int K, int N, int nseg);
@@ -120,6 +120,31 @@ int pir_rss_select_share(const uint64_t *x, const uint64_t *y, const void *b, ui
                          void *stream);                                                    /* RSS.inl:542-569 */
 /* maxpool over gathered pools [n/k windows][k] (modified in place like the reference), RSS.inl:867-1017 */
 int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
+
+/* ---- TPC (2 parties) and FPC (4 parties), src/mpc/TPC.inl, FPC.inl ------- */
+/* TPC value: [2 parties][n] (one share each).  FPC value: [4 parties][3 shares][n],
+ * party q's share s holding component (q + s + 1) % 4.  Placeholder precompute
+ * as in the reference (Precompute.h).  pir_set_mode selects the fused form
+ * (one limb product per party with the truncation in its epilogue) or the
+ * reference's op-by-op form; algo picks the GEMM of the op-by-op form. */
+int pir_tpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream); /* TPC.inl:551-572, out [2][n] */
+int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream);       /* TPC.inl:370-384 */
+/* matmul, TPC.inl:574-583 + localMatMul :1059-1090 (transpose flags as in pir_gemm) */
+int pir_tpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
+                   int trunc, int algo, void *stream);
+/* convolution(kFprop), TPC.inl:644-690, 788-826 */
+int pir_tpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
+                        int Dout, int stride, int pad, int trunc, int algo, void *stream);
+int pir_fpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream); /* FPC.inl:582-604, out [4][n] */
+int pir_fpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream);       /* FPC.inl:566-580 */
+/* reshareFPC, FPC.inl:1183-1260: z, zp [4][n] per party, added into out [4][3][n]; 0 <= i < j < 4 */
+int pir_fpc_reshare(const uint64_t *z, const uint64_t *zp, size_t n, int i, int j, uint64_t *out, void *stream);
+/* matmul, FPC.inl:635-666 */
+int pir_fpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
+                   int trunc, int algo, void *stream);
+/* convolution(kFprop), FPC.inl:744-782, 861-927 */
+int pir_fpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
+                        int Dout, int stride, int pad, int trunc, int algo, void *stream);
 
 /* ---- layers and network (src/nn/FCLayer.cu etc., NeuralNetwork.cu) -------------- */
 typedef struct pir_net pir_net;

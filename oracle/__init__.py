@@ -7,7 +7,9 @@ a literal per-party C++ restatement of the reference's RSS protocol
 path is checked against -- never the thing measured or shipped.
 
 Array conventions: an RSS value for all three parties is a numpy array of
-shape (3, 2, n) (party, share, element); a party-local value is (3, n).
+shape (3, 2, n) (party, share, element); a party-local value is (3, n).  A
+TPC value (two parties, src/mpc/TPC.inl) is (2, n); an FPC value (four
+parties, three shares each, src/mpc/FPC.inl) is (4, 3, n).
 """
 import ctypes
 import os
@@ -72,6 +74,17 @@ _SIGS = {
     "ref_cnn_forward": [_u64p, _u64p, _u64p] + [_i] * 9,
     "ref_relu_forward": [_u64p, _sz, _u64p, _u8p],
     "ref_maxpool_forward": [_u64p, _u64p, _u8p] + [_i] * 7,
+    "ref_tpc_init_list": [_u64p, _sz, _dp, _i, _i],
+    "ref_tpc_reconstruct": [_u64p, _sz, _u64p],
+    "ref_tpc_divide_public": [_u64p, _sz, _u64],
+    "ref_tpc_matmul": [_u64p, _sz, _u64p, _sz, _u64p] + [_i] * 7,
+    "ref_tpc_convolution": [_u64p, _sz, _u64p, _sz, _u64p, _sz] + [_i] * 9,
+    "ref_fpc_init_list": [_u64p, _sz, _dp, _i, _i],
+    "ref_fpc_reconstruct": [_u64p, _sz, _u64p],
+    "ref_fpc_divide_public": [_u64p, _sz, _u64],
+    "ref_fpc_reshare": [_u64p, _u64p, _sz, _i, _i, _u64p],
+    "ref_fpc_matmul": [_u64p, _sz, _u64p, _sz, _u64p] + [_i] * 7,
+    "ref_fpc_convolution": [_u64p, _sz, _u64p, _sz, _u64p, _sz] + [_i] * 9,
 }
 _RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_scalar_divide": _u64, "ref_version": _i}
 
@@ -430,3 +443,99 @@ def net_forward(layers, batch, fp, params, x_shares):
         else:
             raise ValueError(kind)
     return out
+
+
+# --- TPC (2 parties x 1 share) and FPC (4 parties x 3 shares) ----------------
+
+_NSH = {"tpc": (2,), "fpc": (4, 3)}
+
+
+def _scheme_init_list(kind, vals, fp, convert):
+    vals = np.ascontiguousarray(vals, np.float64).ravel()
+    x = np.zeros(_NSH[kind] + (vals.size,), np.uint64)
+    getattr(lib(), f"ref_{kind}_init_list")(_p(x), vals.size, _p(vals), int(convert), fp)
+    return x
+
+
+def tpc_init_list(vals, fp, convert=True):
+    """TPC initializer-list constructor (TPC.inl:341-363): party A holds the values."""
+    return _scheme_init_list("tpc", vals, fp, convert)
+
+
+def fpc_init_list(vals, fp, convert=True):
+    """FPC initializer-list constructor (FPC.inl:527-561)."""
+    return _scheme_init_list("fpc", vals, fp, convert)
+
+
+def tpc_reconstruct(x):
+    x = _u(x)
+    out = np.zeros((2, x.shape[-1]), np.uint64)
+    lib().ref_tpc_reconstruct(_p(x), x.shape[-1], _p(out))
+    return out
+
+
+def fpc_reconstruct(x):
+    x = _u(x)
+    out = np.zeros((4, x.shape[-1]), np.uint64)
+    lib().ref_fpc_reconstruct(_p(x), x.shape[-1], _p(out))
+    return out
+
+
+def tpc_divide_public(a, d):
+    a = _u(a).copy()
+    lib().ref_tpc_divide_public(_p(a), a.shape[-1], int(d))
+    return a
+
+
+def fpc_divide_public(a, d):
+    a = _u(a).copy()
+    lib().ref_fpc_divide_public(_p(a), a.shape[-1], int(d))
+    return a
+
+
+def fpc_reshare(z, zp, i, j, out=None):
+    """reshareFPC (FPC.inl:1183-1260) of per-party z, z' (4, n), added into out."""
+    z, zp = _u(z), _u(zp)
+    n = z.shape[-1]
+    out = np.zeros((4, 3, n), np.uint64) if out is None else _u(out).copy()
+    lib().ref_fpc_reshare(_p(z), _p(zp), n, i, j, _p(out))
+    return out
+
+
+def _scheme_matmul(kind, a, b, M, N, K, ta, tb, tc, trunc):
+    a, b = _u(a), _u(b)
+    c = np.zeros(_NSH[kind] + (M * N,), np.uint64)
+    getattr(lib(), f"ref_{kind}_matmul")(_p(a), a.shape[-1], _p(b), b.shape[-1], _p(c), M, N, K, int(ta), int(tb),
+                                         int(tc), trunc)
+    return c
+
+
+def tpc_matmul(a, b, M, N, K, ta, tb, tc, trunc):
+    """matmul (TPC.inl:574-583, localMatMul :1059-1090)."""
+    return _scheme_matmul("tpc", a, b, M, N, K, ta, tb, tc, trunc)
+
+
+def fpc_matmul(a, b, M, N, K, ta, tb, tc, trunc):
+    """matmul (FPC.inl:635-666)."""
+    return _scheme_matmul("fpc", a, b, M, N, K, ta, tb, tc, trunc)
+
+
+def _scheme_conv(kind, A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
+    A, B = _u(A), _u(B)
+    OH = (H + 2 * pad - f) // stride + 1
+    OW = (W + 2 * pad - f) // stride + 1
+    n = batch * OH * OW * Dout
+    C = np.zeros(_NSH[kind] + (n,), np.uint64)
+    getattr(lib(), f"ref_{kind}_convolution")(_p(A), A.shape[-1], _p(B), B.shape[-1], _p(C), n, batch, H, W, f, Din,
+                                              Dout, stride, pad, trunc)
+    return C
+
+
+def tpc_convolution(A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
+    """convolution(kFprop) (TPC.inl:644-690, 788-826)."""
+    return _scheme_conv("tpc", A, B, batch, H, W, f, Din, Dout, stride, pad, trunc)
+
+
+def fpc_convolution(A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
+    """convolution(kFprop) (FPC.inl:744-782, 861-927)."""
+    return _scheme_conv("fpc", A, B, batch, H, W, f, Din, Dout, stride, pad, trunc)

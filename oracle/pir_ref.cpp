@@ -732,6 +732,321 @@ void maxpool(Rs<u64> input, Rs<u64> result, Rs<U> dresult, int k) {
     round(true);
 }
 
+// ---------------------------------------------------------------------------
+// TPC: two parties, one share each (src/mpc/TPC.h, TPC.inl).  A value is
+// [2 parties][n]; party A = 0, party B = 1.  Precompute placeholders:
+// getMatrixBeaverTriple x = y = 1, z = K (Precompute.h:73-91);
+// getConvBeaverTriple_fprop x = y = z = 0 (:16-33); getDividedShares r = 1,
+// r' = d (:107-119).  TPC::fill puts the value on party A only (TPC.inl:103-106).
+
+enum { TA = 0, TB = 1 };
+
+struct TpBuf {
+    std::vector<u64> d[2];
+    explicit TpBuf(size_t n) {
+        for (auto &v : d) v.assign(n, 0);
+    }
+    size_t size() const { return d[0].size(); }
+};
+
+struct Tp {
+    u64 *p[2];
+    size_t n;
+};
+
+inline Tp tv(TpBuf &b) { return Tp{{b.d[0].data(), b.d[1].data()}, b.size()}; }
+inline Tp wrapTp(u64 *a, size_t n) { return Tp{{a, a + n}, n}; }
+
+// TPCBase::fill, TPC.inl:103-106
+void t_fill(Tp x, u64 v) {
+    for (size_t k = 0; k < x.n; k++) {
+        x.p[TA][k] = v;
+        x.p[TB][k] = 0;
+    }
+}
+// share-wise TPC += / -= TPC (TPC.inl:228-242)
+void t_op_t(Tp x, Tp y, Op op) {
+    for (int q = 0; q < 2; q++)
+        for (size_t k = 0; k < x.n; k++) x.p[q][k] = apply_op<u64>(op, x.p[q][k], y.p[q][k]);
+}
+// TPC += / -= DeviceData: party A only (TPC.inl:173-189).  `y` is per party.
+void t_op_l(Tp x, u64 *const y[2], Op op) {
+    for (size_t k = 0; k < x.n; k++) x.p[TA][k] = apply_op<u64>(op, x.p[TA][k], y[TA][k]);
+}
+// reconstruct, TPC.inl:551-572: own share + the other party's share.
+void t_reconstruct(Tp in, u64 *const out[2]) {
+    for (int q = 0; q < 2; q++)
+        for (size_t k = 0; k < in.n; k++) out[q][k] = in.p[q][k] + in.p[1 - q][k];
+}
+// dividePublic(TPC, T), TPC.inl:370-384
+void t_divide_public(Tp a, u64 d) {
+    size_t n = a.n;
+    TpBuf r(n), rPrime(n), rec(n);
+    t_fill(tv(rPrime), d);
+    t_fill(tv(r), 1);
+    t_op_t(a, tv(rPrime), SUB);
+    u64 *rp[2] = {rec.d[0].data(), rec.d[1].data()};
+    t_reconstruct(a, rp);
+    for (int q = 0; q < 2; q++)
+        for (size_t k = 0; k < n; k++) rp[q][k] = sdiv<u64>(rp[q][k], d);
+    for (int q = 0; q < 2; q++) std::fill(a.p[q], a.p[q] + n, (u64)0);
+    t_op_t(a, tv(r), ADD);
+    t_op_l(a, rp, ADD);
+}
+
+// localMatMul, TPC.inl:1059-1090 (Beaver product on opened e = x + a, f = y + b)
+void t_local_matmul(Tp a, Tp b, Tp c, int M, int N, int K, bool ta, bool tb, bool tc) {
+    TpBuf x(a.n), y(b.n), z(c.n);
+    t_fill(tv(x), 1);
+    t_fill(tv(y), 1);
+    t_fill(tv(z), (u64)K);  // z.fill(shared), shared = K for every transpose combination
+    TpBuf e(a.n), f(b.n), temp(c.n);
+    t_op_t(tv(x), a, ADD);
+    t_op_t(tv(y), b, ADD);
+    u64 *ep[2] = {e.d[0].data(), e.d[1].data()}, *fp[2] = {f.d[0].data(), f.d[1].data()};
+    u64 *tp[2] = {temp.d[0].data(), temp.d[1].data()};
+    t_reconstruct(tv(x), ep);
+    t_reconstruct(tv(y), fp);
+    t_op_t(tv(x), a, SUB);
+    t_op_t(tv(y), b, SUB);
+    for (int q = 0; q < 2; q++) std::fill(c.p[q], c.p[q] + c.n, (u64)0);
+    t_op_t(c, tv(z), ADD);
+    for (int q = 0; q < 2; q++) gemm(M, N, K, ep[q], ta, fp[q], tb, tp[q], tc);
+    t_op_l(c, tp, ADD);
+    for (int q = 0; q < 2; q++) gemm(M, N, K, ep[q], ta, y.d[q].data(), tb, tp[q], tc);
+    t_op_l(c, tp, SUB);
+    for (int q = 0; q < 2; q++) gemm(M, N, K, x.d[q].data(), ta, fp[q], tb, tp[q], tc);
+    t_op_l(c, tp, SUB);
+}
+
+// matmul, TPC.inl:574-583
+void t_matmul(Tp a, Tp b, Tp c, int M, int N, int K, bool ta, bool tb, bool tc, int trunc) {
+    t_local_matmul(a, b, c, M, N, K, ta, tb, tc);
+    t_divide_public(c, (u64)1 << trunc);
+}
+
+// localFprop, TPC.inl:644-690, and convolution(kFprop), TPC.inl:788-826
+void t_convolution(Tp A, Tp B, Tp C, int batch, int H, int W, int f, int Din, int Dout, int stride, int pad,
+                   int trunc) {
+    for (int q = 0; q < 2; q++) std::fill(C.p[q], C.p[q] + C.n, (u64)0);  // C.zero()
+    TpBuf x(A.n), y(B.n), z(C.n);                                          // triple: all zero
+    TpBuf e(A.n), fo(B.n), temp(C.n);
+    t_op_t(tv(x), A, ADD);
+    t_op_t(tv(y), B, ADD);
+    u64 *ep[2] = {e.d[0].data(), e.d[1].data()}, *fp[2] = {fo.d[0].data(), fo.d[1].data()};
+    u64 *tp[2] = {temp.d[0].data(), temp.d[1].data()};
+    t_reconstruct(tv(x), ep);
+    t_reconstruct(tv(y), fp);
+    t_op_t(tv(x), A, SUB);
+    t_op_t(tv(y), B, SUB);
+    for (int q = 0; q < 2; q++) std::fill(C.p[q], C.p[q] + C.n, (u64)0);
+    t_op_t(C, tv(z), ADD);
+    for (int q = 0; q < 2; q++) conv_fprop(ep[q], fp[q], tp[q], batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+    t_op_l(C, tp, ADD);
+    // *C.getShare(0) -= temp on every party (TPC.inl:675-687)
+    for (int q = 0; q < 2; q++) {
+        conv_fprop(ep[q], y.d[q].data(), tp[q], batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+        for (size_t k = 0; k < C.n; k++) C.p[q][k] -= tp[q][k];
+    }
+    for (int q = 0; q < 2; q++) {
+        conv_fprop(x.d[q].data(), fp[q], tp[q], batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+        for (size_t k = 0; k < C.n; k++) C.p[q][k] -= tp[q][k];
+    }
+    t_divide_public(C, (u64)1 << trunc);
+}
+
+// ---------------------------------------------------------------------------
+// FPC: four parties, three shares each (src/mpc/FPC.h, FPC.inl).  A value is
+// [4 parties][3 shares][n].  Party q's share s holds component (q + s + 1) % 4
+// of x = x0 + x1 + x2 + x3 (FPC.inl:119-124 fill, :497-506 shareH).
+
+enum { FA = 0, FB = 1, FC = 2, FD = 3 };
+
+struct FpBuf {
+    std::vector<u64> d[4][3];
+    explicit FpBuf(size_t n) {
+        for (auto &pp : d)
+            for (auto &v : pp) v.assign(n, 0);
+    }
+    size_t size() const { return d[0][0].size(); }
+};
+
+struct Fp {
+    u64 *s[4][3];
+    size_t n;
+};
+
+inline Fp fv(FpBuf &b) {
+    Fp r;
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) r.s[q][i] = b.d[q][i].data();
+    r.n = b.size();
+    return r;
+}
+inline Fp wrapFp(u64 *a, size_t n) {
+    Fp r;
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) r.s[q][i] = a + (3 * q + i) * n;
+    r.n = n;
+    return r;
+}
+
+// FPC.inl:400-441
+inline int f_next(int p) { return (p + 1) % 4; }
+inline int f_prev(int p) { return (p + 3) % 4; }
+inline bool f_opposites(int i, int j) { return (i - j) == 2 || (j - i) == 2; }
+// FPC.inl:451-495 (only called with i < j)
+int f_partyG(int i, int j) {
+    if (i == 0 && j == 1) return 2;
+    if (i == 0 && j == 2) return 1;
+    if (i == 0 && j == 3) return 1;
+    if (i == 1 && j == 2) return 3;
+    if (i == 1 && j == 3) return 2;
+    return 0;  // (2, 3)
+}
+int f_partyH(int i, int j) {
+    if (i == 0 && j == 1) return 3;
+    if (i == 0 && j == 2) return 3;
+    if (i == 0 && j == 3) return 2;
+    if (i == 1 && j == 2) return 0;
+    if (i == 1 && j == 3) return 0;
+    return 1;  // (2, 3)
+}
+// FPC.inl:497-506: the share slot of party `p` that holds component H
+inline int f_shareH(int i, int j, int p) { return (3 - p + f_partyH(i, j)) % 4; }
+
+// the slot in which party q keeps a public value (FPC.inl:119-124, 186-208)
+inline int f_public_slot(int q) { return q == FB ? 2 : (q == FC ? 1 : (q == FD ? 0 : -1)); }
+
+void f_zero(Fp x) {
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) std::fill(x.s[q][i], x.s[q][i] + x.n, (u64)0);
+}
+// FPCBase::fill, FPC.inl:119-124
+void f_fill(Fp x, u64 v) {
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) std::fill(x.s[q][i], x.s[q][i] + x.n, i == f_public_slot(q) ? v : (u64)0);
+}
+// setPublic / initializer list, FPC.inl:126-155, 527-561
+void f_init_list(Fp x, const double *vals, bool convert, int fp) {
+    f_zero(x);
+    for (int q = 0; q < 4; q++) {
+        int slot = f_public_slot(q);
+        if (slot < 0) continue;
+        for (size_t k = 0; k < x.n; k++) x.s[q][slot][k] = convert ? to_fixed<u64>(vals[k], fp) : (u64)(s64)vals[k];
+    }
+}
+// share-wise FPC += / -= FPC (FPC.inl:302-318)
+void f_op_f(Fp x, Fp y, Op op) {
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++)
+            for (size_t k = 0; k < x.n; k++) x.s[q][i][k] = apply_op<u64>(op, x.s[q][i][k], y.s[q][i][k]);
+}
+// FPC += DeviceData: B -> shareC, C -> shareB, D -> shareA (FPC.inl:226-250)
+void f_op_l(Fp x, u64 *const y[4], Op op) {
+    for (int q = 0; q < 4; q++) {
+        int slot = f_public_slot(q);
+        if (slot < 0) continue;
+        for (size_t k = 0; k < x.n; k++) x.s[q][slot][k] = apply_op<u64>(op, x.s[q][slot][k], y[q][k]);
+    }
+}
+// reconstruct, FPC.inl:582-604: own three shares + the previous party's share 0
+void f_reconstruct(Fp in, u64 *const out[4]) {
+    for (int q = 0; q < 4; q++)
+        for (size_t k = 0; k < in.n; k++)
+            out[q][k] = in.s[q][0][k] + in.s[q][1][k] + in.s[q][2][k] + in.s[f_prev(q)][0][k];
+}
+// dividePublic(FPC, T), FPC.inl:566-580
+void f_divide_public(Fp a, u64 d) {
+    size_t n = a.n;
+    FpBuf r(n), rPrime(n);
+    f_fill(fv(rPrime), d);
+    f_fill(fv(r), 1);
+    f_op_f(a, fv(rPrime), SUB);
+    std::vector<u64> rec[4];
+    u64 *rp[4];
+    for (int q = 0; q < 4; q++) {
+        rec[q].assign(n, 0);
+        rp[q] = rec[q].data();
+    }
+    f_reconstruct(a, rp);
+    for (int q = 0; q < 4; q++)
+        for (size_t k = 0; k < n; k++) rp[q][k] = sdiv<u64>(rp[q][k], d);
+    f_zero(a);
+    f_op_f(a, fv(r), ADD);
+    f_op_l(a, rp, ADD);
+}
+// reshareFPC, FPC.inl:1183-1260 with zero PRGs: parties i, j and G add their
+// mask into the slot of component H; G receives party i's mask.
+void f_reshare(u64 *const z[4], u64 *const zPrime[4], size_t n, int i, int j, Fp out) {
+    int G = f_partyG(i, j), H = f_partyH(i, j);
+    bool opp = f_opposites(i, j);
+    for (int q = 0; q < 4; q++) {
+        if (q == H) continue;
+        int slot = f_shareH(i, j, q);
+        const u64 *mask = nullptr;
+        if (q == i || q == j) mask = opp ? zPrime[q] : z[q];
+        else if (q == G) mask = opp ? zPrime[i] : z[i];
+        for (size_t k = 0; k < n; k++) out.s[q][slot][k] += mask[k];
+    }
+}
+void f_reshare_all(u64 *const z[4], u64 *const zPrime[4], size_t n, Fp out) {
+    const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    for (auto &pr : pairs) f_reshare(z, zPrime, n, pr[0], pr[1], out);
+}
+
+// matmul, FPC.inl:635-666
+void f_matmul(Fp a, Fp b, Fp c, int M, int N, int K, bool ta, bool tb, bool tc, int trunc) {
+    size_t mn = (size_t)M * N;
+    std::vector<u64> zb[4], zpb[4], temp(mn);
+    u64 *z[4], *zp[4];
+    for (int q = 0; q < 4; q++) {
+        zb[q].assign(mn, 0);
+        zpb[q].assign(mn, 0);
+        z[q] = zb[q].data();
+        zp[q] = zpb[q].data();
+        for (int s = 0; s < 3; s++) gemm(M, N, K, a.s[q][s], ta, b.s[q][s], tb, c.s[q][s], tc);
+        const int zt[2][2] = {{0, 1}, {1, 0}}, zpt[2][2] = {{0, 2}, {2, 0}};
+        for (auto &t : zt) {
+            gemm(M, N, K, a.s[q][t[0]], ta, b.s[q][t[1]], tb, temp.data(), tc);
+            for (size_t k = 0; k < mn; k++) z[q][k] += temp[k];
+        }
+        for (auto &t : zpt) {
+            gemm(M, N, K, a.s[q][t[0]], ta, b.s[q][t[1]], tb, temp.data(), tc);
+            for (size_t k = 0; k < mn; k++) zp[q][k] += temp[k];
+        }
+    }
+    f_reshare_all(z, zp, mn, c);
+    f_divide_public(c, (u64)1 << trunc);
+}
+
+// localFprop, FPC.inl:744-782, and convolution(kFprop), FPC.inl:861-927
+void f_convolution(Fp A, Fp B, Fp C, int batch, int H, int W, int f, int Din, int Dout, int stride, int pad,
+                   int trunc) {
+    size_t outN = C.n;
+    std::vector<u64> zb[4], zpb[4], acc(outN);
+    u64 *z[4], *zp[4];
+    for (int q = 0; q < 4; q++) {
+        zb[q].assign(outN, 0);
+        zpb[q].assign(outN, 0);
+        z[q] = zb[q].data();
+        zp[q] = zpb[q].data();
+        for (int s = 0; s < 3; s++)
+            conv_fprop(A.s[q][s], B.s[q][s], C.s[q][s], batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+        const int zt[2][2] = {{0, 1}, {1, 0}}, zpt[2][2] = {{0, 2}, {2, 0}};
+        for (auto &t : zt) {
+            conv_fprop(A.s[q][t[0]], B.s[q][t[1]], acc.data(), batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+            for (size_t k = 0; k < outN; k++) z[q][k] += acc[k];
+        }
+        for (auto &t : zpt) {
+            conv_fprop(A.s[q][t[0]], B.s[q][t[1]], acc.data(), batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+            for (size_t k = 0; k < outN; k++) zp[q][k] += acc[k];
+        }
+    }
+    f_reshare_all(z, zp, outN, C);
+    f_divide_public(C, (u64)1 << trunc);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -910,6 +1225,57 @@ void ref_maxpool_forward(const u64 *input, u64 *act, uint8_t *max_prime, int bat
     maxpool(pools.v(), A, emp.v(), eps);
     for (int q = 0; q < 3; q++)
         for (int i = 0; i < 2; i++) truncate_cols(emp.v().s[q][i], MP.s[q][i], eps, pool * pool);
+}
+
+
+// ---- TPC ([2][n]) and FPC ([4][3][n]) ----
+void ref_tpc_init_list(u64 *x, size_t n, const double *vals, int convert, int fp) {
+    // TPC.inl:341-363: party A holds the values, party B zeros
+    for (size_t k = 0; k < n; k++) {
+        x[k] = convert ? to_fixed<u64>(vals[k], fp) : (u64)(s64)vals[k];
+        x[n + k] = 0;
+    }
+}
+void ref_tpc_reconstruct(const u64 *in, size_t n, u64 *out) {
+    u64 *o[2] = {out, out + n};
+    t_reconstruct(wrapTp((u64 *)in, n), o);
+}
+void ref_tpc_divide_public(u64 *a, size_t n, u64 d) { t_divide_public(wrapTp(a, n), d); }
+void ref_tpc_matmul(const u64 *a, size_t a_n, const u64 *b, size_t b_n, u64 *c, int M, int N, int K, int ta, int tb,
+                    int tc, int trunc) {
+    t_matmul(wrapTp((u64 *)a, a_n), wrapTp((u64 *)b, b_n), wrapTp(c, (size_t)M * N), M, N, K, ta != 0, tb != 0,
+             tc != 0, trunc);
+}
+void ref_tpc_convolution(const u64 *A, size_t a_n, const u64 *B, size_t b_n, u64 *C, size_t c_n, int batch, int H,
+                         int W, int f, int Din, int Dout, int stride, int pad, int trunc) {
+    t_convolution(wrapTp((u64 *)A, a_n), wrapTp((u64 *)B, b_n), wrapTp(C, c_n), batch, H, W, f, Din, Dout, stride,
+                  pad, trunc);
+}
+void ref_fpc_init_list(u64 *x, size_t n, const double *vals, int convert, int fp) {
+    f_init_list(wrapFp(x, n), vals, convert != 0, fp);
+}
+void ref_fpc_reconstruct(const u64 *in, size_t n, u64 *out) {
+    u64 *o[4] = {out, out + n, out + 2 * n, out + 3 * n};
+    f_reconstruct(wrapFp((u64 *)in, n), o);
+}
+void ref_fpc_divide_public(u64 *a, size_t n, u64 d) { f_divide_public(wrapFp(a, n), d); }
+void ref_fpc_reshare(const u64 *z, const u64 *zp, size_t n, int i, int j, u64 *out) {
+    u64 *zz[4], *zzp[4];
+    for (int q = 0; q < 4; q++) {
+        zz[q] = (u64 *)z + q * n;
+        zzp[q] = (u64 *)zp + q * n;
+    }
+    f_reshare(zz, zzp, n, i, j, wrapFp(out, n));
+}
+void ref_fpc_matmul(const u64 *a, size_t a_n, const u64 *b, size_t b_n, u64 *c, int M, int N, int K, int ta, int tb,
+                    int tc, int trunc) {
+    f_matmul(wrapFp((u64 *)a, a_n), wrapFp((u64 *)b, b_n), wrapFp(c, (size_t)M * N), M, N, K, ta != 0, tb != 0,
+             tc != 0, trunc);
+}
+void ref_fpc_convolution(const u64 *A, size_t a_n, const u64 *B, size_t b_n, u64 *C, size_t c_n, int batch, int H,
+                         int W, int f, int Din, int Dout, int stride, int pad, int trunc) {
+    f_convolution(wrapFp((u64 *)A, a_n), wrapFp((u64 *)B, b_n), wrapFp(C, c_n), batch, H, W, f, Din, Dout, stride,
+                  pad, trunc);
 }
 
 }  // extern "C"

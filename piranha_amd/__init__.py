@@ -87,6 +87,15 @@ _SIGS = {
     "pir_net_set_mode": ([_vp, _i], _i),
     "pir_net_profile_layers": ([_vp, _vp, _i], _i),
     "pir_net_profile_gemm": ([_vp, _vp, _vp, _vp], _i),
+    "pir_tpc_reconstruct": ([_vp, _sz, _vp, _vp], _i),
+    "pir_tpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
+    "pir_tpc_matmul": ([_vp, _vp, _vp] + [_i] * 8 + [_vp], _i),
+    "pir_tpc_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
+    "pir_fpc_reconstruct": ([_vp, _sz, _vp, _vp], _i),
+    "pir_fpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
+    "pir_fpc_reshare": ([_vp, _vp, _sz, _i, _i, _vp, _vp], _i),
+    "pir_fpc_matmul": ([_vp, _vp, _vp] + [_i] * 8 + [_vp], _i),
+    "pir_fpc_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
 }
 
 _lib = None

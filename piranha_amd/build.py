@@ -21,6 +21,7 @@ SOURCES = [
     "elementwise.hip",
     "runtime.cpp",
     "protocol.cpp",
+    "tpc_fpc.cpp",
     "layers.cpp",
     "capi.cpp",
 ]

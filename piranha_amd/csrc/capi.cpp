@@ -8,6 +8,7 @@
 #include "layers.h"
 #include "protocol.h"
 #include "ring_gemm.h"
+#include "tpc_fpc.h"
 
 using namespace pir;
 
@@ -115,11 +116,12 @@ size_t pir_limb_packed_bytes(int K, int N, int nseg) { return limb_b_bytes(N, ns
 
 int pir_limb_pack(const uint64_t *const *b0, const uint64_t *const *b1, int nseg, int K, int N, int row_major,
                   int8_t *packed, void *stream) {
-    PIR_REQUIRE(nseg >= 1 && nseg <= 6 && K > 0 && N > 0, "pir_limb_pack: invalid shape");
+    PIR_REQUIRE(nseg >= 1 && nseg <= LIMB_MAX_SEG && K > 0 && N > 0, "pir_limb_pack: invalid shape");
     LimbBSrc s{};
     for (int i = 0; i < nseg; i++) {
-        s.b0[i] = b0[i];
-        s.b1[i] = b1 ? b1[i] : nullptr;
+        s.b[i][0] = b0[i];
+        s.b[i][1] = b1 ? b1[i] : nullptr;
+        s.coef[i][0] = s.coef[i][1] = 1;
     }
     s.nseg = nseg;
     s.K = K;
@@ -133,7 +135,7 @@ int pir_limb_gemm(const uint64_t *const *a, int nseg, int M, int K, int a_row_ma
                   uint64_t *C, int tc, int split, void *stream) {
     return guarded([&]() -> int {
         Scope sc(stream);
-        PIR_REQUIRE(nseg >= 1 && nseg <= 6 && M >= 0 && K > 0 && N >= 0, "pir_limb_gemm: invalid shape");
+        PIR_REQUIRE(nseg >= 1 && nseg <= LIMB_MAX_SEG && M >= 0 && K > 0 && N >= 0, "pir_limb_gemm: invalid shape");
         LimbA la{};
         for (int i = 0; i < nseg; i++) la.seg[i] = a[i];
         la.nseg = nseg;
@@ -393,6 +395,92 @@ int pir_rss_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
         RSS<u64> B = RSS<u64>::wrap((u64 *)w, (size_t)Dout * f * f * Din);
         RSS<u64> C = RSS<u64>::wrap(c, (size_t)batch * OH * OW * Dout);
         convolution(A, B, C, kFprop, batch, H, W, f, Din, Dout, stride, pad, trunc);
+        return (int)OK;
+    });
+}
+
+// ---- TPC / FPC ----------------------------------------------------------------
+
+int pir_tpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        tpc_reconstruct(x, n, out);
+        return (int)OK;
+    });
+}
+
+int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        tpc_divide_public(a, n, d);
+        return (int)OK;
+    });
+}
+
+int pir_tpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
+                   int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(M > 0 && N > 0 && K > 0 && trunc >= 1 && trunc <= 62, "pir_tpc_matmul: invalid arguments");
+        tpc_matmul(a, b, c, M, N, K, ta != 0, tb != 0, tc != 0, trunc);
+        return (int)OK;
+    });
+}
+
+int pir_tpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
+                        int Dout, int stride, int pad, int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(batch > 0 && H > 0 && W > 0 && f > 0 && Din > 0 && Dout > 0 && stride > 0 && pad >= 0 &&
+                        trunc >= 1 && trunc <= 62 && (H + 2 * pad - f) >= 0 && (W + 2 * pad - f) >= 0,
+                    "pir_tpc_convolution: invalid geometry");
+        tpc_convolution(a, w, c, batch, H, W, f, Din, Dout, stride, pad, trunc);
+        return (int)OK;
+    });
+}
+
+int pir_fpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        fpc_reconstruct(x, n, out);
+        return (int)OK;
+    });
+}
+
+int pir_fpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        fpc_divide_public(a, n, d);
+        return (int)OK;
+    });
+}
+
+int pir_fpc_reshare(const uint64_t *z, const uint64_t *zp, size_t n, int i, int j, uint64_t *out, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        fpc_reshare(z, zp, n, i, j, out);
+        return (int)OK;
+    });
+}
+
+int pir_fpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
+                   int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(M > 0 && N > 0 && K > 0 && trunc >= 1 && trunc <= 62, "pir_fpc_matmul: invalid arguments");
+        fpc_matmul(a, b, c, M, N, K, ta != 0, tb != 0, tc != 0, trunc);
+        return (int)OK;
+    });
+}
+
+int pir_fpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
+                        int Dout, int stride, int pad, int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(batch > 0 && H > 0 && W > 0 && f > 0 && Din > 0 && Dout > 0 && stride > 0 && pad >= 0 &&
+                        trunc >= 1 && trunc <= 62 && (H + 2 * pad - f) >= 0 && (W + 2 * pad - f) >= 0,
+                    "pir_fpc_convolution: invalid geometry");
+        fpc_convolution(a, w, c, batch, H, W, f, Din, Dout, stride, pad, trunc);
         return (int)OK;
     });
 }

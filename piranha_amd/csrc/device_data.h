@@ -141,6 +141,14 @@ class MultiLaunch {
     }
     template <typename T>
     void add(DeviceData<T> &x, const typename NoDeduce<DeviceData<T>>::type *y, u64 scalar = 0) {
+        if (m_.count == MULTI_OP_MAX) {
+            set_error("MultiLaunch: more than 12 arrays in one launch");
+            throw PirError(ERR_STATE);
+        }
+        if (m_.count > 0 && x.size() != m_.n) {  // one element count per launch
+            set_error("MultiLaunch: arrays of different lengths in one launch");
+            throw PirError(ERR_STATE);
+        }
         int a = m_.count++;
         m_.x[a] = x.data();
         m_.xs[a] = x.stride();

@@ -7,14 +7,15 @@ namespace pir {
 
 enum EwOp { OP_ADD = 0, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV, OP_SET };
 
-// Up to six strided element-op pairs in one launch: x[a][i*xs] op= y[a][i*ys]
-// (or op= scalar[a] when y[a] is null).
+// Up to MULTI_OP_MAX strided element-op pairs in one launch:
+// x[a][i*xs] op= y[a][i*ys] (or op= scalar[a] when y[a] is null).
+constexpr int MULTI_OP_MAX = 12;  // every share array of a 4-party value
 struct MultiOp {
-    void *x[6];
-    long long xs[6];
-    const void *y[6];
-    long long ys[6];
-    u64 scalar[6];
+    void *x[MULTI_OP_MAX];
+    long long xs[MULTI_OP_MAX];
+    const void *y[MULTI_OP_MAX];
+    long long ys[MULTI_OP_MAX];
+    u64 scalar[MULTI_OP_MAX];
     int count;
     int op;
     size_t n;

@@ -38,8 +38,8 @@ __device__ __forceinline__ T ew_apply(int op, T x, T y) {
     return x;
 }
 
-// Up to 6 independent (x, y) array pairs per launch (one per party share), so
-// an RSS op over all three parties is one launch.
+// Up to MULTI_OP_MAX independent (x, y) array pairs per launch (one per party
+// share), so an op over every share of every party is one launch.
 template <typename T>
 __global__ __launch_bounds__(BLK) void multi_op_kernel(MultiOp m) {
     size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(BLK) void multi_op_kernel(MultiOp m) {
 }
 
 int multi_op(const MultiOp &m, int elem_bytes, hipStream_t s) {
-    PIR_REQUIRE(m.count >= 1 && m.count <= 6, "multi_op: 1..6 arrays");
+    PIR_REQUIRE(m.count >= 1 && m.count <= MULTI_OP_MAX, "multi_op: 1..12 arrays");
     if (m.n == 0) return OK;
     dim3 g(g1(m.n), m.count);
     if (elem_bytes == 8) hipLaunchKernelGGL(multi_op_kernel<u64>, g, dim3(BLK), 0, s, m);

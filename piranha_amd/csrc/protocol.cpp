@@ -211,10 +211,10 @@ size_t packed_rss_weights_bytes(int K, int N) { return limb_b_bytes(N, 6, round_
 int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed) {
     LimbBSrc s{};
     for (int p = 0; p < 3; p++) {
-        s.b0[2 * p] = B.getShare(p, 0)->data();
-        s.b1[2 * p] = B.getShare(p, 1)->data();
-        s.b0[2 * p + 1] = B.getShare(p, 0)->data();
-        s.b1[2 * p + 1] = nullptr;
+        s.b[2 * p][0] = B.getShare(p, 0)->data();
+        s.b[2 * p][1] = B.getShare(p, 1)->data();
+        s.b[2 * p + 1][0] = B.getShare(p, 0)->data();
+        s.coef[2 * p][0] = s.coef[2 * p][1] = s.coef[2 * p + 1][0] = 1;
     }
     s.nseg = 6;
     s.K = K;
@@ -260,8 +260,10 @@ int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int 
         a.row_major = ta ? 1 : 0;
     }
     Epilogue e{};
-    e.mode = EPI_RSS_TRUNC;
+    e.mode = EPI_TRUNC;
     e.out = out.base();
+    e.nout = 6;
+    e.base_mask = (1u << (2 * PARTY_A + 0)) | (1u << (2 * PARTY_C + 1));  // A(1+q, 0), C(0, 1+q)
     e.tc = tc ? 1 : 0;
     e.trunc_bits = trunc_bits;
     e.bias = bias ? bias->base() : nullptr;

@@ -15,11 +15,14 @@
 
 namespace pir {
 
-// Operand A description for the limb GEMM: up to 6 K-segments (one per RSS
-// share array of the three parties), each either a dense matrix or the
-// implicit im2col view of an NHWC activation tensor.
+constexpr int LIMB_MAX_SEG = 12;  // K-segments per product (9 for the 4-party matmul)
+constexpr int LIMB_MAX_TERMS = 4;  // share arrays summed into one packed B segment
+
+// Operand A description for the limb GEMM: up to LIMB_MAX_SEG K-segments (one
+// per share array), each either a dense matrix or the implicit im2col view of
+// an NHWC activation tensor.
 struct LimbA {
-    const u64 *seg[6];
+    const u64 *seg[LIMB_MAX_SEG];
     int nseg;
     int M, K;        // per-segment logical shape
     int Kp;          // K rounded up to the K-step (32)
@@ -29,30 +32,35 @@ struct LimbA {
     int H, W, Cin, fh, fw, ph, pw, stride, OH, OW;
 };
 
-// Operand B (weights) description for the packing kernel: per segment the
-// value is b0[idx] + (b1 ? b1[idx] : 0); B(k,n) = row_major ? p[k*N+n] : p[n*K+k].
+// Operand B (weights) description for the packing kernel: segment s is the
+// mod-2^64 combination sum_t coef[s][t] * b[s][t] of up to LIMB_MAX_TERMS share
+// arrays (null pointers are skipped); B(k,n) = row_major ? p[k*N+n] : p[n*K+k].
 struct LimbBSrc {
-    const u64 *b0[6];
-    const u64 *b1[6];
+    const u64 *b[LIMB_MAX_SEG][LIMB_MAX_TERMS];
+    int coef[LIMB_MAX_SEG][LIMB_MAX_TERMS];
     int nseg;
     int K, N, Kp;
     int row_major;
 };
 
 // Epilogue modes.
-enum EpiMode { EPI_RAW = 0, EPI_RSS_TRUNC = 1 };
+enum EpiMode { EPI_RAW = 0, EPI_TRUNC = 1 };
 
 struct Epilogue {
     int mode;
     // C(m,n) / out(m,n) stored row-major (ldc = N) if tc, else column-major (ldc = M)
     u64 *C;
     int tc;
-    // EPI_RSS_TRUNC: out is [3][2][M*N]; share (p,s) = base(p,s) + bias(p,s)[n], with
-    // base A=(1+q,0) B=(0,0) C=(0,1+q), q = trunc((raw - 2^k) / 2^k)
-    // (dividePublic3out3 with placeholder r = 1, r' = 2^k, RSS.inl:1076-1094).
+    // EPI_TRUNC: out holds `nout` share arrays of M*N; array j receives
+    // (bit j of base_mask ? 1 + q : 0) + (bias ? bias[j*N + n] : 0) with
+    // q = trunc((raw - 2^k) / 2^k), the dividePublic of the placeholder
+    // precompute (r = 1, r' = 2^k): RSS.inl:1076-1094 (arrays A.sA, C.sB),
+    // TPC.inl:370-384 (party A), FPC.inl:566-580 (one party's share 0/1/2).
     u64 *out;
+    int nout;
+    unsigned base_mask;
     int trunc_bits;
-    const u64 *bias;  // [3][2][N] or null
+    const u64 *bias;  // [nout][N] or null
 };
 
 constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;

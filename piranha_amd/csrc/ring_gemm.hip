@@ -61,20 +61,17 @@ __device__ __forceinline__ void transpose8x8(const u64 w[8], u64 out[8]) {
     out[7] = (u64)c3 | ((u64)d3 << 32);
 }
 
-// RSS truncation + bias epilogue for one output element (see Epilogue).
-__device__ __forceinline__ void rss_trunc_store(const Epilogue &e, size_t MN, int N, int m, int n, u64 raw) {
+// Truncation + bias epilogue for one output element (see Epilogue).
+__device__ __forceinline__ void trunc_store(const Epilogue &e, size_t MN, int N, int m, int n, u64 raw) {
     const int k = e.trunc_bits;
     u64 q = trunc_div_pow2(raw - ((u64)1 << k), k);
     u64 base = (u64)1 + q;
     size_t idx = e.tc ? (size_t)m * N + n : (size_t)n * (MN / N) + m;
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            u64 v = ((p == PARTY_A && s == 0) || (p == PARTY_C && s == 1)) ? base : (u64)0;
-            if (e.bias) v += e.bias[(size_t)(2 * p + s) * N + n];
-            e.out[(size_t)(2 * p + s) * MN + idx] = v;
-        }
+    for (int j = 0; j < e.nout; j++) {
+        u64 v = ((e.base_mask >> j) & 1u) ? base : (u64)0;
+        if (e.bias) v += e.bias[(size_t)j * N + n];
+        e.out[(size_t)j * MN + idx] = v;
+    }
 }
 
 __device__ __forceinline__ void epilogue_store(const Epilogue &e, int M, int N, int m, int n, u64 raw) {
@@ -82,7 +79,7 @@ __device__ __forceinline__ void epilogue_store(const Epilogue &e, int M, int N, 
         if (e.tc) e.C[(size_t)m * N + n] = raw;
         else e.C[(size_t)n * M + m] = raw;
     } else {
-        rss_trunc_store(e, (size_t)M * N, N, m, n, raw);
+        trunc_store(e, (size_t)M * N, N, m, n, raw);
     }
 }
 
@@ -108,8 +105,9 @@ __global__ __launch_bounds__(256) void limb_pack_b_kernel(LimbBSrc src, int8_t *
         u64 v = 0;
         if (n < src.N && k < src.K) {
             size_t idx = src.row_major ? (size_t)k * src.N + n : (size_t)n * src.K + k;
-            v = src.b0[seg][idx];
-            if (src.b1[seg]) v += src.b1[seg][idx];
+#pragma unroll
+            for (int t = 0; t < LIMB_MAX_TERMS; t++)
+                if (src.b[seg][t]) v += (u64)(s64)src.coef[seg][t] * src.b[seg][t][idx];
         }
         w[j] = recode_digits(v);
     }
@@ -122,7 +120,7 @@ __global__ __launch_bounds__(256) void limb_pack_b_kernel(LimbBSrc src, int8_t *
 }
 
 int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
-    PIR_REQUIRE(src.nseg >= 1 && src.nseg <= 6, "limb_pack_b: nseg must be 1..6");
+    PIR_REQUIRE(src.nseg >= 1 && src.nseg <= LIMB_MAX_SEG, "limb_pack_b: nseg must be 1..12");
     PIR_REQUIRE(src.Kp % LIMB_BK == 0 && src.Kp >= src.K, "limb_pack_b: Kp must be a multiple of 32 >= K");
     int nks = src.nseg * src.Kp / LIMB_BK;
     size_t total = (size_t)((src.N + LIMB_BN - 1) / LIMB_BN) * LIMB_BN * nks * (LIMB_BK / 8);
@@ -381,10 +379,11 @@ int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &e
 
 int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split, u64 *workspace,
               hipStream_t s) {
-    PIR_REQUIRE(a.nseg >= 1 && a.nseg <= 6, "limb_gemm: nseg must be 1..6");
+    PIR_REQUIRE(a.nseg >= 1 && a.nseg <= LIMB_MAX_SEG, "limb_gemm: nseg must be 1..12");
     PIR_REQUIRE(a.Kp % LIMB_BK == 0 && a.Kp >= a.K, "limb_gemm: Kp must be a multiple of 32 >= K");
-    PIR_REQUIRE(epi.mode != EPI_RSS_TRUNC || (epi.trunc_bits >= 1 && epi.trunc_bits <= 62),
+    PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.trunc_bits >= 1 && epi.trunc_bits <= 62),
                 "limb_gemm: truncation bits must be in [1, 62]");
+    PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.nout >= 1 && epi.nout <= 32), "limb_gemm: bad output share count");
     if (a.M == 0 || N == 0) return OK;
     const int nks = a.nseg * a.Kp / LIMB_BK;
     split = limb_effective_split(nks, split);

@@ -378,3 +378,105 @@ def test_network_matches_plaintext_twin(rng, name, layers, batch, fp):
         assert not act[2, 0].any()
         if layers[i][0] != "fc":
             assert not act[1, 1].any()
+
+
+# ---------------------------------------------------------------------------
+# TPC (2 parties) and FPC (4 parties): matmul, convolution, truncation
+# (TPC.inl / FPC.inl).  The reference runs its Func.cu suite typed over
+# RSS / TPC / FPC / OPC, so the same KATs apply to every scheme.
+
+SCHEMES = {
+    "tpc": (O.tpc_init_list, O.tpc_reconstruct, O.tpc_matmul, O.tpc_convolution, O.tpc_divide_public),
+    "fpc": (O.fpc_init_list, O.fpc_reconstruct, O.fpc_matmul, O.fpc_convolution, O.fpc_divide_public),
+}
+
+
+def _every_party(recon, fp=FP):
+    return recon.view(np.int64) / float(1 << fp)
+
+
+@pytest.mark.parametrize("kind", sorted(SCHEMES))
+def test_scheme_kats(kind):
+    init, recon, mm, conv, div = SCHEMES[kind]
+    for r in _every_party(recon(init([1, 1, 2, 8, 12.2, 1], FP))):  # Func.cu:14-31
+        assert np.allclose(r, [1, 1, 2, 8, 12.2, 1], atol=EPS)
+    c = mm(init([1] * 6, FP), init([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], FP), 2, 4, 3, 0, 0, 0, FP)
+    for r in _every_party(recon(c)):  # Func.cu:33-54
+        assert np.allclose(r, [1, 1, 1, 1, 0, 0, 1, 1], atol=EPS)
+    vals = [-2.786461, -1.280988, -2.209210, 0.049379, 0.241369, 1.617007, -0.572261, 0.705014, -1.176370,
+            -0.814461, 0.992866, 0.856274]
+    c = mm(init(np.eye(4).ravel(), FP), init(vals, FP), 4, 3, 4, 1, 1, 1, FP)
+    for r in _every_party(recon(c)):  # Func.cu:56-76
+        assert np.allclose(r, vals, atol=EPS)
+    im = [1, 2, 1, 2, 3, 2, 1, 2, 1] * 2
+    c = conv(init(im, FP), init([1, 0, 1, 0, 1, 0, 1, 0, 1], FP), 2, 3, 3, 3, 1, 1, 1, 1, FP)
+    for r in _every_party(recon(c)):  # Func.cu:177-218
+        assert np.allclose(r, [4, 6, 4, 6, 7, 6, 4, 6, 4] * 2, atol=EPS)
+    for r in _every_party(recon(div(init([8, 16, 24, -24], FP), 8))):  # Func.cu:99-114
+        assert np.allclose(r, [1, 2, 3, -3], atol=EPS)
+    for r in _every_party(recon(div(init([1 << 22], FP), 1 << 21))):  # Func.cu:116-131
+        assert np.allclose(r, [2], atol=EPS)
+
+
+def test_fpc_reshare_kat():
+    """unitTests.cu:44-88: z, z' known to every party; reshareFPC(i, j) puts z
+    (or z' when i, j are opposite parties) into the sharing."""
+    z = (np.array([1, 2, 5, -3], np.int64) << FP).view(np.uint64)
+    zp = (np.array([7, -3, -1, 4], np.int64) << FP).view(np.uint64)
+    Z, ZP = np.tile(z, (4, 1)), np.tile(zp, (4, 1))
+    for i, j in [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]:
+        out = O.fpc_reconstruct(O.fpc_reshare(Z, ZP, i, j))
+        want = zp if (j - i) == 2 else z
+        for q in range(4):
+            assert np.array_equal(out[q], want), (i, j, q)
+
+
+def _wrapped_product(X, W, M, N, K, ta, tb):
+    """X.W mod 2^64 with the gemm layout flags (transpose = row-major storage)."""
+    A = X.reshape(M, K) if ta else X.reshape(K, M).T
+    B = W.reshape(K, N) if tb else W.reshape(N, K).T
+    return (A.astype(object) @ B.astype(object)) % (1 << 64)
+
+
+def _trunc_closed_form(raw, k):
+    return np.array([(1 + _ieee_trunc_div((int(v) - (1 << k)) % (1 << 64), 1 << k)) % (1 << 64) for v in raw],
+                    np.uint64)
+
+
+@pytest.mark.parametrize("ta,tb,tc", [(1, 1, 1), (0, 0, 0), (1, 0, 1), (0, 1, 0)])
+def test_tpc_matmul_closed_form(rng, ta, tb, tc):
+    """TPC's Beaver product is exact for any sharing: party A ends with
+    1 + trunc((XW - 2^k) / 2^k), party B with 0 (ring identity of
+    TPC.inl:1059-1090 with x = y = 1, z = K, then dividePublic)."""
+    M, N, K, k = 5, 7, 9, 13
+    a, b = rand_u64(rng, 2, M * K), rand_u64(rng, 2, K * N)
+    X, W = a[0] + a[1], b[0] + b[1]
+    c = O.tpc_matmul(a, b, M, N, K, ta, tb, tc, k)
+    P = _wrapped_product(X, W, M, N, K, ta, tb)
+    raw = (P if tc else P.T).ravel()
+    assert np.array_equal(c[0], _trunc_closed_form(raw, k))
+    assert not c[1].any()
+
+
+def test_tpc_convolution_closed_form(rng):
+    b_, H, W, f, Din, Dout, s, p, k = 2, 6, 5, 3, 2, 3, 2, 1, 20
+    A, B = rand_u64(rng, 2, b_ * H * W * Din), rand_u64(rng, 2, Dout * f * f * Din)
+    c = O.tpc_convolution(A, B, b_, H, W, f, Din, Dout, s, p, k)
+    raw = O.conv_fprop(A[0] + A[1], B[0] + B[1], b_, H, W, Din, Dout, f, f, p, p, s)
+    assert np.array_equal(c[0], _trunc_closed_form(raw, k))
+    assert not c[1].any()
+
+
+def test_fpc_matmul_public_inputs_closed_form(rng):
+    """Values entered as in the reference's tests (FPC initializer list: only
+    component x0 is nonzero): every cross term vanishes and each of parties
+    B, C, D keeps 1 + trunc((XW - 2^k) / 2^k) in its x0 slot; A holds zeros."""
+    M, N, K, k = 4, 6, 8, 26
+    X, W = rng.integers(-(1 << 30), 1 << 30, M * K), rng.integers(-(1 << 30), 1 << 30, K * N)
+    a, b = O.fpc_init_list(X, 0, convert=False), O.fpc_init_list(W, 0, convert=False)
+    c = O.fpc_matmul(a, b, M, N, K, 1, 1, 1, k)
+    want = _trunc_closed_form(_wrapped_product(X.astype(np.uint64), W.astype(np.uint64), M, N, K, 1, 1).ravel(), k)
+    assert not c[0].any()
+    for q, slot in ((1, 2), (2, 1), (3, 0)):
+        assert np.array_equal(c[q, slot], want)
+        assert not np.delete(c[q], slot, axis=0).any()

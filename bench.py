@@ -209,10 +209,11 @@ def main():
     k_ms, k_ops, k_n = prof[dom]
     avg_ms = k_ms / k_n
     achieved = k_ops / k_n / (avg_ms * 1e-3) / 1e12
-    log("limb GEMM kernels: " + ", ".join(f"<{v}> {prof[v][2]} launches {prof[v][0]:.3f} ms "
+    log("limb GEMM kernels: " + ", ".join(f"<{v % 4},{1 if v >= 4 else 2}> {prof[v][2]} launches {prof[v][0]:.3f} ms "
                                           f"{prof[v][1] / prof[v][0] / 1e9:.0f} TOP/s" for v in sorted(prof)))
     log("per-layer ms: " + ", ".join(f"{layers[i][0]}{i}:{m:.3f}" for i, m in enumerate(layer_ms)))
-    roofline = {"bound": "mfma", "kernel": f"limb_gemm_kernel<{dom}> (3-party implicit-GEMM conv, int8 limbs)",
+    kname = f"limb_gemm_kernel<{dom % 4}, {1 if dom >= 4 else 2}>"
+    roofline = {"bound": "mfma", "kernel": f"{kname} (3-party implicit-GEMM conv, int8 limbs)",
                 "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                 "traffic": read_traffic(dom), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
                 "ops_per_launch": k_ops / k_n,

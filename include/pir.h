@@ -186,10 +186,13 @@ int pir_net_replay(pir_net *net);
  * ms[i] = device time of layer i (count = number of layers) */
 int pir_net_profile_layers(pir_net *net, float *ms, int count);
 /* one eager forward with HIP events around every limb GEMM kernel launch on
- * the network's stream; per A-operand variant v (PIR_LIMB_* below): ms[v] =
- * summed kernel time, ops[v] = summed algorithmic int8 ops (2*36*M*N*nseg*K),
- * launches[v] = launch count.  Arrays hold PIR_LIMB_VARIANTS entries. */
-enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_C32, PIR_LIMB_VARIANTS };
+ * the network's stream; per kernel variant v: ms[v] = summed kernel time,
+ * ops[v] = summed algorithmic int8 ops (2*36*M*N*nseg*K), launches[v] = launch
+ * count.  v = A-operand form (PIR_LIMB_DENSE_ROW .. PIR_LIMB_CONV_C32), plus
+ * PIR_LIMB_WIDE for the 32x128 workgroup tile (else 64x64).  Arrays hold
+ * PIR_LIMB_VARIANTS entries. */
+enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_C32, PIR_LIMB_WIDE = 4,
+       PIR_LIMB_VARIANTS = 8 };
 int pir_net_profile_gemm(pir_net *net, double *ms, double *ops, int *launches);
 int pir_net_set_mode(pir_net *net, int mode);
 

@@ -249,12 +249,13 @@ class Network:
         """One eager forward with HIP events around every limb GEMM launch.
 
         Returns {variant: (kernel_ms, algorithmic_int8_ops, launches)} for the
-        variants launched (0 dense row-major A, 1 dense column-major A, 2 conv,
-        3 conv with Cin % 32 == 0)."""
-        ms, ops = np.zeros(4, np.float64), np.zeros(4, np.float64)
-        n = np.zeros(4, np.int32)
+        variants launched: A-operand form (0 dense row-major, 1 dense
+        column-major, 2 conv, 3 conv with Cin % 32 == 0), plus 4 for the
+        32x128 workgroup tile (limb_gemm_kernel<form, 1>; else <form, 2>)."""
+        ms, ops = np.zeros(8, np.float64), np.zeros(8, np.float64)
+        n = np.zeros(8, np.int32)
         check(lib().pir_net_profile_gemm(self.h, ms.ctypes.data, ops.ctypes.data, n.ctypes.data))
-        return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(4) if n[v]}
+        return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(8) if n[v]}
 
     def _read(self, fn, layer, dtype):
         p, n = _vp(), _sz()

@@ -131,25 +131,28 @@ int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Limb MFMA GEMM.  Workgroup tile 64x64, 4 waves each owning a 32x32 output
-// block computed with v_mfma_i32_32x32x32_i8; K-step 32.  The next K-step's
-// global loads are issued before the current step's MFMAs and staged into LDS
-// after them, so HBM/L2 latency hides behind the matrix work.  LDS rows are
-// padded from 32 to 48 bytes: the 16-byte operand reads of 16 consecutive rows
-// then fall on distinct banks.
+// Limb MFMA GEMM.  256 threads = 4 waves, each owning a 32x32 output block
+// computed with v_mfma_i32_32x32x32_i8; K-step 32.  Two workgroup shapes:
+//   WM = 2: 64x64 tile, waves 2 (M) x 2 (N);
+//   WM = 1: 32x128 tile, waves 1 x 4, used when the packed-N-tile count is
+//           even: every A row is loaded and recoded once per 128 columns.
+// A is recoded and byte-transposed into 8 limb planes in LDS (shared by the
+// waves of a row block).  B is pre-packed in the MFMA operand layout, so each
+// wave loads its own 16 bytes per plane straight from global memory into
+// registers; only A passes through LDS.  The next K-step's A and B loads are
+// issued before the current step's MFMAs.
 
-constexpr int LDS_ROW = 48;
-constexpr int LDS_PLANE = LIMB_BM * LDS_ROW;
+constexpr int LDS_ROW = 32;
 
-// A-operand loader: each thread owns 8 consecutive k of one row of the tile.
+// A-operand loader: each thread owns KPT consecutive k of one row of the tile.
 // Loads are branch-free (indices are clamped into the operand and the value is
-// masked to zero), so all eight are in flight together.  The K-step position
+// masked to zero), so all of them are in flight together.  The K-step position
 // (segment, step within segment) is uniform across the workgroup.
 //   AMODE 0: dense row-major A, 1: dense column-major A,
 //   AMODE 2: implicit im2col, any Cin (per-element decode),
 //   AMODE 3: implicit im2col with Cin % 32 == 0 (one (r, s) per K-step, decoded
 //            once per workgroup and advanced incrementally).
-template <int AMODE>
+template <int AMODE, int KPT>
 struct ALoader {
     const LimbA &a;
     int am, akq, kps;
@@ -203,21 +206,21 @@ struct ALoader {
             }
         }
     }
-    __device__ void load(u64 w[8]) const {
+    __device__ void load(u64 w[KPT]) const {
         const u64 *__restrict__ src = a.seg[seg];
-        const int kb = kk * LIMB_BK + akq * 8;
+        const int kb = kk * LIMB_BK + akq * KPT;
         if (AMODE == 3) {
             int ih = ih0 + r, iw = iw0 + s;
             bool ok = row_ok && kb < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
             const u64 *p = src + (((size_t)cb * a.H + (ok ? ih : 0)) * a.W + (ok ? iw : 0)) * a.Cin +
-                           (ok ? c0 + akq * 8 : 0);
+                           (ok ? c0 + akq * KPT : 0);
             const u64 msk = ok ? ~0ull : 0ull;
 #pragma unroll
-            for (int j = 0; j < 8; j++) w[j] = p[j] & msk;
+            for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
         } else if (AMODE == 2) {
             const int fwc = a.fw * a.Cin;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
+            for (int j = 0; j < KPT; j++) {
                 int k = kb + j;
                 int rr = k / fwc;
                 int rem = k - rr * fwc;
@@ -231,7 +234,7 @@ struct ALoader {
         } else {
             const int m = row_ok ? am : 0;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
+            for (int j = 0; j < KPT; j++) {
                 int k = kb + j;
                 bool ok = row_ok && k < a.K;
                 int kc = ok ? k : 0;
@@ -242,74 +245,71 @@ struct ALoader {
     }
 };
 
-__device__ __forceinline__ void stage_a(int8_t *As, int arow, int akq, const u64 w[8]) {
-    u64 d[8], limbs[8];
+// Recode KPT consecutive k of one row and store limb plane i's bytes for them
+// at As + i * plane + arow * LDS_ROW + akq * KPT.
+template <int KPT>
+__device__ __forceinline__ void stage_a(int8_t *As, int plane, int arow, int akq, const u64 w[KPT]) {
+    u64 d[KPT];
 #pragma unroll
-    for (int j = 0; j < 8; j++) d[j] = recode_digits(w[j]);
-    transpose8x8(d, limbs);
+    for (int j = 0; j < KPT; j++) d[j] = recode_digits(w[j]);
+    int8_t *dst = As + arow * LDS_ROW + akq * KPT;
+    if (KPT == 8) {
+        u64 limbs[8];
+        transpose8x8(d, limbs);
 #pragma unroll
-    for (int i = 0; i < 8; i++) *(u64 *)(As + i * LDS_PLANE + arow * LDS_ROW + akq * 8) = limbs[i];
-}
-
-__device__ __forceinline__ void stage_b(int8_t *Bs, int tid, int4 b0, int4 b1, int4 b2, int4 b3) {
-    // 16-byte chunk ch of the packed [limb][64 n][32 k] block
-    int ch = tid;
-    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b0;
-    ch += 256;
-    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b1;
-    ch += 256;
-    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b2;
-    ch += 256;
-    *(int4 *)(Bs + (ch >> 7) * LDS_PLANE + ((ch >> 1) & 63) * LDS_ROW + (ch & 1) * 16) = b3;
+        for (int i = 0; i < 8; i++) *(u64 *)(dst + i * plane) = limbs[i];
+    } else {
+        u32 l[8];
+        transpose4((u32)d[0], (u32)d[1], (u32)d[2], (u32)d[3], l[0], l[1], l[2], l[3]);
+        transpose4((u32)(d[0] >> 32), (u32)(d[1] >> 32), (u32)(d[2] >> 32), (u32)(d[3] >> 32), l[4], l[5], l[6],
+                   l[7]);
+#pragma unroll
+        for (int i = 0; i < 8; i++) *(u32 *)(dst + i * plane) = l[i];
+    }
 }
 
 // One workgroup covers at most LIMB_FLUSH_STEPS K-steps (8192 k), so the eight
 // int32 group accumulators never overflow and no 64-bit running sum is needed
 // inside the loop; longer K is split across workgroups (split-K) and the 64-bit
 // partials are summed by split_reduce_kernel.  With <= 256 registers per lane
-// (128 accumulators + VGPRs) two workgroups share a CU, so one's staging
+// (128 accumulators + operands) two workgroups share a CU, so one's staging
 // overlaps the other's MFMAs.
-template <int AMODE>
+template <int AMODE, int WM>
 __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t *__restrict__ Bp, int N, int nks,
                                                            int steps_per_split, Epilogue epi, u64 *__restrict__ part) {
-    __shared__ __attribute__((aligned(16))) int8_t As[8 * LDS_PLANE];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[8 * LDS_PLANE];
+    constexpr int TM = 32 * WM, TN = 128 / WM, KPT = 4 * WM;
+    constexpr int KG = LIMB_BK / KPT;  // threads per row for staging
+    constexpr int PLANE = TM * LDS_ROW;
+    constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
+    __shared__ __attribute__((aligned(16))) int8_t As[8 * PLANE];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.x * LIMB_BM, nt = blockIdx.y;
+    const int wm = wave / (4 / WM), wn = wave % (4 / WM);
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN + wn * 32;
     const int M = a.M;
     const int ks_begin = blockIdx.z * steps_per_split;
     const int ks_end = min(nks, ks_begin + steps_per_split);
 
-    int arow, akq;
-    if (AMODE == 1) {
-        arow = tid & 63;
-        akq = tid >> 6;
-    } else {
-        arow = tid >> 2;
-        akq = tid & 3;
-    }
-    ALoader<AMODE> L(a, m0 + arow, akq);
+    const int arow = AMODE == 1 ? tid % TM : tid / KG;
+    const int akq = AMODE == 1 ? tid / TM : tid % KG;
+    ALoader<AMODE, KPT> L(a, m0 + arow, akq);
 
-    const int4 *bblk = (const int4 *)(Bp + (size_t)nt * nks * (8 * LIMB_BN * LIMB_BK)) + tid;
-    u64 w[8];
-    int4 b0, b1, b2, b3;
+    // this lane's 16 bytes of each B limb plane: column n0 + lane % 32, k half lane / 32
+    const int8_t *bw = Bp + (size_t)(n0 / LIMB_BN) * nks * STEP + ((n0 % LIMB_BN) + (lane & 31)) * LIMB_BK +
+                       (lane >> 5) * 16;
+    u64 w[KPT];
+    v4i bv[8], bn[8];
     if (ks_begin < ks_end) {
         L.init(ks_begin);
         L.load(w);
-        const int4 *bp = bblk + (size_t)ks_begin * 1024;
-        b0 = bp[0];
-        b1 = bp[256];
-        b2 = bp[512];
-        b3 = bp[768];
-        stage_a(As, arow, akq, w);
-        stage_b(Bs, tid, b0, b1, b2, b3);
+        const int8_t *bp = bw + (size_t)ks_begin * STEP;
+#pragma unroll
+        for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+        stage_a<KPT>(As, PLANE, arow, akq, w);
     }
     __syncthreads();
 
     const int a_off = (wm * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 16;
-    const int b_off = (wn * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 16;
     v16i acc[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -318,34 +318,30 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         const bool more = ks + 1 < ks_end;
         if (more) L.advance();
         L.load(w);
-        const int4 *bp = bblk + (size_t)(more ? ks + 1 : ks) * 1024;
-        b0 = bp[0];
-        b1 = bp[256];
-        b2 = bp[512];
-        b3 = bp[768];
-
-        v4i bv[8];
+        const int8_t *bp = bw + (size_t)(more ? ks + 1 : ks) * STEP;
 #pragma unroll
-        for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(Bs + j * LDS_PLANE + b_off);
+        for (int j = 0; j < 8; j++) bn[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+
+        // acc[i+j] += A_i . B_j for i + j < 8, A planes streamed from LDS one at a time
         v4i av = *(const v4i *)(As + a_off);
-        // acc[i+j] += A_i . B_j for i + j < 8, A planes streamed one at a time
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             v4i an = av;
-            if (i < 7) an = *(const v4i *)(As + (i + 1) * LDS_PLANE + a_off);
+            if (i < 7) an = *(const v4i *)(As + (i + 1) * PLANE + a_off);
 #pragma unroll
             for (int j = 0; j < 8 - i; j++)
                 acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv[j], acc[i + j], 0, 0, 0);
             av = an;
         }
-        __syncthreads();  // every wave has consumed this step's LDS operands
-        stage_a(As, arow, akq, w);
-        stage_b(Bs, tid, b0, b1, b2, b3);
+        __syncthreads();  // every wave has consumed this step's A planes
+        stage_a<KPT>(As, PLANE, arow, akq, w);
+#pragma unroll
+        for (int j = 0; j < 8; j++) bv[j] = bn[j];
         __syncthreads();
     }
 
     // ---- epilogue: C = sum_s acc_s << 8s; 32x32 MFMA layout, lane -> column, register -> row
-    const int n = nt * LIMB_BN + wn * 32 + (lane & 31);
+    const int n = n0 + (lane & 31);
     if (n >= N) return;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -389,9 +385,12 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     split = limb_effective_split(nks, split);
     const int per = (nks + split - 1) / split;
     PIR_REQUIRE(split == 1 || workspace, "limb_gemm: split-K needs a workspace");
-    dim3 grid((a.M + LIMB_BM - 1) / LIMB_BM, (N + LIMB_BN - 1) / LIMB_BN, split);
+    // 32x128 tiles when the packed 64-column tiles pair up, else 64x64
+    const int wide = (((N + LIMB_BN - 1) / LIMB_BN) % 2) == 0;
+    dim3 grid(wide ? (a.M + 31) / 32 : (a.M + 63) / 64, wide ? (N + 127) / 128 : (N + 63) / 64, split);
     u64 *part = split > 1 ? workspace : nullptr;
-    const int variant = a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1);
+    const int amode = a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1);
+    const int variant = amode + (wide ? 4 : 0);
     GemmTimer &timer = ctx().timer;
     GemmTimer::Rec rec{nullptr, nullptr, 2.0 * 36 * a.M * (double)N * a.nseg * a.K, variant};
     if (timer.on) {
@@ -399,12 +398,19 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
         PIR_HIP(hipEventCreate(&rec.stop));
         PIR_HIP(hipEventRecord(rec.start, s));
     }
+#define PIR_LIMB_LAUNCH(AM, WM)                                                                                \
+    hipLaunchKernelGGL((limb_gemm_kernel<AM, WM>), grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part)
     switch (variant) {
-        case 0: hipLaunchKernelGGL(limb_gemm_kernel<0>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
-        case 1: hipLaunchKernelGGL(limb_gemm_kernel<1>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
-        case 2: hipLaunchKernelGGL(limb_gemm_kernel<2>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
-        default: hipLaunchKernelGGL(limb_gemm_kernel<3>, grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part); break;
+        case 0: PIR_LIMB_LAUNCH(0, 2); break;
+        case 1: PIR_LIMB_LAUNCH(1, 2); break;
+        case 2: PIR_LIMB_LAUNCH(2, 2); break;
+        case 3: PIR_LIMB_LAUNCH(3, 2); break;
+        case 4: PIR_LIMB_LAUNCH(0, 1); break;
+        case 5: PIR_LIMB_LAUNCH(1, 1); break;
+        case 6: PIR_LIMB_LAUNCH(2, 1); break;
+        default: PIR_LIMB_LAUNCH(3, 1); break;
     }
+#undef PIR_LIMB_LAUNCH
     PIR_LAUNCH_CHECK();
     if (timer.on) {
         PIR_HIP(hipEventRecord(rec.stop, s));

@@ -39,7 +39,8 @@ struct GemmTimer {
     struct Rec {
         hipEvent_t start, stop;
         double ops;   // algorithmic int8 ops of the launch: 2 * 36 * M * N * nseg * K
-        int variant;  // A-operand form (0 dense row-major, 1 dense col-major, 2 conv, 3 conv Cin%32==0)
+        int variant;  // A-operand form (0 dense row-major, 1 dense col-major, 2 conv, 3 conv Cin%32==0),
+                      // + 4 for the 32x128 workgroup tile
     };
     bool on = false;
     std::vector<Rec> recs;

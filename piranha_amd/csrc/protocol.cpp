@@ -224,9 +224,12 @@ int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *pac
     return limb_pack_b(s, packed, S());
 }
 
+// Split-K factor for an M x N product of nks K-steps: enough workgroups for
+// two per CU (512 on 256 CUs; the limb kernel runs two workgroups per CU so
+// one's staging hides under the other's MFMAs), at least 8 K-steps each.
 int choose_split(int M, int N, int nks) {
     int tiles = ((M + LIMB_BM - 1) / LIMB_BM) * ((N + LIMB_BN - 1) / LIMB_BN);
-    if (tiles >= 256) return 1;
+    if (tiles >= 512) return 1;
     int want = (512 + tiles - 1) / tiles;
     int cap = nks / 8;  // keep at least 8 K-steps per split
     if (cap < 1) cap = 1;

@@ -137,10 +137,12 @@ int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
 //   WM = 1: 32x128 tile, waves 1 x 4, used when the packed-N-tile count is
 //           even: every A row is loaded and recoded once per 128 columns.
 // A is recoded and byte-transposed into 8 limb planes in LDS (shared by the
-// waves of a row block).  B is pre-packed in the MFMA operand layout, so each
-// wave loads its own 16 bytes per plane straight from global memory into
-// registers; only A passes through LDS.  The next K-step's A and B loads are
-// issued before the current step's MFMAs.
+// waves of a row block), double-buffered so a K-step needs one barrier.  B is
+// pre-packed in the MFMA operand layout, so each wave loads its own 16 bytes
+// per plane straight from global memory into registers; only A passes through
+// LDS.  The next K-step's A and B loads are issued before the current step's
+// MFMAs, and a wave raises its issue priority while it issues MFMAs, so the
+// other workgroup's wave on the same SIMD stages in the MFMA shadow.
 
 constexpr int LDS_ROW = 32;
 
@@ -281,7 +283,8 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     constexpr int KG = LIMB_BK / KPT;  // threads per row for staging
     constexpr int PLANE = TM * LDS_ROW;
     constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
-    __shared__ __attribute__((aligned(16))) int8_t As[8 * PLANE];
+    __shared__ __attribute__((aligned(16))) int8_t As2[2][8 * PLANE];
+    int cur = 0;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / (4 / WM), wn = wave % (4 / WM);
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         const int8_t *bp = bw + (size_t)ks_begin * STEP;
 #pragma unroll
         for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
-        stage_a<KPT>(As, PLANE, arow, akq, w);
+        stage_a<KPT>(As2[0], PLANE, arow, akq, w);
     }
     __syncthreads();
 
@@ -323,7 +326,9 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         for (int j = 0; j < 8; j++) bn[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
 
         // acc[i+j] += A_i . B_j for i + j < 8, A planes streamed from LDS one at a time
+        const int8_t *As = As2[cur];
         v4i av = *(const v4i *)(As + a_off);
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             v4i an = av;
@@ -333,8 +338,10 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
                 acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv[j], acc[i + j], 0, 0, 0);
             av = an;
         }
-        __syncthreads();  // every wave has consumed this step's A planes
-        stage_a<KPT>(As, PLANE, arow, akq, w);
+        __builtin_amdgcn_s_setprio(0);
+        // the other buffer was last read in the previous step, before the barrier
+        stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, w);
+        cur ^= 1;
 #pragma unroll
         for (int j = 0; j < 8; j++) bv[j] = bn[j];
         __syncthreads();

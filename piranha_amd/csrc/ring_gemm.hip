@@ -233,6 +233,13 @@ struct ALoader {
                 size_t idx = ok ? (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + cc : 0;
                 w[j] = src[idx] & (ok ? ~0ull : 0ull);
             }
+        } else if (AMODE == 0 && (a.K % KPT) == 0) {
+            // a thread's KPT consecutive k are all inside K or all outside: one contiguous run
+            bool ok = row_ok && kb < a.K;
+            const u64 *p = src + (size_t)(row_ok ? am : 0) * a.lda + (ok ? kb : 0);
+            const u64 msk = ok ? ~0ull : 0ull;
+#pragma unroll
+            for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
         } else {
             const int m = row_ok ? am : 0;
 #pragma unroll
@@ -286,11 +293,21 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     __shared__ __attribute__((aligned(16))) int8_t As2[2][8 * PLANE];
     int cur = 0;
 
+    // XCD-aware tile order: the dispatcher deals consecutive workgroup ids
+    // round-robin over the 8 XCDs, so give each XCD a contiguous run of logical
+    // tiles (M fastest): workgroups that share a B slice then share one L2.
+    int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    {
+        const int total = gridDim.x * gridDim.y * gridDim.z, per_xcd = total / 8;
+        if (bid < per_xcd * 8) bid = (bid % 8) * per_xcd + bid / 8;
+    }
+    const int bx = bid % gridDim.x, by = (bid / gridDim.x) % gridDim.y, bz = bid / (gridDim.x * gridDim.y);
+
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / (4 / WM), wn = wave % (4 / WM);
-    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN + wn * 32;
+    const int m0 = bx * TM, n0 = by * TN + wn * 32;
     const int M = a.M;
-    const int ks_begin = blockIdx.z * steps_per_split;
+    const int ks_begin = bz * steps_per_split;
     const int ks_end = min(nks, ks_begin + steps_per_split);
 
     const int arow = AMODE == 1 ? tid % TM : tid / KG;
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         u64 raw = 0;
 #pragma unroll
         for (int q = 0; q < 8; q++) raw += ((u64)(s64)acc[q][r]) << (8 * q);
-        if (part) part[((size_t)blockIdx.z * M + m) * N + n] = raw;
+        if (part) part[((size_t)bz * M + m) * N + n] = raw;
         else epilogue_store(epi, M, N, m, n, raw);
     }
 }

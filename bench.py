@@ -39,14 +39,14 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "limb_gemm_conv_vgg16_b64.md")
 
 
-def read_traffic(variant):
-    """HBM bytes per launch of limb_gemm_kernel<variant> in this workload, from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a line
-    `traffic_bytes_per_launch[<v>] = N`), or None when it is absent."""
+def read_traffic(kname):
+    """HBM bytes per launch of kernel `kname` (e.g. "limb_gemm_kernel<3, 1>") in
+    this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (a line `traffic_bytes_per_launch[<kname>] = N`), or None when it is absent."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                key = f"traffic_bytes_per_launch[<{variant}>] ="
+                key = f"traffic_bytes_per_launch[{kname}] ="
                 if line.strip().startswith(key):
                     return float(line.split("=", 1)[1].split()[0])
     except OSError:
@@ -169,8 +169,9 @@ def cpu_baseline(C, batch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    # ~0.7 s of replays first: the GPU clocks ramp up over the first few hundred ms
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--gemm-sizes", default="512,1024,2048,4096,8192")
     ap.add_argument("--plain-steps", type=int, default=2)
@@ -215,7 +216,7 @@ def main():
     kname = f"limb_gemm_kernel<{dom % 4}, {1 if dom >= 4 else 2}>"
     roofline = {"bound": "mfma", "kernel": f"{kname} (3-party implicit-GEMM conv, int8 limbs)",
                 "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
-                "traffic": read_traffic(dom), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
+                "traffic": read_traffic(kname), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
                 "ops_per_launch": k_ops / k_n,
                 "share_of_forward": k_ms / float(layer_ms.sum())}
     del net

@@ -15,6 +15,9 @@ Also reported on the same line:
   * roofline of the dominant kernel (the limb implicit-GEMM convolution),
     timed per launch with HIP events; its HBM traffic per launch comes from
     the committed rocprofv3 counter summary under profiles/;
+  * hbm_roofline: achieved HBM bandwidth of the fused ReLU and maxpool kernels;
+  * fp64_error: the reconstructed VGG16 logits against an FP64 forward of the
+    same network (weights decoded from the shares);
   * cpu_baseline: the CPU oracle on VGG16 at a small batch, host cores.
 """
 import argparse
@@ -52,6 +55,81 @@ def read_traffic(kname):
     except OSError:
         pass
     return None
+
+
+def decode(shares, fp):
+    """Party A's reconstruction (shareA + shareB + party C's shareA) as doubles."""
+    v = (shares[0, 0] + shares[0, 1] + shares[2, 0]).view(np.int64)
+    return v / float(1 << fp)
+
+
+def fp64_forward(layers, batch, params, x):
+    """The same network in plain FP64 (NHWC cross-correlation with zero padding,
+    FC x.W + b, ReLU, max-pooling): the plaintext reference the reconstructed
+    fixed-point outputs are reported against.  params[i] = (W, b) as doubles."""
+    a = np.asarray(x, np.float64).reshape(batch, -1)
+    for i, l in enumerate(layers):
+        if l[0] == "cnn":
+            H, W, Cin, Cout, f, s, p = l[1:]
+            t = np.pad(a.reshape(batch, H, W, Cin), ((0, 0), (p, p), (p, p), (0, 0)))
+            OH, OW = (H + 2 * p - f) // s + 1, (W + 2 * p - f) // s + 1
+            win = np.lib.stride_tricks.sliding_window_view(t, (f, f), axis=(1, 2))[:, ::s, ::s][:, :OH, :OW]
+            cols = win.transpose(0, 1, 2, 4, 5, 3).reshape(batch * OH * OW, f * f * Cin)
+            a = (cols @ params[i][0].reshape(Cout, f * f * Cin).T).reshape(batch, -1)
+        elif l[0] == "fc":
+            a = a.reshape(batch, l[1]) @ params[i][0].reshape(l[1], l[2])
+            if params[i][1] is not None:
+                a = a + params[i][1]
+        elif l[0] == "relu":
+            a = np.maximum(a, 0.0)
+        else:
+            H, W, C, pool, s = l[1:]
+            OH, OW = (H - pool) // s + 1, (W - pool) // s + 1
+            t = a.reshape(batch, H, W, C)
+            win = np.lib.stride_tricks.sliding_window_view(t, (pool, pool), axis=(1, 2))[:, ::s, ::s][:, :OH, :OW]
+            a = win.max(axis=(-2, -1)).reshape(batch, -1)
+    return a
+
+
+def fp64_error(net, layers, batch, fp, x):
+    """Max |decoded 3PC output - FP64 forward| over the final layer, with the
+    weights decoded from the network's own shares."""
+    params = {}
+    for i, l in enumerate(layers):
+        if l[0] in ("fc", "cnn"):
+            w, b = net.weights(i)
+            params[i] = (decode(w, fp), None if b is None else decode(b, fp))
+    ref = fp64_forward(layers, batch, params, x).ravel()
+    got = decode(net.activation(len(layers) - 1), fp)
+    err = float(np.max(np.abs(got - ref)))
+    return {"max_abs_err": err, "max_abs_ref": float(np.max(np.abs(ref))),
+            "rel_err": err / max(float(np.max(np.abs(ref))), 1e-30), "outputs": int(ref.size)}
+
+
+def hbm_roofline(layers, batch, layer_ms):
+    """Achieved HBM bandwidth of the fused elementwise / comparison kernels from
+    per-layer HIP-event times: ReLU 102 B per element (three parties: two 8-byte
+    input and output shares each, two 1-byte derivative shares), maxpool 264 B
+    per window (three parties: 4 input values x 2 shares x 8 B read, 2 x 8 B
+    written, 4 x 2 derivative bytes), against 8 TB/s."""
+    out = {}
+    for kind, per in (("relu", 102.0), ("maxpool", 264.0)):
+        byts = ms = 0.0
+        for i, l in enumerate(layers):
+            if l[0] != kind:
+                continue
+            if kind == "relu":
+                units = batch * l[1]
+            else:
+                H, W, C, pool, s = l[1:]
+                units = batch * ((H - pool) // s + 1) * ((W - pool) // s + 1) * C
+            byts += units * per
+            ms += float(layer_ms[i])
+        if ms > 0:
+            gbs = byts / (ms * 1e-3) / 1e9
+            out[kind] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                         "bytes_per_forward": byts, "ms_per_forward": ms}
+    return out
 
 
 def copy_calibration(P, n):
@@ -219,6 +297,13 @@ def main():
                 "traffic": read_traffic(kname), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
                 "ops_per_launch": k_ops / k_n,
                 "share_of_forward": k_ms / float(layer_ms.sum())}
+    hbm = hbm_roofline(layers, batch, layer_ms)
+    for k, v in hbm.items():
+        log(f"{k}: {v['achieved']:.0f} GB/s ({100 * v['frac']:.1f}% of HBM peak)")
+    xin = np.random.default_rng(1).uniform(0.0, 1.0, batch * C.input_size(layers))  # make_net's input
+    acc = fp64_error(net, layers, batch, fp, xin)
+    log(f"reconstructed logits vs FP64 forward: max abs err {acc['max_abs_err']:.3g} "
+        f"(max |logit| {acc['max_abs_ref']:.3g})")
     del net
 
     result = {
@@ -231,6 +316,8 @@ def main():
                                "He-init weights, all 3 parties on one GPU, hipGraph forward",
                    "global_batch": batch, "parallelism": "none (1 GPU, 3 simulated parties)"},
         "roofline": roofline,
+        "hbm_roofline": hbm,
+        "fp64_error": acc,
     }
 
     if not args.quick:

@@ -84,3 +84,20 @@ def test_graph_replay_matches_eager():
     net.replay()
     net.synchronize()
     check_against_oracle(net, layers, batch, fp, params, x2)
+
+
+def test_vgg16_full_batch_fused_equals_plain():
+    """The measured configuration itself (VGG16-CIFAR, batch 64, FP26): the
+    fused form replayed from its hipGraph equals the reference's op-by-op form
+    share for share on every layer."""
+    layers, batch, fp = C.VGG16_CIFAR, 64, 26
+    x = np.random.default_rng(21).uniform(0.0, 1.0, batch * C.input_size(layers))
+    fused, _ = build(layers, batch, fp, seed=4, mode=P.MODE_FUSED)
+    fused.forward(x)
+    fused.capture()
+    fused.replay()
+    fused.synchronize()
+    plain, _ = build(layers, batch, fp, seed=4, mode=P.MODE_PLAIN)
+    plain.forward(x)
+    for i in range(len(layers)):
+        assert np.array_equal(fused.activation(i), plain.activation(i)), f"layer {i} ({layers[i][0]}) differs"

@@ -166,11 +166,13 @@ def time_eager(net, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True):
+def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True, autotune=True):
     net = P.Network(C.to_spec(layers), batch, fp, seed=seed, mode=mode)
     rng = np.random.default_rng(seed)
     net.forward(rng.uniform(0.0, 1.0, batch * C.input_size(layers)))  # setPublic of U[0,1) inputs
     net.synchronize()
+    if autotune and mode == P.MODE_FUSED:
+        net.autotune()  # one eager forward timing candidate GEMM plans; outputs unchanged
     if capture:
         net.capture()
     return net
@@ -257,6 +259,7 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one x += y element-op launch over this many u64 (counter calibration)")
     ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true", help="heuristic GEMM plans only")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -272,7 +275,8 @@ def main():
 
     layers, batch, fp = C.VGG16_CIFAR, args.batch, 26
     log(f"VGG16-CIFAR b{batch}: building (fused limb-MFMA, hipGraph)")
-    net = make_net(P, C, layers, batch, fp, P.MODE_FUSED)
+    net = make_net(P, C, layers, batch, fp, P.MODE_FUSED, autotune=not args.no_autotune)
+    log(f"autotuned GEMM shapes: {P.autotune_entries()}")
     t = time_graph(P, net, args.steps, args.warmup)
     images_s = batch / t
     log(f"VGG16 fused: {t * 1e3:.3f} ms/forward, {images_s:.1f} images/s")

@@ -196,6 +196,20 @@ enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_
 int pir_net_profile_gemm(pir_net *net, double *ms, double *ops, int *launches);
 int pir_net_set_mode(pir_net *net, int mode);
 
+/* ---- limb GEMM autotuner (no counterpart in the reference: CUTLASS tile
+ * shapes are fixed at compile time there) ------------------------------------
+ * Each limb GEMM shape runs with a plan: workgroup tile (32x128 or 64x64) and
+ * split-K factor.  With tuning on, the first eager launch of an untuned shape
+ * times the candidate plans on the real operands and records the fastest in a
+ * process-wide table; later launches, and hipGraph captures, use the table.
+ * pir_net_autotune runs one eager forward of the network with tuning on. */
+int pir_net_autotune(pir_net *net);
+int pir_autotune_set(int on); /* for the free-standing pir_* calls */
+int pir_autotune_entries(void);
+int pir_autotune_clear(void);
+int pir_autotune_save(const char *path); /* text, one shape per line */
+int pir_autotune_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

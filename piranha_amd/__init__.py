@@ -87,6 +87,12 @@ _SIGS = {
     "pir_net_set_mode": ([_vp, _i], _i),
     "pir_net_profile_layers": ([_vp, _vp, _i], _i),
     "pir_net_profile_gemm": ([_vp, _vp, _vp, _vp], _i),
+    "pir_net_autotune": ([_vp], _i),
+    "pir_autotune_set": ([_i], _i),
+    "pir_autotune_entries": ([], _i),
+    "pir_autotune_clear": ([], _i),
+    "pir_autotune_save": ([ctypes.c_char_p], _i),
+    "pir_autotune_load": ([ctypes.c_char_p], _i),
     "pir_tpc_reconstruct": ([_vp, _sz, _vp, _vp], _i),
     "pir_tpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
     "pir_tpc_matmul": ([_vp, _vp, _vp] + [_i] * 8 + [_vp], _i),
@@ -236,6 +242,10 @@ class Network:
     def capture(self, dev_ptr=None):
         check(lib().pir_net_capture(self.h, dev_ptr))
 
+    def autotune(self):
+        """One eager forward with the limb-GEMM autotuner on (see include/pir.h)."""
+        check(lib().pir_net_autotune(self.h))
+
     def replay(self):
         check(lib().pir_net_replay(self.h))
 
@@ -283,3 +293,19 @@ class Network:
             b = np.empty((3, 2, bn.value), np.uint64)
             check(lib().pir_memcpy_d2h(b.ctypes.data, bp.value, b.nbytes, None))
         return w, b
+
+
+def autotune_entries():
+    return lib().pir_autotune_entries()
+
+
+def autotune_clear():
+    check(lib().pir_autotune_clear())
+
+
+def autotune_save(path):
+    check(lib().pir_autotune_save(os.fsencode(path)))
+
+
+def autotune_load(path):
+    check(lib().pir_autotune_load(os.fsencode(path)))

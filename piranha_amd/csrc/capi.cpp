@@ -148,11 +148,7 @@ int pir_limb_gemm(const uint64_t *const *a, int nseg, int M, int K, int a_row_ma
         e.mode = EPI_RAW;
         e.C = C;
         e.tc = tc;
-        int nks = nseg * la.Kp / LIMB_BK;
-        if (split <= 0) split = choose_split(M, N, nks);
-        split = limb_effective_split(nks, split);
-        DevBuf ws(split > 1 ? (size_t)split * M * N * 8 : 0);
-        return limb_gemm(la, packed, N, e, split, (u64 *)ws.get(), (hipStream_t)stream);
+        return limb_run(la, packed, N, e, split, (hipStream_t)stream);
     });
 }
 
@@ -207,10 +203,7 @@ int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int
         e.mode = EPI_RAW;
         e.C = y;
         e.tc = 1;
-        int nks = la.Kp / LIMB_BK;
-        int split = limb_effective_split(nks, choose_split(M, Dout, nks));
-        DevBuf ws(split > 1 ? (size_t)split * M * Dout * 8 : 0);
-        return limb_gemm(la, (const int8_t *)packed.get(), Dout, e, split, (u64 *)ws.get(), (hipStream_t)stream);
+        return limb_run(la, (const int8_t *)packed.get(), Dout, e, 0, (hipStream_t)stream);
     });
 }
 
@@ -787,6 +780,34 @@ int pir_net_profile_gemm(pir_net *n, double *ms, double *ops, int *launches) {
         return (int)OK;
     });
 }
+
+int pir_net_autotune(pir_net *n) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_autotune: invalid arguments");
+        NetScope ns(n);
+        n->ctx.tune = true;
+        int rc = guarded([&]() -> int {
+            n->net->run();
+            return (int)OK;
+        });
+        n->ctx.tune = false;
+        PIR_TRY(rc);
+        PIR_HIP(hipStreamSynchronize(n->ctx.stream));
+        return (int)OK;
+    });
+}
+
+int pir_autotune_set(int on) {
+    ctx().tune = on != 0;
+    return OK;
+}
+int pir_autotune_entries(void) { return limb_tune_entries(); }
+int pir_autotune_clear(void) {
+    limb_tune_clear();
+    return OK;
+}
+int pir_autotune_save(const char *path) { return limb_tune_save(path); }
+int pir_autotune_load(const char *path) { return limb_tune_load(path); }
 
 int pir_net_replay(pir_net *n) {
     PIR_REQUIRE(n && n->exec, "pir_net_replay: no captured graph");

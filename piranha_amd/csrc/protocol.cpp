@@ -224,20 +224,6 @@ int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *pac
     return limb_pack_b(s, packed, S());
 }
 
-// Split-K factor for an M x N product of nks K-steps: enough workgroups for
-// two per CU (512 on 256 CUs; the limb kernel runs two workgroups per CU so
-// one's staging hides under the other's MFMAs), at least 8 K-steps each.
-int choose_split(int M, int N, int nks) {
-    int tiles = ((M + LIMB_BM - 1) / LIMB_BM) * ((N + LIMB_BN - 1) / LIMB_BN);
-    if (tiles >= 512) return 1;
-    int want = (512 + tiles - 1) / tiles;
-    int cap = nks / 8;  // keep at least 8 K-steps per split
-    if (cap < 1) cap = 1;
-    if (want > cap) want = cap;
-    if (want > 64) want = 64;
-    return want < 1 ? 1 : want;
-}
-
 int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
                      int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split) {
     LimbA a{};
@@ -270,11 +256,7 @@ int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int 
     e.tc = tc ? 1 : 0;
     e.trunc_bits = trunc_bits;
     e.bias = bias ? bias->base() : nullptr;
-    int nks = a.nseg * a.Kp / LIMB_BK;
-    if (split <= 0) split = choose_split(M, N, nks);
-    split = limb_effective_split(nks, split);
-    DevBuf ws(split > 1 ? (size_t)split * M * N * sizeof(u64) : 0);
-    return limb_gemm(a, packed, N, e, split, (u64 *)ws.get(), S());
+    return limb_run(a, packed, N, e, split, S());
 }
 
 // RSS.inl:1096-1110: three gemms per party into a scratch accumulator

@@ -70,7 +70,6 @@ RSS<T> &RSS<T>::operator&=(const RSS<T> &r) {
 int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed);
 size_t packed_rss_weights_bytes(int K, int N);
 // Choose a split-K factor for an M x N output with `nks` K-steps.
-int choose_split(int M, int N, int nks);
 // Runs the fused 3-party product + truncation (+ bias) on packed weights.
 // conv == nullptr: dense A (row-major if ta).  out layout row-major if tc.
 struct ConvShape {

@@ -90,11 +90,36 @@ inline size_t limb_b_bytes(int N, int nseg, int Kp) {
 
 int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s);
 
-// Runs the limb GEMM.  `split` > 1 partitions the K-steps across workgroups and
-// reduces the 64-bit partials (exact: addition mod 2^64 commutes); it needs a
-// workspace of split*M*N*8 bytes.
-int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split, u64 *workspace,
-              hipStream_t s);
+// Launch plan of one limb GEMM: workgroup tile (wide: 32x128, only when the
+// packed 64-column B tiles pair up; else 64x64) and split-K factor.
+struct LimbPlan {
+    int wide;
+    int split;
+};
+
+// The plan for this shape: the autotuned entry when one exists, else the
+// heuristic (split_hint > 0 forces that split factor).
+LimbPlan limb_plan(const LimbA &a, int N, int split_hint = 0);
+// Workspace bytes a plan needs (split * M * N * 8 when split > 1).
+size_t limb_workspace_bytes(const LimbPlan &p, int M, int N);
+
+// Runs the limb GEMM with `plan`.  plan.split > 1 partitions the K-steps
+// across workgroups and reduces the 64-bit partials (exact: addition mod 2^64
+// commutes) from `workspace`.
+int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, const LimbPlan &plan,
+              u64 *workspace, hipStream_t s);
+
+// Plan + workspace (context allocator) + launch.  With autotuning switched on
+// in the context (and no graph capture open), an untuned shape first times
+// every candidate plan on the real operands (each run overwrites the same
+// output) and records the fastest in the process-wide table.
+int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split_hint, hipStream_t s);
+
+// Process-wide autotune table.
+int limb_tune_entries();
+void limb_tune_clear();
+int limb_tune_save(const char *path);
+int limb_tune_load(const char *path);
 
 // Plain VALU forms with the reference's layout conventions (gemm.cuh:64-112,
 // conv.cuh:359-401).

@@ -6,6 +6,13 @@
 // exact CDNA4 forms (see ring_gemm.h): a 64-bit VALU kernel and the signed-digit
 // int8-limb MFMA kernel.
 #include "ring_gemm.h"
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "runtime.h"
 
 namespace pir {
@@ -397,8 +404,113 @@ int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &e
     return OK;
 }
 
-int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split, u64 *workspace,
-              hipStream_t s) {
+// ---------------------------------------------------------------------------
+// Launch plans and the autotune table
+
+namespace {
+
+// Heuristic split-K factor: enough workgroups for two per CU (512 on 256 CUs;
+// the kernel runs two workgroups per CU so one's staging hides under the
+// other's MFMAs), at least 8 K-steps each.
+int heuristic_split(int M, int N, int nks) {
+    int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+    if (tiles >= 512) return 1;
+    int want = (512 + tiles - 1) / tiles;
+    int cap = nks / 8;
+    if (cap < 1) cap = 1;
+    if (want > cap) want = cap;
+    if (want > 64) want = 64;
+    return want < 1 ? 1 : want;
+}
+
+bool can_be_wide(int N) { return (((N + LIMB_BN - 1) / LIMB_BN) % 2) == 0; }
+
+std::string shape_key(const LimbA &a, int N) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d", a.M, N, a.K, a.Kp,
+                  a.nseg, a.lda, a.row_major, a.conv, a.H, a.W, a.Cin, a.fh, a.fw, a.ph, a.pw, a.stride, a.OH, a.OW);
+    return buf;
+}
+
+std::mutex g_tune_mu;
+std::map<std::string, LimbPlan> g_tuned;
+
+bool tuned_plan(const std::string &key, LimbPlan *p) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it == g_tuned.end()) return false;
+    *p = it->second;
+    return true;
+}
+
+}  // namespace
+
+LimbPlan limb_plan(const LimbA &a, int N, int split_hint) {
+    const int nks = a.nseg * a.Kp / LIMB_BK;
+    LimbPlan p;
+    if (split_hint <= 0 && tuned_plan(shape_key(a, N), &p)) return p;
+    p.wide = can_be_wide(N) ? 1 : 0;
+    p.split = limb_effective_split(nks, split_hint > 0 ? split_hint : heuristic_split(a.M, N, nks));
+    return p;
+}
+
+size_t limb_workspace_bytes(const LimbPlan &p, int M, int N) {
+    return p.split > 1 ? (size_t)p.split * M * N * sizeof(u64) : 0;
+}
+
+int limb_tune_entries() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return (int)g_tuned.size();
+}
+
+void limb_tune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
+
+// Text format, one shape per line: the 18 integers of the shape key (M N K Kp nseg lda
+// row_major conv H W Cin fh fw ph pw stride OH OW), ':', wide, split.
+int limb_tune_save(const char *path) {
+    PIR_REQUIRE(path, "autotune save: null path");
+    std::FILE *f = std::fopen(path, "w");
+    PIR_REQUIRE(f, "autotune save: cannot open file");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto &kv : g_tuned) std::fprintf(f, "%s : %d %d\n", kv.first.c_str(), kv.second.wide, kv.second.split);
+    std::fclose(f);
+    return OK;
+}
+
+int limb_tune_load(const char *path) {
+    PIR_REQUIRE(path, "autotune load: null path");
+    std::FILE *f = std::fopen(path, "r");
+    PIR_REQUIRE(f, "autotune load: cannot open file");
+    char line[512];
+    std::map<std::string, LimbPlan> got;
+    bool bad = false;
+    while (std::fgets(line, sizeof line, f)) {
+        std::string l(line);
+        size_t c = l.find(':');
+        if (c == std::string::npos) continue;
+        int v[18], wide = 0, split = 0;
+        if (std::sscanf(l.c_str(), "%d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2],
+                        &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13], &v[14],
+                        &v[15], &v[16], &v[17]) != 18 ||
+            std::sscanf(l.c_str() + c + 1, "%d %d", &wide, &split) != 2 || split < 1 || (wide != 0 && wide != 1) ||
+            (wide && !can_be_wide(v[1]))) {
+            bad = true;
+            break;
+        }
+        got[l.substr(0, c - 1)] = LimbPlan{wide, split};
+    }
+    std::fclose(f);
+    PIR_REQUIRE(!bad, "autotune load: malformed entry");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto &kv : got) g_tuned[kv.first] = kv.second;
+    return OK;
+}
+
+int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, const LimbPlan &plan,
+              u64 *workspace, hipStream_t s) {
     PIR_REQUIRE(a.nseg >= 1 && a.nseg <= LIMB_MAX_SEG, "limb_gemm: nseg must be 1..12");
     PIR_REQUIRE(a.Kp % LIMB_BK == 0 && a.Kp >= a.K, "limb_gemm: Kp must be a multiple of 32 >= K");
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.trunc_bits >= 1 && epi.trunc_bits <= 62),
@@ -406,11 +518,11 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.nout >= 1 && epi.nout <= 32), "limb_gemm: bad output share count");
     if (a.M == 0 || N == 0) return OK;
     const int nks = a.nseg * a.Kp / LIMB_BK;
-    split = limb_effective_split(nks, split);
+    const int split = limb_effective_split(nks, plan.split);
     const int per = (nks + split - 1) / split;
     PIR_REQUIRE(split == 1 || workspace, "limb_gemm: split-K needs a workspace");
-    // 32x128 tiles when the packed 64-column tiles pair up, else 64x64
-    const int wide = (((N + LIMB_BN - 1) / LIMB_BN) % 2) == 0;
+    PIR_REQUIRE(!plan.wide || can_be_wide(N), "limb_gemm: the 32x128 tile needs an even number of 64-column tiles");
+    const int wide = plan.wide;
     dim3 grid(wide ? (a.M + 31) / 32 : (a.M + 63) / 64, wide ? (N + 127) / 128 : (N + 63) / 64, split);
     u64 *part = split > 1 ? workspace : nullptr;
     const int amode = a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1);
@@ -442,6 +554,63 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     }
     if (split > 1) PIR_TRY(rss_trunc_epilogue(workspace, a.M, N, split, epi, s));
     return OK;
+}
+
+int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split_hint, hipStream_t s) {
+    if (a.M == 0 || N == 0) return OK;
+    Context &c = ctx();
+    const std::string key = shape_key(a, N);
+    LimbPlan dummy;
+    if (c.tune && split_hint <= 0 && !c.alloc.capturing && !tuned_plan(key, &dummy)) {
+        // candidates: both tile shapes, split factors around the heuristic
+        const int nks = a.nseg * a.Kp / LIMB_BK;
+        const int h = heuristic_split(a.M, N, nks);
+        std::vector<LimbPlan> cand;
+        for (int wide = can_be_wide(N) ? 1 : 0; wide >= 0; wide--)
+            for (int f : {h / 4, h / 2, h, 2 * h, 4 * h}) {
+                if (f < 1 || f > 64) continue;
+                LimbPlan p{wide, limb_effective_split(nks, f)};
+                bool dup = false;
+                for (auto &q : cand) dup |= (q.wide == p.wide && q.split == p.split);
+                if (!dup) cand.push_back(p);
+            }
+        hipEvent_t e0, e1;
+        PIR_HIP(hipEventCreate(&e0));
+        PIR_HIP(hipEventCreate(&e1));
+        LimbPlan best = cand[0];
+        float best_ms = 1e30f;
+        int rc = OK;
+        for (auto &p : cand) {
+            DevBuf ws(limb_workspace_bytes(p, a.M, N));
+            rc = limb_gemm(a, packedB, N, epi, p, (u64 *)ws.get(), s);  // warm-up
+            float ms = 1e30f;
+            for (int r = 0; r < 2 && rc == OK; r++) {
+                if (hipEventRecord(e0, s) != hipSuccess) rc = ERR_HIP;
+                if (rc == OK) rc = limb_gemm(a, packedB, N, epi, p, (u64 *)ws.get(), s);
+                float t = 0;
+                if (rc == OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                 hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+                    rc = ERR_HIP;
+                if (rc == OK && t < ms) ms = t;
+            }
+            if (rc != OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = p;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc != OK) {
+            set_error("limb GEMM autotune: a candidate launch failed");
+            return rc;
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = best;
+    }
+    LimbPlan p = limb_plan(a, N, split_hint);
+    DevBuf ws(limb_workspace_bytes(p, a.M, N));
+    return limb_gemm(a, packedB, N, epi, p, (u64 *)ws.get(), s);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,6 +49,7 @@ struct GemmTimer {
 struct Context {
     hipStream_t stream = nullptr;
     GemmTimer timer;
+    bool tune = false;  // autotune untuned limb GEMM shapes on their first eager launch
     Allocator alloc;
     int mode = MODE_FUSED;
     int algo = ALGO_LIMB;

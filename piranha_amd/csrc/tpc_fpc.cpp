@@ -79,9 +79,7 @@ void gemm_op(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, 
     e.mode = EPI_RAW;
     e.C = C;
     e.tc = tc ? 1 : 0;
-    int split = limb_effective_split(a.Kp / LIMB_BK, choose_split(M, N, a.Kp / LIMB_BK));
-    DevBuf ws(split > 1 ? (size_t)split * M * N * sizeof(u64) : 0);
-    check(limb_gemm(a, (const int8_t *)packed.get(), N, e, split, (u64 *)ws.get(), S()));
+    check(limb_run(a, (const int8_t *)packed.get(), N, e, 0, S()));
 }
 
 struct Geom {
@@ -125,9 +123,7 @@ void conv_op(const u64 *x, const u64 *w, u64 *y, const Geom &g) {
     e.mode = EPI_RAW;
     e.C = y;
     e.tc = 1;
-    int split = limb_effective_split(a.Kp / LIMB_BK, choose_split(M, g.Dout, a.Kp / LIMB_BK));
-    DevBuf ws(split > 1 ? (size_t)split * M * g.Dout * sizeof(u64) : 0);
-    check(limb_gemm(a, (const int8_t *)packed.get(), g.Dout, e, split, (u64 *)ws.get(), S()));
+    check(limb_run(a, (const int8_t *)packed.get(), g.Dout, e, 0, S()));
 }
 
 // One term of a fused product: A segment times sum_t coef_t * B_t.
@@ -186,10 +182,7 @@ void fused_product(const std::vector<Seg> &segs, bool ta, bool tb, const Geom *c
     e.base_mask = base_mask;
     e.trunc_bits = trunc;
     e.tc = conv ? 1 : (tc ? 1 : 0);
-    int nks = nseg * a.Kp / LIMB_BK;
-    int split = limb_effective_split(nks, choose_split(M, N, nks));
-    DevBuf ws(split > 1 ? (size_t)split * M * N * sizeof(u64) : 0);
-    check(limb_gemm(a, (const int8_t *)packed.get(), N, e, split, (u64 *)ws.get(), S()));
+    check(limb_run(a, (const int8_t *)packed.get(), N, e, 0, S()));
 }
 
 int log2_pow2(u64 d) {

@@ -52,3 +52,26 @@ def test_version_and_error_string(libpir):
     libpir.pir_last_error.restype = ctypes.c_char_p
     assert libpir.pir_version() >= 1
     assert isinstance(libpir.pir_last_error(), bytes)
+
+
+def test_autotune_table_save_load_roundtrip(libpir, tmp_path):
+    """The autotune table is host-side: load, count, save and reload a plan file
+    (no GPU needed); malformed entries are rejected."""
+    for name in ("pir_autotune_entries", "pir_autotune_clear", "pir_autotune_save", "pir_autotune_load"):
+        getattr(libpir, name).restype = ctypes.c_int
+    libpir.pir_autotune_save.argtypes = [ctypes.c_char_p]
+    libpir.pir_autotune_load.argtypes = [ctypes.c_char_p]
+    assert libpir.pir_autotune_clear() == 0
+    good = tmp_path / "plans.txt"
+    good.write_text("4096 256 1152 1152 6 0 1 1 8 8 128 3 3 1 1 1 8 8 : 1 2\n"
+                    "65536 64 576 576 6 0 1 1 32 32 64 3 3 1 1 1 32 32 : 0 1\n")
+    assert libpir.pir_autotune_load(str(good).encode()) == 0
+    assert libpir.pir_autotune_entries() == 2
+    out = tmp_path / "saved.txt"
+    assert libpir.pir_autotune_save(str(out).encode()) == 0
+    assert sorted(out.read_text().splitlines()) == sorted(good.read_text().splitlines())
+    bad = tmp_path / "bad.txt"
+    bad.write_text("65536 64 576 576 6 0 1 1 32 32 64 3 3 1 1 1 32 32 : 1 1\n")  # 32x128 tile with N = 64
+    assert libpir.pir_autotune_load(str(bad).encode()) != 0
+    assert libpir.pir_autotune_entries() == 2
+    assert libpir.pir_autotune_clear() == 0 and libpir.pir_autotune_entries() == 0

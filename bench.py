@@ -15,7 +15,8 @@ Also reported on the same line:
   * roofline of the dominant kernel (the limb implicit-GEMM convolution),
     timed per launch with HIP events; its HBM traffic per launch comes from
     the committed rocprofv3 counter summary under profiles/;
-  * hbm_roofline: achieved HBM bandwidth of the fused ReLU and maxpool kernels;
+  * hbm_roofline: achieved HBM bandwidth of the fused ReLU kernel (standalone)
+    and of the maxpool kernels (each with the ReLU fused after it);
   * fp64_error: the reconstructed VGG16 logits against an FP64 forward of the
     same network (weights decoded from the shares);
   * cpu_baseline: the CPU oracle on VGG16 at a small batch, host cores.
@@ -106,29 +107,48 @@ def fp64_error(net, layers, batch, fp, x):
             "rel_err": err / max(float(np.max(np.abs(ref))), 1e-30), "outputs": int(ref.size)}
 
 
-def hbm_roofline(layers, batch, layer_ms):
-    """Achieved HBM bandwidth of the fused elementwise / comparison kernels from
-    per-layer HIP-event times: ReLU 102 B per element (three parties: two 8-byte
-    input and output shares each, two 1-byte derivative shares), maxpool 264 B
-    per window (three parties: 4 input values x 2 shares x 8 B read, 2 x 8 B
-    written, 4 x 2 derivative bytes), against 8 TB/s."""
+def hbm_roofline(P, layers, batch, layer_ms):
+    """Achieved HBM bandwidth of the fused elementwise / comparison kernels.
+
+    relu_kernel: the standalone fused ReLU (pir_rss_relu) on the first VGG16
+    activation's size, 102 B per element for the three parties (two 8-byte
+    input and output shares each, two 1-byte derivative shares).  In the
+    network the ReLU layers run inside the GEMM epilogue or the maxpool kernel
+    before them.  maxpool: the maxpool layers (with the fused ReLU that follows
+    each of them) from per-layer HIP-event times, 264 B per 2x2 window (4 values
+    x 2 shares x 8 B read, 2 x 8 B written, 4 x 2 derivative bytes, three
+    parties) plus 54 B for the fused ReLU's output."""
     out = {}
-    for kind, per in (("relu", 102.0), ("maxpool", 264.0)):
-        byts = ms = 0.0
-        for i, l in enumerate(layers):
-            if l[0] != kind:
-                continue
-            if kind == "relu":
-                units = batch * l[1]
-            else:
-                H, W, C, pool, s = l[1:]
-                units = batch * ((H - pool) // s + 1) * ((W - pool) // s + 1) * C
-            byts += units * per
-            ms += float(layer_ms[i])
-        if ms > 0:
-            gbs = byts / (ms * 1e-3) / 1e9
-            out[kind] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                         "bytes_per_forward": byts, "ms_per_forward": ms}
+    L = P.lib()
+    n = batch * 32 * 32 * 64
+    x = P.to_dev(np.random.default_rng(3).integers(0, 2**64, (3, 2, n), dtype=np.uint64))
+    y, d = P.Buffer((3, 2, n), np.uint64), P.Buffer((3, 2, n), np.uint8)
+    for _ in range(3):
+        P.check(L.pir_rss_relu(x.ptr, n, y.ptr, d.ptr, 1, None))
+    P.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.check(L.pir_rss_relu(x.ptr, n, y.ptr, d.ptr, 1, None))
+    P.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    gbs = 102.0 * n / (ms * 1e-3) / 1e9
+    out["relu_kernel"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                          "elements": n, "ms": ms}
+    del x, y, d
+    byts = mp_ms = 0.0
+    for i, l in enumerate(layers):
+        if l[0] != "maxpool":
+            continue
+        H, W, C, pool, s = l[1:]
+        windows = batch * ((H - pool) // s + 1) * ((W - pool) // s + 1) * C
+        fused = i + 1 < len(layers) and layers[i + 1][0] == "relu"
+        byts += windows * (264.0 + (54.0 if fused else 0.0))
+        mp_ms += float(layer_ms[i])
+    if mp_ms > 0:
+        gbs = byts / (mp_ms * 1e-3) / 1e9
+        out["maxpool"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                          "bytes_per_forward": byts, "ms_per_forward": mp_ms}
     return out
 
 
@@ -301,7 +321,7 @@ def main():
                 "traffic": read_traffic(kname), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
                 "ops_per_launch": k_ops / k_n,
                 "share_of_forward": k_ms / float(layer_ms.sum())}
-    hbm = hbm_roofline(layers, batch, layer_ms)
+    hbm = hbm_roofline(P, layers, batch, layer_ms)
     for k, v in hbm.items():
         log(f"{k}: {v['achieved']:.0f} GB/s ({100 * v['frac']:.1f}% of HBM peak)")
     xin = np.random.default_rng(1).uniform(0.0, 1.0, batch * C.input_size(layers))  # make_net's input

@@ -183,7 +183,8 @@ int pir_net_weights(pir_net *net, int layer, const uint64_t **shares, size_t *n,
 int pir_net_capture(pir_net *net, const uint64_t *input_shares);
 int pir_net_replay(pir_net *net);
 /* one eager forward with a HIP event after each layer on the network's stream;
- * ms[i] = device time of layer i (count = number of layers) */
+ * ms[i] = device time of layer i (count = number of layers).  In the fused form
+ * a ReLU layer is computed inside the layer before it and reports ~0 ms. */
 int pir_net_profile_layers(pir_net *net, float *ms, int count);
 /* one eager forward with HIP events around every limb GEMM kernel launch on
  * the network's stream; per kernel variant v: ms[v] = summed kernel time,

@@ -734,11 +734,14 @@ int pir_net_profile_layers(pir_net *n, float *ms, int count) {
         std::vector<hipEvent_t> ev(count + 1);
         for (auto &e : ev) PIR_HIP(hipEventCreate(&e));
         PIR_HIP(hipEventRecord(ev[0], s));
-        auto &L = n->net->layers;
-        for (int i = 0; i < count; i++) {
-            L[i]->forward(i == 0 ? n->net->input : *L[i - 1]->getActivation());
-            PIR_HIP(hipEventRecord(ev[i + 1], s));
-        }
+        // the same schedule as a forward (a ReLU fused into the layer before it
+        // gets the event right after its producer's, i.e. ~0 ms)
+        hipError_t rec = hipSuccess;
+        n->net->run([&](size_t i) {
+            hipError_t e = hipEventRecord(ev[i + 1], s);
+            if (rec == hipSuccess) rec = e;
+        });
+        PIR_HIP(rec);
         PIR_HIP(hipEventSynchronize(ev[count]));
         for (int i = 0; i < count; i++) PIR_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
         for (auto &e : ev) (void)hipEventDestroy(e);

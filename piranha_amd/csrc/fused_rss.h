@@ -22,8 +22,11 @@ template <typename U>
 int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s);
 template <typename U>
 int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s);
+// relu_out / relu_d (optional): the following ReLU layer computed on the pooled
+// values in the same pass (ReLU output shares and dReLU bits, [3][2][outN]).
 template <typename U>
-int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s);
+int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s,
+                        u64 *relu_out = nullptr, u8 *relu_d = nullptr);
 template <typename T>
 int fused_reconstruct(const T *x, size_t n, T *out, hipStream_t s);
 int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s);

@@ -181,7 +181,9 @@ int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipSt
 // tournament and keeps the first f*f derivative entries (truncate_cols).
 template <typename U, int K>
 __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
-                                                                 u64 *__restrict__ act, U *__restrict__ mprime) {
+                                                                 u64 *__restrict__ act, U *__restrict__ mprime,
+                                                                 u64 *__restrict__ relu_out,
+                                                                 u8 *__restrict__ relu_d) {
     const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
     size_t o = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (o >= outN) return;
@@ -208,6 +210,14 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
     }
     tournament<U, K>(e, d);
     store3(act, outN, o, e[0]);
+    if (relu_out) {  // the following ReLU layer on the same element
+        Sh3<u8> rb = drelu_<u8>(e[0]);
+        store3(relu_d, outN, o, rb);
+        Sh3<u64> zero;
+#pragma unroll
+        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+        store3(relu_out, outN, o, select_<u8>(zero, e[0], rb));
+    }
     const size_t mpN = outN * f2;
 #pragma unroll
     for (int j = 0; j < K; j++)
@@ -215,7 +225,9 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
 }
 
 template <typename U>
-int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s) {
+int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s, u64 *relu_out,
+                        u8 *relu_d) {
+    PIR_REQUIRE(!relu_out == !relu_d, "maxpool layer: a fused ReLU needs both outputs");
     PIR_REQUIRE(g.f >= 2 && g.stride >= 1 && g.H >= g.f && g.W >= g.f, "maxpool layer: invalid geometry");
     PIR_REQUIRE(g.OH == (g.H - g.f) / g.stride + 1 && g.OW == (g.W - g.f) / g.stride + 1,
                 "maxpool layer: inconsistent output size");
@@ -223,8 +235,8 @@ int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime,
     if (outN == 0) return OK;
     int k = expanded_pool_size(g.f);
     switch (k) {
-        case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime); break;
-        case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime); break;
+        case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d); break;
+        case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d); break;
         default: set_error("maxpool layer: only 2x2 and 3x3 pools are supported"); return ERR_UNSUPPORTED;
     }
     PIR_LAUNCH_CHECK();
@@ -412,7 +424,7 @@ template int fused_select<u8>(const u64 *, const u64 *, const u8 *, u64 *, size_
 template int fused_select<u64>(const u64 *, const u64 *, const u64 *, u64 *, size_t, hipStream_t);
 template int fused_maxpool_pools<u8>(u64 *, size_t, u64 *, u8 *, int, hipStream_t);
 template int fused_maxpool_pools<u64>(u64 *, size_t, u64 *, u64 *, int, hipStream_t);
-template int fused_maxpool_layer<u8>(const u64 *, const MaxpoolGeom &, u64 *, u8 *, hipStream_t);
+template int fused_maxpool_layer<u8>(const u64 *, const MaxpoolGeom &, u64 *, u8 *, hipStream_t, u64 *, u8 *);
 template int fused_reconstruct<u64>(const u64 *, size_t, u64 *, hipStream_t);
 template int fused_reconstruct<u8>(const u8 *, size_t, u8 *, hipStream_t);
 template int fused_rss_and<u8>(u8 *, const u8 *, size_t, hipStream_t);

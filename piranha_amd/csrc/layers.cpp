@@ -39,18 +39,30 @@ void FCLayer::printLayer() {
            layerNum + 1, conf.inputDim, conf.outputDim, conf.batchSize);
 }
 
+// matmul + dividePublic3out3 + elementVectorAdd of both bias shares (and the
+// next ReLU layer when relu_out is given), one limb-GEMM launch
+void FCLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
+    const int M = (int)conf.batchSize, N = (int)conf.outputDim, K = (int)conf.inputDim;
+    if (!packed_valid) {
+        packed = DevBuf(packed_rss_weights_bytes(K, N));
+        check(pack_rss_weights(weights, K, N, true, (int8_t *)packed.get()));
+        packed_valid = true;
+    }
+    check(rss_limb_product(input, true, nullptr, M, N, K, (const int8_t *)packed.get(), ctx().frac_bits, &biases,
+                           activations, true, 0, relu_out, relu_d));
+}
+
+bool FCLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
+    if (!(fused_limb() && input.base() && relu.activations.base() && relu.reluPrime.base())) return false;
+    limbForward(input, relu.activations.base(), relu.reluPrime.base());
+    return true;
+}
+
 void FCLayer::forward(const RSS64 &input) {
     const int M = (int)conf.batchSize, N = (int)conf.outputDim, K = (int)conf.inputDim;
     const int fp = ctx().frac_bits;
     if (fused_limb() && input.base()) {
-        if (!packed_valid) {
-            packed = DevBuf(packed_rss_weights_bytes(K, N));
-            check(pack_rss_weights(weights, K, N, true, (int8_t *)packed.get()));
-            packed_valid = true;
-        }
-        // matmul + dividePublic3out3 + elementVectorAdd of both bias shares, one kernel
-        check(rss_limb_product(input, true, nullptr, M, N, K, (const int8_t *)packed.get(), fp, &biases, activations,
-                               true));
+        limbForward(input, nullptr, nullptr);
         return;
     }
     activations.zero();
@@ -89,20 +101,33 @@ void CNNLayer::printLayer() {
            conf.stride, conf.padding, conf.batchSize, outW(), outH(), conf.outputFeatures);
 }
 
+// implicit-GEMM convolution + dividePublic3out3 (and the next ReLU layer when
+// relu_out is given), one limb-GEMM launch
+void CNNLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
+    const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
+    const int K = f * f * Din;
+    if (!packed_valid) {
+        packed = DevBuf(packed_rss_weights_bytes(K, Dout));
+        check(pack_rss_weights(weights, K, Dout, false, (int8_t *)packed.get()));
+        packed_valid = true;
+    }
+    ConvShape cs{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, Din, f, (int)conf.stride,
+                 (int)conf.padding, outH(), outW()};
+    check(rss_limb_product(input, true, &cs, cs.batch * cs.OH * cs.OW, Dout, K, (const int8_t *)packed.get(),
+                           ctx().frac_bits, nullptr, activations, true, 0, relu_out, relu_d));
+}
+
+bool CNNLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
+    if (!(fused_limb() && input.base() && relu.activations.base() && relu.reluPrime.base())) return false;
+    limbForward(input, relu.activations.base(), relu.reluPrime.base());
+    return true;
+}
+
 void CNNLayer::forward(const RSS64 &input) {
     const int fp = ctx().frac_bits;
     const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
     if (fused_limb() && input.base()) {
-        const int K = f * f * Din;
-        if (!packed_valid) {
-            packed = DevBuf(packed_rss_weights_bytes(K, Dout));
-            check(pack_rss_weights(weights, K, Dout, false, (int8_t *)packed.get()));
-            packed_valid = true;
-        }
-        ConvShape cs{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, Din, f, (int)conf.stride,
-                     (int)conf.padding, outH(), outW()};
-        check(rss_limb_product(input, true, &cs, cs.batch * cs.OH * cs.OW, Dout, K, (const int8_t *)packed.get(), fp,
-                               nullptr, activations, true));
+        limbForward(input, nullptr, nullptr);
         return;
     }
     activations.zero();
@@ -140,6 +165,17 @@ void MaxpoolLayer::printLayer() {
     printf("----------------------------------------------\n(%d) Maxpool Layer\t  %zu x %zu x %zu\n\t\t\t  %zu  \t\t(Pooling Size)\n"
            "\t\t\t  %zu \t\t(Stride)\n\t\t\t  %zu\t\t(Batch Size)\n",
            layerNum + 1, conf.imageHeight, conf.imageWidth, conf.features, conf.poolSize, conf.stride, conf.batchSize);
+}
+
+bool MaxpoolLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
+    if (!(ctx().mode == MODE_FUSED && input.base() && relu.activations.base() && relu.reluPrime.base()))
+        return false;
+    const u64 pad_value = (u64)(s64)(-10LL * (1LL << ctx().frac_bits));
+    MaxpoolGeom g{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, (int)conf.features,
+                  (int)conf.poolSize, (int)conf.stride, outH(), outW(), pad_value};
+    check(fused_maxpool_layer<u8>(input.base(), g, activations.base(), maxPrime.base(), ctx().stream,
+                                  relu.activations.base(), relu.reluPrime.base()));
+    return true;
 }
 
 void MaxpoolLayer::forward(const RSS64 &input) {
@@ -201,10 +237,18 @@ void NeuralNetwork::forward(const std::vector<double> &data) {
     run();
 }
 
-void NeuralNetwork::run() {
-    if (layers.empty()) return;
-    layers[0]->forward(input);
-    for (size_t i = 1; i < layers.size(); i++) layers[i]->forward(*layers[i - 1]->getActivation());
+void NeuralNetwork::run(const std::function<void(size_t)> &after_layer) {
+    for (size_t i = 0; i < layers.size(); i++) {
+        const RSS64 &in = i == 0 ? input : *layers[i - 1]->getActivation();
+        ReLULayer *next = i + 1 < layers.size() ? dynamic_cast<ReLULayer *>(layers[i + 1].get()) : nullptr;
+        if (next && next->inputSize() == layers[i]->getActivation()->size() && layers[i]->forwardWithReLU(in, *next)) {
+            if (after_layer) after_layer(i);
+            i++;
+        } else {
+            layers[i]->forward(in);
+        }
+        if (after_layer) after_layer(i);
+    }
 }
 
 void NeuralNetwork::printNetwork() {

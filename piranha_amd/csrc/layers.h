@@ -3,6 +3,7 @@
 // ReLULayer.cu, MaxpoolLayer.cu; NeuralNetwork.cu:32-133).
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -33,6 +34,8 @@ struct MaxpoolConfig {
         : imageHeight(h), imageWidth(w), features(c), poolSize(pool), stride(s), batchSize(b) {}
 };
 
+class ReLULayer;
+
 class Layer {
   public:
     int layerNum = 0;
@@ -48,6 +51,10 @@ class Layer {
     virtual size_t inputSize() const = 0;
     // Called after the weights were written (invalidates packed forms).
     virtual void weightsChanged() {}
+    // Fused form only: forward() plus the ReLU layer that follows, in one pass
+    // (the ReLU layer's activations and reluPrime are written here, bit for bit
+    // what its own forward would write).  Returns false when this layer cannot.
+    virtual bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) { return false; }
 };
 
 class FCLayer : public Layer {
@@ -55,6 +62,7 @@ class FCLayer : public Layer {
     FCLayer(const FCConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS64 *getWeights() override { return &weights; }
     RSS64 *getBiases() override { return &biases; }
@@ -65,6 +73,7 @@ class FCLayer : public Layer {
     RSS64 activations, weights, biases;
 
   private:
+    void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d);
     void initialize(int seed);
     DevBuf packed;
     bool packed_valid = false;
@@ -75,6 +84,7 @@ class CNNLayer : public Layer {
     CNNLayer(const CNNConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS64 *getWeights() override { return &weights; }
     size_t inputSize() const override {
@@ -88,6 +98,7 @@ class CNNLayer : public Layer {
     RSS64 weights, activations;
 
   private:
+    void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d);
     void initialize(int seed);
     DevBuf packed;
     bool packed_valid = false;
@@ -112,6 +123,7 @@ class MaxpoolLayer : public Layer {
     MaxpoolLayer(const MaxpoolConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS<u8> *getDerivative() override { return &maxPrime; }
     size_t inputSize() const override {
@@ -136,8 +148,10 @@ class NeuralNetwork {
     ~NeuralNetwork();
     // NeuralNetwork::forward, NeuralNetwork.cu:108-133
     void forward(const std::vector<double> &data);
-    // forward from the input shares already held in `input`
-    void run();
+    // forward from the input shares already held in `input`; in the fused form a
+    // ReLU layer is evaluated inside the layer before it.  after_layer(i), when
+    // given, is called once every layer i has been issued (in order).
+    void run(const std::function<void(size_t)> &after_layer = nullptr);
     void printNetwork();
     std::vector<std::unique_ptr<Layer>> layers;
     RSS64 input;

@@ -225,7 +225,8 @@ int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *pac
 }
 
 int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
-                     int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split) {
+                     int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split, u64 *relu_out,
+                     u8 *relu_d) {
     LimbA a{};
     for (int p = 0; p < 3; p++)
         for (int i = 0; i < 2; i++) a.seg[2 * p + i] = A.getShare(p, i)->data();
@@ -256,6 +257,8 @@ int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int 
     e.tc = tc ? 1 : 0;
     e.trunc_bits = trunc_bits;
     e.bias = bias ? bias->base() : nullptr;
+    e.relu_out = relu_out;
+    e.relu_d = relu_d;
     return limb_run(a, packed, N, e, split, S());
 }
 

@@ -75,7 +75,10 @@ size_t packed_rss_weights_bytes(int K, int N);
 struct ConvShape {
     int batch, H, W, Cin, f, stride, pad, OH, OW;
 };
+// relu_out / relu_d (optional, contiguous [3][2][M*N]): the following ReLU
+// layer evaluated on the product's output in the same epilogue.
 int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
-                     int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split = 0);
+                     int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split = 0, u64 *relu_out = nullptr,
+                     u8 *relu_d = nullptr);
 
 }  // namespace pir

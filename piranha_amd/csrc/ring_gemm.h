@@ -61,6 +61,11 @@ struct Epilogue {
     unsigned base_mask;
     int trunc_bits;
     const u64 *bias;  // [nout][N] or null
+    // Optional ReLU of the output (ReLU layer fused into its producer; the six
+    // RSS arrays only, nout == 6): relu_out [3][2][M*N] = x * [x >= 0] and
+    // relu_d [3][2][M*N] = dReLU bits (RSS.inl:806-865), both in out's layout.
+    u64 *relu_out;
+    u8 *relu_d;
 };
 
 constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "rss_device.h"
 #include "runtime.h"
 
 namespace pir {
@@ -74,6 +75,24 @@ __device__ __forceinline__ void trunc_store(const Epilogue &e, size_t MN, int N,
     u64 q = trunc_div_pow2(raw - ((u64)1 << k), k);
     u64 base = (u64)1 + q;
     size_t idx = e.tc ? (size_t)m * N + n : (size_t)n * (MN / N) + m;
+    if (e.relu_out) {
+        // the following ReLU layer on the same element: dReLU + selectShare
+        Sh3<u64> x;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            u64 v = ((e.base_mask >> j) & 1u) ? base : (u64)0;
+            if (e.bias) v += e.bias[(size_t)j * N + n];
+            e.out[(size_t)j * MN + idx] = v;
+            x.s[j >> 1][j & 1] = v;
+        }
+        Sh3<u8> b = drelu_<u8>(x);
+        store3(e.relu_d, MN, idx, b);
+        Sh3<u64> zero;
+#pragma unroll
+        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+        store3(e.relu_out, MN, idx, select_<u8>(zero, x, b));
+        return;
+    }
     for (int j = 0; j < e.nout; j++) {
         u64 v = ((e.base_mask >> j) & 1u) ? base : (u64)0;
         if (e.bias) v += e.bias[(size_t)j * N + n];
@@ -516,6 +535,8 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.trunc_bits >= 1 && epi.trunc_bits <= 62),
                 "limb_gemm: truncation bits must be in [1, 62]");
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.nout >= 1 && epi.nout <= 32), "limb_gemm: bad output share count");
+    PIR_REQUIRE(!epi.relu_out || (epi.mode == EPI_TRUNC && epi.nout == 6 && epi.relu_d),
+                "limb_gemm: a fused ReLU needs the six RSS output arrays");
     if (a.M == 0 || N == 0) return OK;
     const int nks = a.nseg * a.Kp / LIMB_BK;
     const int split = limb_effective_split(nks, plan.split);

@@ -157,12 +157,7 @@ int pir_gemm(int M, int N, int K, const uint64_t *A, int ta, const uint64_t *B, 
     if (algo == PIR_ALGO_PLAIN) return plain_gemm(M, N, K, A, ta, B, tb, C, tc, (hipStream_t)stream);
     return guarded([&]() -> int {
         Scope sc(stream);
-        PIR_REQUIRE(M >= 0 && N >= 0 && K > 0, "pir_gemm: invalid shape");
-        DevBuf packed(pir_limb_packed_bytes(K, N, 1));
-        const uint64_t *b0[1] = {B};
-        PIR_TRY(pir_limb_pack(b0, nullptr, 1, K, N, tb, (int8_t *)packed.get(), stream));
-        const uint64_t *a0[1] = {A};
-        return pir_limb_gemm(a0, 1, M, K, ta, (const int8_t *)packed.get(), N, C, tc, 0, stream);
+        return limb_ring_gemm(M, N, K, A, ta != 0, B, tb != 0, C, tc != 0, (hipStream_t)stream);
     });
 }
 
@@ -173,37 +168,7 @@ int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int
     return guarded([&]() -> int {
         Scope sc(stream);
         PIR_REQUIRE(dilation == 1, "pir_conv_fprop: the limb form supports dilation 1");
-        PIR_REQUIRE(b > 0 && H > 0 && W > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 && stride > 0,
-                    "pir_conv_fprop: invalid geometry");
-        int OH = (H + 2 * ph - fh) / stride + 1, OW = (W + 2 * pw - fw) / stride + 1;
-        PIR_REQUIRE(OH > 0 && OW > 0, "pir_conv_fprop: empty output");
-        int K = fh * fw * Din, M = b * OH * OW;
-        DevBuf packed(pir_limb_packed_bytes(K, Dout, 1));
-        const uint64_t *b0[1] = {w};
-        PIR_TRY(pir_limb_pack(b0, nullptr, 1, K, Dout, 0, (int8_t *)packed.get(), stream));
-        LimbA la{};
-        la.seg[0] = x;
-        la.nseg = 1;
-        la.M = M;
-        la.K = K;
-        la.Kp = round_up(K, LIMB_BK);
-        la.conv = 1;
-        la.row_major = 1;
-        la.H = H;
-        la.W = W;
-        la.Cin = Din;
-        la.fh = fh;
-        la.fw = fw;
-        la.ph = ph;
-        la.pw = pw;
-        la.stride = stride;
-        la.OH = OH;
-        la.OW = OW;
-        Epilogue e{};
-        e.mode = EPI_RAW;
-        e.C = y;
-        e.tc = 1;
-        return limb_run(la, (const int8_t *)packed.get(), Dout, e, 0, (hipStream_t)stream);
+        return limb_ring_conv(x, w, y, b, H, W, Din, Dout, fh, fw, ph, pw, stride, (hipStream_t)stream);
     });
 }
 

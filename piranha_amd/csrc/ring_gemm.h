@@ -120,6 +120,14 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
 // output) and records the fastest in the process-wide table.
 int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, int split_hint, hipStream_t s);
 
+// gpu::gemm (gemm.cuh:64-112) and gpu::conv_fprop (conv.cuh:380-401) in the
+// limb form: B packed into a temporary, plan, launch with a raw epilogue.
+// Transpose flags as in gpu::gemm (set = row-major storage).
+int limb_ring_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc,
+                   hipStream_t s);
+int limb_ring_conv(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
+                   int ph, int pw, int stride, hipStream_t s);
+
 // Process-wide autotune table.
 int limb_tune_entries();
 void limb_tune_clear();

@@ -634,6 +634,77 @@ int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, 
     return limb_gemm(a, packedB, N, epi, p, (u64 *)ws.get(), s);
 }
 
+int limb_ring_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc,
+                   hipStream_t s) {
+    PIR_REQUIRE(M >= 0 && N >= 0 && K > 0, "limb gemm: invalid shape");
+    LimbBSrc src{};
+    src.b[0][0] = B;
+    src.coef[0][0] = 1;
+    src.nseg = 1;
+    src.K = K;
+    src.N = N;
+    src.Kp = round_up(K, LIMB_BK);
+    src.row_major = tb ? 1 : 0;
+    DevBuf packed(limb_b_bytes(N, 1, src.Kp));
+    PIR_TRY(limb_pack_b(src, (int8_t *)packed.get(), s));
+    LimbA a{};
+    a.seg[0] = A;
+    a.nseg = 1;
+    a.M = M;
+    a.K = K;
+    a.Kp = src.Kp;
+    a.lda = ta ? K : M;
+    a.row_major = ta ? 1 : 0;
+    Epilogue e{};
+    e.mode = EPI_RAW;
+    e.C = C;
+    e.tc = tc ? 1 : 0;
+    return limb_run(a, (const int8_t *)packed.get(), N, e, 0, s);
+}
+
+int limb_ring_conv(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
+                   int ph, int pw, int stride, hipStream_t s) {
+    PIR_REQUIRE(b > 0 && H > 0 && W > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 && stride > 0 && ph >= 0 &&
+                    pw >= 0,
+                "limb conv: invalid geometry");
+    const int OH = (H + 2 * ph - fh) / stride + 1, OW = (W + 2 * pw - fw) / stride + 1;
+    PIR_REQUIRE(H + 2 * ph >= fh && W + 2 * pw >= fw && OH > 0 && OW > 0, "limb conv: empty output");
+    const int K = fh * fw * Din, M = b * OH * OW;
+    LimbBSrc src{};
+    src.b[0][0] = w;
+    src.coef[0][0] = 1;
+    src.nseg = 1;
+    src.K = K;
+    src.N = Dout;
+    src.Kp = round_up(K, LIMB_BK);
+    src.row_major = 0;  // KRSC weights: B(k, n) = w[n * K + k]
+    DevBuf packed(limb_b_bytes(Dout, 1, src.Kp));
+    PIR_TRY(limb_pack_b(src, (int8_t *)packed.get(), s));
+    LimbA a{};
+    a.seg[0] = x;
+    a.nseg = 1;
+    a.M = M;
+    a.K = K;
+    a.Kp = src.Kp;
+    a.conv = 1;
+    a.row_major = 1;
+    a.H = H;
+    a.W = W;
+    a.Cin = Din;
+    a.fh = fh;
+    a.fw = fw;
+    a.ph = ph;
+    a.pw = pw;
+    a.stride = stride;
+    a.OH = OH;
+    a.OW = OW;
+    Epilogue e{};
+    e.mode = EPI_RAW;
+    e.C = y;
+    e.tc = 1;
+    return limb_run(a, (const int8_t *)packed.get(), Dout, e, 0, s);
+}
+
 // ---------------------------------------------------------------------------
 // Plain 64-bit VALU GEMM: 64x64 tile, K-step 16, 4x4 outputs per thread.
 

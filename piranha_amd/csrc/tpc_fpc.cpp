@@ -53,33 +53,8 @@ Ops ops(int op) { return Ops(op); }
 // One ring product C = A.B mod 2^64 of the op-by-op form: gpu::gemm
 // (gemm.cuh:64-112) with the plain VALU kernel, or the limb kernel.
 void gemm_op(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc) {
-    if (ctx().algo == ALGO_PLAIN) {
-        check(plain_gemm(M, N, K, A, ta, B, tb, C, tc, S()));
-        return;
-    }
-    LimbBSrc s{};
-    s.b[0][0] = B;
-    s.coef[0][0] = 1;
-    s.nseg = 1;
-    s.K = K;
-    s.N = N;
-    s.Kp = round_up(K, LIMB_BK);
-    s.row_major = tb ? 1 : 0;
-    DevBuf packed(limb_b_bytes(N, 1, s.Kp));
-    check(limb_pack_b(s, (int8_t *)packed.get(), S()));
-    LimbA a{};
-    a.seg[0] = A;
-    a.nseg = 1;
-    a.M = M;
-    a.K = K;
-    a.Kp = s.Kp;
-    a.lda = ta ? K : M;
-    a.row_major = ta ? 1 : 0;
-    Epilogue e{};
-    e.mode = EPI_RAW;
-    e.C = C;
-    e.tc = tc ? 1 : 0;
-    check(limb_run(a, (const int8_t *)packed.get(), N, e, 0, S()));
+    if (ctx().algo == ALGO_PLAIN) check(plain_gemm(M, N, K, A, ta, B, tb, C, tc, S()));
+    else check(limb_ring_gemm(M, N, K, A, ta, B, tb, C, tc, S()));
 }
 
 struct Geom {
@@ -88,42 +63,10 @@ struct Geom {
 
 // gpu::conv_fprop (conv.cuh:380-401) of the op-by-op form.
 void conv_op(const u64 *x, const u64 *w, u64 *y, const Geom &g) {
-    if (ctx().algo == ALGO_PLAIN) {
+    if (ctx().algo == ALGO_PLAIN)
         check(plain_conv_fprop(x, w, y, g.batch, g.H, g.W, g.Din, g.Dout, g.f, g.f, g.pad, g.pad, g.stride, 1, S()));
-        return;
-    }
-    int K = g.f * g.f * g.Din, M = g.batch * g.OH * g.OW;
-    LimbBSrc s{};
-    s.b[0][0] = w;
-    s.coef[0][0] = 1;
-    s.nseg = 1;
-    s.K = K;
-    s.N = g.Dout;
-    s.Kp = round_up(K, LIMB_BK);
-    s.row_major = 0;
-    DevBuf packed(limb_b_bytes(g.Dout, 1, s.Kp));
-    check(limb_pack_b(s, (int8_t *)packed.get(), S()));
-    LimbA a{};
-    a.seg[0] = x;
-    a.nseg = 1;
-    a.M = M;
-    a.K = K;
-    a.Kp = s.Kp;
-    a.conv = 1;
-    a.row_major = 1;
-    a.H = g.H;
-    a.W = g.W;
-    a.Cin = g.Din;
-    a.fh = a.fw = g.f;
-    a.ph = a.pw = g.pad;
-    a.stride = g.stride;
-    a.OH = g.OH;
-    a.OW = g.OW;
-    Epilogue e{};
-    e.mode = EPI_RAW;
-    e.C = y;
-    e.tc = 1;
-    check(limb_run(a, (const int8_t *)packed.get(), g.Dout, e, 0, S()));
+    else
+        check(limb_ring_conv(x, w, y, g.batch, g.H, g.W, g.Din, g.Dout, g.f, g.f, g.pad, g.pad, g.stride, S()));
 }
 
 // One term of a fused product: A segment times sum_t coef_t * B_t.

@@ -157,6 +157,7 @@ __global__ __launch_bounds__(BLK) void expand_compare_kernel(const U *b, const U
 }
 
 int expand_compare(const void *b, const void *inv, void *out, size_t n, size_t out_n, int eb, hipStream_t s) {
+    if (n == 0 && out_n == 0) return OK;  // empty ranges: nothing to expand
     PIR_REQUIRE(n > 0 && out_n % (2 * n) == 0, "expandCompare: expanded size must be a multiple of 2*b.size");
     size_t f = (out_n / n) / 2;
     if (eb == 1) hipLaunchKernelGGL(expand_compare_kernel<u8>, dim3(g1(out_n)), dim3(BLK), 0, s, (const u8 *)b, (const u8 *)inv, (u8 *)out, n, f);

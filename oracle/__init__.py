@@ -85,6 +85,17 @@ _SIGS = {
     "ref_fpc_reshare": [_u64p, _u64p, _sz, _i, _i, _u64p],
     "ref_fpc_matmul": [_u64p, _sz, _u64p, _sz, _u64p] + [_i] * 7,
     "ref_fpc_convolution": [_u64p, _sz, _u64p, _sz, _u64p, _sz] + [_i] * 9,
+    "ref_conv_dgrad": [_u64p, _u64p, _u64p] + [_i] * 13,
+    "ref_conv_wgrad": [_u64p, _u64p, _u64p] + [_i] * 13,
+    "ref_reduce_sum": [_u64p, _u64p, _i, _i, _i],
+    "ref_vector_expand": [_u64p, _sz, _u64p, _sz],
+    "ref_maxpool_row2im": [_u64p, _sz, _u64p, _sz] + [_i] * 6,
+    "ref_convolution_grad": [_i, _u64p, _sz, _u64p, _sz, _u64p, _sz] + [_i] * 9,
+    "ref_fc_backward": [_u64p] * 5 + [_i] * 5,
+    "ref_cnn_backward": [_u64p] * 4 + [_i] * 10,
+    "ref_relu_backward": [_u64p, _u8p, _sz, _u64p],
+    "ref_maxpool_backward": [_u64p, _u8p, _u64p] + [_i] * 6,
+    "ref_reveal_softmax_grad": [_u64p, _dp, _sz, _u64p, _i, _i, _i],
 }
 _RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_scalar_divide": _u64, "ref_version": _i}
 
@@ -443,6 +454,148 @@ def net_forward(layers, batch, fp, params, x_shares):
         else:
             raise ValueError(kind)
     return out
+
+
+# --- backward (training) path ------------------------------------------------
+
+def conv_dgrad(dy, w, b, OH, OW, Dout, fh, fw, Din, ph, pw, stride, dil, H, W):
+    """gpu::conv_dgrad (conv.cuh:405-446): dx [b][H][W][Din]."""
+    dy, w = _u(dy), _u(w)
+    dx = np.zeros(b * H * W * Din, np.uint64)
+    lib().ref_conv_dgrad(_p(dy), _p(w), _p(dx), b, OH, OW, Dout, fh, fw, Din, ph, pw, stride, dil, H, W)
+    return dx
+
+
+def conv_wgrad(dy, x, b, OH, OW, Dout, H, W, Din, fh, fw, ph, pw, stride, dil):
+    """gpu::conv_wgrad (conv.cuh:448-490): dw [Dout][fh][fw][Din]."""
+    dy, x = _u(dy), _u(x)
+    dw = np.zeros(Dout * fh * fw * Din, np.uint64)
+    lib().ref_conv_wgrad(_p(dy), _p(x), _p(dw), b, OH, OW, Dout, H, W, Din, fh, fw, ph, pw, stride, dil)
+    return dw
+
+
+def reduce_sum(a, b, reduce_rows, rows, cols):
+    """gpu::reduceSum (matrix.cuh:109-126); accumulates into a copy of b."""
+    a, b = _u(a), _u(b).copy()
+    lib().ref_reduce_sum(_p(a), _p(b), int(reduce_rows), rows, cols)
+    return b
+
+
+def vector_expand(a, factor):
+    a = _u(a)
+    b = np.zeros(a.size * factor, np.uint64)
+    lib().ref_vector_expand(_p(a), a.size, _p(b), factor)
+    return b
+
+
+def maxpool_row2im(rows, out, image_height, image_width, f, Din, batch, stride):
+    """gpu::maxpool_row2im (convolution.cuh:110-151); adds into a copy of out."""
+    rows, out = _u(rows), _u(out).copy()
+    lib().ref_maxpool_row2im(_p(rows), rows.size, _p(out), out.size, image_height, image_width, f, Din, batch, stride)
+    return out
+
+
+CONV_DGRAD, CONV_WGRAD = 1, 2
+
+
+def convolution_grad(op, A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
+    """RSS convolution(kDgrad / kWgrad) (RSS.inl:641-790): A = delta, B = filter
+    (dgrad) or forward input (wgrad)."""
+    A, B = _u(A), _u(B)
+    n = batch * H * W * Din if op == CONV_DGRAD else Dout * f * f * Din
+    C = np.zeros((3, 2, n), np.uint64)
+    lib().ref_convolution_grad(op, _p(A), A.shape[-1], _p(B), B.shape[-1], _p(C), n, batch, H, W, f, Din, Dout,
+                               stride, pad, trunc)
+    return C
+
+
+def fc_backward(delta, x, W, b, batch, d_in, d_out, fp, lr):
+    """FCLayer::backward (FCLayer.cu:133-224) -> (deltas, new W, new bias)."""
+    delta, x = _u(delta), _u(x)
+    W, b = _u(W).copy(), _u(b).copy()
+    deltas = np.zeros((3, 2, batch * d_in), np.uint64)
+    lib().ref_fc_backward(_p(delta), _p(x), _p(W), _p(b), _p(deltas), batch, d_in, d_out, fp, lr)
+    return deltas, W, b
+
+
+def cnn_backward(delta, x, W, batch, H, Wd, Din, Dout, f, stride, pad, fp, lr):
+    """CNNLayer::backward (CNNLayer.cu:118-195) -> (deltas, new W)."""
+    delta, x = _u(delta), _u(x)
+    W = _u(W).copy()
+    deltas = np.zeros((3, 2, batch * H * Wd * Din), np.uint64)
+    lib().ref_cnn_backward(_p(delta), _p(x), _p(W), _p(deltas), batch, H, Wd, Din, Dout, f, stride, pad, fp, lr)
+    return deltas, W
+
+
+def relu_backward(delta, relu_prime):
+    """ReLULayer::backward (ReLULayer.cu:91-123)."""
+    delta, rp = _u(delta), _u(relu_prime, np.uint8)
+    n = delta.shape[-1]
+    deltas = np.zeros((3, 2, n), np.uint64)
+    lib().ref_relu_backward(_p(delta), _p(rp), n, _p(deltas))
+    return deltas
+
+
+def maxpool_backward(delta, max_prime, batch, H, W, C, pool, stride):
+    """MaxpoolLayer::backward (MaxpoolLayer.cu:133-180)."""
+    delta, mp = _u(delta), _u(max_prime, np.uint8)
+    deltas = np.zeros((3, 2, batch * H * W * C), np.uint64)
+    lib().ref_maxpool_backward(_p(delta), _p(mp), _p(deltas), batch, H, W, C, pool, stride)
+    return deltas
+
+
+def reveal_softmax_grad(act, labels, n_classes, fp, log_mini_batch):
+    """NeuralNetwork::_reveal_softmax_grad (NeuralNetwork.cu:207-267)."""
+    act = _u(act)
+    labels = np.ascontiguousarray(labels, np.float64).ravel()
+    n = act.shape[-1]
+    assert labels.size == n
+    deltas = np.zeros((3, 2, n), np.uint64)
+    lib().ref_reveal_softmax_grad(_p(act), _p(labels), n, _p(deltas), n_classes, fp, log_mini_batch)
+    return deltas
+
+
+def log_mini_batch(batch):
+    """LOG_MINI_BATCH = log2(MINI_BATCH_SIZE) stored in an int (model.cu:39)."""
+    return int(np.log2(batch))
+
+
+def net_backward(layers, batch, fp, params, x_shares, fwd, labels, lr):
+    """NeuralNetwork::backward (NeuralNetwork.cu:136-149, 435-467) over the
+    oracle layers: reveal-softmax delta, then every layer's backward in
+    reverse order, each with the previous layer's activation (the network
+    input for layer 0).  fwd is net_forward's output for x_shares.
+
+    Returns (deltas, new_params): deltas[i] is layer i's outgoing delta
+    (layer 0 included), new_params the updated (W, b) per layer.
+    """
+    L = len(layers)
+    n_classes = fwd[-1][0].shape[-1] // batch
+    delta = reveal_softmax_grad(fwd[-1][0], labels, n_classes, fp, log_mini_batch(batch))
+    deltas = [None] * L
+    new_params = list(params)
+    for i in range(L - 1, -1, -1):
+        kind = layers[i][0]
+        inp = fwd[i - 1][0] if i > 0 else x_shares
+        if kind == "fc":
+            W, b = params[i]
+            d, W2, b2 = fc_backward(delta, inp, W, b, batch, layers[i][1], layers[i][2], fp, lr)
+            new_params[i] = (W2, b2)
+        elif kind == "cnn":
+            W, _ = params[i]
+            H, Wd, Cin, Cout, f, s, p = layers[i][1:]
+            d, W2 = cnn_backward(delta, inp, W, batch, H, Wd, Cin, Cout, f, s, p, fp, lr)
+            new_params[i] = (W2, None)
+        elif kind == "relu":
+            d = relu_backward(delta, fwd[i][1])
+        elif kind == "maxpool":
+            H, Wd, C, pool, s = layers[i][1:]
+            d = maxpool_backward(delta, fwd[i][1], batch, H, Wd, C, pool, s)
+        else:
+            raise ValueError(kind)
+        deltas[i] = d
+        delta = d
+    return deltas, new_params
 
 
 # --- TPC (2 parties x 1 share) and FPC (4 parties x 3 shares) ----------------

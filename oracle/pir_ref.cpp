@@ -1047,6 +1047,213 @@ void f_convolution(Fp A, Fp B, Fp C, int batch, int H, int W, int f, int Din, in
     f_divide_public(C, (u64)1 << trunc);
 }
 
+
+// ===========================================================================
+// Backward (training) path: gpu::conv_dgrad / conv_wgrad, reduceSum,
+// vectorExpand, maxpool_row2im, RSS convolution(kDgrad / kWgrad), the layer
+// backward passes and NeuralNetwork::_reveal_softmax_grad.
+
+// gpu::conv_dgrad (conv.cuh:405-446): CUTLASS Dgrad of the NHWC x KRSC
+// cross-correlation, dx [b][H][W][Din] = sum over (p, q, k, r, s) with
+// h = p*stride - ph + r*dil, w = q*stride - pw + s*dil of dy[b][p][q][k] * w[k][r][s][c].
+void conv_dgrad(const u64 *dy, const u64 *w, u64 *dx, int b, int OH, int OW, int Dout, int fh, int fw, int Din,
+                int ph, int pw, int stride, int dil, int H, int W) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int n = 0; n < b; n++)
+        for (int h = 0; h < H; h++)
+            for (int x = 0; x < W; x++)
+                for (int c = 0; c < Din; c++) {
+                    u64 acc = 0;
+                    for (int r = 0; r < fh; r++) {
+                        int hp = h + ph - r * dil;
+                        if (hp < 0 || hp % stride) continue;
+                        int p = hp / stride;
+                        if (p >= OH) continue;
+                        for (int s = 0; s < fw; s++) {
+                            int wq = x + pw - s * dil;
+                            if (wq < 0 || wq % stride) continue;
+                            int q = wq / stride;
+                            if (q >= OW) continue;
+                            const u64 *dp = dy + (((size_t)n * OH + p) * OW + q) * Dout;
+                            for (int k = 0; k < Dout; k++) acc += dp[k] * w[(((size_t)k * fh + r) * fw + s) * Din + c];
+                        }
+                    }
+                    dx[(((size_t)n * H + h) * W + x) * Din + c] = acc;
+                }
+}
+
+// gpu::conv_wgrad (conv.cuh:448-490): dw [Dout][fh][fw][Din] =
+// sum over (n, p, q) of dy[n][p][q][k] * x[n][p*stride - ph + r*dil][q*stride - pw + s*dil][c].
+void conv_wgrad(const u64 *dy, const u64 *x, u64 *dw, int b, int OH, int OW, int Dout, int H, int W, int Din,
+                int fh, int fw, int ph, int pw, int stride, int dil) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < Dout; k++)
+        for (int r = 0; r < fh; r++)
+            for (int s = 0; s < fw; s++)
+                for (int c = 0; c < Din; c++) {
+                    u64 acc = 0;
+                    for (int n = 0; n < b; n++)
+                        for (int p = 0; p < OH; p++) {
+                            int h = p * stride - ph + r * dil;
+                            if (h < 0 || h >= H) continue;
+                            for (int q = 0; q < OW; q++) {
+                                int wq = q * stride - pw + s * dil;
+                                if (wq < 0 || wq >= W) continue;
+                                acc += dy[(((size_t)n * OH + p) * OW + q) * Dout + k] *
+                                       x[(((size_t)n * H + h) * W + wq) * Din + c];
+                            }
+                        }
+                    dw[(((size_t)k * fh + r) * fw + s) * Din + c] = acc;
+                }
+}
+
+// gpu::reduceSum (matrix.cuh:109-126, 331-351): accumulates into b.
+template <typename T>
+void reduce_sum(const T *a, T *b, bool reduceRows, int rows, int cols) {
+    int out = reduceRows ? cols : rows;
+    for (int col = 0; col < out; col++)
+        for (int i = 0; i < (reduceRows ? rows : cols); i++)
+            b[col] += reduceRows ? a[(size_t)cols * i + col] : a[(size_t)cols * col + i];
+}
+
+// gpu::vectorExpand (matrix.cuh:128-139, 354-380): b[i*f + j] = a[i].
+template <typename T>
+void vector_expand(View<T> a, View<T> b, size_t factor) {
+    for (size_t i = 0; i < a.n; i++)
+        for (size_t j = 0; j < factor; j++) b[i * factor + j] = a[i];
+}
+
+// gpu::maxpool_row2im (convolution.cuh:110-151, 376-410): every pool entry is
+// added (atomicAdd, exact mod 2^n in any order) into its image position.
+template <typename T>
+void maxpool_row2im(View<T> rows, View<T> output, int imageHeight, int imageWidth, int filterSize, int Din,
+                    int batchSize, int stride) {
+    int widthKernels = ((imageWidth - filterSize) / stride) + 1;
+    int heightKernels = ((imageHeight - filterSize) / stride) + 1;
+    size_t windows = (size_t)batchSize * Din * widthKernels * heightKernels;
+    int f2 = filterSize * filterSize;
+    for (size_t cw = 0; cw < windows; cw++)
+        for (int v = 0; v < f2; v++) {
+            int batch = (int)(cw / ((size_t)widthKernels * heightKernels * Din));
+            int rem = (int)(cw % ((size_t)widthKernels * heightKernels * Din));
+            int windowIdx = rem / Din;
+            int windowCol = windowIdx / heightKernels;
+            int windowRow = windowIdx % heightKernels;
+            int channel = rem % Din;
+            int valCol = windowCol * stride + v / filterSize;
+            int valRow = windowRow * stride + v % filterSize;
+            size_t outputIdx = (size_t)batch * imageHeight * imageWidth * Din + (size_t)valCol * imageHeight * Din +
+                               (size_t)valRow * Din + channel;
+            output[outputIdx] = (T)(output[outputIdx] + rows[cw * f2 + v]);
+        }
+}
+
+// localDgrad / localWgrad + convolution(kDgrad / kWgrad), RSS.inl:641-703, 705-790.
+// A is the incoming delta [b][OH][OW][Dout]; B the filter (dgrad) or the forward
+// input (wgrad).  The output is written into C after dividePublic3out3.
+void convolution_grad(int op, Rs<u64> A, Rs<u64> B, Rs<u64> C, int batch, int H, int W, int f, int Din, int Dout,
+                      int stride, int pad, int trunc) {
+    int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
+    size_t outN = C.size();
+    LocBuf<u64> local(outN);
+    std::vector<u64> acc(outN);
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int q = 0; q < 3; q++)
+        for (auto &t : terms) {
+            const u64 *a = A.s[q][t[0]].base + A.s[q][t[0]].off, *bb = B.s[q][t[1]].base + B.s[q][t[1]].off;
+            if (op == 1)
+                conv_dgrad(a, bb, acc.data(), batch, OH, OW, Dout, f, f, Din, pad, pad, stride, 1, H, W);
+            else
+                conv_wgrad(a, bb, acc.data(), batch, OH, OW, Dout, H, W, Din, f, f, pad, pad, stride, 1);
+            for (size_t k = 0; k < outN; k++) local.d[q][k] += acc[k];
+        }
+    divide_public_3out3(local.v(), (u64)1 << trunc, C);
+}
+
+// FCLayer::backward, FCLayer.cu:133-224.  W [in][out] and bias [out] are
+// updated in place; deltas [batch][in] receives the gradient for the previous
+// layer.  The bias gradient is gpu::reduceSum(delta, db, false, out, batch),
+// which reads delta as [out][batch] rows (reference indexing kept).
+void fc_backward(Rs<u64> delta, Rs<u64> input, Rs<u64> W, Rs<u64> bias, Rs<u64> deltas, int batch, int in, int out,
+                 int fp, int lr) {
+    r_zero(deltas);
+    matmul(delta, W, deltas, batch, in, out, true, false, true, fp);
+    RsBuf<u64> dW((size_t)out * in);
+    matmul(delta, input, dW.v(), out, in, batch, false, true, false, fp + lr);
+    r_op_r(W, dW.v(), SUB);
+    RsBuf<u64> db((size_t)out);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            reduce_sum(delta.s[q][i].base + delta.s[q][i].off, db.d[q][i].data(), false, out, batch);
+    divide_public(db.v(), (u64)1 << lr);
+    r_op_r(bias, db.v(), SUB);
+}
+
+// CNNLayer::backward, CNNLayer.cu:118-195 (the bias update is commented out in
+// the reference).
+void cnn_backward(Rs<u64> delta, Rs<u64> input, Rs<u64> W, Rs<u64> deltas, int batch, int H, int Wd, int Din,
+                  int Dout, int f, int stride, int pad, int fp, int lr) {
+    r_zero(deltas);
+    convolution_grad(1, delta, W, deltas, batch, H, Wd, f, Din, Dout, stride, pad, fp);
+    RsBuf<u64> dF(W.size());
+    convolution_grad(2, delta, input, dF.v(), batch, H, Wd, f, Din, Dout, stride, pad, fp);
+    divide_public(dF.v(), (u64)1 << lr);
+    r_op_r(W, dF.v(), SUB);
+}
+
+// ReLULayer::backward, ReLULayer.cu:91-123
+void relu_backward(Rs<u64> delta, Rs<uint8_t> relu_prime, Rs<u64> deltas) {
+    r_zero(deltas);
+    RsBuf<u64> zeros(delta.size());
+    select_share(zeros.v(), delta, relu_prime, deltas);
+}
+
+// MaxpoolLayer::backward, MaxpoolLayer.cu:133-180.  The layer passes
+// (imageWidth, imageHeight) into row2im's (imageHeight, imageWidth) slots.
+void maxpool_backward(Rs<u64> delta, Rs<uint8_t> max_prime, Rs<u64> deltas, int batch, int H, int W, int C,
+                      int pool, int stride) {
+    r_zero(deltas);
+    RsBuf<u64> deltaPools(max_prime.size());
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) vector_expand(delta.s[q][i], deltaPools.v().s[q][i], (size_t)pool * pool);
+    RsBuf<u64> zeros(deltaPools.size());
+    select_share(zeros.v(), deltaPools.v(), max_prime, deltaPools.v());
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) maxpool_row2im(deltaPools.v().s[q][i], deltas.s[q][i], W, H, pool, C, batch, stride);
+}
+
+// NeuralNetwork::_reveal_softmax_grad, NeuralNetwork.cu:207-267.  Every party
+// opens the logits, computes softmax in FP64 on the host (std::exp, summed in
+// element order per row), converts back with to_fixed_functor (functors.h:116-123)
+// and subtracts the public labels; then dividePublic by 2^LOG_MINI_BATCH when
+// LOG_MINI_BATCH > 1.
+void reveal_softmax_grad(Rs<u64> act, const double *labels, Rs<u64> deltas, int nClasses, int fp, int logMiniBatch) {
+    size_t n = act.size();
+    RsBuf<u64> x(n);
+    r_op_r(x.v(), act, ADD);
+    LocBuf<u64> revealed(n);
+    reconstruct(x.v(), revealed.v());
+    LocBuf<u64> sm(n);
+    const double scale = (double)(1 << fp);
+    for (int q = 0; q < 3; q++) {
+        std::vector<double> fa(n), sums(n / nClasses, 0.0);
+        for (size_t i = 0; i < n; i++) fa[i] = (double)(s64)revealed.d[q][i] / scale;
+        for (size_t i = 0; i < n; i++) {
+            fa[i] = std::exp(fa[i]);
+            sums[i / nClasses] += fa[i];
+        }
+        for (size_t i = 0; i < n; i++) fa[i] /= sums[i / nClasses];
+        // (T)(x * (1 << FLOAT_PRECISION)) for 0 <= x <= 1
+        for (size_t i = 0; i < n; i++) sm.d[q][i] = (u64)(s64)(fa[i] * scale);
+    }
+    RsBuf<u64> labelShare(n);
+    r_set_public(labelShare.v(), labels, fp);
+    r_zero(deltas);
+    r_op_l(deltas, sm.v(), ADD);
+    r_op_r(deltas, labelShare.v(), SUB);
+    if (logMiniBatch > 1) divide_public(deltas, (u64)1 << logMiniBatch);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1276,6 +1483,60 @@ void ref_fpc_convolution(const u64 *A, size_t a_n, const u64 *B, size_t b_n, u64
                          int W, int f, int Din, int Dout, int stride, int pad, int trunc) {
     f_convolution(wrapFp((u64 *)A, a_n), wrapFp((u64 *)B, b_n), wrapFp(C, c_n), batch, H, W, f, Din, Dout, stride,
                   pad, trunc);
+}
+
+// --- backward (training) path --------------------------------------------------
+void ref_conv_dgrad(const u64 *dy, const u64 *w, u64 *dx, int b, int OH, int OW, int Dout, int fh, int fw, int Din,
+                    int ph, int pw, int stride, int dil, int H, int W) {
+    conv_dgrad(dy, w, dx, b, OH, OW, Dout, fh, fw, Din, ph, pw, stride, dil, H, W);
+}
+void ref_conv_wgrad(const u64 *dy, const u64 *x, u64 *dw, int b, int OH, int OW, int Dout, int H, int W, int Din,
+                    int fh, int fw, int ph, int pw, int stride, int dil) {
+    conv_wgrad(dy, x, dw, b, OH, OW, Dout, H, W, Din, fh, fw, ph, pw, stride, dil);
+}
+void ref_reduce_sum(const u64 *a, u64 *b, int reduceRows, int rows, int cols) {
+    reduce_sum(a, b, reduceRows != 0, rows, cols);
+}
+void ref_vector_expand(u64 *a, size_t n, u64 *b, size_t factor) {
+    vector_expand(View<u64>{a, 0, 1, n}, View<u64>{b, 0, 1, n * factor}, factor);
+}
+void ref_maxpool_row2im(u64 *rows, size_t rows_n, u64 *out, size_t out_n, int imageHeight, int imageWidth, int f,
+                        int Din, int batch, int stride) {
+    maxpool_row2im(View<u64>{rows, 0, 1, rows_n}, View<u64>{out, 0, 1, out_n}, imageHeight, imageWidth, f, Din, batch,
+                   stride);
+}
+void ref_convolution_grad(int op, const u64 *A, size_t a_n, const u64 *B, size_t b_n, u64 *C, size_t c_n, int batch,
+                          int H, int W, int f, int Din, int Dout, int stride, int pad, int trunc) {
+    convolution_grad(op, wrapRs((u64 *)A, a_n), wrapRs((u64 *)B, b_n), wrapRs(C, c_n), batch, H, W, f, Din, Dout,
+                     stride, pad, trunc);
+}
+void ref_fc_backward(const u64 *delta, const u64 *input, u64 *W, u64 *bias, u64 *deltas, int batch, int in, int out,
+                     int fp, int lr) {
+    fc_backward(wrapRs((u64 *)delta, (size_t)batch * out), wrapRs((u64 *)input, (size_t)batch * in),
+                wrapRs(W, (size_t)in * out), wrapRs(bias, (size_t)out), wrapRs(deltas, (size_t)batch * in), batch, in,
+                out, fp, lr);
+}
+void ref_cnn_backward(const u64 *delta, const u64 *input, u64 *W, u64 *deltas, int batch, int H, int Wd, int Din,
+                      int Dout, int f, int stride, int pad, int fp, int lr) {
+    int OH = (H + 2 * pad - f) / stride + 1, OW = (Wd + 2 * pad - f) / stride + 1;
+    size_t inN = (size_t)batch * H * Wd * Din;
+    cnn_backward(wrapRs((u64 *)delta, (size_t)batch * OH * OW * Dout), wrapRs((u64 *)input, inN),
+                 wrapRs(W, (size_t)Dout * f * f * Din), wrapRs(deltas, inN), batch, H, Wd, Din, Dout, f, stride, pad,
+                 fp, lr);
+}
+void ref_relu_backward(const u64 *delta, const uint8_t *relu_prime, size_t n, u64 *deltas) {
+    relu_backward(wrapRs((u64 *)delta, n), wrapRs((uint8_t *)relu_prime, n), wrapRs(deltas, n));
+}
+void ref_maxpool_backward(const u64 *delta, const uint8_t *max_prime, u64 *deltas, int batch, int H, int W, int C,
+                          int pool, int stride) {
+    int OH = (H - pool) / stride + 1, OW = (W - pool) / stride + 1;
+    size_t outN = (size_t)batch * C * OH * OW;
+    maxpool_backward(wrapRs((u64 *)delta, outN), wrapRs((uint8_t *)max_prime, outN * pool * pool),
+                     wrapRs(deltas, (size_t)batch * H * W * C), batch, H, W, C, pool, stride);
+}
+void ref_reveal_softmax_grad(const u64 *act, const double *labels, size_t n, u64 *deltas, int nClasses, int fp,
+                             int logMiniBatch) {
+    reveal_softmax_grad(wrapRs((u64 *)act, n), labels, wrapRs(deltas, n), nClasses, fp, logMiniBatch);
 }
 
 }  // extern "C"

@@ -4,7 +4,8 @@ doubles the reference's loadShareFromFile parses, util.cuh:271-289).
 
 Run in a container that has the reference checkout:
     python tests/golden/make_fixtures.py /root/reference
-Only data files are read; no reference code is imported or run.
+Only data files and the numeric brace lists of two GPU.cu known-answer tests
+are read; no reference code is imported, compiled or run.
 """
 import os
 import sys
@@ -38,7 +39,50 @@ FILES = {
     "test_convlayer_input": "conv_input",
     "test_convlayer_weight": "conv_weight",
     "test_convlayer_output": "conv_output",
+    # backward fixtures (Layer.cu:118-180, 379-420, 772-840; files/test/secureml
+    # gradin_layer*: dL/d(output of layer k) of one SecureML training step)
+    "test_fclayer_gradin": "fc_gradin",
+    "test_fclayer_gradout": "fc_gradout",
+    "test_fclayer_newweight": "fc_newweight",
+    "test_fclayer_newbias": "fc_newbias",
+    "test_relulayer_gradin": "relu_gradin",
+    "test_relulayer_gradout": "relu_gradout",
+    "test_maxpoollayer_gradin": "maxpool_gradin",
+    "test_maxpoollayer_gradout": "maxpool_gradout",
+    "test_convlayer_gradin": "conv_gradin",
+    "test_convlayer_gradout": "conv_gradout",
+    "test_convlayer_gradweight": "conv_gradweight",
+    "test_convlayer_newweight": "conv_newweight",
+    "secureml/labels": "secureml_labels",
+    "secureml/gradin_layer0": "secureml_gradin0",
+    "secureml/gradin_layer1": "secureml_gradin1",
+    "secureml/gradin_layer2": "secureml_gradin2",
+    "secureml/gradin_layer3": "secureml_gradin3",
+    "secureml/gradin_layer4": "secureml_gradin4",
+    "secureml/gradin_layer5": "secureml_gradin5",
 }
+
+# Known-answer vectors written inline in the reference's src/test/GPU.cu: the
+# brace lists of the named test are read as numbers (no code is run).
+KATS = {
+    "CutlassConvDgrad2": {"delta": "dgrad2_delta", "filters": "dgrad2_filters", "expected": "dgrad2_expected"},
+    "CutlassConvWgrad2": {"delta": "wgrad2_delta", "input": "wgrad2_input", "expected": "wgrad2_expected"},
+}
+
+
+def kat_arrays(ref):
+    import re
+
+    text = open(os.path.join(ref, "src", "test", "GPU.cu")).read()
+    out = {}
+    for test, names in KATS.items():
+        start = text.index("TYPED_TEST(GPUTest, %s)" % test)
+        end = text.find("TYPED_TEST(", start + 1)
+        block = text[start:end if end > 0 else len(text)]
+        for var, key in names.items():
+            m = re.search(r"\b%s\s*=\s*\{([^}]*)\}" % var, block)
+            out[key] = np.array([float(t) for t in m.group(1).replace(",", " ").split()], np.float64)
+    return out
 
 
 def main(ref):
@@ -47,6 +91,7 @@ def main(ref):
     for rel, key in FILES.items():
         with open(os.path.join(base, rel)) as f:
             arrays[key] = np.array([float(t) for t in f.read().split()], np.float64)
+    arrays.update(kat_arrays(ref))
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures.npz")
     np.savez_compressed(out, **arrays)
     print(out, os.path.getsize(out), "bytes")

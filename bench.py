@@ -43,6 +43,11 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "limb_gemm_conv_vgg16_b64.md")
 
 
+def kernel_args(P, v):
+    """Template arguments of limb_gemm_kernel for a profile_gemm variant."""
+    return f"{v % P.LIMB_WIDE}, {1 if v >= P.LIMB_WIDE else 2}"
+
+
 def read_traffic(kname):
     """HBM bytes per launch of kernel `kname` (e.g. "limb_gemm_kernel<3, 1>") in
     this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
@@ -312,10 +317,10 @@ def main():
     k_ms, k_ops, k_n = prof[dom]
     avg_ms = k_ms / k_n
     achieved = k_ops / k_n / (avg_ms * 1e-3) / 1e12
-    log("limb GEMM kernels: " + ", ".join(f"<{v % 4},{1 if v >= 4 else 2}> {prof[v][2]} launches {prof[v][0]:.3f} ms "
+    log("limb GEMM kernels: " + ", ".join(f"<{kernel_args(P, v)}> {prof[v][2]} launches {prof[v][0]:.3f} ms "
                                           f"{prof[v][1] / prof[v][0] / 1e9:.0f} TOP/s" for v in sorted(prof)))
     log("per-layer ms: " + ", ".join(f"{layers[i][0]}{i}:{m:.3f}" for i, m in enumerate(layer_ms)))
-    kname = f"limb_gemm_kernel<{dom % 4}, {1 if dom >= 4 else 2}>"
+    kname = f"limb_gemm_kernel<{kernel_args(P, dom)}>"
     roofline = {"bound": "mfma", "kernel": f"{kname} (3-party implicit-GEMM conv, int8 limbs)",
                 "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                 "traffic": read_traffic(kname), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,

@@ -53,6 +53,17 @@ int pir_gemm(int M, int N, int K, const uint64_t *A, int transpose_a, const uint
 int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int H, int W, int Din, int Dout,
                    int fh, int fw, int ph, int pw, int stride, int dilation, int algo, void *stream);
 
+/* gpu::conv_dgrad, src/gpu/conv.cuh:405-446: dy[b][OH][OW][Dout] and
+ * w[Dout][fh][fw][Din] -> dx[b][H][W][Din] = the gradient of conv_fprop's input.
+ * The limb form needs dilation 1. */
+int pir_conv_dgrad(const uint64_t *dy, const uint64_t *w, uint64_t *dx, int b, int OH, int OW, int Dout, int fh,
+                   int fw, int Din, int ph, int pw, int stride, int dilation, int H, int W, int algo, void *stream);
+
+/* gpu::conv_wgrad, src/gpu/conv.cuh:448-490: dy[b][OH][OW][Dout] and
+ * x[b][H][W][Din] -> dw[Dout][fh][fw][Din] = the gradient of conv_fprop's filter. */
+int pir_conv_wgrad(const uint64_t *dy, const uint64_t *x, uint64_t *dw, int b, int OH, int OW, int Dout, int H,
+                   int W, int Din, int fh, int fw, int ph, int pw, int stride, int dilation, int algo, void *stream);
+
 /* The RSS local product of one party, localMatMul (src/mpc/RSS.inl:1096-1110):
  * C = A0.B0 + A0.B1 + A1.B0 mod 2^64, all row-major (A: MxK, B: KxN, C: MxN).
  * algo 0 runs the reference's form (three plain GEMMs, one launch each, plus
@@ -87,6 +98,16 @@ int pir_stride_pad(const uint64_t *in, uint64_t *out, size_t n_in, int stride, i
 size_t pir_maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);
 int pir_maxpool_im2row(const uint64_t *im, uint64_t *out, int H, int W, int f, int Din, int batch, int stride,
                        int padding, uint64_t pad_value, void *stream);
+/* gpu::reduceSum, matrix.cuh:109-126, 331-351: b[j] += sum_i (reduce_rows ?
+ * a[cols*i + j] (j < cols) : a[cols*j + i] (j < rows)) */
+int pir_reduce_sum(const uint64_t *a, uint64_t *b, int reduce_rows, int rows, int cols, void *stream);
+/* gpu::vectorExpand, matrix.cuh:128-139, 354-380: b[i*factor + j] = a[i] */
+int pir_vector_expand(const uint64_t *a, size_t n, uint64_t *b, int factor, void *stream);
+/* gpu::maxpool_row2im, convolution.cuh:110-151, 376-410: adds each pool entry
+ * rows[window*f*f + v] into its image position of out (parameter order and the
+ * height / width naming of the reference's host function) */
+int pir_maxpool_row2im(const uint64_t *rows, uint64_t *out, int image_height, int image_width, int f, int Din,
+                       int batch, int stride, void *stream);
 
 /* ---- DeviceData element ops (src/gpu/DeviceData.h:131-209) ----------------- */
 enum { PIR_OP_ADD = 0, PIR_OP_SUB, PIR_OP_MUL, PIR_OP_XOR, PIR_OP_AND, PIR_OP_SHR, PIR_OP_SHL, PIR_OP_DIV };
@@ -114,6 +135,13 @@ int pir_rss_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int
 /* convolution(kFprop), RSS.inl:705-803 */
 int pir_rss_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
                         int Dout, int stride, int pad, int trunc, int algo, void *stream);
+/* convolution(A, B, C, op, ...), RSS.inl:641-803, for cutlass::conv::Operator
+ * kFprop (A = input, B = filter, C = output), kDgrad (A = delta [b][OH][OW][Dout],
+ * B = filter, C = input gradient [b][H][W][Din]) or kWgrad (A = delta, B = forward
+ * input, C = filter gradient [Dout][f][f][Din]); truncation by 2^trunc */
+enum { PIR_CONV_FPROP = 0, PIR_CONV_DGRAD = 1, PIR_CONV_WGRAD = 2 };
+int pir_rss_convolution_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *c, int batch, int H, int W, int f,
+                           int Din, int Dout, int stride, int pad, int trunc, int algo, void *stream);
 int pir_rss_drelu(const uint64_t *x, size_t n, void *result, int elem_bytes, void *stream);  /* RSS.inl:806-852 */
 int pir_rss_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int elem_bytes, void *stream); /* :854-865 */
 int pir_rss_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int elem_bytes,
@@ -179,9 +207,29 @@ int pir_net_activation(pir_net *net, int layer, const uint64_t **shares, size_t 
 int pir_net_derivative(pir_net *net, int layer, const uint8_t **shares, size_t *n);
 int pir_net_weights(pir_net *net, int layer, const uint64_t **shares, size_t *n, const uint64_t **bias_shares,
                     size_t *n_bias);
-/* hipGraph capture of the whole forward (input from pir_net_forward_shares' buffer) */
+/* weights (and FC biases) from host share arrays [3][2][n] */
+int pir_net_set_weight_shares(pir_net *net, int layer, const uint64_t *weight_shares, const uint64_t *bias_shares);
+/* hipGraph capture of the whole forward (input from pir_net_forward_shares' buffer).
+ * A weight change (set_weights, a backward pass) drops the graph; capture again. */
 int pir_net_capture(pir_net *net, const uint64_t *input_shares);
 int pir_net_replay(pir_net *net);
+
+/* ---- training (NeuralNetwork.cu:136-149, 207-267, 435-467) ----------------
+ * log_learning_rate (main.cu:48, default 5): updates divide by 2^log_lr. */
+int pir_net_set_log_learning_rate(pir_net *net, int log_lr);
+/* NeuralNetwork::backward after a forward: labels (batch x classes doubles,
+ * one-hot) become public shares; the reveal-softmax delta, then every layer's
+ * Layer::backward in reverse order with its weight update. */
+int pir_net_backward(pir_net *net, const double *labels, size_t n);
+/* the same with label shares already on the device [3][2][batch*classes] */
+int pir_net_backward_shares(pir_net *net, const uint64_t *label_shares);
+/* Layer::backward(delta, forwardInput) of one layer (FCLayer.cu:133-224,
+ * CNNLayer.cu:118-195, ReLULayer.cu:91-123, MaxpoolLayer.cu:133-180): delta and
+ * forward_input are device shares [3][2][n] of the layer's output / input size */
+int pir_net_layer_backward(pir_net *net, int layer, const uint64_t *delta, const uint64_t *forward_input);
+/* Layer::getDelta: device shares of dL/d(input of layer) after a backward pass;
+ * layer == number of layers gives the reveal-softmax delta (dL/d network output) */
+int pir_net_delta(pir_net *net, int layer, const uint64_t **shares, size_t *n);
 /* one eager forward with a HIP event after each layer on the network's stream;
  * ms[i] = device time of layer i (count = number of layers).  In the fused form
  * a ReLU layer is computed inside the layer before it and reports ~0 ms. */
@@ -189,11 +237,11 @@ int pir_net_profile_layers(pir_net *net, float *ms, int count);
 /* one eager forward with HIP events around every limb GEMM kernel launch on
  * the network's stream; per kernel variant v: ms[v] = summed kernel time,
  * ops[v] = summed algorithmic int8 ops (2*36*M*N*nseg*K), launches[v] = launch
- * count.  v = A-operand form (PIR_LIMB_DENSE_ROW .. PIR_LIMB_CONV_C32), plus
- * PIR_LIMB_WIDE for the 32x128 workgroup tile (else 64x64).  Arrays hold
- * PIR_LIMB_VARIANTS entries. */
-enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_C32, PIR_LIMB_WIDE = 4,
-       PIR_LIMB_VARIANTS = 8 };
+ * count.  v = A-operand form (PIR_LIMB_DENSE_ROW .. PIR_LIMB_WGRAD), plus
+ * PIR_LIMB_WIDE for the 32x128 workgroup tile (else 64x64); the kernel is
+ * limb_gemm_kernel<form, wide ? 1 : 2>.  Arrays hold PIR_LIMB_VARIANTS entries. */
+enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_C32, PIR_LIMB_DGRAD,
+       PIR_LIMB_DGRAD_C32, PIR_LIMB_WGRAD, PIR_LIMB_WIDE = 8, PIR_LIMB_VARIANTS = 16 };
 int pir_net_profile_gemm(pir_net *net, double *ms, double *ops, int *launches);
 int pir_net_set_mode(pir_net *net, int mode);
 

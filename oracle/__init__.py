@@ -560,18 +560,23 @@ def log_mini_batch(batch):
     return int(np.log2(batch))
 
 
-def net_backward(layers, batch, fp, params, x_shares, fwd, labels, lr):
+def net_backward(layers, batch, fp, params, x_shares, fwd, labels, lr, out_delta=None):
     """NeuralNetwork::backward (NeuralNetwork.cu:136-149, 435-467) over the
     oracle layers: reveal-softmax delta, then every layer's backward in
     reverse order, each with the previous layer's activation (the network
-    input for layer 0).  fwd is net_forward's output for x_shares.
+    input for layer 0).  fwd is net_forward's output for x_shares.  With
+    out_delta given, the backward pass starts from it instead of the
+    reveal-softmax delta of the labels.
 
     Returns (deltas, new_params): deltas[i] is layer i's outgoing delta
     (layer 0 included), new_params the updated (W, b) per layer.
     """
     L = len(layers)
     n_classes = fwd[-1][0].shape[-1] // batch
-    delta = reveal_softmax_grad(fwd[-1][0], labels, n_classes, fp, log_mini_batch(batch))
+    if out_delta is None:
+        delta = reveal_softmax_grad(fwd[-1][0], labels, n_classes, fp, log_mini_batch(batch))
+    else:
+        delta = _u(out_delta)
     deltas = [None] * L
     new_params = list(params)
     for i in range(L - 1, -1, -1):

@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(_HERE, "libpir_hip.so")
 ALGO_PLAIN, ALGO_LIMB = 0, 1
 MODE_FUSED, MODE_PLAIN = 0, 1
 LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL = 0, 1, 2, 3
+CONV_FPROP, CONV_DGRAD, CONV_WGRAD = 0, 1, 2
+LIMB_WIDE, LIMB_VARIANTS = 8, 16  # kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
 OP_ADD, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV = range(8)
 
 _vp = ctypes.c_void_p
@@ -45,6 +47,11 @@ _SIGS = {
     "pir_set_mode": ([_i], _i),
     "pir_gemm": ([_i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp], _i),
     "pir_conv_fprop": ([_vp, _vp, _vp] + [_i] * 12 + [_vp], _i),
+    "pir_conv_dgrad": ([_vp, _vp, _vp] + [_i] * 14 + [_vp], _i),
+    "pir_conv_wgrad": ([_vp, _vp, _vp] + [_i] * 14 + [_vp], _i),
+    "pir_reduce_sum": ([_vp, _vp, _i, _i, _i, _vp], _i),
+    "pir_vector_expand": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_maxpool_row2im": ([_vp, _vp] + [_i] * 6 + [_vp], _i),
     "pir_ring_gemm3": ([_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "pir_limb_packed_bytes": ([_i, _i, _i], _sz),
     "pir_limb_pack": ([_vp, _vp, _i, _i, _i, _i, _vp, _vp], _i),
@@ -66,6 +73,7 @@ _SIGS = {
     "pir_rss_divide_public_3out3": ([_vp, _sz, _i, _vp, _vp], _i),
     "pir_rss_matmul": ([_vp, _vp, _vp] + [_i] * 8 + [_vp], _i),
     "pir_rss_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
+    "pir_rss_convolution_op": ([_i, _vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
     "pir_rss_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_rss_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),
     "pir_rss_select_share": ([_vp, _vp, _vp, _vp, _sz, _i, _vp], _i),
@@ -82,6 +90,12 @@ _SIGS = {
     "pir_net_derivative": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
     "pir_net_weights": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_vp),
                          ctypes.POINTER(_sz)], _i),
+    "pir_net_set_weight_shares": ([_vp, _i, _vp, _vp], _i),
+    "pir_net_set_log_learning_rate": ([_vp, _i], _i),
+    "pir_net_backward": ([_vp, _vp, _sz], _i),
+    "pir_net_backward_shares": ([_vp, _vp], _i),
+    "pir_net_layer_backward": ([_vp, _i, _vp, _vp], _i),
+    "pir_net_delta": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
     "pir_net_capture": ([_vp, _vp], _i),
     "pir_net_replay": ([_vp], _i),
     "pir_net_set_mode": ([_vp, _i], _i),
@@ -260,12 +274,44 @@ class Network:
 
         Returns {variant: (kernel_ms, algorithmic_int8_ops, launches)} for the
         variants launched: A-operand form (0 dense row-major, 1 dense
-        column-major, 2 conv, 3 conv with Cin % 32 == 0), plus 4 for the
-        32x128 workgroup tile (limb_gemm_kernel<form, 1>; else <form, 2>)."""
-        ms, ops = np.zeros(8, np.float64), np.zeros(8, np.float64)
-        n = np.zeros(8, np.int32)
+        column-major, 2 conv, 3 conv with Cin % 32 == 0, 4 dgrad, 5 dgrad with
+        Cin % 32 == 0, 6 wgrad), plus LIMB_WIDE for the 32x128 workgroup tile
+        (limb_gemm_kernel<form, 1>; else <form, 2>)."""
+        ms, ops = np.zeros(LIMB_VARIANTS, np.float64), np.zeros(LIMB_VARIANTS, np.float64)
+        n = np.zeros(LIMB_VARIANTS, np.int32)
         check(lib().pir_net_profile_gemm(self.h, ms.ctypes.data, ops.ctypes.data, n.ctypes.data))
-        return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(8) if n[v]}
+        return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(LIMB_VARIANTS) if n[v]}
+
+    # ---- training ---------------------------------------------------------
+
+    def set_log_learning_rate(self, log_lr):
+        check(lib().pir_net_set_log_learning_rate(self.h, log_lr))
+
+    def set_weight_shares(self, layer, w, b=None):
+        """Weights (and FC biases) from share arrays (3, 2, n) uint64."""
+        w = np.ascontiguousarray(w, np.uint64)
+        bp = None
+        if b is not None:
+            b = np.ascontiguousarray(b, np.uint64)
+            bp = b.ctypes.data
+        check(lib().pir_net_set_weight_shares(self.h, layer, w.ctypes.data, bp))
+
+    def backward(self, labels):
+        """NeuralNetwork::backward after a forward: labels (batch x classes)."""
+        labels = np.ascontiguousarray(labels, np.float64).ravel()
+        check(lib().pir_net_backward(self.h, labels.ctypes.data, labels.size))
+
+    def backward_shares(self, dev_ptr):
+        check(lib().pir_net_backward_shares(self.h, dev_ptr))
+
+    def layer_backward(self, layer, delta_ptr, input_ptr):
+        """Layer::backward(delta, forwardInput) of one layer (device shares)."""
+        check(lib().pir_net_layer_backward(self.h, layer, delta_ptr, input_ptr))
+
+    def delta(self, layer):
+        """dL/d(input of layer) after a backward pass (layer == num_layers: the
+        reveal-softmax delta at the network output)."""
+        return self._read(lib().pir_net_delta, layer, np.uint64)
 
     def _read(self, fn, layer, dtype):
         p, n = _vp(), _sz()

@@ -12,6 +12,11 @@
 
 using namespace pir;
 
+static_assert(PIR_LIMB_VARIANTS == LIMB_VARIANTS && PIR_LIMB_WIDE == LIMB_VARIANT_WIDE,
+              "kernel variant numbering of include/pir.h and ring_gemm.h");
+static_assert(PIR_CONV_FPROP == kFprop && PIR_CONV_DGRAD == kDgrad && PIR_CONV_WGRAD == kWgrad,
+              "convolution operator numbering");
+
 namespace {
 
 // Settings of the default context used by the free-standing pir_rss_* calls.
@@ -53,6 +58,20 @@ struct NetScope {
     explicit NetScope(pir_net *n) { set_ctx(&n->ctx); }
     ~NetScope() { set_ctx(nullptr); }
 };
+
+// A captured forward holds the weights packed at capture time: any weight
+// change (set_weights, a backward pass) drops it, and replay then reports that
+// the network must be captured again.
+void drop_graph(pir_net *n) {
+    if (n->exec) {
+        (void)hipGraphExecDestroy(n->exec);
+        n->exec = nullptr;
+    }
+    if (n->graph) {
+        (void)hipGraphDestroy(n->graph);
+        n->graph = nullptr;
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -154,7 +173,11 @@ int pir_limb_gemm(const uint64_t *const *a, int nseg, int M, int K, int a_row_ma
 
 int pir_gemm(int M, int N, int K, const uint64_t *A, int ta, const uint64_t *B, int tb, uint64_t *C, int tc, int algo,
              void *stream) {
-    if (algo == PIR_ALGO_PLAIN) return plain_gemm(M, N, K, A, ta, B, tb, C, tc, (hipStream_t)stream);
+    if (algo == PIR_ALGO_PLAIN)
+        return guarded([&]() -> int {
+            Scope sc(stream, ALGO_PLAIN);
+            return plain_gemm(M, N, K, A, ta, B, tb, C, tc, (hipStream_t)stream);
+        });
     return guarded([&]() -> int {
         Scope sc(stream);
         return limb_ring_gemm(M, N, K, A, ta != 0, B, tb != 0, C, tc != 0, (hipStream_t)stream);
@@ -164,12 +187,59 @@ int pir_gemm(int M, int N, int K, const uint64_t *A, int ta, const uint64_t *B, 
 int pir_conv_fprop(const uint64_t *x, const uint64_t *w, uint64_t *y, int b, int H, int W, int Din, int Dout, int fh,
                    int fw, int ph, int pw, int stride, int dilation, int algo, void *stream) {
     if (algo == PIR_ALGO_PLAIN)
-        return plain_conv_fprop(x, w, y, b, H, W, Din, Dout, fh, fw, ph, pw, stride, dilation, (hipStream_t)stream);
+        return guarded([&]() -> int {
+            Scope sc(stream, ALGO_PLAIN);
+            return plain_conv_fprop(x, w, y, b, H, W, Din, Dout, fh, fw, ph, pw, stride, dilation,
+                                    (hipStream_t)stream);
+        });
     return guarded([&]() -> int {
         Scope sc(stream);
         PIR_REQUIRE(dilation == 1, "pir_conv_fprop: the limb form supports dilation 1");
         return limb_ring_conv(x, w, y, b, H, W, Din, Dout, fh, fw, ph, pw, stride, (hipStream_t)stream);
     });
+}
+
+int pir_conv_dgrad(const uint64_t *dy, const uint64_t *w, uint64_t *dx, int b, int OH, int OW, int Dout, int fh,
+                   int fw, int Din, int ph, int pw, int stride, int dilation, int H, int W, int algo, void *stream) {
+    if (algo == PIR_ALGO_PLAIN)
+        return guarded([&]() -> int {
+            Scope sc(stream, ALGO_PLAIN);
+            return plain_conv_dgrad(dy, w, dx, b, OH, OW, Dout, fh, fw, Din, ph, pw, stride, dilation, H, W,
+                                    (hipStream_t)stream);
+        });
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(dilation == 1, "pir_conv_dgrad: the limb form supports dilation 1");
+        return limb_ring_dgrad(dy, w, dx, b, OH, OW, Dout, fh, fw, Din, ph, pw, stride, H, W, (hipStream_t)stream);
+    });
+}
+
+int pir_conv_wgrad(const uint64_t *dy, const uint64_t *x, uint64_t *dw, int b, int OH, int OW, int Dout, int H,
+                   int W, int Din, int fh, int fw, int ph, int pw, int stride, int dilation, int algo, void *stream) {
+    if (algo == PIR_ALGO_PLAIN)
+        return guarded([&]() -> int {
+            Scope sc(stream, ALGO_PLAIN);
+            return plain_conv_wgrad(dy, x, dw, b, OH, OW, Dout, H, W, Din, fh, fw, ph, pw, stride, dilation,
+                                    (hipStream_t)stream);
+        });
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(dilation == 1, "pir_conv_wgrad: the limb form supports dilation 1");
+        return limb_ring_wgrad(dy, x, dw, b, OH, OW, Dout, H, W, Din, fh, fw, ph, pw, stride, (hipStream_t)stream);
+    });
+}
+
+int pir_reduce_sum(const uint64_t *a, uint64_t *b, int reduce_rows, int rows, int cols, void *stream) {
+    return reduce_sum(a, b, reduce_rows, rows, cols, (hipStream_t)stream);
+}
+
+int pir_vector_expand(const uint64_t *a, size_t n, uint64_t *b, int factor, void *stream) {
+    return vector_expand(a, n, b, factor, (hipStream_t)stream);
+}
+
+int pir_maxpool_row2im(const uint64_t *rows, uint64_t *out, int image_height, int image_width, int f, int Din,
+                       int batch, int stride, void *stream) {
+    return maxpool_row2im(rows, out, image_height, image_width, f, Din, batch, stride, (hipStream_t)stream);
 }
 
 int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint64_t *A1, const uint64_t *B0,
@@ -353,6 +423,30 @@ int pir_rss_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
         RSS<u64> B = RSS<u64>::wrap((u64 *)w, (size_t)Dout * f * f * Din);
         RSS<u64> C = RSS<u64>::wrap(c, (size_t)batch * OH * OW * Dout);
         convolution(A, B, C, kFprop, batch, H, W, f, Din, Dout, stride, pad, trunc);
+        return (int)OK;
+    });
+}
+
+int pir_rss_convolution_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *c, int batch, int H, int W, int f,
+                           int Din, int Dout, int stride, int pad, int trunc, int algo, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        PIR_REQUIRE(op == PIR_CONV_FPROP || op == PIR_CONV_DGRAD || op == PIR_CONV_WGRAD,
+                    "pir_rss_convolution_op: unknown operator");
+        PIR_REQUIRE(batch > 0 && H > 0 && W > 0 && f > 0 && Din > 0 && Dout > 0 && stride > 0 && pad >= 0,
+                    "pir_rss_convolution_op: invalid geometry");
+        PIR_REQUIRE(trunc >= 1 && trunc <= 62, "pir_rss_convolution_op: truncation must be in [1, 62]");
+        int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
+        PIR_REQUIRE(H + 2 * pad >= f && W + 2 * pad >= f && OH > 0 && OW > 0, "pir_rss_convolution_op: empty output");
+        size_t in_n = (size_t)batch * H * W * Din, w_n = (size_t)Dout * f * f * Din,
+               out_n = (size_t)batch * OH * OW * Dout;
+        size_t na = op == PIR_CONV_FPROP ? in_n : out_n;
+        size_t nb = op == PIR_CONV_WGRAD ? in_n : w_n;
+        size_t nc = op == PIR_CONV_FPROP ? out_n : (op == PIR_CONV_DGRAD ? in_n : w_n);
+        RSS<u64> A = RSS<u64>::wrap((u64 *)a, na);
+        RSS<u64> B = RSS<u64>::wrap((u64 *)b, nb);
+        RSS<u64> C = RSS<u64>::wrap(c, nc);
+        convolution(A, B, C, (ConvOperator)op, batch, H, W, f, Din, Dout, stride, pad, trunc);
         return (int)OK;
     });
 }
@@ -584,8 +678,75 @@ int pir_net_set_weights(pir_net *n, int layer, const double *w, size_t nw, const
             L->getBiases()->setPublic(std::vector<double>(b, b + nb), n->ctx.frac_bits);
         }
         L->weightsChanged();
+        drop_graph(n);
         return (int)OK;
     });
+}
+
+int pir_net_set_weight_shares(pir_net *n, int layer, const uint64_t *w, const uint64_t *b) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && w && layer >= 0 && layer < pir_net_num_layers(n), "pir_net_set_weight_shares: bad layer");
+        NetScope ns(n);
+        Layer *L = n->net->layers[layer].get();
+        PIR_REQUIRE(L->getWeights(), "pir_net_set_weight_shares: layer has no weights");
+        PIR_REQUIRE(!b || L->getBiases(), "pir_net_set_weight_shares: layer has no biases");
+        L->getWeights()->copy_from_host(w);
+        if (b) L->getBiases()->copy_from_host(b);
+        L->weightsChanged();
+        drop_graph(n);
+        return (int)OK;
+    });
+}
+
+int pir_net_set_log_learning_rate(pir_net *n, int log_lr) {
+    PIR_REQUIRE(n && log_lr >= 1 && log_lr <= 30, "pir_net_set_log_learning_rate: log_lr must be in [1, 30]");
+    n->ctx.log_lr = log_lr;
+    return OK;
+}
+
+int pir_net_backward(pir_net *n, const double *labels, size_t count) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && labels, "pir_net_backward: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        n->net->backward(std::vector<double>(labels, labels + count));
+        return (int)OK;
+    });
+}
+
+int pir_net_backward_shares(pir_net *n, const uint64_t *label_shares) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && label_shares && !n->net->layers.empty(), "pir_net_backward_shares: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        size_t count = n->net->layers.back()->getActivation()->size();
+        RSS64 labels = RSS64::wrap((u64 *)label_shares, count);
+        n->net->backwardShares(labels);
+        return (int)OK;
+    });
+}
+
+int pir_net_layer_backward(pir_net *n, int layer, const uint64_t *delta, const uint64_t *forward_input) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && delta && forward_input && layer >= 0 && layer < pir_net_num_layers(n),
+                    "pir_net_layer_backward: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        Layer *L = n->net->layers[layer].get();
+        RSS64 d = RSS64::wrap((u64 *)delta, L->getActivation()->size());
+        RSS64 x = RSS64::wrap((u64 *)forward_input, L->inputSize());
+        L->backward(d, x);
+        return (int)OK;
+    });
+}
+
+int pir_net_delta(pir_net *n, int layer, const uint64_t **shares, size_t *count) {
+    PIR_REQUIRE(n && shares && count && layer >= 0 && layer <= pir_net_num_layers(n), "pir_net_delta: bad layer");
+    RSS64 *d = layer == pir_net_num_layers(n) ? &n->net->outputDelta : n->net->layers[layer]->getDelta();
+    PIR_REQUIRE(d->size() > 0, "pir_net_delta: no backward pass has run");
+    *shares = d->base();
+    *count = d->size();
+    return OK;
 }
 
 int pir_net_forward(pir_net *n, const double *input, size_t count) {
@@ -663,14 +824,7 @@ int pir_net_capture(pir_net *n, const uint64_t *shares) {
         // one eager pass packs the weights and warms the allocator cache
         n->net->run();
         PIR_HIP(hipStreamSynchronize(s));
-        if (n->exec) {
-            (void)hipGraphExecDestroy(n->exec);
-            n->exec = nullptr;
-        }
-        if (n->graph) {
-            (void)hipGraphDestroy(n->graph);
-            n->graph = nullptr;
-        }
+        drop_graph(n);
         PIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         n->ctx.alloc.capturing = true;
         int rc = guarded([&]() -> int {
@@ -778,7 +932,7 @@ int pir_autotune_save(const char *path) { return limb_tune_save(path); }
 int pir_autotune_load(const char *path) { return limb_tune_load(path); }
 
 int pir_net_replay(pir_net *n) {
-    PIR_REQUIRE(n && n->exec, "pir_net_replay: no captured graph");
+    PIR_REQUIRE(n && n->exec, "pir_net_replay: no captured graph (a weight change drops it; capture again)");
     PIR_HIP(hipGraphLaunch(n->exec, n->ctx.stream));
     return OK;
 }

@@ -35,5 +35,13 @@ int stride_pad(const u64 *in, u64 *out, size_t n_in, int stride, int pad, u64 pv
 size_t maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);
 int maxpool_im2row(const u64 *im, u64 *out, int H, int W, int f, int Din, int batch, int stride, int padding, u64 pv,
                    hipStream_t s);
+// gpu::reduceSum (matrix.cuh:109-126, 331-351): b[col] += sum over the other axis.
+int reduce_sum(const u64 *a, u64 *b, int reduce_rows, int rows, int cols, hipStream_t s);
+// gpu::vectorExpand (matrix.cuh:128-139, 354-380): b[i*f + j] = a[i].
+int vector_expand(const u64 *a, size_t n, u64 *b, int factor, hipStream_t s);
+// gpu::maxpool_row2im (convolution.cuh:110-151, 376-410): adds every pool
+// entry into its image position (the kernel-side height/width naming).
+int maxpool_row2im(const u64 *rows, u64 *out, int imageHeight, int imageWidth, int f, int Din, int batch, int stride,
+                   hipStream_t s);
 
 }  // namespace pir

@@ -282,4 +282,85 @@ int maxpool_im2row(const u64 *im, u64 *out, int H, int W, int f, int Din, int ba
     return OK;
 }
 
+// matrix.cuh:109-126 -- b[col] += sum_i (reduceRows ? a[cols*i + col] : a[cols*col + i])
+__global__ __launch_bounds__(BLK) void reduce_sum_kernel(const u64 *a, u64 *b, int reduceRows, int rows, int cols) {
+    int col = blockIdx.x * BLK + threadIdx.x;
+    int out = reduceRows ? cols : rows;
+    if (col >= out) return;
+    u64 acc = b[col];
+    if (reduceRows)
+        for (int i = 0; i < rows; i++) acc += a[(size_t)cols * i + col];
+    else
+        for (int i = 0; i < cols; i++) acc += a[(size_t)cols * col + i];
+    b[col] = acc;
+}
+
+int reduce_sum(const u64 *a, u64 *b, int reduce_rows, int rows, int cols, hipStream_t s) {
+    PIR_REQUIRE(rows >= 0 && cols >= 0, "reduceSum: negative dimension");
+    int out = reduce_rows ? cols : rows;
+    if (out == 0) return OK;
+    hipLaunchKernelGGL(reduce_sum_kernel, dim3(g1(out)), dim3(BLK), 0, s, a, b, reduce_rows, rows, cols);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// matrix.cuh:128-139, one thread per output element
+__global__ __launch_bounds__(BLK) void vector_expand_kernel(const u64 *a, u64 *b, size_t total, int f) {
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t < total) b[t] = a[t / f];
+}
+
+int vector_expand(const u64 *a, size_t n, u64 *b, int factor, hipStream_t s) {
+    PIR_REQUIRE(factor >= 1, "vectorExpand: factor must be >= 1");
+    size_t total = n * (size_t)factor;
+    if (total == 0) return OK;
+    hipLaunchKernelGGL(vector_expand_kernel, dim3(g1(total)), dim3(BLK), 0, s, a, b, total, factor);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// convolution.cuh:110-151 gathered per output position: the reference adds
+// rows[window * f*f + v] into output[b][valCol][valRow][c] with atomics; here
+// each position sums the pool entries that land on it (the same sum mod 2^64)
+// and adds it once, so no atomics are needed.  Kernel-side names: windowCol
+// runs over widthKernels = (imageWidth - f)/stride + 1, windowRow over
+// heightKernels = (imageHeight - f)/stride + 1, and
+// outputIdx = b*H*W*Din + valCol*imageHeight*Din + valRow*Din + c.
+__global__ __launch_bounds__(BLK) void row2im_kernel(const u64 *rows, u64 *out, int iH, int iW, int f, int Din,
+                                                     int batch, int stride, int wK, int hK) {
+    size_t total = (size_t)batch * iH * iW * Din;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= total) return;
+    int c = (int)(t % Din);
+    size_t r = t / Din;
+    int valRow = (int)(r % iH);
+    r /= iH;
+    int valCol = (int)(r % iW);
+    int b = (int)(r / iW);
+    int wc_lo = valCol - f + 1 > 0 ? (valCol - f + 1 + stride - 1) / stride : 0;
+    int wc_hi = min(wK - 1, valCol / stride);
+    int wr_lo = valRow - f + 1 > 0 ? (valRow - f + 1 + stride - 1) / stride : 0;
+    int wr_hi = min(hK - 1, valRow / stride);
+    u64 acc = 0;
+    for (int wc = wc_lo; wc <= wc_hi; wc++)
+        for (int wr = wr_lo; wr <= wr_hi; wr++) {
+            int v = (valCol - wc * stride) * f + (valRow - wr * stride);
+            size_t win = (size_t)b * wK * hK * Din + ((size_t)wc * hK + wr) * Din + c;
+            acc += rows[win * f * f + v];
+        }
+    out[t] += acc;
+}
+
+int maxpool_row2im(const u64 *rows, u64 *out, int imageHeight, int imageWidth, int f, int Din, int batch, int stride,
+                   hipStream_t s) {
+    PIR_REQUIRE(f >= 1 && stride >= 1 && Din > 0 && batch > 0 && imageHeight >= f && imageWidth >= f,
+                "maxpool_row2im: invalid geometry");
+    int wK = (imageWidth - f) / stride + 1, hK = (imageHeight - f) / stride + 1;
+    size_t total = (size_t)batch * imageHeight * imageWidth * Din;
+    hipLaunchKernelGGL(row2im_kernel, dim3(g1(total)), dim3(BLK), 0, s, rows, out, imageHeight, imageWidth, f, Din,
+                       batch, stride, wK, hK);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 }  // namespace pir

@@ -38,4 +38,23 @@ template <typename U>
 int fused_convex_comb(u64 *d, const u64 *c, const U *e, size_t n, hipStream_t s);
 int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s);
 
+// ---- backward (training) path ----
+// ReLULayer::backward (ReLULayer.cu:91-123): out = selectShare(0, delta, reluPrime).
+template <typename U>
+int fused_relu_backward(const u64 *delta, const U *relu_prime, size_t n, u64 *out, hipStream_t s);
+// MaxpoolLayer::backward (MaxpoolLayer.cu:133-180): vectorExpand, selectShare
+// with maxPrime and maxpool_row2im in one pass, gathered per input element.
+template <typename U>
+int fused_maxpool_backward(const u64 *delta, const U *max_prime, const MaxpoolGeom &g, u64 *out, hipStream_t s);
+// FCLayer.cu:205-218: db = reduceSum(delta, false, outDim, batch) per share,
+// dividePublic(db, 2^lr), biases -= db.
+int fused_fc_bias_update(const u64 *delta, u64 *bias, int out_dim, int batch, int lr, hipStream_t s);
+// NeuralNetwork::_reveal_softmax_grad (NeuralNetwork.cu:207-267): open the
+// logits, softmax per row of n_classes in FP64, back to fixed point, minus the
+// label shares, then dividePublic by 2^lmb when lmb > 1.
+int fused_softmax_grad(const u64 *act, const u64 *labels, int rows, int n_classes, int fp, int lmb, u64 *out,
+                       hipStream_t s);
+// The FP64 softmax of each party's opened logits ([3][n] -> [3][n] fixed point).
+int softmax_fixed(const u64 *revealed, int rows, int n_classes, int fp, u64 *out, hipStream_t s);
+
 }  // namespace pir

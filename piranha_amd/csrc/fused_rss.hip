@@ -243,6 +243,203 @@ int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime,
     return OK;
 }
 
+// ---------------------------------------------------------------------------
+// Backward (training) path
+
+template <typename U>
+__global__ __launch_bounds__(EW_BLOCK) void relu_backward_kernel(const u64 *__restrict__ delta,
+                                                                 const U *__restrict__ rp, size_t n,
+                                                                 u64 *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<u64> zero;
+#pragma unroll
+    for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+    store3(out, n, i, select_<U>(zero, load3(delta, n, i), load3(rp, n, i)));
+}
+
+template <typename U>
+int fused_relu_backward(const u64 *delta, const U *relu_prime, size_t n, u64 *out, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL((relu_backward_kernel<U>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, delta, relu_prime, n, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// One thread per input element (b, h, w, c): every pooling window (oh, ow)
+// that covers it contributes selectShare(0, delta[window], maxPrime[window][tap])
+// with tap = (h - oh*stride)*f + (w - ow*stride) -- the entry vectorExpand +
+// selectShare produce for that position, which maxpool_row2im then adds there.
+template <typename U>
+__global__ __launch_bounds__(EW_BLOCK) void maxpool_backward_kernel(const u64 *__restrict__ delta,
+                                                                    const U *__restrict__ mp, MaxpoolGeom g,
+                                                                    u64 *__restrict__ out) {
+    const size_t inN = (size_t)g.batch * g.H * g.W * g.C;
+    size_t t = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (t >= inN) return;
+    int c = (int)(t % g.C);
+    size_t r = t / g.C;
+    int w = (int)(r % g.W);
+    r /= g.W;
+    int h = (int)(r % g.H);
+    int b = (int)(r / g.H);
+    const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    const int f2 = g.f * g.f;
+    int oh_lo = h - g.f + 1 > 0 ? (h - g.f + 1 + g.stride - 1) / g.stride : 0;
+    int oh_hi = min(g.OH - 1, h / g.stride);
+    int ow_lo = w - g.f + 1 > 0 ? (w - g.f + 1 + g.stride - 1) / g.stride : 0;
+    int ow_hi = min(g.OW - 1, w / g.stride);
+    Sh3<u64> acc, zero;
+#pragma unroll
+    for (int p = 0; p < 3; p++) acc.s[p][0] = acc.s[p][1] = zero.s[p][0] = zero.s[p][1] = 0;
+    for (int oh = oh_lo; oh <= oh_hi; oh++)
+        for (int ow = ow_lo; ow <= ow_hi; ow++) {
+            size_t o = (((size_t)b * g.OH + oh) * g.OW + ow) * g.C + c;
+            int j = (h - oh * g.stride) * g.f + (w - ow * g.stride);
+            Sh3<u64> v = select_<U>(zero, load3(delta, outN, o), load3(mp, outN * f2, o * f2 + j));
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                acc.s[p][0] += v.s[p][0];
+                acc.s[p][1] += v.s[p][1];
+            }
+        }
+    store3(out, inN, t, acc);
+}
+
+template <typename U>
+int fused_maxpool_backward(const u64 *delta, const U *max_prime, const MaxpoolGeom &g, u64 *out, hipStream_t s) {
+    PIR_REQUIRE(g.f >= 1 && g.stride >= 1 && g.H >= g.f && g.W >= g.f, "maxpool backward: invalid geometry");
+    PIR_REQUIRE(g.OH == (g.H - g.f) / g.stride + 1 && g.OW == (g.W - g.f) / g.stride + 1,
+                "maxpool backward: inconsistent output size");
+    size_t inN = (size_t)g.batch * g.H * g.W * g.C;
+    if (inN == 0) return OK;
+    hipLaunchKernelGGL((maxpool_backward_kernel<U>), dim3(ew_grid(inN)), dim3(EW_BLOCK), 0, s, delta, max_prime, g,
+                       out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// FCLayer.cu:205-218 for one bias element o: every share array sums delta
+// entries o*batch .. o*batch + batch-1 (reduceSum with reduceRows = false reads
+// delta as outDim rows of batch), then dividePublic and the subtraction.
+__global__ __launch_bounds__(EW_BLOCK) void fc_bias_update_kernel(const u64 *__restrict__ delta,
+                                                                  u64 *__restrict__ bias, int out_dim, int batch,
+                                                                  int lr) {
+    int o = blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (o >= out_dim) return;
+    const size_t n = (size_t)out_dim * batch;
+    Sh3<u64> db;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int sh = 0; sh < 2; sh++) {
+            const u64 *a = delta + (size_t)(2 * p + sh) * n + (size_t)o * batch;
+            u64 acc = 0;
+            for (int i = 0; i < batch; i++) acc += a[i];
+            db.s[p][sh] = acc;
+        }
+    db = divide_public_pow2_(db, lr);
+    Sh3<u64> bv = load3(bias, (size_t)out_dim, (size_t)o);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        bv.s[p][0] -= db.s[p][0];
+        bv.s[p][1] -= db.s[p][1];
+    }
+    store3(bias, (size_t)out_dim, (size_t)o, bv);
+}
+
+int fused_fc_bias_update(const u64 *delta, u64 *bias, int out_dim, int batch, int lr, hipStream_t s) {
+    PIR_REQUIRE(out_dim >= 0 && batch >= 0 && lr >= 1 && lr <= 62, "FC bias update: invalid arguments");
+    if (out_dim == 0) return OK;
+    hipLaunchKernelGGL(fc_bias_update_kernel, dim3(ew_grid(out_dim)), dim3(EW_BLOCK), 0, s, delta, bias, out_dim, batch,
+                       lr);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// Softmax of one row of opened logits v[0..nc) in the reference's FP64 steps:
+// x = (double)(int64)v / 2^fp, e = exp(x) summed in element order, e / sum,
+// back with (uint64)(e / sum * 2^fp).  exp is evaluated twice per element
+// (sum pass, output pass) so no per-row storage is needed; both evaluations
+// are the same function of the same input.
+template <typename F>
+__device__ __forceinline__ void softmax_row(F logit, int nc, int fp, u64 *sm, size_t stride) {
+    const double scale = (double)(1 << fp);
+    double sum = 0.0;
+    for (int i = 0; i < nc; i++) sum += exp((double)(s64)logit(i) / scale);
+    for (int i = 0; i < nc; i++) {
+        double e = exp((double)(s64)logit(i) / scale);
+        sm[(size_t)i * stride] = (u64)(s64)((e / sum) * scale);
+    }
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void softmax_fixed_kernel(const u64 *__restrict__ rev, int rows, int nc, int fp,
+                                                                 u64 *__restrict__ out) {
+    int t = blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (t >= 3 * rows) return;
+    int q = t / rows, row = t - q * rows;
+    const size_t n = (size_t)rows * nc;
+    const u64 *v = rev + (size_t)q * n + (size_t)row * nc;
+    softmax_row([&](int i) { return v[i]; }, nc, fp, out + (size_t)q * n + (size_t)row * nc, 1);
+}
+
+int softmax_fixed(const u64 *revealed, int rows, int n_classes, int fp, u64 *out, hipStream_t s) {
+    PIR_REQUIRE(rows >= 0 && n_classes >= 1 && fp >= 0 && fp <= 30, "softmax: invalid arguments");
+    if (rows == 0) return OK;
+    hipLaunchKernelGGL(softmax_fixed_kernel, dim3(ew_grid((size_t)3 * rows)), dim3(EW_BLOCK), 0, s, revealed, rows,
+                       n_classes, fp, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// One thread per row: parties A and C open the row (the only opened values
+// that reach an output share: deltas += softmax touches A.sA and C.sB),
+// deltas = (A.sA, C.sB = softmax) - labels, then the dividePublic.
+__global__ __launch_bounds__(EW_BLOCK) void softmax_grad_kernel(const u64 *__restrict__ act,
+                                                                const u64 *__restrict__ labels, int rows, int nc,
+                                                                int fp, int lmb, u64 *__restrict__ out) {
+    int row = blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (row >= rows) return;
+    const size_t n = (size_t)rows * nc, base = (size_t)row * nc;
+    // A opens sA + sB + (C's sA); C opens sA + sB + (B's sA)
+    const u64 *A0 = act + 0 * n + base, *A1 = act + 1 * n + base, *B0 = act + 2 * n + base;
+    const u64 *C0 = act + 4 * n + base, *C1 = act + 5 * n + base;
+    // softmax values are staged in the output's A.sA and C.sB slots
+    u64 *oA = out + 0 * n + base, *oC = out + 5 * n + base;
+    softmax_row([&](int i) { return A0[i] + A1[i] + C0[i]; }, nc, fp, oA, 1);
+    softmax_row([&](int i) { return C0[i] + C1[i] + B0[i]; }, nc, fp, oC, 1);
+    for (int i = 0; i < nc; i++) {
+        Sh3<u64> d;
+#pragma unroll
+        for (int p = 0; p < 3; p++) d.s[p][0] = d.s[p][1] = 0;
+        d.s[PARTY_A][0] = oA[i];
+        d.s[PARTY_C][1] = oC[i];
+        Sh3<u64> L = load3(labels, n, base + i);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            d.s[p][0] -= L.s[p][0];
+            d.s[p][1] -= L.s[p][1];
+        }
+        if (lmb > 1) d = divide_public_pow2_(d, lmb);
+        store3(out, n, base + i, d);
+    }
+}
+
+int fused_softmax_grad(const u64 *act, const u64 *labels, int rows, int n_classes, int fp, int lmb, u64 *out,
+                       hipStream_t s) {
+    PIR_REQUIRE(rows >= 0 && n_classes >= 1 && fp >= 0 && fp <= 30 && lmb >= 0 && lmb <= 62,
+                "softmax gradient: invalid arguments");
+    PIR_REQUIRE(act != out, "softmax gradient: output must not alias the activations");
+    if (rows == 0) return OK;
+    hipLaunchKernelGGL(softmax_grad_kernel, dim3(ew_grid(rows)), dim3(EW_BLOCK), 0, s, act, labels, rows, n_classes, fp,
+                       lmb, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+template int fused_relu_backward<u8>(const u64 *, const u8 *, size_t, u64 *, hipStream_t);
+template int fused_maxpool_backward<u8>(const u64 *, const u8 *, const MaxpoolGeom &, u64 *, hipStream_t);
+
 int expanded_pool_size(int f) {
     // pow(ceil(log2(f*f)), 2) (convolution.cuh:288, MaxpoolLayer.cu:69)
     int f2 = f * f, c = 0;

@@ -1,10 +1,11 @@
-// layers.cpp -- FC / CNN / ReLU / Maxpool forward layers and the network.
+// layers.cpp -- FC / CNN / ReLU / Maxpool layers (forward and backward) and the network.
 #include "layers.h"
 
 #include <cmath>
 #include <cstdio>
 #include <random>
 
+#include "elementwise.h"
 #include "fused_rss.h"
 
 namespace pir {
@@ -73,6 +74,52 @@ void FCLayer::forward(const RSS64 &input) {
                                      ctx().stream));
 }
 
+// FCLayer::backward, FCLayer.cu:133-224: deltas = delta . W^T (truncated by
+// 2^FP), W -= delta^T . X (truncated by 2^(FP + log_lr)), and the bias update
+// of reduceSum / dividePublic.  The fused form runs the two products as limb
+// GEMMs, the second with the subtraction in its epilogue, and the bias update
+// as one kernel.
+void FCLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
+    const int B = (int)conf.batchSize, in = (int)conf.inputDim, out = (int)conf.outputDim;
+    const int fp = ctx().frac_bits, lr = ctx().log_lr;
+    ensureDeltas();
+    if (fused_limb() && delta.base() && forwardInput.base()) {
+        // (1) M = batch, N = in, K = out; B(k = o, n = i) = W[i*out + o]
+        DevBuf pw(packed_rss_weights_bytes(out, in));
+        check(pack_rss_weights(weights, out, in, false, (int8_t *)pw.get()));
+        check(rss_limb_product(delta, true, nullptr, B, in, out, (const int8_t *)pw.get(), fp, nullptr, deltas, true));
+        // (2) M = out, N = in, K = batch; A(m = o, k = b) = delta[b*out + o], B(k, n) = X[b*in + i],
+        // the M x N result column-major is W's [in][out] layout
+        DevBuf px(packed_rss_weights_bytes(B, in));
+        check(pack_rss_weights(forwardInput, B, in, true, (int8_t *)px.get()));
+        LimbA a{};
+        a.M = out;
+        a.K = B;
+        a.Kp = round_up(B, LIMB_BK);
+        a.lda = out;
+        a.row_major = 0;
+        Epilogue e = rss_trunc_epi(weights.base(), fp + lr, false);
+        e.sub_out = 1;
+        check(rss_limb_run(delta, a, (const int8_t *)px.get(), in, e));
+        // (3)
+        check(fused_fc_bias_update(delta.base(), biases.base(), out, B, lr, ctx().stream));
+        packed_valid = false;
+        return;
+    }
+    deltas.zero();
+    matmul(delta, weights, deltas, B, in, out, true, false, true, (u64)fp);
+    RSS64 dW((size_t)out * in);
+    matmul(delta, forwardInput, dW, out, in, B, false, true, false, (u64)(fp + lr));
+    weights -= dW;
+    RSS64 db((size_t)out);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(reduce_sum(delta.getShare(q, i)->data(), db.getShare(q, i)->data(), 0, out, B, ctx().stream));
+    dividePublic(db, (u64)1 << lr);
+    biases -= db;
+    packed_valid = false;
+}
+
 // ---------------------------------------------------------------------------
 // CNN (CNNLayer.cu:23-115); weights are [Dout][f][f][Din], no bias.
 
@@ -135,6 +182,41 @@ void CNNLayer::forward(const RSS64 &input) {
                 (int)conf.imageWidth, f, Din, Dout, (int)conf.stride, (int)conf.padding, fp);
 }
 
+// CNNLayer::backward, CNNLayer.cu:118-195: deltas = convolution(kDgrad),
+// dF = convolution(kWgrad), dividePublic(dF, 2^log_lr), W -= dF (the
+// reference's bias update is commented out).  The fused form runs dgrad and
+// wgrad as limb GEMMs, the dividePublic and the subtraction in wgrad's epilogue.
+void CNNLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
+    const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
+    const int B = (int)conf.batchSize, H = (int)conf.imageHeight, W = (int)conf.imageWidth;
+    const int st = (int)conf.stride, pad = (int)conf.padding, fp = ctx().frac_bits, lr = ctx().log_lr;
+    ensureDeltas();
+    if (fused_limb() && delta.base() && forwardInput.base()) {
+        convolution(delta, weights, deltas, kDgrad, B, H, W, f, Din, Dout, st, pad, fp);
+        LimbA a = limb_wgrad_view(B, H, W, f, Din, Dout, st, pad);
+        LimbBSrc g{};
+        g.K = a.K;
+        g.N = Dout;
+        g.Kp = a.Kp;
+        g.row_major = 1;
+        DevBuf packed(limb_b_bytes(Dout, 6, g.Kp));
+        check(pack_rss_operand(delta, g, (int8_t *)packed.get()));
+        Epilogue e = rss_trunc_epi(weights.base(), fp, false);
+        e.div_bits = lr;
+        e.sub_out = 1;
+        check(rss_limb_run(forwardInput, a, (const int8_t *)packed.get(), Dout, e));
+        packed_valid = false;
+        return;
+    }
+    deltas.zero();
+    convolution(delta, weights, deltas, kDgrad, B, H, W, f, Din, Dout, st, pad, fp);
+    RSS64 dF(weights.size());
+    convolution(delta, forwardInput, dF, kWgrad, B, H, W, f, Din, Dout, st, pad, fp);
+    dividePublic(dF, (u64)1 << lr);
+    weights -= dF;
+    packed_valid = false;
+}
+
 // ---------------------------------------------------------------------------
 // ReLU (ReLULayer.cu:21-88)
 
@@ -149,6 +231,18 @@ void ReLULayer::printLayer() {
 void ReLULayer::forward(const RSS64 &input) {
     if (ctx().mode != MODE_FUSED) activations.zero();
     ReLU(input, activations, reluPrime);
+}
+
+// ReLULayer::backward, ReLULayer.cu:91-123: deltas = selectShare(0, delta, reluPrime)
+void ReLULayer::backward(const RSS64 &delta, const RSS64 &) {
+    ensureDeltas();
+    if (ctx().mode == MODE_FUSED && delta.base()) {
+        check(fused_relu_backward<u8>(delta.base(), reluPrime.base(), delta.size(), deltas.base(), ctx().stream));
+        return;
+    }
+    deltas.zero();
+    RSS64 zeros(delta.size());
+    selectShare(zeros, delta, reluPrime, deltas);
 }
 
 // ---------------------------------------------------------------------------
@@ -209,6 +303,33 @@ void MaxpoolLayer::forward(const RSS64 &input) {
                                 ctx().stream));
 }
 
+// MaxpoolLayer::backward, MaxpoolLayer.cu:133-180: vectorExpand of delta over
+// each pool, selectShare with maxPrime, maxpool_row2im into deltas.
+void MaxpoolLayer::backward(const RSS64 &delta, const RSS64 &) {
+    ensureDeltas();
+    const int f = (int)conf.poolSize;
+    if (ctx().mode == MODE_FUSED && delta.base()) {
+        MaxpoolGeom g{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, (int)conf.features, f,
+                      (int)conf.stride, outH(), outW(), 0};
+        check(fused_maxpool_backward<u8>(delta.base(), maxPrime.base(), g, deltas.base(), ctx().stream));
+        return;
+    }
+    deltas.zero();
+    RSS64 deltaPools(maxPrime.size());
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(vector_expand(delta.getShare(q, i)->data(), delta.size(), deltaPools.getShare(q, i)->data(), f * f,
+                                ctx().stream));
+    RSS64 zeros(deltaPools.size());
+    selectShare(zeros, deltaPools, maxPrime, deltaPools);
+    // the layer passes (imageWidth, imageHeight) into row2im's (imageHeight, imageWidth)
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(maxpool_row2im(deltaPools.getShare(q, i)->data(), deltas.getShare(q, i)->data(),
+                                 (int)conf.imageWidth, (int)conf.imageHeight, f, (int)conf.features,
+                                 (int)conf.batchSize, (int)conf.stride, ctx().stream));
+}
+
 // ---------------------------------------------------------------------------
 // Network (NeuralNetwork.cu:32-133)
 
@@ -249,6 +370,56 @@ void NeuralNetwork::run(const std::function<void(size_t)> &after_layer) {
         }
         if (after_layer) after_layer(i);
     }
+}
+
+void NeuralNetwork::backward(const std::vector<double> &labels) {
+    RSS64 labelShare(labels.size());
+    labelShare.setPublic(labels, ctx().frac_bits);
+    backwardShares(labelShare);
+}
+
+void NeuralNetwork::backwardShares(const RSS64 &labelShare) {
+    if (layers.size() < 2) {
+        set_error("network backward: needs at least two layers");
+        throw PirError(ERR_UNSUPPORTED);
+    }
+    if (labelShare.size() != layers.back()->getActivation()->size() || batch <= 0 ||
+        labelShare.size() % (size_t)batch != 0) {
+        set_error("network backward: labels must have batch x classes entries");
+        throw PirError(ERR_ARG);
+    }
+    if (outputDelta.size() != labelShare.size()) outputDelta.resize(labelShare.size());
+    _reveal_softmax_grad(labelShare, outputDelta);
+    _backward_pass(outputDelta);
+}
+
+void NeuralNetwork::_reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas) {
+    const RSS64 &act = *layers.back()->getActivation();
+    const size_t n = act.size();
+    const int nClasses = (int)(n / (size_t)batch);
+    const int lmb = (int)std::log2((double)batch);  // LOG_MINI_BATCH, model.cu:39
+    const int fp = ctx().frac_bits;
+    if (ctx().mode == MODE_FUSED && act.base() && labels.base() && deltas.base()) {
+        check(fused_softmax_grad(act.base(), labels.base(), batch, nClasses, fp, lmb, deltas.base(), ctx().stream));
+        return;
+    }
+    RSS64 x(n);
+    x += act;
+    PartyData<u64> revealed(n);
+    reconstruct(x, revealed);
+    PartyData<u64> sm(n);
+    check(softmax_fixed(revealed.base(), batch, nClasses, fp, sm.base(), ctx().stream));
+    deltas.zero();
+    deltas += sm;
+    deltas -= labels;
+    if (lmb > 1) dividePublic(deltas, (u64)1 << lmb);
+}
+
+void NeuralNetwork::_backward_pass(const RSS64 &deltas) {
+    const size_t L = layers.size();
+    layers[L - 1]->backward(deltas, *layers[L - 2]->getActivation());
+    for (size_t i = L - 2; i > 0; i--) layers[i]->backward(*layers[i + 1]->getDelta(), *layers[i - 1]->getActivation());
+    layers[0]->backward(*layers[1]->getDelta(), input);
 }
 
 void NeuralNetwork::printNetwork() {

@@ -1,6 +1,6 @@
-// layers.h -- forward layers and network for 3-party RSS inference, with the
-// reference's layer API (src/nn/Layer.h:10-30; FCLayer.cu, CNNLayer.cu,
-// ReLULayer.cu, MaxpoolLayer.cu; NeuralNetwork.cu:32-133).
+// layers.h -- layers and network for 3-party RSS inference and training, with
+// the reference's layer API (src/nn/Layer.h:10-30; FCLayer.cu, CNNLayer.cu,
+// ReLULayer.cu, MaxpoolLayer.cu; NeuralNetwork.cu:32-149, 207-267, 435-467).
 #pragma once
 
 #include <functional>
@@ -43,7 +43,12 @@ class Layer {
     virtual ~Layer() = default;
     virtual void printLayer() = 0;
     virtual void forward(const RSS64 &input) = 0;
+    // Layer::backward: `delta` is dL/d(this layer's output), `forwardInput` the
+    // input of the forward pass; writes dL/d(input) into getDelta() and applies
+    // the weight update (learning rate 2^-log_lr of the context).
+    virtual void backward(const RSS64 &delta, const RSS64 &forwardInput) = 0;
     virtual RSS64 *getActivation() = 0;
+    RSS64 *getDelta() { return &deltas; }
     virtual RSS64 *getWeights() { return nullptr; }
     virtual RSS64 *getBiases() { return nullptr; }
     // reluPrime / maxPrime of ReLU and Maxpool layers (Share<uint8_t>)
@@ -55,6 +60,13 @@ class Layer {
     // (the ReLU layer's activations and reluPrime are written here, bit for bit
     // what its own forward would write).  Returns false when this layer cannot.
     virtual bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) { return false; }
+
+  protected:
+    // allocated on the first backward pass (inference never needs it)
+    void ensureDeltas() {
+        if (deltas.size() != inputSize()) deltas.resize(inputSize());
+    }
+    RSS64 deltas;
 };
 
 class FCLayer : public Layer {
@@ -62,6 +74,7 @@ class FCLayer : public Layer {
     FCLayer(const FCConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
     bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS64 *getWeights() override { return &weights; }
@@ -84,6 +97,7 @@ class CNNLayer : public Layer {
     CNNLayer(const CNNConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
     bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS64 *getWeights() override { return &weights; }
@@ -109,6 +123,7 @@ class ReLULayer : public Layer {
     ReLULayer(const ReLUConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
     RSS64 *getActivation() override { return &activations; }
     RSS<u8> *getDerivative() override { return &reluPrime; }
     size_t inputSize() const override { return conf.batchSize * conf.inputDim; }
@@ -123,6 +138,7 @@ class MaxpoolLayer : public Layer {
     MaxpoolLayer(const MaxpoolConfig &conf, int layerNum, int seed);
     void printLayer() override;
     void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
     bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS<u8> *getDerivative() override { return &maxPrime; }
@@ -152,10 +168,22 @@ class NeuralNetwork {
     // ReLU layer is evaluated inside the layer before it.  after_layer(i), when
     // given, is called once every layer i has been issued (in order).
     void run(const std::function<void(size_t)> &after_layer = nullptr);
+    // NeuralNetwork::backward, NeuralNetwork.cu:136-149: labels (one-hot,
+    // batch x classes doubles) become public shares, then the reveal-softmax
+    // delta and the backward pass with weight updates.
+    void backward(const std::vector<double> &labels);
+    // the same from label shares already on the device ([3][2][batch*classes])
+    void backwardShares(const RSS64 &labelShare);
     void printNetwork();
+    // dL/d(network output) of the last backward (input of the last layer's backward)
+    RSS64 outputDelta;
     std::vector<std::unique_ptr<Layer>> layers;
     RSS64 input;
     int batch;
+
+  private:
+    void _reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas);  // NeuralNetwork.cu:207-267
+    void _backward_pass(const RSS64 &deltas);                        // NeuralNetwork.cu:446-467
 };
 
 }  // namespace pir

@@ -208,8 +208,8 @@ void dividePublic3out3(PartyData<u64> &a, u64 d, RSS64 &out) {
 
 size_t packed_rss_weights_bytes(int K, int N) { return limb_b_bytes(N, 6, round_up(K, LIMB_BK)); }
 
-int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed) {
-    LimbBSrc s{};
+int pack_rss_operand(const RSS64 &B, const LimbBSrc &geom, int8_t *packed) {
+    LimbBSrc s = geom;
     for (int p = 0; p < 3; p++) {
         s.b[2 * p][0] = B.getShare(p, 0)->data();
         s.b[2 * p][1] = B.getShare(p, 1)->data();
@@ -217,25 +217,76 @@ int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *pac
         s.coef[2 * p][0] = s.coef[2 * p][1] = s.coef[2 * p + 1][0] = 1;
     }
     s.nseg = 6;
+    return limb_pack_b(s, packed, S());
+}
+
+int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed) {
+    LimbBSrc s{};
     s.K = K;
     s.N = N;
     s.Kp = round_up(K, LIMB_BK);
     s.row_major = b_row_major ? 1 : 0;
-    return limb_pack_b(s, packed, S());
+    return pack_rss_operand(B, s, packed);
+}
+
+Epilogue rss_trunc_epi(u64 *out, int trunc_bits, bool tc) {
+    Epilogue e{};
+    e.mode = EPI_TRUNC;
+    e.out = out;
+    e.nout = 6;
+    e.base_mask = (1u << (2 * PARTY_A + 0)) | (1u << (2 * PARTY_C + 1));  // A(1+q, 0), C(0, 1+q)
+    e.tc = tc ? 1 : 0;
+    e.trunc_bits = trunc_bits;
+    return e;
+}
+
+int rss_limb_run(const RSS64 &A, LimbA a, const int8_t *packed, int N, const Epilogue &e, int split) {
+    for (int p = 0; p < 3; p++)
+        for (int i = 0; i < 2; i++) a.seg[2 * p + i] = A.getShare(p, i)->data();
+    a.nseg = 6;
+    return limb_run(a, packed, N, e, split, S());
+}
+
+static LimbA conv_view(int conv, int batch, int H, int W, int f, int Cin, int stride, int pad) {
+    LimbA a{};
+    a.conv = conv;
+    a.row_major = 1;
+    a.H = H;
+    a.W = W;
+    a.Cin = Cin;
+    a.fh = a.fw = f;
+    a.ph = a.pw = pad;
+    a.stride = stride;
+    a.OH = (H + 2 * pad - f) / stride + 1;
+    a.OW = (W + 2 * pad - f) / stride + 1;
+    return a;
+}
+
+LimbA limb_dgrad_view(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad) {
+    LimbA a = conv_view(LIMB_DGRAD, batch, H, W, f, Dout, stride, pad);
+    a.M = batch * H * W;
+    a.K = f * f * Dout;
+    a.Kp = round_up(a.K, LIMB_BK);
+    return a;
+}
+
+LimbA limb_wgrad_view(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad) {
+    LimbA a = conv_view(LIMB_WGRAD, batch, H, W, f, Din, stride, pad);
+    a.M = f * f * Din;
+    a.K = batch * a.OH * a.OW;
+    a.Kp = round_up(a.K, LIMB_BK);
+    return a;
 }
 
 int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
                      int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split, u64 *relu_out,
                      u8 *relu_d) {
     LimbA a{};
-    for (int p = 0; p < 3; p++)
-        for (int i = 0; i < 2; i++) a.seg[2 * p + i] = A.getShare(p, i)->data();
-    a.nseg = 6;
     a.M = M;
     a.K = K;
     a.Kp = round_up(K, LIMB_BK);
     if (conv) {
-        a.conv = 1;
+        a.conv = LIMB_FPROP;
         a.row_major = 1;
         a.H = conv->H;
         a.W = conv->W;
@@ -249,17 +300,11 @@ int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int 
         a.lda = ta ? K : M;
         a.row_major = ta ? 1 : 0;
     }
-    Epilogue e{};
-    e.mode = EPI_TRUNC;
-    e.out = out.base();
-    e.nout = 6;
-    e.base_mask = (1u << (2 * PARTY_A + 0)) | (1u << (2 * PARTY_C + 1));  // A(1+q, 0), C(0, 1+q)
-    e.tc = tc ? 1 : 0;
-    e.trunc_bits = trunc_bits;
+    Epilogue e = rss_trunc_epi(out.base(), trunc_bits, tc);
     e.bias = bias ? bias->base() : nullptr;
     e.relu_out = relu_out;
     e.relu_d = relu_d;
-    return limb_run(a, packed, N, e, split, S());
+    return rss_limb_run(A, a, packed, N, e, split);
 }
 
 // RSS.inl:1096-1110: three gemms per party into a scratch accumulator
@@ -308,16 +353,79 @@ void localFprop(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batch, in
     }
 }
 
-// RSS.inl:705-803 (kFprop)
+// RSS.inl:641-672
+void localDgrad(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batch, int OH, int OW, int Dout, int fh,
+                int fw, int Din, int ph, int pw, int stride, int dil, int H, int W) {
+    PartyData<u64> acc(C.size());
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int t = 0; t < 3; t++) {
+        for (int q = 0; q < 3; q++)
+            check(plain_conv_dgrad(A.getShare(q, terms[t][0])->data(), B.getShare(q, terms[t][1])->data(),
+                                   acc[q].data(), batch, OH, OW, Dout, fh, fw, Din, ph, pw, stride, dil, H, W, S()));
+        C += acc;
+    }
+}
+
+// RSS.inl:674-703
+void localWgrad(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batch, int OH, int OW, int Dout, int H, int W,
+                int Din, int fh, int fw, int ph, int pw, int stride, int dil) {
+    PartyData<u64> acc(C.size());
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    for (int t = 0; t < 3; t++) {
+        for (int q = 0; q < 3; q++)
+            check(plain_conv_wgrad(A.getShare(q, terms[t][0])->data(), B.getShare(q, terms[t][1])->data(),
+                                   acc[q].data(), batch, OH, OW, Dout, H, W, Din, fh, fw, ph, pw, stride, dil, S()));
+        C += acc;
+    }
+}
+
+static void local_conv(ConvOperator op, const RSS64 &A, const RSS64 &B, PartyData<u64> &local, int batch, int H,
+                       int W, int f, int Din, int Dout, int stride, int pad, int OH, int OW) {
+    switch (op) {
+        case kFprop: localFprop(A, B, local, batch, H, W, Din, Dout, f, f, pad, pad, stride, 1); break;
+        case kDgrad: localDgrad(A, B, local, batch, OH, OW, Dout, f, f, Din, pad, pad, stride, 1, H, W); break;
+        default: localWgrad(A, B, local, batch, OH, OW, Dout, H, W, Din, f, f, pad, pad, stride, 1); break;
+    }
+}
+
+// The dgrad and wgrad products in the fused limb form: one GEMM launch over the
+// six share segments with the dividePublic3out3 epilogue.
+void rss_conv_grad_limb(ConvOperator op, const RSS64 &A, const RSS64 &B, RSS64 &C, int batch, int H, int W, int f,
+                        int Din, int Dout, int stride, int pad, int trunc) {
+    LimbBSrc g{};
+    LimbA a;
+    int N;
+    if (op == kDgrad) {  // A = delta (gathered), B = filter in (r, s, dout)-major order
+        a = limb_dgrad_view(batch, H, W, f, Din, Dout, stride, pad);
+        N = Din;
+        g.dgrad_filter = 1;
+        g.kfh = g.kfw = f;
+        g.kc = Dout;
+    } else {  // A = forward input (transposed im2col), B = delta [b*OH*OW][Dout]
+        a = limb_wgrad_view(batch, H, W, f, Din, Dout, stride, pad);
+        N = Dout;
+        g.row_major = 1;
+    }
+    g.K = a.K;
+    g.N = N;
+    g.Kp = a.Kp;
+    DevBuf packed(limb_b_bytes(N, 6, g.Kp));
+    const RSS64 &Aop = op == kDgrad ? A : B;
+    const RSS64 &Bop = op == kDgrad ? B : A;
+    check(pack_rss_operand(Bop, g, (int8_t *)packed.get()));
+    check(rss_limb_run(Aop, a, (const int8_t *)packed.get(), N, rss_trunc_epi(C.base(), trunc, op == kDgrad)));
+}
+
+// RSS.inl:705-803
 void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int batch, int H, int W, int f, int Din,
                  int Dout, int stride, int pad, int trunc) {
-    if (op != kFprop) {
-        set_error("convolution: only kFprop is implemented");
-        throw PirError(ERR_UNSUPPORTED);
-    }
     int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
     if (fused_mode() && A.base() && B.base() && C.base()) {
         if (ctx().algo == ALGO_LIMB) {
+            if (op != kFprop) {
+                rss_conv_grad_limb(op, A, B, C, batch, H, W, f, Din, Dout, stride, pad, trunc);
+                return;
+            }
             int K = f * f * Din;
             DevBuf packed(packed_rss_weights_bytes(K, Dout));
             check(pack_rss_weights(B, K, Dout, false, (int8_t *)packed.get()));
@@ -327,12 +435,12 @@ void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int 
             return;
         }
         PartyData<u64> local(C.size());
-        localFprop(A, B, local, batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+        local_conv(op, A, B, local, batch, H, W, f, Din, Dout, stride, pad, OH, OW);
         check(fused_divide_3out3(local.base(), local.size(), trunc, C.base(), S()));
         return;
     }
     PartyData<u64> local(C.size());
-    localFprop(A, B, local, batch, H, W, Din, Dout, f, f, pad, pad, stride, 1);
+    local_conv(op, A, B, local, batch, H, W, f, Din, Dout, stride, pad, OH, OW);
     dividePublic3out3(local, (u64)1 << trunc, C);
 }
 

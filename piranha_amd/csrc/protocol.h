@@ -18,7 +18,8 @@ namespace pir {
 
 typedef RSS<u64> RSS64;
 
-enum ConvOperator { kFprop = 0 };
+// cutlass::conv::Operator (RSS.inl:705-790)
+enum ConvOperator { kFprop = 0, kDgrad = 1, kWgrad = 2 };
 
 template <typename T>
 void reconstruct(const RSS<T> &in, PartyData<T> &out);
@@ -35,6 +36,15 @@ void matmul(const RSS64 &a, const RSS64 &b, RSS64 &c, int M, int N, int K, bool 
 void localFprop(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batchSize, int imageHeight, int imageWidth,
                 int Din, int Dout, int filterHeight, int filterWidth, int paddingHeight, int paddingWidth, int stride,
                 int dilation);
+// RSS.inl:641-703
+void localDgrad(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batchSize, int outputHeight, int outputWidth,
+                int Dout, int filterHeight, int filterWidth, int Din, int paddingHeight, int paddingWidth, int stride,
+                int dilation, int imageHeight, int imageWidth);
+void localWgrad(const RSS64 &A, const RSS64 &B, PartyData<u64> &C, int batchSize, int outputHeight, int outputWidth,
+                int Dout, int imageHeight, int imageWidth, int Din, int filterHeight, int filterWidth,
+                int paddingHeight, int paddingWidth, int stride, int dilation);
+// kFprop: A = input, B = filter, C = output; kDgrad: A = delta, B = filter,
+// C = input gradient; kWgrad: A = delta, B = forward input, C = filter gradient.
 void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int batchSize, int imageHeight,
                  int imageWidth, int filterSize, int Din, int Dout, int stride, int padding, int truncation);
 template <typename U>
@@ -69,7 +79,19 @@ RSS<T> &RSS<T>::operator&=(const RSS<T> &r) {
 // limb form [B_p0 + B_p1 ; B_p0] for p = A, B, C.
 int pack_rss_weights(const RSS64 &B, int K, int N, bool b_row_major, int8_t *packed);
 size_t packed_rss_weights_bytes(int K, int N);
-// Choose a split-K factor for an M x N output with `nks` K-steps.
+// Same 6-segment packing with the operand geometry given by `geom` (K, N, Kp,
+// row_major or the dgrad filter view; the share pointers are filled in here).
+int pack_rss_operand(const RSS64 &B, const LimbBSrc &geom, int8_t *packed);
+// Fused 3-party product over an A view `a` (geometry and conv form set by the
+// caller; the six share segments are filled in here from A) into the
+// epilogue `e` (mode EPI_TRUNC with the RSS output layout).
+int rss_limb_run(const RSS64 &A, LimbA a, const int8_t *packed, int N, const Epilogue &e, int split = 0);
+// The 3out3 output epilogue of the fused product: out [3][2][M*N] receives
+// A(1+q, 0), B(0, 0), C(0, 1+q).
+Epilogue rss_trunc_epi(u64 *out, int trunc_bits, bool tc);
+// Geometry of the dgrad / wgrad limb views of a convolution layer.
+LimbA limb_dgrad_view(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad);
+LimbA limb_wgrad_view(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad);
 // Runs the fused 3-party product + truncation (+ bias) on packed weights.
 // conv == nullptr: dense A (row-major if ta).  out layout row-major if tc.
 struct ConvShape {

@@ -19,8 +19,21 @@ constexpr int LIMB_MAX_SEG = 12;  // K-segments per product (9 for the 4-party m
 constexpr int LIMB_MAX_TERMS = 4;  // share arrays summed into one packed B segment
 
 // Operand A description for the limb GEMM: up to LIMB_MAX_SEG K-segments (one
-// per share array), each either a dense matrix or the implicit im2col view of
-// an NHWC activation tensor.
+// per share array), each either a dense matrix or an implicit-GEMM view of an
+// NHWC tensor.
+enum LimbConv {
+    LIMB_DENSE = 0,
+    // fprop im2col of x [b][H][W][Cin]: m = (b, oh, ow), k = (r*fw + s)*Cin + c
+    LIMB_FPROP = 1,
+    // dgrad gather of dy [b][OH][OW][Cin] (Cin = the layer's output channels):
+    // m = (b, h, w) of the H x W input image, k = (r*fw + s)*Cin + c,
+    // A(m, k) = dy[b][p][q][c] where h + ph - r = p*stride and w + pw - s = q*stride
+    LIMB_DGRAD = 2,
+    // wgrad transposed im2col of x [b][H][W][Cin]: m = (r*fw + s)*Cin + c,
+    // k = (b, p, q) over the OH x OW output, A(m, k) = x[b][p*stride-ph+r][q*stride-pw+s][c]
+    LIMB_WGRAD = 3,
+};
+
 struct LimbA {
     const u64 *seg[LIMB_MAX_SEG];
     int nseg;
@@ -28,19 +41,22 @@ struct LimbA {
     int Kp;          // K rounded up to the K-step (32)
     int lda;         // dense: leading dimension
     int row_major;   // dense: A(m,k) = row_major ? p[m*lda+k] : p[k*lda+m]
-    int conv;        // 1: implicit im2col (NHWC input, k = (r*fw + s)*Cin + c)
+    int conv;        // LimbConv
     int H, W, Cin, fh, fw, ph, pw, stride, OH, OW;
 };
 
 // Operand B (weights) description for the packing kernel: segment s is the
 // mod-2^64 combination sum_t coef[s][t] * b[s][t] of up to LIMB_MAX_TERMS share
-// arrays (null pointers are skipped); B(k,n) = row_major ? p[k*N+n] : p[n*K+k].
+// arrays (null pointers are skipped); B(k,n) = row_major ? p[k*N+n] : p[n*K+k],
+// or, with dgrad_filter set, the filter [kc][kfh][kfw][N] read as
+// B((r*kfw + s)*kc + c, n) = p[((c*kfh + r)*kfw + s)*N + n].
 struct LimbBSrc {
     const u64 *b[LIMB_MAX_SEG][LIMB_MAX_TERMS];
     int coef[LIMB_MAX_SEG][LIMB_MAX_TERMS];
     int nseg;
     int K, N, Kp;
     int row_major;
+    int dgrad_filter, kfh, kfw, kc;
 };
 
 // Epilogue modes.
@@ -61,6 +77,13 @@ struct Epilogue {
     unsigned base_mask;
     int trunc_bits;
     const u64 *bias;  // [nout][N] or null
+    // div_bits > 0: the truncated value v = 1 + q is then divided by
+    // 2^div_bits as dividePublic does on those shares (RSS.inl:422-436):
+    // v' = 1 + trunc((v - 2^div_bits) / 2^div_bits).
+    int div_bits;
+    // sub_out: subtract the result shares from `out` instead of storing them
+    // (a weight update W -= dW in the GEMM that produces dW); bias must be null.
+    int sub_out;
     // Optional ReLU of the output (ReLU layer fused into its producer; the six
     // RSS arrays only, nout == 6): relu_out [3][2][M*N] = x * [x >= 0] and
     // relu_d [3][2][M*N] = dReLU bits (RSS.inl:806-865), both in out's layout.
@@ -69,6 +92,9 @@ struct Epilogue {
 };
 
 constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;
+constexpr int LIMB_MAX_SPLIT = 512;     // split-K factors the planner considers
+constexpr int LIMB_VARIANT_WIDE = 8;    // kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
+constexpr int LIMB_VARIANTS = 16;
 constexpr int LIMB_FLUSH_STEPS = 256;  // 256 * 32 = 8192 k per int32 accumulation chunk
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -127,6 +153,11 @@ int limb_ring_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, boo
                    hipStream_t s);
 int limb_ring_conv(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
                    int ph, int pw, int stride, hipStream_t s);
+// gpu::conv_dgrad / conv_wgrad (conv.cuh:405-490) in the limb form (dil = 1).
+int limb_ring_dgrad(const u64 *dy, const u64 *w, u64 *dx, int b, int OH, int OW, int Dout, int fh, int fw, int Din,
+                    int ph, int pw, int stride, int H, int W, hipStream_t s);
+int limb_ring_wgrad(const u64 *dy, const u64 *x, u64 *dw, int b, int OH, int OW, int Dout, int H, int W, int Din,
+                    int fh, int fw, int ph, int pw, int stride, hipStream_t s);
 
 // Process-wide autotune table.
 int limb_tune_entries();
@@ -139,6 +170,10 @@ int limb_tune_load(const char *path);
 int plain_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc, hipStream_t s);
 int plain_conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
                      int ph, int pw, int stride, int dil, hipStream_t s);
+int plain_conv_dgrad(const u64 *dy, const u64 *w, u64 *dx, int b, int OH, int OW, int Dout, int fh, int fw, int Din,
+                     int ph, int pw, int stride, int dil, int H, int W, hipStream_t s);
+int plain_conv_wgrad(const u64 *dy, const u64 *x, u64 *dw, int b, int OH, int OW, int Dout, int H, int W, int Din,
+                     int fh, int fw, int ph, int pw, int stride, int dil, hipStream_t s);
 
 // Standalone truncation epilogue over a raw (3-of-3 summed) product, used by
 // the split-K path and by callers that already hold Sum_p raw_p.

@@ -74,7 +74,13 @@ __device__ __forceinline__ void trunc_store(const Epilogue &e, size_t MN, int N,
     const int k = e.trunc_bits;
     u64 q = trunc_div_pow2(raw - ((u64)1 << k), k);
     u64 base = (u64)1 + q;
+    if (e.div_bits) base = (u64)1 + trunc_div_pow2(base - ((u64)1 << e.div_bits), e.div_bits);
     size_t idx = e.tc ? (size_t)m * N + n : (size_t)n * (MN / N) + m;
+    if (e.sub_out) {
+        for (int j = 0; j < e.nout; j++)
+            if ((e.base_mask >> j) & 1u) e.out[(size_t)j * MN + idx] -= base;
+        return;
+    }
     if (e.relu_out) {
         // the following ReLU layer on the same element: dReLU + selectShare
         Sh3<u64> x;
@@ -130,7 +136,15 @@ __global__ __launch_bounds__(256) void limb_pack_b_kernel(LimbBSrc src, int8_t *
         int k = kbase + j;
         u64 v = 0;
         if (n < src.N && k < src.K) {
-            size_t idx = src.row_major ? (size_t)k * src.N + n : (size_t)n * src.K + k;
+            size_t idx;
+            if (src.dgrad_filter) {
+                int fwc = src.kfw * src.kc;
+                int r = k / fwc, rem = k - r * fwc;
+                int sc = rem / src.kc, c = rem - sc * src.kc;
+                idx = (((size_t)c * src.kfh + r) * src.kfw + sc) * src.N + n;
+            } else {
+                idx = src.row_major ? (size_t)k * src.N + n : (size_t)n * src.K + k;
+            }
 #pragma unroll
             for (int t = 0; t < LIMB_MAX_TERMS; t++)
                 if (src.b[seg][t]) v += (u64)(s64)src.coef[seg][t] * src.b[seg][t][idx];
@@ -147,6 +161,8 @@ __global__ __launch_bounds__(256) void limb_pack_b_kernel(LimbBSrc src, int8_t *
 
 int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
     PIR_REQUIRE(src.nseg >= 1 && src.nseg <= LIMB_MAX_SEG, "limb_pack_b: nseg must be 1..12");
+    PIR_REQUIRE(!src.dgrad_filter || (src.kfh > 0 && src.kfw > 0 && src.kc > 0 && src.K == src.kfh * src.kfw * src.kc),
+                "limb_pack_b: dgrad filter view needs K = kfh * kfw * kc");
     PIR_REQUIRE(src.Kp % LIMB_BK == 0 && src.Kp >= src.K, "limb_pack_b: Kp must be a multiple of 32 >= K");
     int nks = src.nseg * src.Kp / LIMB_BK;
     size_t total = (size_t)((src.N + LIMB_BN - 1) / LIMB_BN) * LIMB_BN * nks * (LIMB_BK / 8);
@@ -179,33 +195,51 @@ constexpr int LDS_ROW = 32;
 //   AMODE 0: dense row-major A, 1: dense column-major A,
 //   AMODE 2: implicit im2col, any Cin (per-element decode),
 //   AMODE 3: implicit im2col with Cin % 32 == 0 (one (r, s) per K-step, decoded
-//            once per workgroup and advanced incrementally).
+//            once per workgroup and advanced incrementally),
+//   AMODE 4: dgrad gather of dy, any channel count (per-element decode),
+//   AMODE 5: dgrad gather with Cin % 32 == 0 (one (r, s) per K-step),
+//   AMODE 6: wgrad transposed im2col: the row fixes (r, s, c), the thread's KPT
+//            consecutive k are consecutive output pixels (b, p, q).
 template <int AMODE, int KPT>
 struct ALoader {
     const LimbA &a;
     int am, akq, kps;
     bool row_ok;
     int seg, kk;          // uniform: segment and K-step within the segment
-    int cb, ih0, iw0;     // conv: image and top-left input coordinate of this row's pixel
-    int r, s, c0;         // AMODE 3: k = kk*32 = (r*fw + s)*Cin + c0
+    int cb, ih0, iw0;     // fprop: image and top-left input coordinate of this row's pixel;
+                          // dgrad: image and (h + ph, w + pw) of this row's pixel
+    int r, s, c0;         // AMODE 3/5: k = kk*32 = (r*fw + s)*Cin + c0; AMODE 6: the row's (r, s, c)
 
     __device__ ALoader(const LimbA &a_, int am_, int akq_) : a(a_), am(am_), akq(akq_) {
         kps = a.Kp / LIMB_BK;
         row_ok = am < a.M;
         cb = ih0 = iw0 = 0;
         r = s = c0 = 0;
-        if (AMODE >= 2) {
-            int m = row_ok ? am : 0;
+        const int m = row_ok ? am : 0;
+        if (AMODE == 2 || AMODE == 3) {
             int ohw = a.OH * a.OW;
             cb = m / ohw;
             int rem = m - cb * ohw;
             int oh = rem / a.OW, ow = rem - (rem / a.OW) * a.OW;
             ih0 = oh * a.stride - a.ph;
             iw0 = ow * a.stride - a.pw;
+        } else if (AMODE == 4 || AMODE == 5) {
+            int hw = a.H * a.W;
+            cb = m / hw;
+            int rem = m - cb * hw;
+            int h = rem / a.W, w = rem - (rem / a.W) * a.W;
+            ih0 = h + a.ph;
+            iw0 = w + a.pw;
+        } else if (AMODE == 6) {
+            int fwc = a.fw * a.Cin;
+            r = m / fwc;
+            int rem = m - r * fwc;
+            s = rem / a.Cin;
+            c0 = rem - s * a.Cin;
         }
     }
     __device__ void decode() {
-        if (AMODE == 3) {
+        if (AMODE == 3 || AMODE == 5) {
             int k = kk * LIMB_BK, fwc = a.fw * a.Cin;
             r = k / fwc;
             int rem = k - r * fwc;
@@ -223,7 +257,7 @@ struct ALoader {
             kk = 0;
             seg++;
             decode();
-        } else if (AMODE == 3) {
+        } else if (AMODE == 3 || AMODE == 5) {
             c0 += LIMB_BK;
             if (c0 == a.Cin) {
                 c0 = 0;
@@ -233,6 +267,20 @@ struct ALoader {
                 }
             }
         }
+    }
+    // dgrad: output pixel (p, q) of dy that tap (rr, ss) of this row's pixel reads
+    __device__ __forceinline__ bool dgrad_src(int rr, int ss, int &p, int &q) const {
+        int hp = ih0 - rr, wq = iw0 - ss;
+        bool ok = hp >= 0 && wq >= 0;
+        if (a.stride == 1) {
+            p = hp;
+            q = wq;
+        } else {
+            ok = ok && (hp % a.stride) == 0 && (wq % a.stride) == 0;
+            p = hp / a.stride;
+            q = wq / a.stride;
+        }
+        return ok && p < a.OH && q < a.OW;
     }
     __device__ void load(u64 w[KPT]) const {
         const u64 *__restrict__ src = a.seg[seg];
@@ -245,7 +293,15 @@ struct ALoader {
             const u64 msk = ok ? ~0ull : 0ull;
 #pragma unroll
             for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
-        } else if (AMODE == 2) {
+        } else if (AMODE == 5) {
+            int pp, qq;
+            bool ok = dgrad_src(r, s, pp, qq) && row_ok && kb < a.K;
+            const u64 *p = src + (((size_t)cb * a.OH + (ok ? pp : 0)) * a.OW + (ok ? qq : 0)) * a.Cin +
+                           (ok ? c0 + akq * KPT : 0);
+            const u64 msk = ok ? ~0ull : 0ull;
+#pragma unroll
+            for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
+        } else if (AMODE == 2 || AMODE == 4) {
             const int fwc = a.fw * a.Cin;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
@@ -254,10 +310,37 @@ struct ALoader {
                 int rem = k - rr * fwc;
                 int ss = rem / a.Cin;
                 int cc = rem - ss * a.Cin;
-                int ih = ih0 + rr, iw = iw0 + ss;
-                bool ok = row_ok && k < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
-                size_t idx = ok ? (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + cc : 0;
+                bool ok;
+                size_t idx;
+                if (AMODE == 2) {
+                    int ih = ih0 + rr, iw = iw0 + ss;
+                    ok = row_ok && k < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
+                    idx = ok ? (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + cc : 0;
+                } else {
+                    int pp, qq;
+                    ok = row_ok && k < a.K && dgrad_src(rr, ss, pp, qq);
+                    idx = ok ? (((size_t)cb * a.OH + pp) * a.OW + qq) * a.Cin + cc : 0;
+                }
                 w[j] = src[idx] & (ok ? ~0ull : 0ull);
+            }
+        } else if (AMODE == 6) {
+            const int ohw = a.OH * a.OW;
+            int n = kb / ohw;
+            int rem = kb - n * ohw;
+            int p = rem / a.OW, q = rem - (rem / a.OW) * a.OW;
+#pragma unroll
+            for (int j = 0; j < KPT; j++) {
+                int ih = p * a.stride - a.ph + r, iw = q * a.stride - a.pw + s;
+                bool ok = row_ok && kb + j < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
+                size_t idx = ok ? (((size_t)n * a.H + ih) * a.W + iw) * a.Cin + c0 : 0;
+                w[j] = src[idx] & (ok ? ~0ull : 0ull);
+                if (++q == a.OW) {
+                    q = 0;
+                    if (++p == a.OH) {
+                        p = 0;
+                        ++n;
+                    }
+                }
             }
         } else if (AMODE == 0 && (a.K % KPT) == 0) {
             // a thread's KPT consecutive k are all inside K or all outside: one contiguous run
@@ -336,8 +419,9 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     const int ks_begin = bz * steps_per_split;
     const int ks_end = min(nks, ks_begin + steps_per_split);
 
-    const int arow = AMODE == 1 ? tid % TM : tid / KG;
-    const int akq = AMODE == 1 ? tid / TM : tid % KG;
+    // column-major forms: consecutive lanes take consecutive rows (contiguous in memory)
+    const int arow = (AMODE == 1 || AMODE == 6) ? tid % TM : tid / KG;
+    const int akq = (AMODE == 1 || AMODE == 6) ? tid / TM : tid % KG;
     ALoader<AMODE, KPT> L(a, m0 + arow, akq);
 
     // this lane's 16 bytes of each B limb plane: column n0 + lane % 32, k half lane / 32
@@ -438,7 +522,7 @@ int heuristic_split(int M, int N, int nks) {
     int cap = nks / 8;
     if (cap < 1) cap = 1;
     if (want > cap) want = cap;
-    if (want > 64) want = 64;
+    if (want > LIMB_MAX_SPLIT) want = LIMB_MAX_SPLIT;
     return want < 1 ? 1 : want;
 }
 
@@ -537,6 +621,9 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.nout >= 1 && epi.nout <= 32), "limb_gemm: bad output share count");
     PIR_REQUIRE(!epi.relu_out || (epi.mode == EPI_TRUNC && epi.nout == 6 && epi.relu_d),
                 "limb_gemm: a fused ReLU needs the six RSS output arrays");
+    PIR_REQUIRE(!epi.sub_out || (epi.mode == EPI_TRUNC && !epi.bias && !epi.relu_out),
+                "limb_gemm: the subtracting epilogue takes no bias and no fused ReLU");
+    PIR_REQUIRE(epi.div_bits >= 0 && epi.div_bits <= 62, "limb_gemm: division bits must be in [0, 62]");
     if (a.M == 0 || N == 0) return OK;
     const int nks = a.nseg * a.Kp / LIMB_BK;
     const int split = limb_effective_split(nks, plan.split);
@@ -546,8 +633,14 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     const int wide = plan.wide;
     dim3 grid(wide ? (a.M + 31) / 32 : (a.M + 63) / 64, wide ? (N + 127) / 128 : (N + 63) / 64, split);
     u64 *part = split > 1 ? workspace : nullptr;
-    const int amode = a.conv ? ((a.Cin % LIMB_BK) == 0 ? 3 : 2) : (a.row_major ? 0 : 1);
-    const int variant = amode + (wide ? 4 : 0);
+    int amode;
+    switch (a.conv) {
+        case LIMB_FPROP: amode = (a.Cin % LIMB_BK) == 0 ? 3 : 2; break;
+        case LIMB_DGRAD: amode = (a.Cin % LIMB_BK) == 0 ? 5 : 4; break;
+        case LIMB_WGRAD: amode = 6; break;
+        default: amode = a.row_major ? 0 : 1; break;
+    }
+    const int variant = amode + (wide ? LIMB_VARIANT_WIDE : 0);
     GemmTimer &timer = ctx().timer;
     GemmTimer::Rec rec{nullptr, nullptr, 2.0 * 36 * a.M * (double)N * a.nseg * a.K, variant};
     if (timer.on) {
@@ -562,10 +655,16 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
         case 1: PIR_LIMB_LAUNCH(1, 2); break;
         case 2: PIR_LIMB_LAUNCH(2, 2); break;
         case 3: PIR_LIMB_LAUNCH(3, 2); break;
-        case 4: PIR_LIMB_LAUNCH(0, 1); break;
-        case 5: PIR_LIMB_LAUNCH(1, 1); break;
-        case 6: PIR_LIMB_LAUNCH(2, 1); break;
-        default: PIR_LIMB_LAUNCH(3, 1); break;
+        case 4: PIR_LIMB_LAUNCH(4, 2); break;
+        case 5: PIR_LIMB_LAUNCH(5, 2); break;
+        case 6: PIR_LIMB_LAUNCH(6, 2); break;
+        case LIMB_VARIANT_WIDE + 0: PIR_LIMB_LAUNCH(0, 1); break;
+        case LIMB_VARIANT_WIDE + 1: PIR_LIMB_LAUNCH(1, 1); break;
+        case LIMB_VARIANT_WIDE + 2: PIR_LIMB_LAUNCH(2, 1); break;
+        case LIMB_VARIANT_WIDE + 3: PIR_LIMB_LAUNCH(3, 1); break;
+        case LIMB_VARIANT_WIDE + 4: PIR_LIMB_LAUNCH(4, 1); break;
+        case LIMB_VARIANT_WIDE + 5: PIR_LIMB_LAUNCH(5, 1); break;
+        default: PIR_LIMB_LAUNCH(6, 1); break;
     }
 #undef PIR_LIMB_LAUNCH
     PIR_LAUNCH_CHECK();
@@ -589,7 +688,7 @@ int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, 
         std::vector<LimbPlan> cand;
         for (int wide = can_be_wide(N) ? 1 : 0; wide >= 0; wide--)
             for (int f : {h / 4, h / 2, h, 2 * h, 4 * h}) {
-                if (f < 1 || f > 64) continue;
+                if (f < 1 || f > LIMB_MAX_SPLIT) continue;
                 LimbPlan p{wide, limb_effective_split(nks, f)};
                 bool dup = false;
                 for (auto &q : cand) dup |= (q.wide == p.wide && q.split == p.split);
@@ -686,7 +785,7 @@ int limb_ring_conv(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int 
     a.M = M;
     a.K = K;
     a.Kp = src.Kp;
-    a.conv = 1;
+    a.conv = LIMB_FPROP;
     a.row_major = 1;
     a.H = H;
     a.W = W;
@@ -705,33 +804,129 @@ int limb_ring_conv(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int 
     return limb_run(a, (const int8_t *)packed.get(), Dout, e, 0, s);
 }
 
+// gpu::conv_dgrad: the limb GEMM M = b*H*W, N = Din, K = fh*fw*Dout over the
+// dgrad gather of dy and the filter read in (r, s, c)-major order.
+int limb_ring_dgrad(const u64 *dy, const u64 *w, u64 *dx, int b, int OH, int OW, int Dout, int fh, int fw, int Din,
+                    int ph, int pw, int stride, int H, int W, hipStream_t s) {
+    PIR_REQUIRE(b > 0 && H > 0 && W > 0 && OH > 0 && OW > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 &&
+                    stride > 0 && ph >= 0 && pw >= 0,
+                "limb dgrad: invalid geometry");
+    const int K = fh * fw * Dout;
+    LimbBSrc src{};
+    src.b[0][0] = w;
+    src.coef[0][0] = 1;
+    src.nseg = 1;
+    src.K = K;
+    src.N = Din;
+    src.Kp = round_up(K, LIMB_BK);
+    src.dgrad_filter = 1;
+    src.kfh = fh;
+    src.kfw = fw;
+    src.kc = Dout;
+    DevBuf packed(limb_b_bytes(Din, 1, src.Kp));
+    PIR_TRY(limb_pack_b(src, (int8_t *)packed.get(), s));
+    LimbA a{};
+    a.seg[0] = dy;
+    a.nseg = 1;
+    a.M = b * H * W;
+    a.K = K;
+    a.Kp = src.Kp;
+    a.conv = LIMB_DGRAD;
+    a.H = H;
+    a.W = W;
+    a.Cin = Dout;
+    a.fh = fh;
+    a.fw = fw;
+    a.ph = ph;
+    a.pw = pw;
+    a.stride = stride;
+    a.OH = OH;
+    a.OW = OW;
+    Epilogue e{};
+    e.mode = EPI_RAW;
+    e.C = dx;
+    e.tc = 1;
+    return limb_run(a, (const int8_t *)packed.get(), Din, e, 0, s);
+}
+
+// gpu::conv_wgrad: the limb GEMM M = fh*fw*Din, N = Dout, K = b*OH*OW over the
+// transposed im2col of x and dy as a row-major K x N operand; the M x N result
+// is written column-major, which is the [Dout][fh][fw][Din] filter layout.
+int limb_ring_wgrad(const u64 *dy, const u64 *x, u64 *dw, int b, int OH, int OW, int Dout, int H, int W, int Din,
+                    int fh, int fw, int ph, int pw, int stride, hipStream_t s) {
+    PIR_REQUIRE(b > 0 && H > 0 && W > 0 && OH > 0 && OW > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 &&
+                    stride > 0 && ph >= 0 && pw >= 0,
+                "limb wgrad: invalid geometry");
+    const int K = b * OH * OW;
+    LimbBSrc src{};
+    src.b[0][0] = dy;
+    src.coef[0][0] = 1;
+    src.nseg = 1;
+    src.K = K;
+    src.N = Dout;
+    src.Kp = round_up(K, LIMB_BK);
+    src.row_major = 1;
+    DevBuf packed(limb_b_bytes(Dout, 1, src.Kp));
+    PIR_TRY(limb_pack_b(src, (int8_t *)packed.get(), s));
+    LimbA a{};
+    a.seg[0] = x;
+    a.nseg = 1;
+    a.M = fh * fw * Din;
+    a.K = K;
+    a.Kp = src.Kp;
+    a.conv = LIMB_WGRAD;
+    a.H = H;
+    a.W = W;
+    a.Cin = Din;
+    a.fh = fh;
+    a.fw = fw;
+    a.ph = ph;
+    a.pw = pw;
+    a.stride = stride;
+    a.OH = OH;
+    a.OW = OW;
+    Epilogue e{};
+    e.mode = EPI_RAW;
+    e.C = dw;
+    e.tc = 0;
+    return limb_run(a, (const int8_t *)packed.get(), Dout, e, 0, s);
+}
+
 // ---------------------------------------------------------------------------
 // Plain 64-bit VALU GEMM: 64x64 tile, K-step 16, 4x4 outputs per thread.
+// blockIdx.z selects a K-chunk of `kchunk` k; with more than one chunk every
+// chunk writes its own M x N partial slab (C + z * M * N), summed afterwards.
 
 struct PlainConv {
     int H, W, Cin, fh, fw, ph, pw, stride, dil, OH, OW;
 };
 
-template <int AMODE, bool TB, bool TC>  // AMODE 0: A column-major, 1: row-major, 2: im2col
-__global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, const u64 *__restrict__ A,
+// AMODE 0: A column-major, 1: row-major, 2: fprop im2col, 3: dgrad gather
+// (m = (b, h, w), k = (r, s, c) over dy [b][OH][OW][Cin]), 4: wgrad transposed
+// im2col (m = (r, s, c), k = (b, p, q)).  BMODE 0: B(k, n) = B[n*K + k],
+// 1: B[k*N + n], 2: the dgrad filter [c][fh][fw][N] at k = (r*fw + s)*Cin + c.
+template <int AMODE, int BMODE, bool TC>
+__global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, int kchunk, const u64 *__restrict__ A,
                                                          const u64 *__restrict__ B, u64 *__restrict__ C,
                                                          PlainConv cv) {
     __shared__ u64 As[16][64 + 1];
     __shared__ u64 Bs[16][64 + 1];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+    C += (size_t)blockIdx.z * M * N;
     u64 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
 
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             int idx = tid + e * 256;
             int mm, kk;
-            if (AMODE == 0) {
+            if (AMODE == 0 || AMODE == 4) {
                 kk = idx >> 6;
                 mm = idx & 63;
             } else {
@@ -740,10 +935,10 @@ __global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, co
             }
             int m = m0 + mm, k = k0 + kk;
             u64 v = 0;
-            if (m < M && k < K) {
+            if (m < M && k < kend) {
                 if (AMODE == 0) v = A[(size_t)k * M + m];
                 else if (AMODE == 1) v = A[(size_t)m * K + k];
-                else {
+                else if (AMODE == 2) {
                     int ohw = cv.OH * cv.OW;
                     int b = m / ohw, rem = m - b * ohw;
                     int oh = rem / cv.OW, ow = rem - oh * cv.OW;
@@ -753,11 +948,33 @@ __global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, co
                     int ih = oh * cv.stride - cv.ph + r * cv.dil, iw = ow * cv.stride - cv.pw + s * cv.dil;
                     if (ih >= 0 && ih < cv.H && iw >= 0 && iw < cv.W)
                         v = A[(((size_t)b * cv.H + ih) * cv.W + iw) * cv.Cin + c];
+                } else if (AMODE == 3) {
+                    int hw = cv.H * cv.W;
+                    int b = m / hw, rem = m - b * hw;
+                    int h = rem / cv.W, w = rem - h * cv.W;
+                    int fwc = cv.fw * cv.Cin;
+                    int r = k / fwc, rm = k - r * fwc;
+                    int s = rm / cv.Cin, c = rm - s * cv.Cin;
+                    int hp = h + cv.ph - r * cv.dil, wq = w + cv.pw - s * cv.dil;
+                    if (hp >= 0 && wq >= 0 && hp % cv.stride == 0 && wq % cv.stride == 0) {
+                        int p = hp / cv.stride, q = wq / cv.stride;
+                        if (p < cv.OH && q < cv.OW) v = A[(((size_t)b * cv.OH + p) * cv.OW + q) * cv.Cin + c];
+                    }
+                } else {
+                    int fwc = cv.fw * cv.Cin;
+                    int r = m / fwc, rm = m - r * fwc;
+                    int s = rm / cv.Cin, c = rm - s * cv.Cin;
+                    int ohw = cv.OH * cv.OW;
+                    int b = k / ohw, rem = k - b * ohw;
+                    int p = rem / cv.OW, q = rem - p * cv.OW;
+                    int ih = p * cv.stride - cv.ph + r * cv.dil, iw = q * cv.stride - cv.pw + s * cv.dil;
+                    if (ih >= 0 && ih < cv.H && iw >= 0 && iw < cv.W)
+                        v = A[(((size_t)b * cv.H + ih) * cv.W + iw) * cv.Cin + c];
                 }
             }
             As[kk][mm] = v;
             int nn;
-            if (TB) {
+            if (BMODE != 0) {
                 kk = idx >> 6;
                 nn = idx & 63;
             } else {
@@ -767,7 +984,16 @@ __global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, co
             int n = n0 + nn;
             k = k0 + kk;
             u64 bvv = 0;
-            if (n < N && k < K) bvv = TB ? B[(size_t)k * N + n] : B[(size_t)n * K + k];
+            if (n < N && k < kend) {
+                if (BMODE == 0) bvv = B[(size_t)n * K + k];
+                else if (BMODE == 1) bvv = B[(size_t)k * N + n];
+                else {
+                    int fwc = cv.fw * cv.Cin;
+                    int r = k / fwc, rm = k - r * fwc;
+                    int s = rm / cv.Cin, c = rm - s * cv.Cin;
+                    bvv = B[(((size_t)c * cv.fh + r) * cv.fw + s) * N + n];
+                }
+            }
             Bs[kk][nn] = bvv;
         }
         __syncthreads();
@@ -797,27 +1023,62 @@ __global__ __launch_bounds__(256) void plain_gemm_kernel(int M, int N, int K, co
         }
 }
 
-template <int AMODE>
-static void launch_plain(dim3 g, hipStream_t s, int M, int N, int K, const u64 *A, const u64 *B, u64 *C, bool tb,
+__global__ __launch_bounds__(256) void plain_split_sum_kernel(const u64 *__restrict__ part, int split, size_t MN,
+                                                              u64 *__restrict__ C) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= MN) return;
+    u64 v = 0;
+    for (int z = 0; z < split; z++) v += part[(size_t)z * MN + t];
+    C[t] = v;
+}
+
+template <int AMODE, int BMODE>
+static void launch_plain(dim3 g, hipStream_t s, int M, int N, int K, int kchunk, const u64 *A, const u64 *B, u64 *C,
                          bool tc, PlainConv cv) {
-    if (tb) {
-        if (tc) hipLaunchKernelGGL((plain_gemm_kernel<AMODE, true, true>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
-        else hipLaunchKernelGGL((plain_gemm_kernel<AMODE, true, false>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
-    } else {
-        if (tc) hipLaunchKernelGGL((plain_gemm_kernel<AMODE, false, true>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
-        else hipLaunchKernelGGL((plain_gemm_kernel<AMODE, false, false>), g, dim3(256), 0, s, M, N, K, A, B, C, cv);
+    if (tc) hipLaunchKernelGGL((plain_gemm_kernel<AMODE, BMODE, true>), g, dim3(256), 0, s, M, N, K, kchunk, A, B, C, cv);
+    else hipLaunchKernelGGL((plain_gemm_kernel<AMODE, BMODE, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, B, C, cv);
+}
+
+// K-chunking for small outputs with long K (wgrad): enough workgroups to cover
+// the chip, each chunk a multiple of the K-step.
+static int plain_split(int M, int N, int K) {
+    long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
+    if (tiles >= 256 || K < 1024) return 1;
+    long want = (512 + tiles - 1) / tiles;
+    long cap = K / 256;
+    if (want > cap) want = cap;
+    return want < 1 ? 1 : (int)want;
+}
+
+template <int AMODE, int BMODE>
+static int run_plain(int M, int N, int K, const u64 *A, const u64 *B, u64 *C, bool tc, PlainConv cv, hipStream_t s) {
+    if (M == 0 || N == 0) return OK;
+    int split = plain_split(M, N, K);
+    int kchunk = round_up((K + split - 1) / split, 16);
+    if (kchunk < 16) kchunk = 16;
+    split = (K + kchunk - 1) / kchunk;
+    if (split < 1) split = 1;
+    dim3 g((M + 63) / 64, (N + 63) / 64, split);
+    if (split == 1) {
+        launch_plain<AMODE, BMODE>(g, s, M, N, K, K > 0 ? K : 16, A, B, C, tc, cv);
+        PIR_LAUNCH_CHECK();
+        return OK;
     }
+    size_t MN = (size_t)M * N;
+    DevBuf part(MN * split * sizeof(u64));
+    launch_plain<AMODE, BMODE>(g, s, M, N, K, kchunk, A, B, (u64 *)part.get(), tc, cv);
+    PIR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(plain_split_sum_kernel, dim3(grid_for(MN, 256)), dim3(256), 0, s, (const u64 *)part.get(),
+                       split, MN, C);
+    PIR_LAUNCH_CHECK();
+    return OK;
 }
 
 int plain_gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc, hipStream_t s) {
     PIR_REQUIRE(M >= 0 && N >= 0 && K >= 0, "gemm: negative dimension");
-    if (M == 0 || N == 0) return OK;
-    dim3 g((M + 63) / 64, (N + 63) / 64);
     PlainConv cv{};
-    if (ta) launch_plain<1>(g, s, M, N, K, A, B, C, tb, tc, cv);
-    else launch_plain<0>(g, s, M, N, K, A, B, C, tb, tc, cv);
-    PIR_LAUNCH_CHECK();
-    return OK;
+    if (ta) return tb ? run_plain<1, 1>(M, N, K, A, B, C, tc, cv, s) : run_plain<1, 0>(M, N, K, A, B, C, tc, cv, s);
+    return tb ? run_plain<0, 1>(M, N, K, A, B, C, tc, cv, s) : run_plain<0, 0>(M, N, K, A, B, C, tc, cv, s);
 }
 
 int plain_conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, int Din, int Dout, int fh, int fw,
@@ -828,12 +1089,30 @@ int plain_conv_fprop(const u64 *x, const u64 *w, u64 *y, int b, int H, int W, in
     cv.OH = (H + 2 * ph - ((fh - 1) * dil + 1)) / stride + 1;
     cv.OW = (W + 2 * pw - ((fw - 1) * dil + 1)) / stride + 1;
     PIR_REQUIRE(cv.OH > 0 && cv.OW > 0, "conv_fprop: empty output");
-    int M = b * cv.OH * cv.OW, N = Dout, K = fh * fw * Din;
-    dim3 g((M + 63) / 64, (N + 63) / 64);
     // filter [Dout][fh][fw][Din] is B(k, n) = w[n*K + k]: column-major K x N
-    launch_plain<2>(g, s, M, N, K, x, w, y, false, true, cv);
-    PIR_LAUNCH_CHECK();
-    return OK;
+    return run_plain<2, 0>(b * cv.OH * cv.OW, Dout, fh * fw * Din, x, w, y, true, cv, s);
+}
+
+// gpu::conv_dgrad (conv.cuh:405-446): dx = sum over taps of dy . w, as the GEMM
+// M = b*H*W, N = Din, K = fh*fw*Dout with the dgrad gather of dy.
+int plain_conv_dgrad(const u64 *dy, const u64 *w, u64 *dx, int b, int OH, int OW, int Dout, int fh, int fw, int Din,
+                     int ph, int pw, int stride, int dil, int H, int W, hipStream_t s) {
+    PIR_REQUIRE(b > 0 && H > 0 && W > 0 && OH > 0 && OW > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 &&
+                    stride > 0 && dil > 0 && ph >= 0 && pw >= 0,
+                "conv_dgrad: invalid geometry");
+    PlainConv cv{H, W, Dout, fh, fw, ph, pw, stride, dil, OH, OW};
+    return run_plain<3, 2>(b * H * W, Din, fh * fw * Dout, dy, w, dx, true, cv, s);
+}
+
+// gpu::conv_wgrad (conv.cuh:448-490): dw^T = im2col(x)^T . dy, the GEMM
+// M = fh*fw*Din, N = Dout, K = b*OH*OW, written column-major into [Dout][fh][fw][Din].
+int plain_conv_wgrad(const u64 *dy, const u64 *x, u64 *dw, int b, int OH, int OW, int Dout, int H, int W, int Din,
+                     int fh, int fw, int ph, int pw, int stride, int dil, hipStream_t s) {
+    PIR_REQUIRE(b > 0 && H > 0 && W > 0 && OH > 0 && OW > 0 && Din > 0 && Dout > 0 && fh > 0 && fw > 0 &&
+                    stride > 0 && dil > 0 && ph >= 0 && pw >= 0,
+                "conv_wgrad: invalid geometry");
+    PlainConv cv{H, W, Din, fh, fw, ph, pw, stride, dil, OH, OW};
+    return run_plain<4, 1>(fh * fw * Din, Dout, b * OH * OW, x, dy, dw, false, cv, s);
 }
 
 }  // namespace pir

@@ -162,6 +162,23 @@ __device__ __forceinline__ Sh3<u64> select_(const Sh3<u64> &x, const Sh3<u64> &y
     return r;
 }
 
+// dividePublic by d = 2^k (RSS.inl:422-436, placeholder r = 1, r' = d of
+// Precompute.h:107-119): a -= r' (fill(d): A.sA, C.sB); t_p = reconstruct(a)_p;
+// a = r + (A.sA, C.sB += t_p / d) with the reference's FP64 truncating division.
+__device__ __forceinline__ Sh3<u64> divide_public_pow2_(Sh3<u64> v, int k) {
+    const u64 d = (u64)1 << k;
+    v.s[PARTY_A][0] -= d;
+    v.s[PARTY_C][1] -= d;
+    u64 tA = v.s[PARTY_A][0] + v.s[PARTY_A][1] + v.s[PARTY_C][0];
+    u64 tC = v.s[PARTY_C][0] + v.s[PARTY_C][1] + v.s[PARTY_B][0];
+    Sh3<u64> o;
+#pragma unroll
+    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
+    o.s[PARTY_A][0] = 1 + trunc_div_pow2(tA, k);
+    o.s[PARTY_C][1] = 1 + trunc_div_pow2(tC, k);
+    return o;
+}
+
 template <typename T>
 __device__ __forceinline__ Sh3<T> load3(const T *__restrict__ base, size_t n, size_t i) {
     Sh3<T> v;

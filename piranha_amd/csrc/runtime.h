@@ -39,8 +39,8 @@ struct GemmTimer {
     struct Rec {
         hipEvent_t start, stop;
         double ops;   // algorithmic int8 ops of the launch: 2 * 36 * M * N * nseg * K
-        int variant;  // A-operand form (0 dense row-major, 1 dense col-major, 2 conv, 3 conv Cin%32==0),
-                      // + 4 for the 32x128 workgroup tile
+        int variant;  // A-operand form (0 dense row-major, 1 dense col-major, 2 conv, 3 conv Cin%32==0,
+                      // 4 dgrad, 5 dgrad Cin%32==0, 6 wgrad), + 8 for the 32x128 workgroup tile
     };
     bool on = false;
     std::vector<Rec> recs;
@@ -54,6 +54,7 @@ struct Context {
     int mode = MODE_FUSED;
     int algo = ALGO_LIMB;
     int frac_bits = 26;  // FLOAT_PRECISION (globals.h:19-21), a runtime value here
+    int log_lr = 5;      // log_learning_rate (main.cu:48), weight updates divide by 2^log_lr
     int error = 0;
 };
 

@@ -203,6 +203,40 @@ def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True, autotune=True)
     return net
 
 
+def bench_training(P, C, layers, batch, fp, steps, warmup, autotune=True):
+    """One training step (NeuralNetwork::forward + backward: reveal-softmax
+    delta, every layer's backward and SGD update), fused limb-MFMA form,
+    captured as one hipGraph; synthetic U[0,1) inputs and one-hot labels."""
+    net = P.Network(C.to_spec(layers), batch, fp, seed=1, mode=P.MODE_FUSED)
+    rng = np.random.default_rng(1)
+    net.forward(rng.uniform(0.0, 1.0, batch * C.input_size(layers)))
+    n_classes = C.output_size(layers)
+    labels = np.zeros((batch, n_classes))
+    labels[np.arange(batch), rng.integers(0, n_classes, batch)] = 1.0
+    net.set_train_labels(labels.ravel())
+    if autotune:
+        net.autotune_train()
+    net.capture_train()
+    t = time_graph_fn(net.replay_train, net, steps, warmup)
+    prof = net.profile_train_gemm()
+    kern = {f"limb_gemm_kernel<{kernel_args(P, v)}>": {"ms": ms, "launches": n, "TOP/s": ops / ms / 1e9}
+            for v, (ms, ops, n) in sorted(prof.items())}
+    return {"workload": f"VGG16-CIFAR training step, batch {batch}: forward + reveal-softmax backward + SGD "
+                        f"update (log_lr 5), hipGraph", "images_per_s": batch / t, "ms_per_step": t * 1e3,
+            "steps": steps, "limb_gemm_per_step": kern}
+
+
+def time_graph_fn(replay, net, steps, warmup):
+    for _ in range(warmup):
+        replay()
+    net.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        replay()
+    net.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
 def bench_ring_gemm(P, sizes, reps):
     L = P.lib()
     out = []
@@ -285,6 +319,7 @@ def main():
                     help="also run one x += y element-op launch over this many u64 (counter calibration)")
     ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="heuristic GEMM plans only")
+    ap.add_argument("--train-only", action="store_true", help="VGG16 training step only (for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -297,6 +332,13 @@ def main():
 
     if args.copy_calib:
         copy_calibration(P, args.copy_calib)
+
+    if args.train_only:
+        tr = bench_training(P, C, C.VGG16_CIFAR, args.batch, 26, args.steps, args.warmup,
+                            autotune=not args.no_autotune)
+        log(f"VGG16 training: {tr['ms_per_step']:.2f} ms/step, {tr['images_per_s']:.1f} images/s")
+        print(json.dumps({"training": tr}), flush=True)
+        return
 
     layers, batch, fp = C.VGG16_CIFAR, args.batch, 26
     log(f"VGG16-CIFAR b{batch}: building (fused limb-MFMA, hipGraph)")
@@ -357,6 +399,13 @@ def main():
         result["plain_images_per_s"] = batch / tp
         result["speedup_vs_plain"] = images_s / (batch / tp)
         log(f"VGG16 plain: {tp * 1e3:.1f} ms/forward, {batch / tp:.1f} images/s, speedup {result['speedup_vs_plain']:.1f}x")
+
+        log("VGG16 training step")
+        tr = bench_training(P, C, layers, batch, fp, max(10, args.steps // 4), 20, autotune=not args.no_autotune)
+        result["training"] = tr
+        log(f"VGG16 training: {tr['ms_per_step']:.2f} ms/step, {tr['images_per_s']:.1f} images/s; " +
+            ", ".join(f"{k} {v['launches']}x {v['ms']:.3f} ms {v['TOP/s']:.0f} TOP/s"
+                      for k, v in tr["limb_gemm_per_step"].items()))
 
         sizes = [int(s) for s in args.gemm_sizes.split(",") if s]
         result["ring_gemm"] = bench_ring_gemm(P, sizes, reps=2)

@@ -230,6 +230,21 @@ int pir_net_layer_backward(pir_net *net, int layer, const uint64_t *delta, const
 /* Layer::getDelta: device shares of dL/d(input of layer) after a backward pass;
  * layer == number of layers gives the reveal-softmax delta (dL/d network output) */
 int pir_net_delta(pir_net *net, int layer, const uint64_t **shares, size_t *n);
+/* Training-step API (the loop body of main.cu:306-340: forward + backward):
+ * labels are kept on the device (setPublic of batch x classes doubles, or
+ * written directly into pir_net_train_labels' buffer), the input is the
+ * network's input buffer (pir_net_input).  capture_train runs one eager step,
+ * then captures forward + backward into a hipGraph replayed by replay_train;
+ * any eager weight change or backward drops it. */
+int pir_net_set_train_labels(pir_net *net, const double *labels, size_t n);
+int pir_net_train_labels(pir_net *net, uint64_t **shares, size_t *n);
+int pir_net_train_step(pir_net *net);
+int pir_net_capture_train(pir_net *net);
+int pir_net_replay_train(pir_net *net);
+/* autotune every limb GEMM shape of a training step / time them like
+ * pir_net_profile_gemm (each runs one eager training step) */
+int pir_net_autotune_train(pir_net *net);
+int pir_net_profile_train_gemm(pir_net *net, double *ms, double *ops, int *launches);
 /* one eager forward with a HIP event after each layer on the network's stream;
  * ms[i] = device time of layer i (count = number of layers).  In the fused form
  * a ReLU layer is computed inside the layer before it and reports ~0 ms. */

@@ -96,6 +96,13 @@ _SIGS = {
     "pir_net_backward_shares": ([_vp, _vp], _i),
     "pir_net_layer_backward": ([_vp, _i, _vp, _vp], _i),
     "pir_net_delta": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
+    "pir_net_set_train_labels": ([_vp, _vp, _sz], _i),
+    "pir_net_train_labels": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
+    "pir_net_train_step": ([_vp], _i),
+    "pir_net_capture_train": ([_vp], _i),
+    "pir_net_replay_train": ([_vp], _i),
+    "pir_net_autotune_train": ([_vp], _i),
+    "pir_net_profile_train_gemm": ([_vp, _vp, _vp, _vp], _i),
     "pir_net_capture": ([_vp, _vp], _i),
     "pir_net_replay": ([_vp], _i),
     "pir_net_set_mode": ([_vp, _i], _i),
@@ -307,6 +314,30 @@ class Network:
     def layer_backward(self, layer, delta_ptr, input_ptr):
         """Layer::backward(delta, forwardInput) of one layer (device shares)."""
         check(lib().pir_net_layer_backward(self.h, layer, delta_ptr, input_ptr))
+
+    def set_train_labels(self, labels):
+        labels = np.ascontiguousarray(labels, np.float64).ravel()
+        check(lib().pir_net_set_train_labels(self.h, labels.ctypes.data, labels.size))
+
+    def train_step(self):
+        """Forward from the input buffer + backward from the stored labels."""
+        check(lib().pir_net_train_step(self.h))
+
+    def capture_train(self):
+        check(lib().pir_net_capture_train(self.h))
+
+    def replay_train(self):
+        check(lib().pir_net_replay_train(self.h))
+
+    def autotune_train(self):
+        check(lib().pir_net_autotune_train(self.h))
+
+    def profile_train_gemm(self):
+        """profile_gemm over one eager training step (forward + backward)."""
+        ms, ops = np.zeros(LIMB_VARIANTS, np.float64), np.zeros(LIMB_VARIANTS, np.float64)
+        n = np.zeros(LIMB_VARIANTS, np.int32)
+        check(lib().pir_net_profile_train_gemm(self.h, ms.ctypes.data, ops.ctypes.data, n.ctypes.data))
+        return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(LIMB_VARIANTS) if n[v]}
 
     def delta(self, layer):
         """dL/d(input of layer) after a backward pass (layer == num_layers: the

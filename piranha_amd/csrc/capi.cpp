@@ -48,8 +48,11 @@ int guarded(F &&f) {
 struct pir_net {
     Context ctx;
     std::unique_ptr<NeuralNetwork> net;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;  // captured forward
     hipGraphExec_t exec = nullptr;
+    hipGraph_t tgraph = nullptr;  // captured training step (forward + backward)
+    hipGraphExec_t texec = nullptr;
+    RSS64 labels;  // label shares of the training-step API
     bool own_stream = false;
 };
 
@@ -63,14 +66,61 @@ struct NetScope {
 // change (set_weights, a backward pass) drops it, and replay then reports that
 // the network must be captured again.
 void drop_graph(pir_net *n) {
-    if (n->exec) {
-        (void)hipGraphExecDestroy(n->exec);
-        n->exec = nullptr;
+    for (hipGraphExec_t *e : {&n->exec, &n->texec})
+        if (*e) {
+            (void)hipGraphExecDestroy(*e);
+            *e = nullptr;
+        }
+    for (hipGraph_t *g : {&n->graph, &n->tgraph})
+        if (*g) {
+            (void)hipGraphDestroy(*g);
+            *g = nullptr;
+        }
+}
+
+// Runs `body` with HIP events around every limb GEMM launch and sums the
+// kernel times per variant (the layout of pir_net_profile_gemm).
+template <typename F>
+int timed_gemms(pir_net *n, double *ms, double *ops, int *launches, F &&body) {
+    GemmTimer &t = n->ctx.timer;
+    t.recs.clear();
+    t.on = true;
+    int rc = guarded(body);
+    t.on = false;
+    hipError_t e = hipStreamSynchronize(n->ctx.stream);
+    for (int v = 0; v < PIR_LIMB_VARIANTS; v++) {
+        ms[v] = 0;
+        ops[v] = 0;
+        launches[v] = 0;
     }
-    if (n->graph) {
-        (void)hipGraphDestroy(n->graph);
-        n->graph = nullptr;
+    for (auto &r : t.recs) {
+        float dt = 0;
+        if (rc == OK && e == hipSuccess && hipEventElapsedTime(&dt, r.start, r.stop) == hipSuccess) {
+            ms[r.variant] += dt;
+            ops[r.variant] += r.ops;
+            launches[r.variant]++;
+        }
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
     }
+    t.recs.clear();
+    if (rc) return rc;
+    PIR_HIP(e);
+    return (int)OK;
+}
+
+void require_train_labels(pir_net *n) {
+    if (n->net->layers.empty() || n->labels.size() != n->net->layers.back()->getActivation()->size()) {
+        set_error("training step: set the labels first (pir_net_set_train_labels)");
+        throw PirError(ERR_STATE);
+    }
+}
+
+// one training step from the network's input buffer and label shares
+void train_step(pir_net *n) {
+    require_train_labels(n);
+    n->net->run();
+    n->net->backwardShares(n->labels);
 }
 }  // namespace
 
@@ -645,8 +695,8 @@ int pir_net_destroy(pir_net *n) {
     {
         NetScope ns(n);
         (void)hipStreamSynchronize(n->ctx.stream);
-        if (n->exec) (void)hipGraphExecDestroy(n->exec);
-        if (n->graph) (void)hipGraphDestroy(n->graph);
+        drop_graph(n);
+        n->labels = RSS64();
         n->net.reset();
     }
     hipStream_t s = n->ctx.stream;
@@ -872,33 +922,105 @@ int pir_net_profile_gemm(pir_net *n, double *ms, double *ops, int *launches) {
     return guarded([&]() -> int {
         PIR_REQUIRE(n && ms && ops && launches, "pir_net_profile_gemm: invalid arguments");
         NetScope ns(n);
-        GemmTimer &t = n->ctx.timer;
-        t.recs.clear();
-        t.on = true;
-        int rc = guarded([&]() -> int {
+        return timed_gemms(n, ms, ops, launches, [&]() -> int {
             n->net->run();
             return (int)OK;
         });
-        t.on = false;
-        hipError_t e = hipStreamSynchronize(n->ctx.stream);
-        for (int v = 0; v < PIR_LIMB_VARIANTS; v++) {
-            ms[v] = 0;
-            ops[v] = 0;
-            launches[v] = 0;
+    });
+}
+
+int pir_net_profile_train_gemm(pir_net *n, double *ms, double *ops, int *launches) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && ms && ops && launches, "pir_net_profile_train_gemm: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        return timed_gemms(n, ms, ops, launches, [&]() -> int {
+            train_step(n);
+            return (int)OK;
+        });
+    });
+}
+
+int pir_net_set_train_labels(pir_net *n, const double *labels, size_t count) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && labels && !n->net->layers.empty(), "pir_net_set_train_labels: invalid arguments");
+        PIR_REQUIRE(count == n->net->layers.back()->getActivation()->size(),
+                    "pir_net_set_train_labels: labels must have batch x classes entries");
+        NetScope ns(n);
+        if (n->labels.size() != count) {
+            drop_graph(n);  // a captured step reads the label buffer
+            n->labels.resize(count);
         }
-        for (auto &r : t.recs) {
-            float dt = 0;
-            if (rc == OK && e == hipSuccess && hipEventElapsedTime(&dt, r.start, r.stop) == hipSuccess) {
-                ms[r.variant] += dt;
-                ops[r.variant] += r.ops;
-                launches[r.variant]++;
-            }
-            (void)hipEventDestroy(r.start);
-            (void)hipEventDestroy(r.stop);
+        n->labels.setPublic(std::vector<double>(labels, labels + count), n->ctx.frac_bits);
+        return (int)OK;
+    });
+}
+
+int pir_net_train_labels(pir_net *n, uint64_t **shares, size_t *count) {
+    PIR_REQUIRE(n && shares && count && n->labels.size() > 0, "pir_net_train_labels: no labels set");
+    *shares = n->labels.base();
+    *count = n->labels.size();
+    return OK;
+}
+
+int pir_net_train_step(pir_net *n) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_train_step: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        train_step(n);
+        return (int)OK;
+    });
+}
+
+int pir_net_capture_train(pir_net *n) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_capture_train: invalid arguments");
+        NetScope ns(n);
+        hipStream_t s = n->ctx.stream;
+        drop_graph(n);
+        // one eager step (a real training step) allocates every buffer the capture needs
+        train_step(n);
+        PIR_HIP(hipStreamSynchronize(s));
+        PIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        n->ctx.alloc.capturing = true;
+        int rc = guarded([&]() -> int {
+            train_step(n);
+            return (int)OK;
+        });
+        n->ctx.alloc.capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
         }
-        t.recs.clear();
-        if (rc) return rc;
         PIR_HIP(e);
+        n->tgraph = g;
+        PIR_HIP(hipGraphInstantiate(&n->texec, g, nullptr, nullptr, 0));
+        return (int)OK;
+    });
+}
+
+int pir_net_replay_train(pir_net *n) {
+    PIR_REQUIRE(n && n->texec, "pir_net_replay_train: no captured training step (capture again after a change)");
+    PIR_HIP(hipGraphLaunch(n->texec, n->ctx.stream));
+    return OK;
+}
+
+int pir_net_autotune_train(pir_net *n) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_autotune_train: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        n->ctx.tune = true;
+        int rc = guarded([&]() -> int {
+            train_step(n);
+            return (int)OK;
+        });
+        n->ctx.tune = false;
+        if (rc) return rc;
+        PIR_HIP(hipStreamSynchronize(n->ctx.stream));
         return (int)OK;
     });
 }

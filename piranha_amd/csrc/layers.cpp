@@ -45,7 +45,8 @@ void FCLayer::printLayer() {
 void FCLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
     const int M = (int)conf.batchSize, N = (int)conf.outputDim, K = (int)conf.inputDim;
     if (!packed_valid) {
-        packed = DevBuf(packed_rss_weights_bytes(K, N));
+        // repacked in place: a captured graph keeps pointing at the same buffer
+        if (packed.bytes() != packed_rss_weights_bytes(K, N)) packed = DevBuf(packed_rss_weights_bytes(K, N));
         check(pack_rss_weights(weights, K, N, true, (int8_t *)packed.get()));
         packed_valid = true;
     }
@@ -154,7 +155,7 @@ void CNNLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
     const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
     const int K = f * f * Din;
     if (!packed_valid) {
-        packed = DevBuf(packed_rss_weights_bytes(K, Dout));
+        if (packed.bytes() != packed_rss_weights_bytes(K, Dout)) packed = DevBuf(packed_rss_weights_bytes(K, Dout));
         check(pack_rss_weights(weights, K, Dout, false, (int8_t *)packed.get()));
         packed_valid = true;
     }

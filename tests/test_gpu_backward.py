@@ -341,3 +341,38 @@ def test_training_reduces_loss():
         net.backward(labels)
         last = loss()
     assert last < first, (first, last)
+
+
+def test_training_step_api_and_graph_replay():
+    """pir_net_train_step equals forward + backward; a captured training step
+    (capture_train runs one eager step, each replay one more) follows the same
+    trajectory as eager steps, replays interleaved with the host reading
+    weights between them."""
+    layers, batch, fp, lr = C.LENET, 4, 13, 5
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0.0, 1.0, batch * C.input_size(layers))
+    labels = one_hot(rng, batch, 10)
+    nets = [P.Network(C.to_spec(layers), batch, fp, seed=8) for _ in range(3)]
+    for net in nets:
+        net.set_log_learning_rate(lr)
+        net.forward(x)  # fills the input buffer
+        net.set_train_labels(labels)
+    eager, api, graph = nets
+    traj = []
+    for step in range(3):
+        eager.forward(x)
+        eager.backward(labels)
+        traj.append([eager.weights(i)[0] for i, l in enumerate(layers) if l[0] in ("fc", "cnn")])
+    for step in range(3):
+        api.train_step()
+        got = [api.weights(i)[0] for i, l in enumerate(layers) if l[0] in ("fc", "cnn")]
+        assert all(np.array_equal(a, b) for a, b in zip(got, traj[step])), f"train_step {step}"
+    graph.capture_train()
+    got = [graph.weights(i)[0] for i, l in enumerate(layers) if l[0] in ("fc", "cnn")]
+    assert all(np.array_equal(a, b) for a, b in zip(got, traj[0]))
+    for step in (1, 2):
+        graph.replay_train()
+        graph.synchronize()
+        got = [graph.weights(i)[0] for i, l in enumerate(layers) if l[0] in ("fc", "cnn")]
+        assert all(np.array_equal(a, b) for a, b in zip(got, traj[step])), f"replay {step}"
+    assert np.array_equal(graph.delta(0), eager.delta(0))

@@ -221,9 +221,15 @@ def bench_training(P, C, layers, batch, fp, steps, warmup, autotune=True):
     prof = net.profile_train_gemm()
     kern = {f"limb_gemm_kernel<{kernel_args(P, v)}>": {"ms": ms, "launches": n, "TOP/s": ops / ms / 1e9}
             for v, (ms, ops, n) in sorted(prof.items())}
+    fwd_ms, sm_ms, bwd_ms = net.profile_train_layers()
+    log("training step per layer (ms) forward: " + ", ".join(f"{l[0]}{i}:{m:.3f}" for i, (l, m) in
+                                                              enumerate(zip(layers, fwd_ms))))
+    log(f"  softmax delta {sm_ms:.3f}; backward: " + ", ".join(f"{l[0]}{i}:{m:.3f}" for i, (l, m) in
+                                                              enumerate(zip(layers, bwd_ms))))
     return {"workload": f"VGG16-CIFAR training step, batch {batch}: forward + reveal-softmax backward + SGD "
                         f"update (log_lr 5), hipGraph", "images_per_s": batch / t, "ms_per_step": t * 1e3,
-            "steps": steps, "limb_gemm_per_step": kern}
+            "steps": steps, "limb_gemm_per_step": kern,
+            "eager_ms": {"forward": float(fwd_ms.sum()), "softmax_delta": sm_ms, "backward": float(bwd_ms.sum())}}
 
 
 def time_graph_fn(replay, net, steps, warmup):

@@ -245,6 +245,10 @@ int pir_net_replay_train(pir_net *net);
  * pir_net_profile_gemm (each runs one eager training step) */
 int pir_net_autotune_train(pir_net *net);
 int pir_net_profile_train_gemm(pir_net *net, double *ms, double *ops, int *launches);
+/* one eager training step with HIP events: ms[i] (i < L) = layer i's forward,
+ * ms[L] = the softmax delta, ms[L + 1 + i] = layer i's backward (count = 2L + 1;
+ * in the fused form a ReLU layer runs inside its neighbour and reports ~0 ms) */
+int pir_net_profile_train_layers(pir_net *net, float *ms, int count);
 /* one eager forward with a HIP event after each layer on the network's stream;
  * ms[i] = device time of layer i (count = number of layers).  In the fused form
  * a ReLU layer is computed inside the layer before it and reports ~0 ms. */

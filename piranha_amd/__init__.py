@@ -103,6 +103,7 @@ _SIGS = {
     "pir_net_replay_train": ([_vp], _i),
     "pir_net_autotune_train": ([_vp], _i),
     "pir_net_profile_train_gemm": ([_vp, _vp, _vp, _vp], _i),
+    "pir_net_profile_train_layers": ([_vp, _vp, _i], _i),
     "pir_net_capture": ([_vp, _vp], _i),
     "pir_net_replay": ([_vp], _i),
     "pir_net_set_mode": ([_vp, _i], _i),
@@ -338,6 +339,14 @@ class Network:
         n = np.zeros(LIMB_VARIANTS, np.int32)
         check(lib().pir_net_profile_train_gemm(self.h, ms.ctypes.data, ops.ctypes.data, n.ctypes.data))
         return {v: (float(ms[v]), float(ops[v]), int(n[v])) for v in range(LIMB_VARIANTS) if n[v]}
+
+    def profile_train_layers(self):
+        """Device ms of one eager training step: (forward per layer, softmax
+        delta, backward per layer)."""
+        L = self.num_layers
+        ms = np.zeros(2 * L + 1, np.float32)
+        check(lib().pir_net_profile_train_layers(self.h, ms.ctypes.data, ms.size))
+        return ms[:L], float(ms[L]), ms[L + 1:]
 
     def delta(self, layer):
         """dL/d(input of layer) after a backward pass (layer == num_layers: the

@@ -941,6 +941,40 @@ int pir_net_profile_train_gemm(pir_net *n, double *ms, double *ops, int *launche
     });
 }
 
+int pir_net_profile_train_layers(pir_net *n, float *ms, int count) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && ms && count == 2 * pir_net_num_layers(n) + 1,
+                    "pir_net_profile_train_layers: invalid arguments");
+        NetScope ns(n);
+        drop_graph(n);
+        require_train_labels(n);
+        hipStream_t s = n->ctx.stream;
+        const int L = pir_net_num_layers(n);
+        std::vector<hipEvent_t> ev(count + 1);
+        for (auto &e : ev) PIR_HIP(hipEventCreate(&e));
+        hipError_t rec = hipSuccess;
+        auto mark = [&](int k) {
+            hipError_t e = hipEventRecord(ev[k], s);
+            if (rec == hipSuccess) rec = e;
+        };
+        // events in issue order: forward layers 0..L-1, the softmax delta, then
+        // the backward steps, which run from layer L-1 down to layer 0
+        int next = 0;
+        mark(next++);
+        n->net->run([&](size_t) { mark(next++); });
+        n->net->backwardShares(n->labels, [&](size_t) { mark(next++); });
+        PIR_HIP(rec);
+        PIR_REQUIRE(next == count + 1, "pir_net_profile_train_layers: unexpected step count");
+        PIR_HIP(hipEventSynchronize(ev[count]));
+        std::vector<float> dt(count);
+        for (int j = 0; j < count; j++) PIR_HIP(hipEventElapsedTime(&dt[j], ev[j], ev[j + 1]));
+        for (int j = 0; j <= L; j++) ms[j] = dt[j];                  // forward, softmax
+        for (int t = 0; t < L; t++) ms[L + 1 + (L - 1 - t)] = dt[L + 1 + t];  // backward of layer L-1-t
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        return (int)OK;
+    });
+}
+
 int pir_net_set_train_labels(pir_net *n, const double *labels, size_t count) {
     return guarded([&]() -> int {
         PIR_REQUIRE(n && labels && !n->net->layers.empty(), "pir_net_set_train_labels: invalid arguments");

@@ -85,26 +85,7 @@ void FCLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
     const int fp = ctx().frac_bits, lr = ctx().log_lr;
     ensureDeltas();
     if (fused_limb() && delta.base() && forwardInput.base()) {
-        // (1) M = batch, N = in, K = out; B(k = o, n = i) = W[i*out + o]
-        DevBuf pw(packed_rss_weights_bytes(out, in));
-        check(pack_rss_weights(weights, out, in, false, (int8_t *)pw.get()));
-        check(rss_limb_product(delta, true, nullptr, B, in, out, (const int8_t *)pw.get(), fp, nullptr, deltas, true));
-        // (2) M = out, N = in, K = batch; A(m = o, k = b) = delta[b*out + o], B(k, n) = X[b*in + i],
-        // the M x N result column-major is W's [in][out] layout
-        DevBuf px(packed_rss_weights_bytes(B, in));
-        check(pack_rss_weights(forwardInput, B, in, true, (int8_t *)px.get()));
-        LimbA a{};
-        a.M = out;
-        a.K = B;
-        a.Kp = round_up(B, LIMB_BK);
-        a.lda = out;
-        a.row_major = 0;
-        Epilogue e = rss_trunc_epi(weights.base(), fp + lr, false);
-        e.sub_out = 1;
-        check(rss_limb_run(delta, a, (const int8_t *)px.get(), in, e));
-        // (3)
-        check(fused_fc_bias_update(delta.base(), biases.base(), out, B, lr, ctx().stream));
-        packed_valid = false;
+        limbBackward(delta, forwardInput, nullptr);
         return;
     }
     deltas.zero();
@@ -119,6 +100,54 @@ void FCLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
     dividePublic(db, (u64)1 << lr);
     biases -= db;
     packed_valid = false;
+}
+
+// The fused form of FCLayer::backward; relu (optional) is the ReLU layer
+// before this one, whose backward runs in the deltas product's epilogue.
+void FCLayer::limbBackward(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer *relu) {
+    const int B = (int)conf.batchSize, in = (int)conf.inputDim, out = (int)conf.outputDim;
+    const int fp = ctx().frac_bits, lr = ctx().log_lr;
+    // (1) deltas: M = batch, N = in, K = out; B(k = o, n = i) = W[i*out + o]
+    DevBuf pw(packed_rss_weights_bytes(out, in));
+    check(pack_rss_weights(weights, out, in, false, (int8_t *)pw.get()));
+    LimbA a1{};
+    a1.M = B;
+    a1.K = out;
+    a1.Kp = round_up(out, LIMB_BK);
+    a1.lda = out;
+    a1.row_major = 1;
+    Epilogue e1 = rss_trunc_epi(deltas.base(), fp, true);
+    if (relu) {
+        e1.bw_relu_d = relu->reluPrime.base();
+        e1.bw_relu_out = relu->getDelta()->base();
+    }
+    check(rss_limb_run(delta, a1, (const int8_t *)pw.get(), in, e1));
+    // (2) M = out, N = in, K = batch; A(m = o, k = b) = delta[b*out + o], B(k, n) = X[b*in + i],
+    // the M x N result column-major is W's [in][out] layout
+    DevBuf px(packed_rss_weights_bytes(B, in));
+    check(pack_rss_weights(forwardInput, B, in, true, (int8_t *)px.get()));
+    LimbA a{};
+    a.M = out;
+    a.K = B;
+    a.Kp = round_up(B, LIMB_BK);
+    a.lda = out;
+    a.row_major = 0;
+    Epilogue e = rss_trunc_epi(weights.base(), fp + lr, false);
+    e.sub_out = 1;
+    check(rss_limb_run(delta, a, (const int8_t *)px.get(), in, e));
+    // (3)
+    check(fused_fc_bias_update(delta.base(), biases.base(), out, B, lr, ctx().stream));
+    packed_valid = false;
+}
+
+bool FCLayer::backwardWithReLU(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer &relu) {
+    if (!(fused_limb() && delta.base() && forwardInput.base() && relu.reluPrime.base() &&
+          relu.inputSize() == inputSize()))
+        return false;
+    ensureDeltas();
+    relu.allocDelta();
+    limbBackward(delta, forwardInput, &relu);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,20 +222,7 @@ void CNNLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
     const int st = (int)conf.stride, pad = (int)conf.padding, fp = ctx().frac_bits, lr = ctx().log_lr;
     ensureDeltas();
     if (fused_limb() && delta.base() && forwardInput.base()) {
-        convolution(delta, weights, deltas, kDgrad, B, H, W, f, Din, Dout, st, pad, fp);
-        LimbA a = limb_wgrad_view(B, H, W, f, Din, Dout, st, pad);
-        LimbBSrc g{};
-        g.K = a.K;
-        g.N = Dout;
-        g.Kp = a.Kp;
-        g.row_major = 1;
-        DevBuf packed(limb_b_bytes(Dout, 6, g.Kp));
-        check(pack_rss_operand(delta, g, (int8_t *)packed.get()));
-        Epilogue e = rss_trunc_epi(weights.base(), fp, false);
-        e.div_bits = lr;
-        e.sub_out = 1;
-        check(rss_limb_run(forwardInput, a, (const int8_t *)packed.get(), Dout, e));
-        packed_valid = false;
+        limbBackward(delta, forwardInput, nullptr);
         return;
     }
     deltas.zero();
@@ -216,6 +232,39 @@ void CNNLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
     dividePublic(dF, (u64)1 << lr);
     weights -= dF;
     packed_valid = false;
+}
+
+// The fused form of CNNLayer::backward; relu (optional) is the ReLU layer
+// before this one, whose backward runs in the dgrad product's epilogue.
+void CNNLayer::limbBackward(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer *relu) {
+    const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
+    const int B = (int)conf.batchSize, H = (int)conf.imageHeight, W = (int)conf.imageWidth;
+    const int st = (int)conf.stride, pad = (int)conf.padding, fp = ctx().frac_bits, lr = ctx().log_lr;
+    rss_conv_grad_limb(kDgrad, delta, weights, deltas, B, H, W, f, Din, Dout, st, pad, fp,
+                       relu ? relu->reluPrime.base() : nullptr, relu ? relu->getDelta()->base() : nullptr);
+    LimbA a = limb_wgrad_view(B, H, W, f, Din, Dout, st, pad);
+    LimbBSrc g{};
+    g.K = a.K;
+    g.N = Dout;
+    g.Kp = a.Kp;
+    g.row_major = 1;
+    DevBuf packed(limb_b_bytes(Dout, 6, g.Kp));
+    check(pack_rss_operand(delta, g, (int8_t *)packed.get()));
+    Epilogue e = rss_trunc_epi(weights.base(), fp, false);
+    e.div_bits = lr;
+    e.sub_out = 1;
+    check(rss_limb_run(forwardInput, a, (const int8_t *)packed.get(), Dout, e));
+    packed_valid = false;
+}
+
+bool CNNLayer::backwardWithReLU(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer &relu) {
+    if (!(fused_limb() && delta.base() && forwardInput.base() && relu.reluPrime.base() &&
+          relu.inputSize() == inputSize()))
+        return false;
+    ensureDeltas();
+    relu.allocDelta();
+    limbBackward(delta, forwardInput, &relu);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -379,7 +428,7 @@ void NeuralNetwork::backward(const std::vector<double> &labels) {
     backwardShares(labelShare);
 }
 
-void NeuralNetwork::backwardShares(const RSS64 &labelShare) {
+void NeuralNetwork::backwardShares(const RSS64 &labelShare, const std::function<void(size_t)> &after_step) {
     if (layers.size() < 2) {
         set_error("network backward: needs at least two layers");
         throw PirError(ERR_UNSUPPORTED);
@@ -391,7 +440,8 @@ void NeuralNetwork::backwardShares(const RSS64 &labelShare) {
     }
     if (outputDelta.size() != labelShare.size()) outputDelta.resize(labelShare.size());
     _reveal_softmax_grad(labelShare, outputDelta);
-    _backward_pass(outputDelta);
+    if (after_step) after_step(0);
+    _backward_pass(outputDelta, after_step);
 }
 
 void NeuralNetwork::_reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas) {
@@ -416,11 +466,27 @@ void NeuralNetwork::_reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas) {
     if (lmb > 1) dividePublic(deltas, (u64)1 << lmb);
 }
 
-void NeuralNetwork::_backward_pass(const RSS64 &deltas) {
-    const size_t L = layers.size();
-    layers[L - 1]->backward(deltas, *layers[L - 2]->getActivation());
-    for (size_t i = L - 2; i > 0; i--) layers[i]->backward(*layers[i + 1]->getDelta(), *layers[i - 1]->getActivation());
-    layers[0]->backward(*layers[1]->getDelta(), input);
+// Layer L-1 takes the output delta, layer i the delta of layer i+1, each with
+// the forward input of its own forward pass (the network input for layer 0).
+// In the fused form a ReLU layer's backward runs inside the product of the
+// layer after it (which reads that ReLU's activation as its forward input).
+void NeuralNetwork::_backward_pass(const RSS64 &deltas, const std::function<void(size_t)> &after_step) {
+    const RSS64 *d = &deltas;
+    size_t i = layers.size();
+    while (i > 0) {
+        i--;
+        const RSS64 &in = i == 0 ? input : *layers[i - 1]->getActivation();
+        ReLULayer *prev = i > 0 ? dynamic_cast<ReLULayer *>(layers[i - 1].get()) : nullptr;
+        if (prev && layers[i]->backwardWithReLU(*d, in, *prev)) {
+            if (after_step) after_step(1 + i);
+            i--;  // the ReLU layer i-1 is done
+            d = prev->getDelta();
+        } else {
+            layers[i]->backward(*d, in);
+            d = layers[i]->getDelta();
+        }
+        if (after_step) after_step(1 + i);
+    }
 }
 
 void NeuralNetwork::printNetwork() {

@@ -60,6 +60,11 @@ class Layer {
     // (the ReLU layer's activations and reluPrime are written here, bit for bit
     // what its own forward would write).  Returns false when this layer cannot.
     virtual bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) { return false; }
+    // Fused form only: backward() plus the backward of the ReLU layer that
+    // precedes this one (its input is that ReLU's activation), in one pass; the
+    // ReLU layer's deltas receive exactly what its own backward would write.
+    virtual bool backwardWithReLU(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer &relu) { return false; }
+    void allocDelta() { ensureDeltas(); }
 
   protected:
     // allocated on the first backward pass (inference never needs it)
@@ -76,6 +81,7 @@ class FCLayer : public Layer {
     void forward(const RSS64 &input) override;
     void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
     bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
+    bool backwardWithReLU(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS64 *getWeights() override { return &weights; }
     RSS64 *getBiases() override { return &biases; }
@@ -87,6 +93,7 @@ class FCLayer : public Layer {
 
   private:
     void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d);
+    void limbBackward(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer *relu);
     void initialize(int seed);
     DevBuf packed;
     bool packed_valid = false;
@@ -99,6 +106,7 @@ class CNNLayer : public Layer {
     void forward(const RSS64 &input) override;
     void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
     bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
+    bool backwardWithReLU(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer &relu) override;
     RSS64 *getActivation() override { return &activations; }
     RSS64 *getWeights() override { return &weights; }
     size_t inputSize() const override {
@@ -113,6 +121,7 @@ class CNNLayer : public Layer {
 
   private:
     void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d);
+    void limbBackward(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer *relu);
     void initialize(int seed);
     DevBuf packed;
     bool packed_valid = false;
@@ -172,8 +181,11 @@ class NeuralNetwork {
     // batch x classes doubles) become public shares, then the reveal-softmax
     // delta and the backward pass with weight updates.
     void backward(const std::vector<double> &labels);
-    // the same from label shares already on the device ([3][2][batch*classes])
-    void backwardShares(const RSS64 &labelShare);
+    // the same from label shares already on the device ([3][2][batch*classes]);
+    // after_step(k), when given, is called once backward step k has been issued
+    // (k = 0: the softmax delta, k = 1 + i: layer i's backward; in the fused form
+    // a ReLU layer's backward runs inside the layer after it).
+    void backwardShares(const RSS64 &labelShare, const std::function<void(size_t)> &after_step = nullptr);
     void printNetwork();
     // dL/d(network output) of the last backward (input of the last layer's backward)
     RSS64 outputDelta;
@@ -183,7 +195,7 @@ class NeuralNetwork {
 
   private:
     void _reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas);  // NeuralNetwork.cu:207-267
-    void _backward_pass(const RSS64 &deltas);                        // NeuralNetwork.cu:446-467
+    void _backward_pass(const RSS64 &deltas, const std::function<void(size_t)> &after_step);  // :446-467
 };
 
 }  // namespace pir

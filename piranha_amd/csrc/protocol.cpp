@@ -391,7 +391,7 @@ static void local_conv(ConvOperator op, const RSS64 &A, const RSS64 &B, PartyDat
 // The dgrad and wgrad products in the fused limb form: one GEMM launch over the
 // six share segments with the dividePublic3out3 epilogue.
 void rss_conv_grad_limb(ConvOperator op, const RSS64 &A, const RSS64 &B, RSS64 &C, int batch, int H, int W, int f,
-                        int Din, int Dout, int stride, int pad, int trunc) {
+                        int Din, int Dout, int stride, int pad, int trunc, const u8 *bw_relu_d, u64 *bw_relu_out) {
     LimbBSrc g{};
     LimbA a;
     int N;
@@ -413,7 +413,10 @@ void rss_conv_grad_limb(ConvOperator op, const RSS64 &A, const RSS64 &B, RSS64 &
     const RSS64 &Aop = op == kDgrad ? A : B;
     const RSS64 &Bop = op == kDgrad ? B : A;
     check(pack_rss_operand(Bop, g, (int8_t *)packed.get()));
-    check(rss_limb_run(Aop, a, (const int8_t *)packed.get(), N, rss_trunc_epi(C.base(), trunc, op == kDgrad)));
+    Epilogue e = rss_trunc_epi(C.base(), trunc, op == kDgrad);
+    e.bw_relu_d = bw_relu_d;
+    e.bw_relu_out = bw_relu_out;
+    check(rss_limb_run(Aop, a, (const int8_t *)packed.get(), N, e));
 }
 
 // RSS.inl:705-803

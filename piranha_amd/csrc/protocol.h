@@ -89,6 +89,12 @@ int rss_limb_run(const RSS64 &A, LimbA a, const int8_t *packed, int N, const Epi
 // The 3out3 output epilogue of the fused product: out [3][2][M*N] receives
 // A(1+q, 0), B(0, 0), C(0, 1+q).
 Epilogue rss_trunc_epi(u64 *out, int trunc_bits, bool tc);
+// convolution(kDgrad / kWgrad) in the fused limb form (one 6-segment product
+// with the dividePublic3out3 epilogue); for kDgrad, bw_relu_d / bw_relu_out
+// (optional) add the preceding ReLU layer's backward to the epilogue.
+void rss_conv_grad_limb(ConvOperator op, const RSS64 &A, const RSS64 &B, RSS64 &C, int batch, int H, int W, int f,
+                        int Din, int Dout, int stride, int pad, int trunc, const u8 *bw_relu_d = nullptr,
+                        u64 *bw_relu_out = nullptr);
 // Geometry of the dgrad / wgrad limb views of a convolution layer.
 LimbA limb_dgrad_view(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad);
 LimbA limb_wgrad_view(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad);

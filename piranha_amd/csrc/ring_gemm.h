@@ -89,6 +89,11 @@ struct Epilogue {
     // relu_d [3][2][M*N] = dReLU bits (RSS.inl:806-865), both in out's layout.
     u64 *relu_out;
     u8 *relu_d;
+    // Optional backward of the ReLU layer that precedes this product's layer
+    // (nout == 6): bw_relu_out [3][2][M*N] = selectShare(0, out, bw_relu_d)
+    // with bw_relu_d that ReLU layer's reluPrime (ReLULayer.cu:91-123).
+    const u8 *bw_relu_d;
+    u64 *bw_relu_out;
 };
 
 constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;

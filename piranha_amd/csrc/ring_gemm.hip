@@ -99,6 +99,20 @@ __device__ __forceinline__ void trunc_store(const Epilogue &e, size_t MN, int N,
         store3(e.relu_out, MN, idx, select_<u8>(zero, x, b));
         return;
     }
+    if (e.bw_relu_out) {
+        // the preceding ReLU layer's backward on the same element
+        Sh3<u64> x, zero;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            u64 v = ((e.base_mask >> j) & 1u) ? base : (u64)0;
+            if (e.bias) v += e.bias[(size_t)j * N + n];
+            e.out[(size_t)j * MN + idx] = v;
+            x.s[j >> 1][j & 1] = v;
+            zero.s[j >> 1][j & 1] = 0;
+        }
+        store3(e.bw_relu_out, MN, idx, select_<u8>(zero, x, load3(e.bw_relu_d, MN, idx)));
+        return;
+    }
     for (int j = 0; j < e.nout; j++) {
         u64 v = ((e.base_mask >> j) & 1u) ? base : (u64)0;
         if (e.bias) v += e.bias[(size_t)j * N + n];
@@ -621,8 +635,10 @@ int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi,
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.nout >= 1 && epi.nout <= 32), "limb_gemm: bad output share count");
     PIR_REQUIRE(!epi.relu_out || (epi.mode == EPI_TRUNC && epi.nout == 6 && epi.relu_d),
                 "limb_gemm: a fused ReLU needs the six RSS output arrays");
-    PIR_REQUIRE(!epi.sub_out || (epi.mode == EPI_TRUNC && !epi.bias && !epi.relu_out),
+    PIR_REQUIRE(!epi.sub_out || (epi.mode == EPI_TRUNC && !epi.bias && !epi.relu_out && !epi.bw_relu_out),
                 "limb_gemm: the subtracting epilogue takes no bias and no fused ReLU");
+    PIR_REQUIRE(!epi.bw_relu_out || (epi.mode == EPI_TRUNC && epi.nout == 6 && epi.bw_relu_d && !epi.relu_out),
+                "limb_gemm: a fused ReLU backward needs the six RSS output arrays and the derivative");
     PIR_REQUIRE(epi.div_bits >= 0 && epi.div_bits <= 62, "limb_gemm: division bits must be in [0, 62]");
     if (a.M == 0 || N == 0) return OK;
     const int nks = a.nseg * a.Kp / LIMB_BK;

@@ -54,6 +54,22 @@ const char *last_error();
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
+// Unsigned division by a runtime constant d >= 1 with one multiply-high:
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, n / d = (mulhi(n, m) + n) >> l,
+// exact for every 32-bit n (the sum is formed in 64 bits).
+struct FastDiv {
+    u32 d = 1, mul = 1, shr = 0;
+    FastDiv() = default;
+    explicit FastDiv(u32 dv) : d(dv) {
+        shr = 0;
+        while (shr < 32 && (1ull << shr) < dv) shr++;
+        mul = (u32)((((u64)1 << 32) * (((u64)1 << shr) - dv)) / dv + 1);
+    }
+    __host__ __device__ __forceinline__ u32 div(u32 n) const {
+        return (u32)(((((u64)n * mul) >> 32) + n) >> shr);
+    }
+};
+
 // Emulates the reference truncation q = (S)((double)(S)t / (double)(S)2^k)
 // (functors.cuh:33-45) in integer arithmetic: the int64 -> double conversion
 // rounds |t| to 53 significant bits (IEEE round-to-nearest-even, sign

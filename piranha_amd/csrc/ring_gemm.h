@@ -43,6 +43,8 @@ struct LimbA {
     int row_major;   // dense: A(m,k) = row_major ? p[m*lda+k] : p[k*lda+m]
     int conv;        // LimbConv
     int H, W, Cin, fh, fw, ph, pw, stride, OH, OW;
+    // divisors of the per-element index decode, set by limb_gemm
+    FastDiv fd_fwc, fd_cin, fd_ohw, fd_ow;
 };
 
 // Operand B (weights) description for the packing kernel: segment s is the

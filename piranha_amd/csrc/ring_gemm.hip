@@ -131,43 +131,59 @@ __device__ __forceinline__ void epilogue_store(const Epilogue &e, int M, int N, 
 
 // ---------------------------------------------------------------------------
 // B packing: Bp[nt][ks][limb][64 n][32 k] int8, zero padded in n and k.
+// One workgroup per (n-tile, K-step) block of 64 x 32 values: the block is read
+// along the operand's contiguous dimension (n for row-major and the dgrad
+// filter view, k for column-major), recoded and staged in LDS, then each
+// thread byte-transposes 8 k of one n and writes its 8 limb words; every
+// plane of the block is 2 KB of contiguous output.
+
+__device__ __forceinline__ u64 pack_src_value(const LimbBSrc &src, int seg, int k, int n) {
+    if (n >= src.N || k >= src.K) return 0;
+    size_t idx;
+    if (src.dgrad_filter) {
+        int fwc = src.kfw * src.kc;
+        int r = k / fwc, rem = k - r * fwc;
+        int sc = rem / src.kc, c = rem - sc * src.kc;
+        idx = (((size_t)c * src.kfh + r) * src.kfw + sc) * src.N + n;
+    } else {
+        idx = src.row_major ? (size_t)k * src.N + n : (size_t)n * src.K + k;
+    }
+    u64 v = 0;
+#pragma unroll
+    for (int t = 0; t < LIMB_MAX_TERMS; t++)
+        if (src.b[seg][t]) v += (u64)(s64)src.coef[seg][t] * src.b[seg][t][idx];
+    return v;
+}
 
 __global__ __launch_bounds__(256) void limb_pack_b_kernel(LimbBSrc src, int8_t *__restrict__ out, int nks) {
-    // one thread per (n, group of 8 k'' within a K-step)
-    const size_t total = (size_t)((src.N + LIMB_BN - 1) / LIMB_BN) * LIMB_BN * nks * (LIMB_BK / 8);
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    int kq = (int)(t % (LIMB_BK / 8));
-    size_t r = t / (LIMB_BK / 8);
-    int ks = (int)(r % nks);
-    int n = (int)(r / nks);
+    __shared__ u64 tile[LIMB_BN][LIMB_BK + 1];
+    const int ks = blockIdx.x, nt = blockIdx.y, tid = threadIdx.x;
     const int kps = src.Kp / LIMB_BK;  // K-steps per segment
-    int seg = ks / kps;
-    int kbase = (ks % kps) * LIMB_BK + kq * 8;
-    u64 w[8];
+    const int seg = ks / kps;
+    const int k0 = (ks % kps) * LIMB_BK, n0 = nt * LIMB_BN;
+    if (src.row_major || src.dgrad_filter) {
+        // n contiguous: a wave reads 64 consecutive n of one k
+        const int nl = tid % LIMB_BN, kl0 = tid / LIMB_BN;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        int k = kbase + j;
-        u64 v = 0;
-        if (n < src.N && k < src.K) {
-            size_t idx;
-            if (src.dgrad_filter) {
-                int fwc = src.kfw * src.kc;
-                int r = k / fwc, rem = k - r * fwc;
-                int sc = rem / src.kc, c = rem - sc * src.kc;
-                idx = (((size_t)c * src.kfh + r) * src.kfw + sc) * src.N + n;
-            } else {
-                idx = src.row_major ? (size_t)k * src.N + n : (size_t)n * src.K + k;
-            }
-#pragma unroll
-            for (int t = 0; t < LIMB_MAX_TERMS; t++)
-                if (src.b[seg][t]) v += (u64)(s64)src.coef[seg][t] * src.b[seg][t][idx];
+        for (int j = 0; j < LIMB_BK / 4; j++) {
+            int kl = kl0 + 4 * j;
+            tile[nl][kl] = recode_digits(pack_src_value(src, seg, k0 + kl, n0 + nl));
         }
-        w[j] = recode_digits(v);
+    } else {
+        // k contiguous: half a wave reads 32 consecutive k of one n
+        const int kl = tid % LIMB_BK, nl0 = tid / LIMB_BK;
+#pragma unroll
+        for (int j = 0; j < LIMB_BN / 8; j++) {
+            int nl = nl0 + 8 * j;
+            tile[nl][kl] = recode_digits(pack_src_value(src, seg, k0 + kl, n0 + nl));
+        }
     }
-    u64 limbs[8];
+    __syncthreads();
+    const int nl = tid / (LIMB_BK / 8), kq = tid % (LIMB_BK / 8);
+    u64 w[8], limbs[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j] = tile[nl][kq * 8 + j];
     transpose8x8(w, limbs);
-    int nt = n / LIMB_BN, nl = n % LIMB_BN;
     int8_t *blk = out + ((size_t)nt * nks + ks) * (8 * LIMB_BN * LIMB_BK);
 #pragma unroll
     for (int i = 0; i < 8; i++) *(u64 *)(blk + (size_t)i * LIMB_BN * LIMB_BK + nl * LIMB_BK + kq * 8) = limbs[i];
@@ -179,9 +195,9 @@ int limb_pack_b(const LimbBSrc &src, int8_t *packed, hipStream_t s) {
                 "limb_pack_b: dgrad filter view needs K = kfh * kfw * kc");
     PIR_REQUIRE(src.Kp % LIMB_BK == 0 && src.Kp >= src.K, "limb_pack_b: Kp must be a multiple of 32 >= K");
     int nks = src.nseg * src.Kp / LIMB_BK;
-    size_t total = (size_t)((src.N + LIMB_BN - 1) / LIMB_BN) * LIMB_BN * nks * (LIMB_BK / 8);
-    if (total == 0) return OK;
-    hipLaunchKernelGGL(limb_pack_b_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, src, packed, nks);
+    const int ntiles = (src.N + LIMB_BN - 1) / LIMB_BN;
+    if (ntiles == 0 || nks == 0) return OK;
+    hipLaunchKernelGGL(limb_pack_b_kernel, dim3(nks, ntiles), dim3(256), 0, s, src, packed, nks);
     PIR_LAUNCH_CHECK();
     return OK;
 }
@@ -320,9 +336,9 @@ struct ALoader {
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
                 int k = kb + j;
-                int rr = k / fwc;
+                int rr = (int)a.fd_fwc.div((u32)k);
                 int rem = k - rr * fwc;
-                int ss = rem / a.Cin;
+                int ss = (int)a.fd_cin.div((u32)rem);
                 int cc = rem - ss * a.Cin;
                 bool ok;
                 size_t idx;
@@ -338,22 +354,34 @@ struct ALoader {
                 w[j] = src[idx] & (ok ? ~0ull : 0ull);
             }
         } else if (AMODE == 6) {
+            // k = (n, p, q): decode the first of the thread's KPT pixels, then
+            // step along the output row (a new row only when q wraps)
             const int ohw = a.OH * a.OW;
-            int n = kb / ohw;
+            int n = (int)a.fd_ohw.div((u32)kb);
             int rem = kb - n * ohw;
-            int p = rem / a.OW, q = rem - (rem / a.OW) * a.OW;
+            int p = (int)a.fd_ow.div((u32)rem);
+            int q = rem - p * a.OW;
+            int ih = p * a.stride - a.ph + r, iw = q * a.stride - a.pw + s;
+            // rowp addresses row ih of image n only when both exist (n < batch
+            // <=> n * OH * OW < K); otherwise it stays at element c0 of the tensor
+            bool hok = ih >= 0 && ih < a.H && n * ohw < a.K;
+            const u64 *rowp = src + (hok ? (((size_t)n * a.H + ih) * a.W) * a.Cin : 0) + c0;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
-                int ih = p * a.stride - a.ph + r, iw = q * a.stride - a.pw + s;
-                bool ok = row_ok && kb + j < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
-                size_t idx = ok ? (((size_t)n * a.H + ih) * a.W + iw) * a.Cin + c0 : 0;
-                w[j] = src[idx] & (ok ? ~0ull : 0ull);
+                bool ok = row_ok && kb + j < a.K && hok && iw >= 0 && iw < a.W;
+                w[j] = rowp[ok ? (size_t)iw * a.Cin : 0] & (ok ? ~0ull : 0ull);
+                iw += a.stride;
                 if (++q == a.OW) {
                     q = 0;
+                    iw = s - a.pw;
+                    ih += a.stride;
                     if (++p == a.OH) {
                         p = 0;
                         ++n;
+                        ih = r - a.ph;
                     }
+                    hok = ih >= 0 && ih < a.H && n * ohw < a.K;
+                    rowp = src + (hok ? (((size_t)n * a.H + ih) * a.W) * a.Cin : 0) + c0;
                 }
             }
         } else if (AMODE == 0 && (a.K % KPT) == 0) {
@@ -626,8 +654,16 @@ int limb_tune_load(const char *path) {
     return OK;
 }
 
-int limb_gemm(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, const LimbPlan &plan,
+int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &epi, const LimbPlan &plan,
               u64 *workspace, hipStream_t s) {
+    LimbA a = a_in;
+    if (a.conv != LIMB_DENSE) {
+        PIR_REQUIRE(a.fw > 0 && a.Cin > 0 && a.OH > 0 && a.OW > 0, "limb_gemm: invalid convolution geometry");
+        a.fd_fwc = FastDiv((u32)(a.fw * a.Cin));
+        a.fd_cin = FastDiv((u32)a.Cin);
+        a.fd_ohw = FastDiv((u32)(a.OH * a.OW));
+        a.fd_ow = FastDiv((u32)a.OW);
+    }
     PIR_REQUIRE(a.nseg >= 1 && a.nseg <= LIMB_MAX_SEG, "limb_gemm: nseg must be 1..12");
     PIR_REQUIRE(a.Kp % LIMB_BK == 0 && a.Kp >= a.K, "limb_gemm: Kp must be a multiple of 32 >= K");
     PIR_REQUIRE(epi.mode != EPI_TRUNC || (epi.trunc_bits >= 1 && epi.trunc_bits <= 62),

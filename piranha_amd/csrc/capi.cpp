@@ -986,6 +986,7 @@ int pir_net_set_train_labels(pir_net *n, const double *labels, size_t count) {
             n->labels.resize(count);
         }
         n->labels.setPublic(std::vector<double>(labels, labels + count), n->ctx.frac_bits);
+        n->net->prepareBackward();
         return (int)OK;
     });
 }
@@ -1013,7 +1014,9 @@ int pir_net_capture_train(pir_net *n) {
         NetScope ns(n);
         hipStream_t s = n->ctx.stream;
         drop_graph(n);
-        // one eager step (a real training step) allocates every buffer the capture needs
+        // persistent buffers first, then one eager step (a real training step)
+        // fills the allocator cache with every temporary the capture needs
+        n->net->prepareBackward();
         train_step(n);
         PIR_HIP(hipStreamSynchronize(s));
         PIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -444,6 +444,13 @@ void NeuralNetwork::backwardShares(const RSS64 &labelShare, const std::function<
     _backward_pass(outputDelta, after_step);
 }
 
+void NeuralNetwork::prepareBackward() {
+    if (layers.empty()) return;
+    size_t n = layers.back()->getActivation()->size();
+    if (outputDelta.size() != n) outputDelta.resize(n);
+    for (auto &l : layers) l->allocDelta();
+}
+
 void NeuralNetwork::_reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas) {
     const RSS64 &act = *layers.back()->getActivation();
     const size_t n = act.size();

@@ -186,6 +186,11 @@ class NeuralNetwork {
     // (k = 0: the softmax delta, k = 1 + i: layer i's backward; in the fused form
     // a ReLU layer's backward runs inside the layer after it).
     void backwardShares(const RSS64 &labelShare, const std::function<void(size_t)> &after_step = nullptr);
+    // Allocates every buffer a backward pass keeps (each layer's deltas and the
+    // output delta) up front, so that the temporaries of a training step come
+    // from the allocator cache in the same order on every step (a captured
+    // step then replays the eager step's allocations exactly).
+    void prepareBackward();
     void printNetwork();
     // dL/d(network output) of the last backward (input of the last layer's backward)
     RSS64 outputDelta;

@@ -236,6 +236,7 @@ struct ALoader {
     int am, akq, kps;
     bool row_ok;
     int seg, kk;          // uniform: segment and K-step within the segment
+    const u64 *src;       // a.seg[seg], reloaded only when the segment changes
     int cb, ih0, iw0;     // fprop: image and top-left input coordinate of this row's pixel;
                           // dgrad: image and (h + ph, w + pw) of this row's pixel
     int r, s, c0;         // AMODE 3/5: k = kk*32 = (r*fw + s)*Cin + c0; AMODE 6: the row's (r, s, c)
@@ -280,12 +281,14 @@ struct ALoader {
     __device__ void init(int ks) {
         seg = ks / kps;
         kk = ks - seg * kps;
+        src = a.seg[seg];
         decode();
     }
     __device__ void advance() {
         if (++kk == kps) {
             kk = 0;
             seg++;
+            src = a.seg[seg < a.nseg ? seg : a.nseg - 1];
             decode();
         } else if (AMODE == 3 || AMODE == 5) {
             c0 += LIMB_BK;
@@ -313,7 +316,6 @@ struct ALoader {
         return ok && p < a.OH && q < a.OW;
     }
     __device__ void load(u64 w[KPT]) const {
-        const u64 *__restrict__ src = a.seg[seg];
         const int kb = kk * LIMB_BK + akq * KPT;
         if (AMODE == 3) {
             int ih = ih0 + r, iw = iw0 + s;
@@ -554,18 +556,28 @@ int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &e
 
 namespace {
 
-// Heuristic split-K factor: enough workgroups for two per CU (512 on 256 CUs;
-// the kernel runs two workgroups per CU so one's staging hides under the
-// other's MFMAs), at least 8 K-steps each.
-int heuristic_split(int M, int N, int nks) {
-    int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-    if (tiles >= 512) return 1;
-    int want = (512 + tiles - 1) / tiles;
-    int cap = nks / 8;
-    if (cap < 1) cap = 1;
-    if (want > cap) want = cap;
-    if (want > LIMB_MAX_SPLIT) want = LIMB_MAX_SPLIT;
-    return want < 1 ? 1 : want;
+// Heuristic split-K factor from a round model: 2 workgroups per CU on 256 CUs
+// run 512 at a time, so a launch of W workgroups takes ceil(W / 512) rounds of
+// (K-steps per workgroup + a fixed prologue / epilogue cost of ~8 K-steps).
+// The split with the fewest rounds x steps wins (the smaller on ties); split-K
+// partials are cheap next to that for the shapes of the network.
+int heuristic_split(int M, int N, int nks, bool wide) {
+    const long tiles = wide ? (long)((M + 31) / 32) * ((N + 127) / 128) : (long)((M + 63) / 64) * ((N + 63) / 64);
+    const int lo = (nks + LIMB_FLUSH_STEPS - 1) / LIMB_FLUSH_STEPS;
+    int hi = nks / 4;
+    if (hi > LIMB_MAX_SPLIT) hi = LIMB_MAX_SPLIT;
+    if (hi < lo) hi = lo;
+    int best = lo;
+    long best_cost = -1;
+    for (int sp = lo; sp <= hi; sp++) {
+        long rounds = (tiles * sp + 511) / 512;
+        long cost = rounds * ((nks + sp - 1) / sp + 8);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = sp;
+        }
+    }
+    return best;
 }
 
 bool can_be_wide(int N) { return (((N + LIMB_BN - 1) / LIMB_BN) % 2) == 0; }
@@ -595,7 +607,7 @@ LimbPlan limb_plan(const LimbA &a, int N, int split_hint) {
     LimbPlan p;
     if (split_hint <= 0 && tuned_plan(shape_key(a, N), &p)) return p;
     p.wide = can_be_wide(N) ? 1 : 0;
-    p.split = limb_effective_split(nks, split_hint > 0 ? split_hint : heuristic_split(a.M, N, nks));
+    p.split = limb_effective_split(nks, split_hint > 0 ? split_hint : heuristic_split(a.M, N, nks, p.wide != 0));
     return p;
 }
 
@@ -736,16 +748,17 @@ int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, 
     if (c.tune && split_hint <= 0 && !c.alloc.capturing && !tuned_plan(key, &dummy)) {
         // candidates: both tile shapes, split factors around the heuristic
         const int nks = a.nseg * a.Kp / LIMB_BK;
-        const int h = heuristic_split(a.M, N, nks);
         std::vector<LimbPlan> cand;
-        for (int wide = can_be_wide(N) ? 1 : 0; wide >= 0; wide--)
-            for (int f : {h / 4, h / 2, h, 2 * h, 4 * h}) {
+        for (int wide = can_be_wide(N) ? 1 : 0; wide >= 0; wide--) {
+            const int h = heuristic_split(a.M, N, nks, wide != 0);
+            for (int f : {1, h / 4, h / 2, h - 1, h, h + 1, 2 * h, 4 * h}) {
                 if (f < 1 || f > LIMB_MAX_SPLIT) continue;
                 LimbPlan p{wide, limb_effective_split(nks, f)};
                 bool dup = false;
                 for (auto &q : cand) dup |= (q.wide == p.wide && q.split == p.split);
                 if (!dup) cand.push_back(p);
             }
+        }
         hipEvent_t e0, e1;
         PIR_HIP(hipEventCreate(&e0));
         PIR_HIP(hipEventCreate(&e1));

@@ -33,7 +33,8 @@ void *Allocator::alloc(size_t bytes) {
         return p;
     }
     if (capturing) {
-        set_error("device allocation requested during hipGraph capture (run the forward once before capturing)");
+        set_error("device allocation of " + std::to_string(sz) +
+                  " bytes requested during hipGraph capture (run the forward once before capturing)");
         throw PirError(ERR_STATE);
     }
     void *p = nullptr;

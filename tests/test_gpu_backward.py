@@ -376,3 +376,28 @@ def test_training_step_api_and_graph_replay():
         got = [graph.weights(i)[0] for i, l in enumerate(layers) if l[0] in ("fc", "cnn")]
         assert all(np.array_equal(a, b) for a, b in zip(got, traj[step])), f"replay {step}"
     assert np.array_equal(graph.delta(0), eager.delta(0))
+
+
+def test_capture_train_first_step_allocations():
+    """capture_train as the first training call: the output delta (batch x 10
+    shares) has the size of the last FC product's split-K workspace (6 splits
+    at K = 256), so the persistent backward buffers must exist before the
+    eager warm-up step for the captured step to find every temporary cached."""
+    layers = [("fc", 32, 256), ("relu", 256), ("fc", 256, 10), ("relu", 10)]
+    batch, fp = 8, 13
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.0, 1.0, batch * 32)
+    labels = one_hot(rng, batch, 10)
+    nets = [P.Network(C.to_spec(layers), batch, fp, seed=2) for _ in range(2)]
+    for net in nets:
+        net.forward(x)
+        net.set_train_labels(labels)
+    eager, graph = nets
+    graph.capture_train()  # one eager step + capture
+    graph.replay_train()
+    graph.synchronize()
+    for _ in range(2):
+        eager.train_step()
+    for i in (0, 2):
+        assert np.array_equal(graph.weights(i)[0], eager.weights(i)[0])
+        assert np.array_equal(graph.weights(i)[1], eager.weights(i)[1])

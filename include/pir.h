@@ -112,7 +112,9 @@ int pir_maxpool_row2im(const uint64_t *rows, uint64_t *out, int image_height, in
 /* ---- DeviceData element ops (src/gpu/DeviceData.h:131-209) ----------------- */
 enum { PIR_OP_ADD = 0, PIR_OP_SUB, PIR_OP_MUL, PIR_OP_XOR, PIR_OP_AND, PIR_OP_SHR, PIR_OP_SHL, PIR_OP_DIV };
 /* x[i*sx] = x[i*sx] op y[i*sy] (y == NULL: op scalar).  elem_bytes 8 or 1.  DIV
- * is the signed division of functors.cuh:33-45, restricted to power-of-two y. */
+ * is the signed division of functors.cuh:33-45 / 94-105, (S)((double)(S)x /
+ * (double)(S)y): exact integer arithmetic for powers of two, IEEE double
+ * division otherwise. */
 int pir_dd_op(int op, int elem_bytes, void *x, long long x_stride, const void *y, long long y_stride,
               uint64_t scalar, size_t n, void *stream);
 
@@ -125,7 +127,7 @@ int pir_rss_reconstruct(const void *x, size_t n, void *out, int elem_bytes, void
 int pir_rss_reshare(const uint64_t *c, size_t n, uint64_t *out, void *stream);             /* RSS.inl:1052-1074 */
 int pir_rss_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream);                 /* RSS.inl:283-320 */
 int pir_rss_and(void *x, const void *y, size_t n, int elem_bytes, void *stream);          /* RSS.inl:330-345 */
-/* dividePublic, RSS.inl:422-455: d (scalar) or d_local ([3][n]) power-of-two denominators */
+/* dividePublic, RSS.inl:422-455: d (scalar) or d_local ([3][n] per-party) denominators */
 int pir_rss_divide_public(uint64_t *a, size_t n, uint64_t d, const uint64_t *d_local, void *stream);
 /* dividePublic3out3, RSS.inl:1076-1094: a = 3-of-3 shares [3][n], denominator 2^k */
 int pir_rss_divide_public_3out3(const uint64_t *a, size_t n, int k, uint64_t *out, void *stream);

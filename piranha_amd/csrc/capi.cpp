@@ -417,14 +417,8 @@ int pir_rss_divide_public(uint64_t *a, size_t n, uint64_t d, const uint64_t *d_l
             PartyData<u64> dv(DeviceData<u64>((u64 *)d_local, n), DeviceData<u64>((u64 *)d_local + n, n),
                               DeviceData<u64>((u64 *)d_local + 2 * n, n));
             if (g_mode == MODE_FUSED) {
-                // the fused kernel needs a contiguous [3][n] denominator block
-                DevBuf flag(sizeof(int));
-                PIR_TRY(fill_u8((u8 *)flag.get(), 0, sizeof(int), (hipStream_t)stream));
-                PIR_TRY(fused_divide_public(a, n, 0, d_local, (int *)flag.get(), (hipStream_t)stream));
-                int bad = 0;
-                PIR_HIP(hipMemcpyAsync(&bad, flag.get(), sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream));
-                PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
-                PIR_REQUIRE(!bad, "dividePublic: denominators must be powers of two");
+                // the fused kernel reads the contiguous [3][n] denominator block
+                PIR_TRY(fused_divide_public(a, n, 0, d_local, (hipStream_t)stream));
                 return (int)OK;
             }
             dividePublic(x, dv);

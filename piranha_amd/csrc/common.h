@@ -95,4 +95,21 @@ __host__ __device__ inline u64 trunc_div_pow2(u64 t, int k) {
     return neg ? (u64)0 - q : q;
 }
 
+// The reference's DeviceData division (functors.cuh:33-45, 94-105):
+// (S)((double)(S)x / (double)(S)y).  A power-of-two y >= 2 goes through
+// trunc_div_pow2; any other y through IEEE-754 double division (correctly
+// rounded on CDNA as on the host) and truncation toward zero.  A quotient
+// outside the int64 range (|y| <= 1 or y == 0) becomes 0x8000000000000000, the
+// value the reference's x86 conversion yields.
+__host__ __device__ inline u64 signed_div_u64(u64 x, u64 y) {
+    if (y >= 2 && (y & (y - 1)) == 0 && y < (1ull << 63)) {
+        int k = 0;
+        while (((u64)1 << k) != y) k++;
+        return trunc_div_pow2(x, k);
+    }
+    double q = (double)(s64)x / (double)(s64)y;
+    if (!(q < 9223372036854775808.0) || !(q >= -9223372036854775808.0)) return (u64)1 << 63;
+    return (u64)(s64)q;
+}
+
 }  // namespace pir

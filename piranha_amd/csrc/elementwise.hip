@@ -26,11 +26,12 @@ __device__ __forceinline__ T ew_apply(int op, T x, T y) {
         case OP_SHR: return (T)((S)x >> y);
         case OP_SHL: return (T)(x << y);
         case OP_DIV: {
-            // functors.cuh:33-45 for a power-of-two divisor y = 2^k
-            int k = 0;
-            while (k < 63 && ((u64)1 << k) != (u64)y) k++;
-            if (sizeof(T) == 8) return (T)trunc_div_pow2((u64)x, k);
-            S q = (S)x / (S)y;  // exact for narrow types (|x| < 2^53)
+            // functors.cuh:33-45 / 94-105: (S)((double)(S)x / (double)(S)y)
+            if (sizeof(T) == 8) return (T)signed_div_u64((u64)x, (u64)y);
+            // narrow types: the double quotient of two small integers truncates
+            // to the integer quotient (C division truncates toward zero too)
+            if ((S)y == 0) return (T)((T)1 << (sizeof(T) * 8 - 1));  // as the 64-bit form
+            S q = (S)x / (S)y;
             return (T)q;
         }
         case OP_SET: return y;
@@ -66,10 +67,6 @@ int multi_op(const MultiOp &m, int elem_bytes, hipStream_t s) {
 
 int dd_op(int op, int elem_bytes, void *x, long long xs, const void *y, long long ys, u64 scalar, size_t n,
           hipStream_t s) {
-    if (op == OP_DIV && y == nullptr) {
-        PIR_REQUIRE(scalar != 0 && (scalar & (scalar - 1)) == 0,
-                    "DeviceData /=: only power-of-two divisors are supported");
-    }
     MultiOp m{};
     m.count = 1;
     m.op = op;

@@ -33,7 +33,7 @@ int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s);
 int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s);
 template <typename T>
 int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s);
-int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, int *bad_flag, hipStream_t s);
+int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s);
 template <typename U>
 int fused_convex_comb(u64 *d, const u64 *c, const U *e, size_t n, hipStream_t s);
 int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s);

@@ -513,13 +513,13 @@ int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s) {
     return OK;
 }
 
-// dividePublic with power-of-two denominators (RSS.inl:422-455 with the
-// placeholder r = 1, r' = d of Precompute.h:107-134).  Per party p:
-// a -= r'; t_p = reconstruct(a)_p; a = r + (A.sA, C.sB += t_p / d).
-// `dv` (per-party local denominators [3][n]) overrides the scalar `d` when set;
-// a denominator that is not a power of two sets *bad.
+// dividePublic (RSS.inl:422-455 with the placeholder r = 1, r' = d of
+// Precompute.h:107-134).  Per party p: a -= r'; t_p = reconstruct(a)_p;
+// a = r + (A.sA, C.sB += t_p / d) with the reference's division
+// (S)((double)(S)t / (double)(S)d).  `dv` (per-party local denominators [3][n])
+// overrides the scalar `d` when set.
 __global__ __launch_bounds__(EW_BLOCK) void divide_public_kernel(u64 *__restrict__ a, size_t n, u64 d,
-                                                                 const u64 *__restrict__ dv, int *__restrict__ bad) {
+                                                                 const u64 *__restrict__ dv) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     Sh3<u64> v = load3(a, n, i);
@@ -535,25 +535,15 @@ __global__ __launch_bounds__(EW_BLOCK) void divide_public_kernel(u64 *__restrict
     Sh3<u64> o;
 #pragma unroll
     for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
-#pragma unroll
-    for (int p = 0; p < 3; p += 2) {  // only A (shareA) and C (shareB) add the opened quotient
-        u64 den = dd[p];
-        if (den == 0 || (den & (den - 1)) != 0) {
-            *bad = 1;
-            continue;
-        }
-        int k = __ffsll((unsigned long long)den) - 1;
-        u64 q = trunc_div_pow2(t[p], k);
-        if (p == PARTY_A) o.s[p][0] = 1 + q;
-        else o.s[p][1] = 1 + q;
-    }
+    // only A (shareA) and C (shareB) add the opened quotient
+    o.s[PARTY_A][0] = 1 + signed_div_u64(t[PARTY_A], dd[PARTY_A]);
+    o.s[PARTY_C][1] = 1 + signed_div_u64(t[PARTY_C], dd[PARTY_C]);
     store3(a, n, i, o);
 }
 
-int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, int *bad_flag, hipStream_t s) {
-    if (!dv) PIR_REQUIRE(d != 0 && (d & (d - 1)) == 0 && d < (1ull << 63), "dividePublic: denominator must be a power of two");
+int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s) {
     if (n == 0) return OK;
-    hipLaunchKernelGGL(divide_public_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, d, dv, bad_flag);
+    hipLaunchKernelGGL(divide_public_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, d, dv);
     PIR_LAUNCH_CHECK();
     return OK;
 }

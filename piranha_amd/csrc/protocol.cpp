@@ -129,9 +129,8 @@ void rss_and_impl(RSS<T> &x, const RSS<T> &y) {
 
 // RSS.inl:422-436
 void dividePublic(RSS64 &a, u64 d) {
-    log2_exact(d);
     if (fused_mode() && a.base()) {
-        check(fused_divide_public(a.base(), a.size(), d, nullptr, nullptr, S()));
+        check(fused_divide_public(a.base(), a.size(), d, nullptr, S()));
         return;
     }
     size_t n = a.size();
@@ -147,23 +146,11 @@ void dividePublic(RSS64 &a, u64 d) {
     a += rec;
 }
 
-// RSS.inl:438-455 (power-of-two denominators, as produced by taylorSeries)
+// RSS.inl:438-455
 void dividePublic(RSS64 &a, PartyData<u64> &den) {
     size_t n = a.size();
     if (fused_mode() && a.base() && den.base()) {
-        DevBuf flag(sizeof(int));
-        check(fill_u8((u8 *)flag.get(), 0, sizeof(int), S()));
-        check(fused_divide_public(a.base(), n, 0, den.base(), (int *)flag.get(), S()));
-        int bad = 0;
-        if (hipMemcpyAsync(&bad, flag.get(), sizeof(int), hipMemcpyDeviceToHost, S()) != hipSuccess ||
-            hipStreamSynchronize(S()) != hipSuccess) {
-            set_error("dividePublic: flag readback failed");
-            throw PirError(ERR_HIP);
-        }
-        if (bad) {
-            set_error("dividePublic: denominators must be powers of two");
-            throw PirError(ERR_UNSUPPORTED);
-        }
+        check(fused_divide_public(a.base(), n, 0, den.base(), S()));
         return;
     }
     RSS64 r(n), rPrime(n);

@@ -128,16 +128,6 @@ void fused_product(const std::vector<Seg> &segs, bool ta, bool tb, const Geom *c
     check(limb_run(a, (const int8_t *)packed.get(), N, e, 0, S()));
 }
 
-int log2_pow2(u64 d) {
-    if (d == 0 || (d & (d - 1)) != 0) {
-        set_error("dividePublic: denominator must be a power of two");
-        throw PirError(ERR_UNSUPPORTED);
-    }
-    int k = 0;
-    while (((u64)1 << k) != d) k++;
-    return k;
-}
-
 Geom make_geom(int batch, int H, int W, int f, int Din, int Dout, int stride, int pad) {
     Geom g{batch, H, W, f, Din, Dout, stride, pad, (H + 2 * pad - f) / stride + 1, (W + 2 * pad - f) / stride + 1};
     return g;
@@ -193,7 +183,6 @@ void tpc_reconstruct(const u64 *x, size_t n, u64 *out) {
 }
 
 void tpc_divide_public(u64 *a, size_t n, u64 d) {
-    log2_pow2(d);
     Block r(2, n), rPrime(2, n), rec(2, n);
     tpc_fill(rPrime, d);
     tpc_fill(r, 1);
@@ -359,7 +348,6 @@ void fpc_reconstruct(const u64 *x, size_t n, u64 *out) {
 }
 
 void fpc_divide_public(u64 *a, size_t n, u64 d) {
-    log2_pow2(d);
     Block r(12, n), rPrime(12, n), rec(4, n);
     fpc_fill(rPrime, d);
     fpc_fill(r, 1);

@@ -100,6 +100,40 @@ def test_divide_public_vec(rng, mode, kind):
     assert np.array_equal(dx.numpy(), O.divide_public_vec(x, d))
 
 
+@pytest.mark.parametrize("d", [3, 9, 7, 1000, 1, 123456789, (1 << 40) + 3])
+def test_divide_public_general_denominator(rng, mode, kind, d):
+    """Non-power-of-two denominators take the IEEE double division of the
+    reference's functor (AveragepoolLayer.cu:81 divides by pool^2, e.g. 9)."""
+    n = 2000
+    x = make(rng, kind, n)
+    x[0, 0, :500] = rand_u64(rng, 500)  # magnitudes beyond 2^53: the int64 -> double rounding
+    edge = np.array([0, 1, (1 << 53) - 1, 1 << 53, (1 << 53) + 1, (1 << 63) - 1, 1 << 63, (1 << 64) - 1], np.uint64)
+    x[0, 0, 500:508] = edge
+    dx = P.to_dev(x)
+    P.check(P.lib().pir_rss_divide_public(dx.ptr, n, d, None, None))
+    assert np.array_equal(dx.numpy(), O.divide_public(x, d))
+
+
+def test_divide_public_vec_general(rng, mode, kind):
+    n = 1000
+    x = make(rng, kind, n)
+    d = rng.integers(1, 1 << 20, size=(3, n)).astype(np.uint64)
+    d[:, :5] = np.array([1, 2, 3, 9, 0], np.uint64)[:, None].T  # includes d = 0 (the x86 0x80..0 result)
+    dx, dd = P.to_dev(x), P.to_dev(d)
+    P.check(P.lib().pir_rss_divide_public(dx.ptr, n, 0, dd.ptr, None))
+    assert np.array_equal(dx.numpy(), O.divide_public_vec(x, d))
+
+
+@pytest.mark.parametrize("d", [3, 9, 10, 1, 2, 1 << 30, (1 << 62) + 1])
+def test_dd_op_divide(rng, d):
+    n = 3000
+    x = rand_u64(rng, n)
+    x[:8] = np.array([0, 1, (1 << 53) - 1, 1 << 53, (1 << 53) + 1, (1 << 63) - 1, 1 << 63, (1 << 64) - 1], np.uint64)
+    dx = P.to_dev(x)
+    P.check(P.lib().pir_dd_op(P.OP_DIV, 8, dx.ptr, 1, None, 0, d, n, None))
+    assert dx.numpy().tolist() == [O.scalar_divide(v, d) for v in x.tolist()]
+
+
 @pytest.mark.parametrize("k", [1, 13, 20, 26, 40])
 def test_divide_public_3out3(rng, mode, k):
     n = 4096

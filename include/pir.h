@@ -98,6 +98,18 @@ int pir_stride_pad(const uint64_t *in, uint64_t *out, size_t n_in, int stride, i
 size_t pir_maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);
 int pir_maxpool_im2row(const uint64_t *im, uint64_t *out, int H, int W, int f, int Din, int batch, int stride,
                        int padding, uint64_t pad_value, void *stream);
+/* gpu::averagepool_im2row, convolution.cuh:339-371 (the reference host's
+ * argument order, width before height); out must hold
+ * pir_averagepool_im2row_size() values: f*f per (image, window, channel) */
+size_t pir_averagepool_im2row_size(int image_width, int image_height, int f, int Din, int batch, int stride,
+                                   int padding);
+int pir_averagepool_im2row(const uint64_t *im, uint64_t *out, int image_width, int image_height, int f, int Din,
+                           int batch, int stride, int padding, void *stream);
+/* gpu::averagepool_expand_delta, convolution.cuh:194-207, 412-437: in[k]
+ * (k = b*Din + c) fills out[b*Din*pool_size + c + j*Din] for j < pool_size;
+ * out_capacity (entries of out) must be at least n_in * pool_size */
+int pir_averagepool_expand_delta(const uint64_t *in, size_t n_in, uint64_t *out, size_t out_capacity, int Din,
+                                 int pool_size, void *stream);
 /* gpu::reduceSum, matrix.cuh:109-126, 331-351: b[j] += sum_i (reduce_rows ?
  * a[cols*i + j] (j < cols) : a[cols*j + i] (j < rows)) */
 int pir_reduce_sum(const uint64_t *a, uint64_t *b, int reduce_rows, int rows, int cols, void *stream);
@@ -113,8 +125,9 @@ int pir_maxpool_row2im(const uint64_t *rows, uint64_t *out, int image_height, in
 enum { PIR_OP_ADD = 0, PIR_OP_SUB, PIR_OP_MUL, PIR_OP_XOR, PIR_OP_AND, PIR_OP_SHR, PIR_OP_SHL, PIR_OP_DIV };
 /* x[i*sx] = x[i*sx] op y[i*sy] (y == NULL: op scalar).  elem_bytes 8 or 1.  DIV
  * is the signed division of functors.cuh:33-45 / 94-105, (S)((double)(S)x /
- * (double)(S)y): exact integer arithmetic for powers of two, IEEE double
- * division otherwise. */
+ * (double)(S)y), restated in integer arithmetic (binary64 rounding of both
+ * operands, round-to-nearest-even quotient, truncation toward zero; a quotient
+ * outside the signed range clamps, 0/0 gives 0). */
 int pir_dd_op(int op, int elem_bytes, void *x, long long x_stride, const void *y, long long y_stride,
               uint64_t scalar, size_t n, void *stream);
 
@@ -178,13 +191,15 @@ int pir_fpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
 
 /* ---- layers and network (src/nn/FCLayer.cu etc., NeuralNetwork.cu) -------------- */
 typedef struct pir_net pir_net;
-enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3 };
+enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3, PIR_LAYER_AVGPOOL = 4 };
 enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
 /* spec: per layer 8 ints {type, a0..a6}:
  *   FC      {0, inputDim, outputDim}                                (FCConfig.h:16)
  *   CNN     {1, H, W, Cin, Cout, filter, stride, padding}           (CNNConfig.h:27)
  *   ReLU    {2, inputDim}                                           (ReLUConfig.h:12)
  *   Maxpool {3, H, W, C, pool, stride}                              (MaxpoolConfig.h:16)
+ *   Avgpool {4, H, W, C, pool, stride}                              (AveragepoolConfig.h:15)
+ * Each layer's input size must equal the previous layer's output size.
  * Weights are initialised as in FCLayer.cu:31-48 / CNNLayer.cu:36-50 with
  * std::default_random_engine(seed + layer) unless set with pir_net_set_weights. */
 int pir_net_create(pir_net **net, const int *spec, int num_layers, int batch, int frac_bits, int seed, int mode,

@@ -96,8 +96,13 @@ _SIGS = {
     "ref_relu_backward": [_u64p, _u8p, _sz, _u64p],
     "ref_maxpool_backward": [_u64p, _u8p, _u64p] + [_i] * 6,
     "ref_reveal_softmax_grad": [_u64p, _dp, _sz, _u64p, _i, _i, _i],
+    "ref_averagepool_im2row": [_u64p, _u64p] + [_i] * 7,
+    "ref_averagepool_expand_delta": [_u64p, _sz, _u64p, _sz, _i, _i],
+    "ref_avgpool_forward": [_u64p, _u64p] + [_i] * 6,
+    "ref_avgpool_backward": [_u64p, _sz, _u64p, _sz, _i, _i],
 }
-_RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_scalar_divide": _u64, "ref_version": _i}
+_RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_averagepool_im2row": _sz,
+        "ref_scalar_divide": _u64, "ref_version": _i}
 
 _lib = None
 
@@ -424,6 +429,42 @@ def maxpool_forward(x, batch, H, W, C, pool, stride, fp):
     return act, mp
 
 
+def averagepool_im2row(im, imageWidth, imageHeight, f, Din, batch, stride, padding=0):
+    """gpu::averagepool_im2row (convolution.cuh:339-371) of one share array."""
+    im = np.ascontiguousarray(im, np.uint64)
+    n = lib().ref_averagepool_im2row(_p(im), ctypes.cast(None, _u64p), imageWidth, imageHeight, f, Din, batch, stride, padding)
+    out = np.zeros(n, np.uint64)
+    lib().ref_averagepool_im2row(_p(im), _p(out), imageWidth, imageHeight, f, Din, batch, stride, padding)
+    return out
+
+
+def averagepool_expand_delta(x, out_n, Din, pool_size):
+    """gpu::averagepool_expand_delta (convolution.cuh:194-207, 412-437) into a
+    zeroed array of out_n entries."""
+    x = np.ascontiguousarray(x, np.uint64)
+    out = np.zeros(out_n, np.uint64)
+    lib().ref_averagepool_expand_delta(_p(x), x.size, _p(out), out_n, Din, pool_size)
+    return out
+
+
+def avgpool_forward(x, batch, H, W, C, pool, stride):
+    """AveragepoolLayer::forward (AveragepoolLayer.cu:49-96)."""
+    x = _u(x)
+    OH = (H - pool) // stride + 1
+    OW = (W - pool) // stride + 1
+    act = np.zeros((3, 2, batch * C * OH * OW), np.uint64)
+    lib().ref_avgpool_forward(_p(x), _p(act), batch, H, W, C, pool, stride)
+    return act
+
+
+def avgpool_backward(delta, batch, H, W, C, pool):
+    """AveragepoolLayer::backward (AveragepoolLayer.cu:98-125)."""
+    delta = _u(delta)
+    deltas = np.zeros((3, 2, batch * H * W * C), np.uint64)
+    lib().ref_avgpool_backward(_p(delta), delta.shape[-1], _p(deltas), deltas.shape[-1], C, pool)
+    return deltas
+
+
 def net_forward(layers, batch, fp, params, x_shares):
     """NeuralNetwork::forward (NeuralNetwork.cu:108-133) over the oracle layers.
 
@@ -451,6 +492,10 @@ def net_forward(layers, batch, fp, params, x_shares):
             H, Wd, C, pool, s = l[1:]
             h, d = maxpool_forward(h, batch, H, Wd, C, pool, s, fp)
             out.append((h, d))
+        elif kind == "avgpool":
+            H, Wd, C, pool, s = l[1:]
+            h = avgpool_forward(h, batch, H, Wd, C, pool, s)
+            out.append((h, None))
         else:
             raise ValueError(kind)
     return out
@@ -596,6 +641,9 @@ def net_backward(layers, batch, fp, params, x_shares, fwd, labels, lr, out_delta
         elif kind == "maxpool":
             H, Wd, C, pool, s = layers[i][1:]
             d = maxpool_backward(delta, fwd[i][1], batch, H, Wd, C, pool, s)
+        elif kind == "avgpool":
+            H, Wd, C, pool, s = layers[i][1:]
+            d = avgpool_backward(delta, batch, H, Wd, C, pool)
         else:
             raise ValueError(kind)
         deltas[i] = d

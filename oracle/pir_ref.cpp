@@ -33,6 +33,7 @@
 #include <cstddef>
 #include <cstring>
 #include <cmath>
+#include <limits>
 #include <vector>
 #include <type_traits>
 #include <algorithm>
@@ -145,10 +146,14 @@ inline T sdiv(T x, T y) {
     // functors.cuh:33-45 / 94-105: (S)((double)(S)x / (double)(S)y)
     typedef typename std::make_signed<T>::type S;
     double q = (double)(S)x / (double)(S)y;
-    // out-of-range conversion is undefined in C++; x86 yields the "indefinite"
-    // integer 0x80..0.  Only reachable for |y| == 1, never on the hot path.
+    // The functor runs on the reference's GPU (thrust::transform over
+    // DeviceData, DeviceData.h:147-150, 182-185), whose double -> integer
+    // conversion clamps to the destination range and maps NaN to 0 (PTX cvt
+    // semantics).  Only reachable for |y| <= 1.
     const double lim = std::ldexp(1.0, (int)(sizeof(S) * 8 - 1));
-    if (!(q < lim) || !(q >= -lim)) return (T)((T)1 << (sizeof(T) * 8 - 1));
+    if (q != q) return 0;
+    if (q >= lim) return (T)std::numeric_limits<S>::max();
+    if (q < -lim) return (T)std::numeric_limits<S>::min();
     return (T)(S)q;
 }
 
@@ -541,13 +546,13 @@ inline int expanded_pool_size(int f) {
     return (int)std::pow(std::ceil(std::log2((double)(f * f))), 2);
 }
 
-// maxpool_im2row, convolution.cuh:57-108 and host :252-304.  The host passes
-// (imageHeight, imageWidth) into the kernel's (imageWidth, imageHeight) slots,
-// which makes the gather plain NHWC: x runs over rows, y over columns.
-// Out-of-image taps are skipped without advancing the output index.
+// The window gather of the maxpool_im2row kernel, convolution.cuh:57-108, as
+// both hosts (:252-304 and :339-371) call it: they pass (imageHeight,
+// imageWidth) into the kernel's (imageWidth, imageHeight) slots, which makes
+// the gather plain NHWC: x runs over rows, y over columns.  Out-of-image taps
+// are skipped without advancing the output index.  Output [batch][window][Din][f*f].
 template <typename T>
-void maxpool_im2row(const T *im, std::vector<T> &out, int H, int W, int f, int Din, int batch, int stride,
-                    int padding, T pad_val) {
+std::vector<T> pool_windows(const T *im, int H, int W, int f, int Din, int batch, int stride, int padding) {
     int kW = H, kH = W;  // kernel-side names after the swap
     int widthKernels = ((kW - f + 2 * padding) / stride) + 1;
     int heightKernels = ((kH - f + 2 * padding) / stride) + 1;
@@ -568,12 +573,42 @@ void maxpool_im2row(const T *im, std::vector<T> &out, int H, int W, int f, int D
                     unpadded[o++] = im[src];
                 }
         }
+    return unpadded;
+}
+
+// maxpool_im2row, convolution.cuh:252-304: the window gather, then each pool
+// padded to pow(ceil(log2(f*f)), 2) entries with pad_val (stride_pad).
+template <typename T>
+void maxpool_im2row(const T *im, std::vector<T> &out, int H, int W, int f, int Din, int batch, int stride,
+                    int padding, T pad_val) {
+    std::vector<T> unpadded = pool_windows(im, H, W, f, Din, batch, stride, padding);
+    size_t pool = (size_t)f * f, groups = unpadded.size() / pool;
     int padded = expanded_pool_size(f);
-    out.assign((size_t)padded * windows * batch * Din, 0);
+    out.assign((size_t)padded * groups, 0);
     if ((size_t)padded == pool) {
         std::copy(unpadded.begin(), unpadded.end(), out.begin());
     } else {
-        stride_pad(unpadded.data(), out.data(), windows * batch * Din, (int)pool, padded - (int)pool, pad_val);
+        stride_pad(unpadded.data(), out.data(), groups, (int)pool, padded - (int)pool, pad_val);
+    }
+}
+
+// averagepool_im2row, convolution.cuh:339-371: host arguments (imageWidth,
+// imageHeight), the same gather and no padding.
+template <typename T>
+void averagepool_im2row(const T *im, std::vector<T> &out, int imageWidth, int imageHeight, int f, int Din, int batch,
+                        int stride, int padding) {
+    out = pool_windows(im, imageHeight, imageWidth, f, Din, batch, stride, padding);
+}
+
+// averagepool_expand_delta, convolution.cuh:194-207: input entry k = b*Din + c
+// fills out[b*Din*P + c + j*Din] for j < P (entries past out_n are not written;
+// the reference's layer sizes never reach them).
+template <typename T>
+void averagepool_expand_delta(const T *in, size_t n_in, T *out, size_t out_n, int Din, int P) {
+    for (size_t k = 0; k < n_in; k++) {
+        size_t b = k / Din, c = k % Din;
+        for (size_t i = b * Din * P + c; i < (b + 1) * Din * P; i += Din)
+            if (i < out_n) out[i] = in[k];
     }
 }
 
@@ -1222,6 +1257,31 @@ void maxpool_backward(Rs<u64> delta, Rs<uint8_t> max_prime, Rs<u64> deltas, int 
         for (int i = 0; i < 2; i++) maxpool_row2im(deltaPools.v().s[q][i], deltas.s[q][i], W, H, pool, C, batch, stride);
 }
 
+// AveragepoolLayer::forward, AveragepoolLayer.cu:49-96: per share
+// averagepool_im2row(input, imageWidth, imageHeight, ...) and reduceSum of each
+// pool into the zeroed activations, then dividePublic by f*f.
+void avgpool_forward(Rs<u64> input, Rs<u64> act, int batch, int H, int W, int C, int pool, int stride) {
+    r_zero(act);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) {
+            std::vector<u64> pools;
+            averagepool_im2row(&input.s[q][i][0], pools, W, H, pool, C, batch, stride, 0);
+            reduce_sum(pools.data(), &act.s[q][i][0], false, (int)act.size(), pool * pool);
+        }
+    divide_public(act, (u64)pool * pool);
+}
+
+// AveragepoolLayer::backward, AveragepoolLayer.cu:98-125: deltas zeroed,
+// averagepool_expand_delta per share, dividePublic by f*f.
+void avgpool_backward(Rs<u64> delta, Rs<u64> deltas, int C, int pool) {
+    r_zero(deltas);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            averagepool_expand_delta(&delta.s[q][i][0], delta.size(), &deltas.s[q][i][0], deltas.size(), C,
+                                     pool * pool);
+    divide_public(deltas, (u64)pool * pool);
+}
+
 // NeuralNetwork::_reveal_softmax_grad, NeuralNetwork.cu:207-267.  Every party
 // opens the logits, computes softmax in FP64 on the host (std::exp, summed in
 // element order per row), converts back with to_fixed_functor (functors.h:116-123)
@@ -1243,8 +1303,13 @@ void reveal_softmax_grad(Rs<u64> act, const double *labels, Rs<u64> deltas, int 
             sums[i / nClasses] += fa[i];
         }
         for (size_t i = 0; i < n; i++) fa[i] /= sums[i / nClasses];
-        // (T)(x * (1 << FLOAT_PRECISION)) for 0 <= x <= 1
-        for (size_t i = 0; i < n; i++) sm.d[q][i] = (u64)(s64)(fa[i] * scale);
+        // to_fixed_functor (T)(x * (1 << FLOAT_PRECISION)), evaluated on the
+        // reference's GPU: the double -> uint64 conversion clamps to the
+        // uint64 range and maps NaN (an overflowed row sum) to 0
+        for (size_t i = 0; i < n; i++) {
+            double f = fa[i] * scale;
+            sm.d[q][i] = !(f > 0.0) ? 0 : (f >= 18446744073709551616.0 ? ~(u64)0 : (u64)f);
+        }
     }
     RsBuf<u64> labelShare(n);
     r_set_public(labelShare.v(), labels, fp);
@@ -1533,6 +1598,24 @@ void ref_maxpool_backward(const u64 *delta, const uint8_t *max_prime, u64 *delta
     size_t outN = (size_t)batch * C * OH * OW;
     maxpool_backward(wrapRs((u64 *)delta, outN), wrapRs((uint8_t *)max_prime, outN * pool * pool),
                      wrapRs(deltas, (size_t)batch * H * W * C), batch, H, W, C, pool, stride);
+}
+size_t ref_averagepool_im2row(const u64 *im, u64 *out, int imageWidth, int imageHeight, int f, int Din, int batch,
+                              int stride, int padding) {
+    std::vector<u64> o;
+    averagepool_im2row(im, o, imageWidth, imageHeight, f, Din, batch, stride, padding);
+    if (out) std::copy(o.begin(), o.end(), out);
+    return o.size();
+}
+void ref_averagepool_expand_delta(const u64 *in, size_t n_in, u64 *out, size_t out_n, int Din, int P) {
+    averagepool_expand_delta(in, n_in, out, out_n, Din, P);
+}
+void ref_avgpool_forward(const u64 *input, u64 *act, int batch, int H, int W, int C, int pool, int stride) {
+    int OH = (H - pool) / stride + 1, OW = (W - pool) / stride + 1;
+    avgpool_forward(wrapRs((u64 *)input, (size_t)batch * H * W * C), wrapRs(act, (size_t)batch * C * OH * OW), batch,
+                    H, W, C, pool, stride);
+}
+void ref_avgpool_backward(const u64 *delta, size_t n_in, u64 *deltas, size_t n_out, int C, int pool) {
+    avgpool_backward(wrapRs((u64 *)delta, n_in), wrapRs(deltas, n_out), C, pool);
 }
 void ref_reveal_softmax_grad(const u64 *act, const double *labels, size_t n, u64 *deltas, int nClasses, int fp,
                              int logMiniBatch) {

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libpir_hip.so")
 
 ALGO_PLAIN, ALGO_LIMB = 0, 1
 MODE_FUSED, MODE_PLAIN = 0, 1
-LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL = 0, 1, 2, 3
+LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL = 0, 1, 2, 3, 4
 CONV_FPROP, CONV_DGRAD, CONV_WGRAD = 0, 1, 2
 LIMB_WIDE, LIMB_VARIANTS = 8, 16  # kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
 OP_ADD, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV = range(8)
@@ -64,6 +64,9 @@ _SIGS = {
     "pir_stride_pad": ([_vp, _vp, _sz, _i, _i, _u64, _vp], _i),
     "pir_maxpool_im2row_size": ([_i] * 7, _sz),
     "pir_maxpool_im2row": ([_vp, _vp] + [_i] * 7 + [_u64, _vp], _i),
+    "pir_averagepool_im2row_size": ([_i] * 7, _sz),
+    "pir_averagepool_im2row": ([_vp, _vp] + [_i] * 7 + [_vp], _i),
+    "pir_averagepool_expand_delta": ([_vp, _sz, _vp, _sz, _i, _i, _vp], _i),
     "pir_dd_op": ([_i, _i, _vp, _ll, _vp, _ll, _u64, _sz, _vp], _i),
     "pir_rss_reconstruct": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_rss_reshare": ([_vp, _sz, _vp, _vp], _i),

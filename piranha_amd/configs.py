@@ -3,13 +3,14 @@ of pir_net_create (include/pir.h), plus a reader for the reference's model JSON
 format (src/util/model.cu:24-115, files/models/*.json).
 
 Layer specs: ("fc", in, out) | ("cnn", H, W, Cin, Cout, f, stride, pad) |
-("relu", n_per_image) | ("maxpool", H, W, C, pool, stride).
+("relu", n_per_image) | ("maxpool", H, W, C, pool, stride) |
+("avgpool", H, W, C, pool, stride).
 """
 import json
 
-from . import LAYER_CNN, LAYER_FC, LAYER_MAXPOOL, LAYER_RELU
+from . import LAYER_AVGPOOL, LAYER_CNN, LAYER_FC, LAYER_MAXPOOL, LAYER_RELU
 
-_TYPES = {"fc": LAYER_FC, "cnn": LAYER_CNN, "relu": LAYER_RELU, "maxpool": LAYER_MAXPOOL}
+_TYPES = {"fc": LAYER_FC, "cnn": LAYER_CNN, "relu": LAYER_RELU, "maxpool": LAYER_MAXPOOL, "avgpool": LAYER_AVGPOOL}
 
 
 def to_spec(layers):
@@ -39,9 +40,10 @@ def output_size(layers):
     return ((H - pool) // s + 1) * ((W - pool) // s + 1) * C
 
 
-def from_model_json(doc, avgpool_as_maxpool=True):
+def from_model_json(doc, avgpool_as_maxpool=False):
     """Reads the reference's model JSON (model.cu:24-115).  Average-pool layers
-    become max-pool layers of the same geometry when `avgpool_as_maxpool`."""
+    become max-pool layers of the same geometry when `avgpool_as_maxpool` (the
+    max-pool variants of the VGG16 / AlexNet CIFAR models)."""
     if isinstance(doc, str):
         with open(doc) as f:
             doc = json.load(f)
@@ -58,8 +60,11 @@ def from_model_json(doc, avgpool_as_maxpool=True):
         elif t == "maxpool" or (t == "averagepool" and avgpool_as_maxpool):
             out.append(("maxpool", l["input_hw"][0], l["input_hw"][1], l["in_channels"], l["pool_hw"][0],
                         l["stride"]))
+        elif t == "averagepool":
+            out.append(("avgpool", l["input_hw"][0], l["input_hw"][1], l["in_channels"], l["pool_hw"][0],
+                        l["stride"]))
         else:
-            raise ValueError(f"layer type {t!r} is not on the inference path")
+            raise ValueError(f"layer type {t!r} is not supported")
     return out
 
 

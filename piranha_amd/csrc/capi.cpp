@@ -343,6 +343,18 @@ int pir_maxpool_im2row(const uint64_t *im, uint64_t *out, int H, int W, int f, i
                        int padding, uint64_t pv, void *stream) {
     return maxpool_im2row(im, out, H, W, f, Din, batch, stride, padding, pv, (hipStream_t)stream);
 }
+size_t pir_averagepool_im2row_size(int image_width, int image_height, int f, int Din, int batch, int stride,
+                                   int padding) {
+    return averagepool_im2row_size(image_width, image_height, f, Din, batch, stride, padding);
+}
+int pir_averagepool_im2row(const uint64_t *im, uint64_t *out, int image_width, int image_height, int f, int Din,
+                           int batch, int stride, int padding, void *stream) {
+    return averagepool_im2row(im, out, image_width, image_height, f, Din, batch, stride, padding, (hipStream_t)stream);
+}
+int pir_averagepool_expand_delta(const uint64_t *in, size_t n_in, uint64_t *out, size_t out_capacity, int Din,
+                                 int pool_size, void *stream) {
+    return averagepool_expand_delta(in, n_in, out, out_capacity, Din, pool_size, (hipStream_t)stream);
+}
 int pir_dd_op(int op, int eb, void *x, long long xs, const void *y, long long ys, uint64_t scalar, size_t n,
               void *stream) {
     PIR_REQUIRE(op >= 0 && op <= PIR_OP_DIV, "pir_dd_op: unknown op");

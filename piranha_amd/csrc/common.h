@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <string>
 
+#include "ring_div.h"
+
 namespace pir {
 
 typedef uint64_t u64;
@@ -69,47 +71,5 @@ struct FastDiv {
         return (u32)(((((u64)n * mul) >> 32) + n) >> shr);
     }
 };
-
-// Emulates the reference truncation q = (S)((double)(S)t / (double)(S)2^k)
-// (functors.cuh:33-45) in integer arithmetic: the int64 -> double conversion
-// rounds |t| to 53 significant bits (IEEE round-to-nearest-even, sign
-// symmetric), the division by 2^k is exact, and the conversion back truncates
-// toward zero.  Valid for 1 <= k <= 62.
-__host__ __device__ inline u64 trunc_div_pow2(u64 t, int k) {
-    bool neg = (s64)t < 0;
-    u64 mag = neg ? (u64)0 - t : t;
-    if (mag >> 53) {
-#if defined(__HIP_DEVICE_COMPILE__)
-        int lz = __clzll((long long)mag);
-#else
-        int lz = __builtin_clzll(mag);
-#endif
-        int shift = 11 - lz;  // bit length - 53
-        u64 rem = mag & ((1ull << shift) - 1);
-        u64 half = 1ull << (shift - 1);
-        u64 base = mag >> shift;
-        if (rem > half || (rem == half && (base & 1))) base++;
-        mag = base << shift;
-    }
-    u64 q = mag >> k;
-    return neg ? (u64)0 - q : q;
-}
-
-// The reference's DeviceData division (functors.cuh:33-45, 94-105):
-// (S)((double)(S)x / (double)(S)y).  A power-of-two y >= 2 goes through
-// trunc_div_pow2; any other y through IEEE-754 double division (correctly
-// rounded on CDNA as on the host) and truncation toward zero.  A quotient
-// outside the int64 range (|y| <= 1 or y == 0) becomes 0x8000000000000000, the
-// value the reference's x86 conversion yields.
-__host__ __device__ inline u64 signed_div_u64(u64 x, u64 y) {
-    if (y >= 2 && (y & (y - 1)) == 0 && y < (1ull << 63)) {
-        int k = 0;
-        while (((u64)1 << k) != y) k++;
-        return trunc_div_pow2(x, k);
-    }
-    double q = (double)(s64)x / (double)(s64)y;
-    if (!(q < 9223372036854775808.0) || !(q >= -9223372036854775808.0)) return (u64)1 << 63;
-    return (u64)(s64)q;
-}
 
 }  // namespace pir

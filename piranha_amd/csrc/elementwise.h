@@ -35,6 +35,16 @@ int stride_pad(const u64 *in, u64 *out, size_t n_in, int stride, int pad, u64 pv
 size_t maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);
 int maxpool_im2row(const u64 *im, u64 *out, int H, int W, int f, int Din, int batch, int stride, int padding, u64 pv,
                    hipStream_t s);
+// gpu::averagepool_im2row (convolution.cuh:339-371): the maxpool_im2row gather
+// with the same (height, width) swap and no padding to a power of two; output
+// f*f x windows x batch x Din entries.
+size_t averagepool_im2row_size(int imageWidth, int imageHeight, int f, int Din, int batch, int stride, int padding);
+int averagepool_im2row(const u64 *im, u64 *out, int imageWidth, int imageHeight, int f, int Din, int batch, int stride,
+                       int padding, hipStream_t s);
+// gpu::averagepool_expand_delta (convolution.cuh:194-207, 412-437): writes the
+// first n_in * pool_size entries of out (which must hold that many).
+int averagepool_expand_delta(const u64 *in, size_t n_in, u64 *out, size_t out_capacity, int Din, int pool_size,
+                             hipStream_t s);
 // gpu::reduceSum (matrix.cuh:109-126, 331-351): b[col] += sum over the other axis.
 int reduce_sum(const u64 *a, u64 *b, int reduce_rows, int rows, int cols, hipStream_t s);
 // gpu::vectorExpand (matrix.cuh:128-139, 354-380): b[i*f + j] = a[i].

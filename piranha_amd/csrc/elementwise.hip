@@ -29,10 +29,13 @@ __device__ __forceinline__ T ew_apply(int op, T x, T y) {
             // functors.cuh:33-45 / 94-105: (S)((double)(S)x / (double)(S)y)
             if (sizeof(T) == 8) return (T)signed_div_u64((u64)x, (u64)y);
             // narrow types: the double quotient of two small integers truncates
-            // to the integer quotient (C division truncates toward zero too)
-            if ((S)y == 0) return (T)((T)1 << (sizeof(T) * 8 - 1));  // as the 64-bit form
-            S q = (S)x / (S)y;
-            return (T)q;
+            // to the integer quotient (C division truncates toward zero too),
+            // clamped to the type's range as the 64-bit form is
+            const int lo = -(1 << (sizeof(T) * 8 - 1)), hi = (1 << (sizeof(T) * 8 - 1)) - 1;
+            int xs = (int)(S)x, ys = (int)(S)y;
+            if (ys == 0) return (T)(S)(xs == 0 ? 0 : (xs < 0 ? lo : hi));
+            int q = xs / ys;
+            return (T)(S)(q > hi ? hi : q);
         }
         case OP_SET: return y;
     }
@@ -275,6 +278,51 @@ int maxpool_im2row(const u64 *im, u64 *out, int H, int W, int f, int Din, int ba
     size_t threads = (size_t)wK * hK * batch * Din;
     hipLaunchKernelGGL(im2row_kernel, dim3(g1(threads)), dim3(BLK), 0, s, im, out, kW, kH, f, Din, batch, stride,
                        padding, wK, hK, padded, pv);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+size_t averagepool_im2row_size(int imageWidth, int imageHeight, int f, int Din, int batch, int stride, int padding) {
+    if (f < 1 || stride < 1) return 0;
+    int wK = ((imageWidth - f + 2 * padding) / stride) + 1, hK = ((imageHeight - f + 2 * padding) / stride) + 1;
+    if (wK <= 0 || hK <= 0 || Din <= 0 || batch <= 0) return 0;
+    return (size_t)f * f * wK * hK * batch * Din;
+}
+
+int averagepool_im2row(const u64 *im, u64 *out, int imageWidth, int imageHeight, int f, int Din, int batch, int stride,
+                       int padding, hipStream_t s) {
+    PIR_REQUIRE(f >= 1 && stride >= 1 && Din > 0 && batch > 0 && padding >= 0, "averagepool_im2row: invalid geometry");
+    // the host passes (imageHeight, imageWidth) into maxpool_im2row's (width, height)
+    int kW = imageHeight, kH = imageWidth;
+    int wK = ((kW - f + 2 * padding) / stride) + 1, hK = ((kH - f + 2 * padding) / stride) + 1;
+    PIR_REQUIRE(wK > 0 && hK > 0, "averagepool_im2row: empty output");
+    size_t threads = (size_t)wK * hK * batch * Din;
+    hipLaunchKernelGGL(im2row_kernel, dim3(g1(threads)), dim3(BLK), 0, s, im, out, kW, kH, f, Din, batch, stride,
+                       padding, wK, hK, f * f, (u64)0);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// averagepool_expand_delta, convolution.cuh:194-207 / :412-437: input entry
+// k = b*Din + c (b over image-window pairs) fills out[b*Din*P + c + j*Din] for
+// j < P.  Gathered: one thread per written output entry.
+__global__ __launch_bounds__(BLK) void avgpool_expand_kernel(const u64 *in, u64 *out, size_t n_out, int Din, int P) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n_out) return;
+    size_t b = i / ((size_t)Din * P);
+    int c = (int)(i % (size_t)Din);
+    out[i] = in[b * Din + c];
+}
+
+int averagepool_expand_delta(const u64 *in, size_t n_in, u64 *out, size_t out_capacity, int Din, int pool_size,
+                             hipStream_t s) {
+    PIR_REQUIRE(Din > 0 && pool_size > 0 && n_in % (size_t)Din == 0,
+                "averagepool_expand_delta: the input must hold whole Din-channel groups");
+    PIR_REQUIRE(n_in * (size_t)pool_size <= out_capacity,
+                "averagepool_expand_delta: the output must hold input size x poolSize entries");
+    size_t n_out = n_in * (size_t)pool_size;
+    if (n_out == 0) return OK;
+    hipLaunchKernelGGL(avgpool_expand_kernel, dim3(g1(n_out)), dim3(BLK), 0, s, in, out, n_out, Din, pool_size);
     PIR_LAUNCH_CHECK();
     return OK;
 }

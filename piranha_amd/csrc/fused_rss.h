@@ -42,6 +42,13 @@ int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s);
 // ReLULayer::backward (ReLULayer.cu:91-123): out = selectShare(0, delta, reluPrime).
 template <typename U>
 int fused_relu_backward(const u64 *delta, const U *relu_prime, size_t n, u64 *out, hipStream_t s);
+// AveragepoolLayer::forward (AveragepoolLayer.cu:49-96) in one pass (pad_value
+// of g unused), optionally with the following ReLU layer.
+int fused_avgpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, hipStream_t s, u64 *relu_out = nullptr,
+                        u8 *relu_d = nullptr);
+// AveragepoolLayer::backward (AveragepoolLayer.cu:98-125): delta [3][2][n_in]
+// -> out [3][2][n_out] (n_out >= n_in * P).
+int fused_avgpool_backward(const u64 *delta, size_t n_in, int C, int P, u64 *out, size_t n_out, hipStream_t s);
 // MaxpoolLayer::backward (MaxpoolLayer.cu:133-180): vectorExpand, selectShare
 // with maxPrime and maxpool_row2im in one pass, gathered per input element.
 template <typename U>
@@ -49,12 +56,11 @@ int fused_maxpool_backward(const u64 *delta, const U *max_prime, const MaxpoolGe
 // FCLayer.cu:205-218: db = reduceSum(delta, false, outDim, batch) per share,
 // dividePublic(db, 2^lr), biases -= db.
 int fused_fc_bias_update(const u64 *delta, u64 *bias, int out_dim, int batch, int lr, hipStream_t s);
-// NeuralNetwork::_reveal_softmax_grad (NeuralNetwork.cu:207-267): open the
-// logits, softmax per row of n_classes in FP64, back to fixed point, minus the
-// label shares, then dividePublic by 2^lmb when lmb > 1.
-int fused_softmax_grad(const u64 *act, const u64 *labels, int rows, int n_classes, int fp, int lmb, u64 *out,
-                       hipStream_t s);
-// The FP64 softmax of each party's opened logits ([3][n] -> [3][n] fixed point).
-int softmax_fixed(const u64 *revealed, int rows, int n_classes, int fp, u64 *out, hipStream_t s);
+// NeuralNetwork::_reveal_softmax_grad (NeuralNetwork.cu:207-267), the device
+// halves around the host softmax: parties A and C open the logits
+// (out [2][n]), and deltas = (A.sA, C.sB = softmax values [2][n]) - labels,
+// then dividePublic by 2^lmb when lmb > 1.
+int fused_open_logits_ac(const u64 *act, size_t n, u64 *out, hipStream_t s);
+int fused_softmax_delta(const u64 *sm, const u64 *labels, size_t n, int lmb, u64 *out, hipStream_t s);
 
 }  // namespace pir

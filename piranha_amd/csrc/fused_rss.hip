@@ -243,6 +243,59 @@ int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime,
     return OK;
 }
 
+// AveragepoolLayer::forward (AveragepoolLayer.cu:49-96): averagepool_im2row,
+// reduceSum over each f x f window in every share and dividePublic by f*f, one
+// thread per output element; the following ReLU layer is applied to the same
+// element when relu_out is set.
+__global__ __launch_bounds__(EW_BLOCK) void avgpool_layer_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
+                                                                 u64 *__restrict__ act, u64 *__restrict__ relu_out,
+                                                                 u8 *__restrict__ relu_d) {
+    const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    size_t o = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (o >= outN) return;
+    const size_t inN = (size_t)g.batch * g.H * g.W * g.C;
+    int c = (int)(o % g.C);
+    size_t r = o / g.C;
+    int ow = (int)(r % g.OW);
+    r /= g.OW;
+    int oh = (int)(r % g.OH);
+    int b = (int)(r / g.OH);
+    Sh3<u64> sum;
+#pragma unroll
+    for (int p = 0; p < 3; p++) sum.s[p][0] = sum.s[p][1] = 0;
+    for (int dh = 0; dh < g.f; dh++)
+        for (int dw = 0; dw < g.f; dw++) {
+            Sh3<u64> e = load3(x, inN, (((size_t)b * g.H + oh * g.stride + dh) * g.W + ow * g.stride + dw) * g.C + c);
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                sum.s[p][0] += e.s[p][0];
+                sum.s[p][1] += e.s[p][1];
+            }
+        }
+    Sh3<u64> v = divide_public_(sum, (u64)g.f * (u64)g.f);
+    store3(act, outN, o, v);
+    if (relu_out) {
+        Sh3<u8> rb = drelu_<u8>(v);
+        store3(relu_d, outN, o, rb);
+        Sh3<u64> zero;
+#pragma unroll
+        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+        store3(relu_out, outN, o, select_<u8>(zero, v, rb));
+    }
+}
+
+int fused_avgpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, hipStream_t s, u64 *relu_out, u8 *relu_d) {
+    PIR_REQUIRE(!relu_out == !relu_d, "averagepool layer: a fused ReLU needs both outputs");
+    PIR_REQUIRE(g.f >= 1 && g.stride >= 1 && g.H >= g.f && g.W >= g.f, "averagepool layer: invalid geometry");
+    PIR_REQUIRE(g.OH == (g.H - g.f) / g.stride + 1 && g.OW == (g.W - g.f) / g.stride + 1,
+                "averagepool layer: inconsistent output size");
+    size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    if (outN == 0) return OK;
+    hipLaunchKernelGGL(avgpool_layer_kernel, dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, relu_out, relu_d);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 // ---------------------------------------------------------------------------
 // Backward (training) path
 
@@ -357,82 +410,79 @@ int fused_fc_bias_update(const u64 *delta, u64 *bias, int out_dim, int batch, in
     return OK;
 }
 
-// Softmax of one row of opened logits v[0..nc) in the reference's FP64 steps:
-// x = (double)(int64)v / 2^fp, e = exp(x) summed in element order, e / sum,
-// back with (uint64)(e / sum * 2^fp).  exp is evaluated twice per element
-// (sum pass, output pass) so no per-row storage is needed; both evaluations
-// are the same function of the same input.
-template <typename F>
-__device__ __forceinline__ void softmax_row(F logit, int nc, int fp, u64 *sm, size_t stride) {
-    const double scale = (double)(1 << fp);
-    double sum = 0.0;
-    for (int i = 0; i < nc; i++) sum += exp((double)(s64)logit(i) / scale);
-    for (int i = 0; i < nc; i++) {
-        double e = exp((double)(s64)logit(i) / scale);
-        sm[(size_t)i * stride] = (u64)(s64)((e / sum) * scale);
-    }
+// Parties A and C open the logits: the only opened values that reach an
+// output share (deltas += softmax touches A.sA and C.sB).  out[0] = A's view
+// sA + sB + (C's sA), out[1] = C's view sA + sB + (B's sA).
+__global__ __launch_bounds__(EW_BLOCK) void open_logits_ac_kernel(const u64 *__restrict__ act, size_t n,
+                                                                  u64 *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const u64 A0 = act[0 * n + i], A1 = act[1 * n + i], B0 = act[2 * n + i];
+    const u64 C0 = act[4 * n + i], C1 = act[5 * n + i];
+    out[i] = A0 + A1 + C0;
+    out[n + i] = C0 + C1 + B0;
 }
 
-__global__ __launch_bounds__(EW_BLOCK) void softmax_fixed_kernel(const u64 *__restrict__ rev, int rows, int nc, int fp,
-                                                                 u64 *__restrict__ out) {
-    int t = blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (t >= 3 * rows) return;
-    int q = t / rows, row = t - q * rows;
-    const size_t n = (size_t)rows * nc;
-    const u64 *v = rev + (size_t)q * n + (size_t)row * nc;
-    softmax_row([&](int i) { return v[i]; }, nc, fp, out + (size_t)q * n + (size_t)row * nc, 1);
-}
-
-int softmax_fixed(const u64 *revealed, int rows, int n_classes, int fp, u64 *out, hipStream_t s) {
-    PIR_REQUIRE(rows >= 0 && n_classes >= 1 && fp >= 0 && fp <= 30, "softmax: invalid arguments");
-    if (rows == 0) return OK;
-    hipLaunchKernelGGL(softmax_fixed_kernel, dim3(ew_grid((size_t)3 * rows)), dim3(EW_BLOCK), 0, s, revealed, rows,
-                       n_classes, fp, out);
+int fused_open_logits_ac(const u64 *act, size_t n, u64 *out, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(open_logits_ac_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, act, n, out);
     PIR_LAUNCH_CHECK();
     return OK;
 }
 
-// One thread per row: parties A and C open the row (the only opened values
-// that reach an output share: deltas += softmax touches A.sA and C.sB),
-// deltas = (A.sA, C.sB = softmax) - labels, then the dividePublic.
-__global__ __launch_bounds__(EW_BLOCK) void softmax_grad_kernel(const u64 *__restrict__ act,
-                                                                const u64 *__restrict__ labels, int rows, int nc,
-                                                                int fp, int lmb, u64 *__restrict__ out) {
-    int row = blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (row >= rows) return;
-    const size_t n = (size_t)rows * nc, base = (size_t)row * nc;
-    // A opens sA + sB + (C's sA); C opens sA + sB + (B's sA)
-    const u64 *A0 = act + 0 * n + base, *A1 = act + 1 * n + base, *B0 = act + 2 * n + base;
-    const u64 *C0 = act + 4 * n + base, *C1 = act + 5 * n + base;
-    // softmax values are staged in the output's A.sA and C.sB slots
-    u64 *oA = out + 0 * n + base, *oC = out + 5 * n + base;
-    softmax_row([&](int i) { return A0[i] + A1[i] + C0[i]; }, nc, fp, oA, 1);
-    softmax_row([&](int i) { return C0[i] + C1[i] + B0[i]; }, nc, fp, oC, 1);
-    for (int i = 0; i < nc; i++) {
-        Sh3<u64> d;
+// deltas = (A.sA = sm[0], C.sB = sm[1]) - labels, then dividePublic by 2^lmb
+// when lmb > 1 (NeuralNetwork.cu:259-264).
+__global__ __launch_bounds__(EW_BLOCK) void softmax_delta_kernel(const u64 *__restrict__ sm,
+                                                                 const u64 *__restrict__ labels, size_t n, int lmb,
+                                                                 u64 *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<u64> d = load3(labels, n, i);
 #pragma unroll
-        for (int p = 0; p < 3; p++) d.s[p][0] = d.s[p][1] = 0;
-        d.s[PARTY_A][0] = oA[i];
-        d.s[PARTY_C][1] = oC[i];
-        Sh3<u64> L = load3(labels, n, base + i);
-#pragma unroll
-        for (int p = 0; p < 3; p++) {
-            d.s[p][0] -= L.s[p][0];
-            d.s[p][1] -= L.s[p][1];
-        }
-        if (lmb > 1) d = divide_public_pow2_(d, lmb);
-        store3(out, n, base + i, d);
+    for (int p = 0; p < 3; p++) {
+        d.s[p][0] = (u64)0 - d.s[p][0];
+        d.s[p][1] = (u64)0 - d.s[p][1];
     }
+    d.s[PARTY_A][0] += sm[i];
+    d.s[PARTY_C][1] += sm[n + i];
+    if (lmb > 1) d = divide_public_pow2_(d, lmb);
+    store3(out, n, i, d);
 }
 
-int fused_softmax_grad(const u64 *act, const u64 *labels, int rows, int n_classes, int fp, int lmb, u64 *out,
-                       hipStream_t s) {
-    PIR_REQUIRE(rows >= 0 && n_classes >= 1 && fp >= 0 && fp <= 30 && lmb >= 0 && lmb <= 62,
-                "softmax gradient: invalid arguments");
-    PIR_REQUIRE(act != out, "softmax gradient: output must not alias the activations");
-    if (rows == 0) return OK;
-    hipLaunchKernelGGL(softmax_grad_kernel, dim3(ew_grid(rows)), dim3(EW_BLOCK), 0, s, act, labels, rows, n_classes, fp,
-                       lmb, out);
+int fused_softmax_delta(const u64 *sm, const u64 *labels, size_t n, int lmb, u64 *out, hipStream_t s) {
+    PIR_REQUIRE(lmb >= 0 && lmb <= 62, "softmax delta: invalid arguments");
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(softmax_delta_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, sm, labels, n, lmb, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// AveragepoolLayer::backward (AveragepoolLayer.cu:98-125): the expanded delta
+// (averagepool_expand_delta, convolution.cuh:194-207: deltas entry
+// b*C*P + c + j*C takes delta[b*C + c] for j < P, b over image-window pairs;
+// entries past n_in * P stay zero), then dividePublic by P over all of deltas.
+// One thread per deltas entry.
+__global__ __launch_bounds__(EW_BLOCK) void avgpool_backward_kernel(const u64 *__restrict__ delta, size_t n_in, int C,
+                                                                    int P, size_t n_out, u64 *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n_out) return;
+    Sh3<u64> v;
+    if (i < n_in * (size_t)P) {
+        size_t b = i / ((size_t)C * P);
+        v = load3(delta, n_in, b * C + (size_t)(i % (size_t)C));
+    } else {
+#pragma unroll
+        for (int p = 0; p < 3; p++) v.s[p][0] = v.s[p][1] = 0;
+    }
+    store3(out, n_out, i, divide_public_(v, (u64)P));
+}
+
+int fused_avgpool_backward(const u64 *delta, size_t n_in, int C, int P, u64 *out, size_t n_out, hipStream_t s) {
+    PIR_REQUIRE(C > 0 && P > 0 && n_in % (size_t)C == 0, "averagepool backward: invalid arguments");
+    PIR_REQUIRE(n_in * (size_t)P <= n_out, "averagepool backward: the expanded delta does not fit the input size");
+    if (n_out == 0) return OK;
+    hipLaunchKernelGGL(avgpool_backward_kernel, dim3(ew_grid(n_out)), dim3(EW_BLOCK), 0, s, delta, n_in, C, P, n_out,
+                       out);
     PIR_LAUNCH_CHECK();
     return OK;
 }

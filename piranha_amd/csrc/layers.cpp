@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <vector>
 
 #include "elementwise.h"
 #include "fused_rss.h"
@@ -381,22 +382,116 @@ void MaxpoolLayer::backward(const RSS64 &delta, const RSS64 &) {
 }
 
 // ---------------------------------------------------------------------------
+// AveragepoolLayer.cu
+
+AveragepoolLayer::AveragepoolLayer(const AveragepoolConfig &c, int n, int)
+    : Layer(n), conf(c),
+      activations(c.batchSize * c.features * ((c.imageWidth - c.poolSize) / c.stride + 1) *
+                  ((c.imageHeight - c.poolSize) / c.stride + 1)) {}
+
+void AveragepoolLayer::printLayer() {
+    printf("----------------------------------------------\n(%d) Averagepool Layer\t  %zu x %zu x %zu\n\t\t\t  %zu  \t\t(Pooling Size)\n"
+           "\t\t\t  %zu \t\t(Stride)\n\t\t\t  %zu\t\t(Batch Size)\n",
+           layerNum + 1, conf.imageHeight, conf.imageWidth, conf.features, conf.poolSize, conf.stride, conf.batchSize);
+}
+
+MaxpoolGeom AveragepoolLayer::geom() const {
+    return MaxpoolGeom{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, (int)conf.features,
+                       (int)conf.poolSize, (int)conf.stride, outH(), outW(), 0};
+}
+
+bool AveragepoolLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
+    if (!(ctx().mode == MODE_FUSED && input.base() && relu.activations.base() && relu.reluPrime.base()))
+        return false;
+    check(fused_avgpool_layer(input.base(), geom(), activations.base(), ctx().stream, relu.activations.base(),
+                              relu.reluPrime.base()));
+    return true;
+}
+
+// AveragepoolLayer.cu:49-96
+void AveragepoolLayer::forward(const RSS64 &input) {
+    const int f = (int)conf.poolSize;
+    if (ctx().mode == MODE_FUSED && input.base()) {
+        check(fused_avgpool_layer(input.base(), geom(), activations.base(), ctx().stream));
+        return;
+    }
+    activations.zero();
+    RSS64 pools(averagepool_im2row_size((int)conf.imageWidth, (int)conf.imageHeight, f, (int)conf.features,
+                                        (int)conf.batchSize, (int)conf.stride, 0));
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(averagepool_im2row(input.getShare(q, i)->data(), pools.getShare(q, i)->data(), (int)conf.imageWidth,
+                                     (int)conf.imageHeight, f, (int)conf.features, (int)conf.batchSize,
+                                     (int)conf.stride, 0, ctx().stream));
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(reduce_sum(pools.getShare(q, i)->data(), activations.getShare(q, i)->data(), 0,
+                             (int)activations.size(), f * f, ctx().stream));
+    dividePublic(activations, (u64)f * (u64)f);
+}
+
+// AveragepoolLayer.cu:98-125
+void AveragepoolLayer::backward(const RSS64 &delta, const RSS64 &) {
+    ensureDeltas();
+    const int P = (int)(conf.poolSize * conf.poolSize);
+    if (ctx().mode == MODE_FUSED && delta.base()) {
+        check(fused_avgpool_backward(delta.base(), delta.size(), (int)conf.features, P, deltas.base(), deltas.size(),
+                                     ctx().stream));
+        return;
+    }
+    deltas.zero();
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(averagepool_expand_delta(delta.getShare(q, i)->data(), delta.size(), deltas.getShare(q, i)->data(),
+                                           deltas.size(), (int)conf.features, P, ctx().stream));
+    dividePublic(deltas, (u64)P);
+}
+
+// ---------------------------------------------------------------------------
 // Network (NeuralNetwork.cu:32-133)
+
+// Geometry of one layer spec row: every size positive, windows inside the
+// (padded) image, so that the layer's size arithmetic and kernels stay in range.
+static bool spec_valid(const int *v) {
+    switch (v[0]) {
+        case 0: return v[1] > 0 && v[2] > 0;
+        case 1:
+            return v[1] > 0 && v[2] > 0 && v[3] > 0 && v[4] > 0 && v[5] > 0 && v[6] > 0 && v[7] >= 0 &&
+                   v[1] + 2 * v[7] >= v[5] && v[2] + 2 * v[7] >= v[5];
+        case 2: return v[1] > 0;
+        case 3:
+        case 4: return v[1] > 0 && v[2] > 0 && v[3] > 0 && v[4] > 0 && v[5] > 0 && v[1] >= v[4] && v[2] >= v[4];
+        default: return true;  // rejected below
+    }
+}
 
 NeuralNetwork::NeuralNetwork(const std::vector<LayerSpec> &spec, int b, int seed) : batch(b) {
     for (size_t i = 0; i < spec.size(); i++) {
         const int *v = spec[i].v;
         int s = seed + (int)i;
+        if (!spec_valid(v)) {
+            set_error("network: layer " + std::to_string(i) + " has an invalid geometry");
+            throw PirError(ERR_ARG);
+        }
         switch (v[0]) {
             case 0: layers.emplace_back(new FCLayer(FCConfig(v[1], b, v[2]), (int)i, s)); break;
             case 1: layers.emplace_back(new CNNLayer(CNNConfig(v[1], v[2], v[3], v[4], v[5], v[6], v[7], b), (int)i, s)); break;
             case 2: layers.emplace_back(new ReLULayer(ReLUConfig(v[1], b), (int)i, s)); break;
             case 3: layers.emplace_back(new MaxpoolLayer(MaxpoolConfig(v[1], v[2], v[3], v[4], v[5], b), (int)i, s)); break;
+            case 4:
+                layers.emplace_back(new AveragepoolLayer(AveragepoolConfig(v[1], v[2], v[3], v[4], v[5], b), (int)i, s));
+                break;
             default:
-                set_error("network: only FC, CNN, ReLU and Maxpool layers are supported");
+                set_error("network: only FC, CNN, ReLU, Maxpool and Averagepool layers are supported");
                 throw PirError(ERR_UNSUPPORTED);
         }
     }
+    for (size_t i = 1; i < layers.size(); i++)
+        if (layers[i]->inputSize() != layers[i - 1]->getActivation()->size()) {
+            set_error("network: layer " + std::to_string(i) + " input size does not match layer " +
+                      std::to_string(i - 1) + "'s output");
+            throw PirError(ERR_ARG);
+        }
     if (!layers.empty()) input.resize(layers[0]->inputSize());
 }
 
@@ -449,26 +544,95 @@ void NeuralNetwork::prepareBackward() {
     size_t n = layers.back()->getActivation()->size();
     if (outputDelta.size() != n) outputDelta.resize(n);
     for (auto &l : layers) l->allocDelta();
+    _prepare_softmax();
+}
+
+NeuralNetwork::HostSoftmax::~HostSoftmax() {
+    if (in_) (void)hipHostFree(in_);
+    if (out_) (void)hipHostFree(out_);
+}
+
+void NeuralNetwork::HostSoftmax::prepare(size_t n, int views, int rows, int nc, int fp) {
+    if (n != n_ || views != views_) {
+        if (in_) hip_check(hipHostFree(in_));
+        if (out_) hip_check(hipHostFree(out_));
+        in_ = out_ = nullptr;
+        n_ = 0;
+        hip_check(hipHostMalloc((void **)&in_, views * n * sizeof(u64), hipHostMallocDefault));
+        hip_check(hipHostMalloc((void **)&out_, views * n * sizeof(u64), hipHostMallocDefault));
+        n_ = n;
+        views_ = views;
+    }
+    rows_ = rows;
+    nc_ = nc;
+    fp_ = fp;
+}
+
+void NeuralNetwork::HostSoftmax::enqueue(const u64 *dev_in, u64 *dev_out, hipStream_t s) {
+    const size_t bytes = (size_t)views_ * n_ * sizeof(u64);
+    hip_check(hipMemcpyAsync(in_, dev_in, bytes, hipMemcpyDeviceToHost, s));
+    hip_check(hipLaunchHostFunc(s, &HostSoftmax::run, this));
+    hip_check(hipMemcpyAsync(dev_out, out_, bytes, hipMemcpyHostToDevice, s));
+}
+
+// NeuralNetwork.cu:220-257 for each view: x = (double)(int64)v / 2^fp
+// (to_double_functor, functors.h:105-114), exp summed in element order per row,
+// each value divided by its row's sum, back with (uint64)(x * 2^fp)
+// (to_fixed_functor, functors.h:116-123; the reference converts on its GPU,
+// which clamps to the uint64 range and maps NaN to 0).
+void NeuralNetwork::HostSoftmax::run(void *self) {
+    const HostSoftmax &h = *(const HostSoftmax *)self;
+    const double scale = (double)(1 << h.fp_);
+    std::vector<double> e((size_t)h.nc_);
+    for (int v = 0; v < h.views_; v++)
+        for (int r = 0; r < h.rows_; r++) {
+            const size_t base = (size_t)v * h.n_ + (size_t)r * h.nc_;
+            double sum = 0.0;
+            for (int i = 0; i < h.nc_; i++) {
+                e[i] = std::exp((double)(s64)h.in_[base + i] / scale);
+                sum += e[i];
+            }
+            for (int i = 0; i < h.nc_; i++) {
+                double q = e[i] / sum;
+                double f = q * scale;
+                u64 fx;
+                if (!(f > 0.0)) fx = 0;  // NaN, zero or negative
+                else if (f >= 18446744073709551616.0) fx = ~0ull;
+                else fx = (u64)f;
+                h.out_[base + i] = fx;
+            }
+        }
+}
+
+void NeuralNetwork::_prepare_softmax() {
+    if (layers.empty() || batch <= 0) return;
+    const size_t n = layers.back()->getActivation()->size();
+    const int nc = (int)(n / (size_t)batch), fp = ctx().frac_bits;
+    if (softmaxOpen_.size() != n) {
+        softmaxOpen_.resize(n);
+        softmaxVals_.resize(n);
+    }
+    softmaxAC_.prepare(n, 2, batch, nc, fp);
+    softmaxAll_.prepare(n, 3, batch, nc, fp);
 }
 
 void NeuralNetwork::_reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas) {
     const RSS64 &act = *layers.back()->getActivation();
     const size_t n = act.size();
-    const int nClasses = (int)(n / (size_t)batch);
     const int lmb = (int)std::log2((double)batch);  // LOG_MINI_BATCH, model.cu:39
-    const int fp = ctx().frac_bits;
+    _prepare_softmax();
     if (ctx().mode == MODE_FUSED && act.base() && labels.base() && deltas.base()) {
-        check(fused_softmax_grad(act.base(), labels.base(), batch, nClasses, fp, lmb, deltas.base(), ctx().stream));
+        check(fused_open_logits_ac(act.base(), n, softmaxOpen_.base(), ctx().stream));
+        softmaxAC_.enqueue(softmaxOpen_.base(), softmaxVals_.base(), ctx().stream);
+        check(fused_softmax_delta(softmaxVals_.base(), labels.base(), n, lmb, deltas.base(), ctx().stream));
         return;
     }
     RSS64 x(n);
     x += act;
-    PartyData<u64> revealed(n);
-    reconstruct(x, revealed);
-    PartyData<u64> sm(n);
-    check(softmax_fixed(revealed.base(), batch, nClasses, fp, sm.base(), ctx().stream));
+    reconstruct(x, softmaxOpen_);
+    softmaxAll_.enqueue(softmaxOpen_.base(), softmaxVals_.base(), ctx().stream);
     deltas.zero();
-    deltas += sm;
+    deltas += softmaxVals_;
     deltas -= labels;
     if (lmb > 1) dividePublic(deltas, (u64)1 << lmb);
 }

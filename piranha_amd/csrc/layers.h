@@ -8,12 +8,14 @@
 #include <string>
 #include <vector>
 
+#include "fused_rss.h"
 #include "protocol.h"
 
 namespace pir {
 
 // Layer configurations, same fields and argument order as the reference
-// (FCConfig.h:16, CNNConfig.h:27, ReLUConfig.h:12, MaxpoolConfig.h:16).
+// (FCConfig.h:16, CNNConfig.h:27, ReLUConfig.h:12, MaxpoolConfig.h:16,
+// AveragepoolConfig.h:15).
 struct FCConfig {
     size_t inputDim, batchSize, outputDim;
     FCConfig(size_t in, size_t batch, size_t out) : inputDim(in), batchSize(batch), outputDim(out) {}
@@ -31,6 +33,12 @@ struct ReLUConfig {
 struct MaxpoolConfig {
     size_t imageHeight, imageWidth, features, poolSize, stride, batchSize;
     MaxpoolConfig(size_t h, size_t w, size_t c, size_t pool, size_t s, size_t b)
+        : imageHeight(h), imageWidth(w), features(c), poolSize(pool), stride(s), batchSize(b) {}
+};
+
+struct AveragepoolConfig {
+    size_t imageHeight, imageWidth, features, poolSize, stride, batchSize;
+    AveragepoolConfig(size_t h, size_t w, size_t c, size_t pool, size_t s, size_t b)
         : imageHeight(h), imageWidth(w), features(c), poolSize(pool), stride(s), batchSize(b) {}
 };
 
@@ -162,6 +170,30 @@ class MaxpoolLayer : public Layer {
     RSS<u8> maxPrime;
 };
 
+// AveragepoolLayer.cu: window sums in every share, dividePublic by f*f; the
+// backward expands each delta over its window (in the reference's own
+// averagepool_expand_delta order) and divides by f*f again.
+class AveragepoolLayer : public Layer {
+  public:
+    AveragepoolLayer(const AveragepoolConfig &conf, int layerNum, int seed);
+    void printLayer() override;
+    void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
+    bool forwardWithReLU(const RSS64 &input, ReLULayer &relu) override;
+    RSS64 *getActivation() override { return &activations; }
+    size_t inputSize() const override {
+        return conf.batchSize * conf.imageHeight * conf.imageWidth * conf.features;
+    }
+    int outH() const { return (int)((conf.imageHeight - conf.poolSize) / conf.stride + 1); }
+    int outW() const { return (int)((conf.imageWidth - conf.poolSize) / conf.stride + 1); }
+
+    AveragepoolConfig conf;
+    RSS64 activations;
+
+  private:
+    MaxpoolGeom geom() const;
+};
+
 // Layer description used by the C ABI: {type, a0..a6} (see include/pir.h).
 struct LayerSpec {
     int v[8];
@@ -199,6 +231,29 @@ class NeuralNetwork {
     int batch;
 
   private:
+    // The reveal-softmax is computed on the host, as in the reference, whose
+    // softmax loop reads the opened logits one by one into host doubles
+    // (NeuralNetwork.cu:220-257): the opened logits go to pinned host memory,
+    // a host function computes the fixed-point softmax per row and the values
+    // come back, all in stream order (a captured training step records the
+    // host function as a graph node).  Buffers are allocated by prepare(),
+    // outside any capture, and keep their addresses for the life of the network.
+    class HostSoftmax {
+      public:
+        ~HostSoftmax();
+        void prepare(size_t n, int views, int rows, int nc, int fp);
+        // dev_in [views][n] opened logits -> dev_out [views][n] fixed-point softmax
+        void enqueue(const u64 *dev_in, u64 *dev_out, hipStream_t s);
+        static void run(void *self);
+
+      private:
+        u64 *in_ = nullptr, *out_ = nullptr;  // pinned host [views][n]
+        size_t n_ = 0;
+        int views_ = 0, rows_ = 0, nc_ = 0, fp_ = 0;
+    };
+    HostSoftmax softmaxAC_, softmaxAll_;  // fused form (views A, C), op-by-op form (3 views)
+    PartyData<u64> softmaxOpen_, softmaxVals_;
+    void _prepare_softmax();
     void _reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas);  // NeuralNetwork.cu:207-267
     void _backward_pass(const RSS64 &deltas, const std::function<void(size_t)> &after_step);  // :446-467
 };

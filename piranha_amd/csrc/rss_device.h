@@ -179,6 +179,21 @@ __device__ __forceinline__ Sh3<u64> divide_public_pow2_(Sh3<u64> v, int k) {
     return o;
 }
 
+// dividePublic by any public d (RSS.inl:422-436 with r = 1, r' = d) with the
+// reference's division (S)((double)(S)t / (double)(S)d) of the opened values.
+__device__ __forceinline__ Sh3<u64> divide_public_(Sh3<u64> v, u64 d) {
+    v.s[PARTY_A][0] -= d;
+    v.s[PARTY_C][1] -= d;
+    u64 tA = v.s[PARTY_A][0] + v.s[PARTY_A][1] + v.s[PARTY_C][0];
+    u64 tC = v.s[PARTY_C][0] + v.s[PARTY_C][1] + v.s[PARTY_B][0];
+    Sh3<u64> o;
+#pragma unroll
+    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
+    o.s[PARTY_A][0] = 1 + signed_div_u64(tA, d);
+    o.s[PARTY_C][1] = 1 + signed_div_u64(tC, d);
+    return o;
+}
+
 template <typename T>
 __device__ __forceinline__ Sh3<T> load3(const T *__restrict__ base, size_t n, size_t i) {
     Sh3<T> v;

@@ -113,4 +113,11 @@ inline void check(int status) {
     if (status != 0) throw PirError(status);
 }
 
+inline void hip_check(hipError_t e) {
+    if (e != hipSuccess) {
+        set_error(std::string("hip: ") + hipGetErrorString(e));
+        throw PirError(ERR_HIP);
+    }
+}
+
 }  // namespace pir

@@ -206,6 +206,25 @@ int pir_net_create(pir_net **net, const int *spec, int num_layers, int batch, in
                    void *stream);
 int pir_net_destroy(pir_net *net);
 int pir_net_num_layers(const pir_net *net);
+/* ---- model files and snapshots ---------------------------------------------- */
+/* loadModel (model.cu:24-115) without building the network: the file's
+ * input_size, num_classes, batch_size, the layer count (*num_layers) and up to
+ * max_layers layer specs (8 ints each, the pir_net_create form).  Layer types
+ * fc, relu, cnn, maxpool, averagepool; ln and res give PIR_ERR_UNSUPPORTED. */
+int pir_model_read(const char *network_filename, int *input_size, int *num_classes, int *batch_size, int *num_layers,
+                   int *spec, int max_layers);
+/* loadModel + the network constructor: batch <= 0 takes the file's batch_size */
+int pir_net_load_model(pir_net **net, const char *network_filename, int batch, int frac_bits, int seed, int mode,
+                       void *stream);
+int pir_net_batch_size(const pir_net *net);
+int pir_net_layer_spec(const pir_net *net, int layer, int *spec); /* spec: 8 ints */
+/* NeuralNetwork::saveSnapshot (NeuralNetwork.cu:85-104, util.cuh:291-303):
+ * path/weight{i} and path/bias{i} (FC) as space-separated reconstructed doubles
+ * in the default stream format; creates the directory */
+int pir_net_save_snapshot(pir_net *net, const char *path);
+/* NeuralNetwork::loadSnapshot (NeuralNetwork.cu:74-83, util.cuh:271-289):
+ * reads those files into public shares; missing values stay zero */
+int pir_net_load_snapshot(pir_net *net, const char *path);
 /* setPublic of weights (and FC biases) from doubles, RSS.inl:122-142 */
 int pir_net_set_weights(pir_net *net, int layer, const double *weights, size_t n_weights, const double *biases,
                         size_t n_biases);

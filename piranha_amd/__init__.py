@@ -84,6 +84,12 @@ _SIGS = {
     "pir_net_create": ([ctypes.POINTER(_vp), _vp, _i, _i, _i, _i, _i, _vp], _i),
     "pir_net_destroy": ([_vp], _i),
     "pir_net_num_layers": ([_vp], _i),
+    "pir_model_read": ([ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp, _i], _i),
+    "pir_net_load_model": ([ctypes.POINTER(_vp), ctypes.c_char_p, _i, _i, _i, _i, _vp], _i),
+    "pir_net_batch_size": ([_vp], _i),
+    "pir_net_layer_spec": ([_vp, _i, _vp], _i),
+    "pir_net_save_snapshot": ([_vp, ctypes.c_char_p], _i),
+    "pir_net_load_snapshot": ([_vp, ctypes.c_char_p], _i),
     "pir_net_set_weights": ([_vp, _i, _vp, _sz, _vp, _sz], _i),
     "pir_net_forward": ([_vp, _vp, _sz], _i),
     "pir_net_forward_shares": ([_vp, _vp], _i),
@@ -212,22 +218,52 @@ def _ptr_array(ptrs):
     return ctypes.cast(arr, _vp), arr
 
 
+def read_model(path):
+    """loadModel's reading of a model file (model.cu:24-115) without building
+    it: (input_size, num_classes, batch_size, [8-int layer spec, ...])."""
+    ins, ncl, bs, nl = _i(), _i(), _i(), _i()
+    check(lib().pir_model_read(os.fsencode(path), ctypes.byref(ins), ctypes.byref(ncl), ctypes.byref(bs),
+                               ctypes.byref(nl), None, 0))
+    spec = np.zeros(8 * max(nl.value, 1), np.int32)
+    check(lib().pir_model_read(os.fsencode(path), None, None, None, ctypes.byref(nl), spec.ctypes.data, nl.value))
+    return ins.value, ncl.value, bs.value, [tuple(int(v) for v in spec[8 * i:8 * i + 8]) for i in range(nl.value)]
+
+
 class Network:
     """Handle on a pir_net (NeuralNetwork of the reference, all three parties)."""
 
-    def __init__(self, spec, batch, frac_bits, seed=0, mode=MODE_FUSED):
-        flat = []
-        for layer in spec:
-            row = list(layer) + [0] * (8 - len(layer))
-            flat.extend(row[:8])
-        self._spec = np.asarray(flat, np.int32)
+    def __init__(self, spec, batch, frac_bits, seed=0, mode=MODE_FUSED, model_file=None):
         h = _vp()
-        check(lib().pir_net_create(ctypes.byref(h), self._spec.ctypes.data, len(spec), batch, frac_bits, seed, mode,
-                                   None))
+        if model_file is not None:  # loadModel (model.cu:24-115); batch <= 0 keeps the file's batch_size
+            check(lib().pir_net_load_model(ctypes.byref(h), os.fsencode(model_file), batch or 0, frac_bits, seed, mode,
+                                           None))
+        else:
+            flat = []
+            for layer in spec:
+                row = list(layer) + [0] * (8 - len(layer))
+                flat.extend(row[:8])
+            self._spec = np.asarray(flat, np.int32)
+            check(lib().pir_net_create(ctypes.byref(h), self._spec.ctypes.data, len(spec), batch, frac_bits, seed,
+                                       mode, None))
         self.h = h.value
-        self.batch = batch
+        self.batch = lib().pir_net_batch_size(self.h)
         self.frac_bits = frac_bits
-        self.spec = [tuple(s) for s in spec]
+        self.spec = [self.layer_spec(i) for i in range(self.num_layers)]
+
+    @classmethod
+    def from_model_file(cls, path, frac_bits, batch=0, seed=0, mode=MODE_FUSED):
+        return cls(None, batch, frac_bits, seed=seed, mode=mode, model_file=path)
+
+    def layer_spec(self, layer):
+        v = np.zeros(8, np.int32)
+        check(lib().pir_net_layer_spec(self.h, layer, v.ctypes.data))
+        return tuple(int(x) for x in v)
+
+    def save_snapshot(self, path):
+        check(lib().pir_net_save_snapshot(self.h, os.fsencode(path)))
+
+    def load_snapshot(self, path):
+        check(lib().pir_net_load_snapshot(self.h, os.fsencode(path)))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -23,6 +23,7 @@ SOURCES = [
     "protocol.cpp",
     "tpc_fpc.cpp",
     "layers.cpp",
+    "model_io.cpp",
     "capi.cpp",
 ]
 

@@ -6,6 +6,7 @@
 #include "elementwise.h"
 #include "fused_rss.h"
 #include "layers.h"
+#include "model_io.h"
 #include "protocol.h"
 #include "ring_gemm.h"
 #include "tpc_fpc.h"
@@ -53,6 +54,7 @@ struct pir_net {
     hipGraph_t tgraph = nullptr;  // captured training step (forward + backward)
     hipGraphExec_t texec = nullptr;
     RSS64 labels;  // label shares of the training-step API
+    std::vector<LayerSpec> spec;
     bool own_stream = false;
 };
 
@@ -661,15 +663,16 @@ int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, 
 
 // ---- network -------------------------------------------------------------------
 
-int pir_net_create(pir_net **out, const int *spec, int num_layers, int batch, int frac_bits, int seed, int mode,
-                   void *stream) {
-    PIR_REQUIRE(out && spec && num_layers > 0 && batch > 0, "pir_net_create: invalid arguments");
+namespace {
+int create_net(pir_net **out, const std::vector<LayerSpec> &sp, int batch, int frac_bits, int seed, int mode,
+               void *stream) {
     PIR_REQUIRE(frac_bits >= 1 && frac_bits <= 30, "pir_net_create: frac_bits must be in [1, 30]");
     PIR_REQUIRE(mode == MODE_FUSED || mode == MODE_PLAIN, "pir_net_create: unknown mode");
     pir_net *n = new pir_net();
     n->ctx.frac_bits = frac_bits;
     n->ctx.mode = mode;
     n->ctx.algo = mode == MODE_FUSED ? ALGO_LIMB : ALGO_PLAIN;
+    n->spec = sp;
     if (stream) {
         n->ctx.stream = (hipStream_t)stream;
     } else {
@@ -682,9 +685,6 @@ int pir_net_create(pir_net **out, const int *spec, int num_layers, int batch, in
     }
     int rc = guarded([&]() -> int {
         NetScope ns(n);
-        std::vector<LayerSpec> sp(num_layers);
-        for (int i = 0; i < num_layers; i++)
-            for (int j = 0; j < 8; j++) sp[i].v[j] = spec[i * 8 + j];
         n->net.reset(new NeuralNetwork(sp, batch, seed));
         return (int)OK;
     });
@@ -694,6 +694,74 @@ int pir_net_create(pir_net **out, const int *spec, int num_layers, int batch, in
     }
     *out = n;
     return OK;
+}
+}  // namespace
+
+int pir_net_create(pir_net **out, const int *spec, int num_layers, int batch, int frac_bits, int seed, int mode,
+                   void *stream) {
+    PIR_REQUIRE(out && spec && num_layers > 0 && batch > 0, "pir_net_create: invalid arguments");
+    std::vector<LayerSpec> sp(num_layers);
+    for (int i = 0; i < num_layers; i++)
+        for (int j = 0; j < 8; j++) sp[i].v[j] = spec[i * 8 + j];
+    return create_net(out, sp, batch, frac_bits, seed, mode, stream);
+}
+
+int pir_model_read(const char *network_filename, int *input_size, int *num_classes, int *batch_size, int *num_layers,
+                   int *spec, int max_layers) {
+    PIR_REQUIRE(network_filename && num_layers && max_layers >= 0 && (spec || max_layers == 0),
+                "pir_model_read: invalid arguments");
+    return guarded([&]() -> int {
+        ModelFile m = load_model_file(network_filename);
+        if (input_size) *input_size = m.input_size;
+        if (num_classes) *num_classes = m.num_classes;
+        if (batch_size) *batch_size = m.batch_size;
+        *num_layers = (int)m.layers.size();
+        for (int i = 0; i < (int)m.layers.size() && i < max_layers; i++)
+            for (int j = 0; j < 8; j++) spec[i * 8 + j] = m.layers[i].v[j];
+        return (int)OK;
+    });
+}
+
+int pir_net_load_model(pir_net **out, const char *network_filename, int batch, int frac_bits, int seed, int mode,
+                       void *stream) {
+    PIR_REQUIRE(out && network_filename, "pir_net_load_model: invalid arguments");
+    ModelFile m;
+    int rc = guarded([&]() -> int {
+        m = load_model_file(network_filename);
+        return (int)OK;
+    });
+    if (rc) return rc;
+    PIR_REQUIRE(!m.layers.empty(), "pir_net_load_model: the model has no layers");
+    if (batch <= 0) batch = m.batch_size;
+    PIR_REQUIRE(batch > 0, "pir_net_load_model: the model's batch_size must be positive");
+    return create_net(out, m.layers, batch, frac_bits, seed, mode, stream);
+}
+
+int pir_net_batch_size(const pir_net *n) { return n && n->net ? n->net->batch : 0; }
+
+int pir_net_layer_spec(const pir_net *n, int layer, int *spec) {
+    PIR_REQUIRE(n && spec && layer >= 0 && layer < (int)n->spec.size(), "pir_net_layer_spec: invalid arguments");
+    for (int j = 0; j < 8; j++) spec[j] = n->spec[layer].v[j];
+    return OK;
+}
+
+int pir_net_save_snapshot(pir_net *n, const char *path) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && path, "pir_net_save_snapshot: invalid arguments");
+        NetScope ns(n);
+        save_snapshot(*n->net, path);
+        return (int)OK;
+    });
+}
+
+int pir_net_load_snapshot(pir_net *n, const char *path) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n && path, "pir_net_load_snapshot: invalid arguments");
+        NetScope ns(n);
+        load_snapshot(*n->net, path);
+        drop_graph(n);
+        return (int)OK;
+    });
 }
 
 int pir_net_destroy(pir_net *n) {

@@ -4,8 +4,10 @@ doubles the reference's loadShareFromFile parses, util.cuh:271-289).
 
 Run in a container that has the reference checkout:
     python tests/golden/make_fixtures.py /root/reference
-Only data files and the numeric brace lists of two GPU.cu known-answer tests
-are read; no reference code is imported, compiled or run.
+Only data files (the fixtures and the model descriptions of files/models/,
+copied as they are to tests/golden/models/) and the numeric brace lists of two
+GPU.cu known-answer tests are read; no reference code is imported, compiled or
+run.
 """
 import os
 import sys
@@ -92,9 +94,19 @@ def main(ref):
         with open(os.path.join(base, rel)) as f:
             arrays[key] = np.array([float(t) for t in f.read().split()], np.float64)
     arrays.update(kat_arrays(ref))
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures.npz")
+    here = os.path.dirname(os.path.abspath(__file__))
+    out = os.path.join(here, "fixtures.npz")
     np.savez_compressed(out, **arrays)
     print(out, os.path.getsize(out), "bytes")
+    # the model description files (data, files/models/*.json) as they are
+    models = os.path.join(here, "models")
+    os.makedirs(models, exist_ok=True)
+    src = os.path.join(ref, "files", "models")
+    for name in sorted(os.listdir(src)):
+        if name.endswith(".json"):
+            with open(os.path.join(src, name), "rb") as f, open(os.path.join(models, name), "wb") as g:
+                g.write(f.read())
+    print(models, len(os.listdir(models)), "model files")
 
 
 if __name__ == "__main__":

@@ -191,7 +191,8 @@ int pir_fpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
 
 /* ---- layers and network (src/nn/FCLayer.cu etc., NeuralNetwork.cu) -------------- */
 typedef struct pir_net pir_net;
-enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3, PIR_LAYER_AVGPOOL = 4 };
+enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3, PIR_LAYER_AVGPOOL = 4,
+       PIR_LAYER_LN = 5 };
 enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
 /* spec: per layer 8 ints {type, a0..a6}:
  *   FC      {0, inputDim, outputDim}                                (FCConfig.h:16)
@@ -199,6 +200,7 @@ enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
  *   ReLU    {2, inputDim}                                           (ReLUConfig.h:12)
  *   Maxpool {3, H, W, C, pool, stride}                              (MaxpoolConfig.h:16)
  *   Avgpool {4, H, W, C, pool, stride}                              (AveragepoolConfig.h:15)
+ *   LN      {5, inputDim}                                           (LNConfig.h:13)
  * Each layer's input size must equal the previous layer's output size.
  * Weights are initialised as in FCLayer.cu:31-48 / CNNLayer.cu:36-50 with
  * std::default_random_engine(seed + layer) unless set with pir_net_set_weights. */
@@ -210,7 +212,7 @@ int pir_net_num_layers(const pir_net *net);
 /* loadModel (model.cu:24-115) without building the network: the file's
  * input_size, num_classes, batch_size, the layer count (*num_layers) and up to
  * max_layers layer specs (8 ints each, the pir_net_create form).  Layer types
- * fc, relu, cnn, maxpool, averagepool; ln and res give PIR_ERR_UNSUPPORTED. */
+ * fc, relu, cnn, maxpool, averagepool, ln; res gives PIR_ERR_UNSUPPORTED. */
 int pir_model_read(const char *network_filename, int *input_size, int *num_classes, int *batch_size, int *num_layers,
                    int *spec, int max_layers);
 /* loadModel + the network constructor: batch <= 0 takes the file's batch_size */
@@ -219,7 +221,8 @@ int pir_net_load_model(pir_net **net, const char *network_filename, int batch, i
 int pir_net_batch_size(const pir_net *net);
 int pir_net_layer_spec(const pir_net *net, int layer, int *spec); /* spec: 8 ints */
 /* NeuralNetwork::saveSnapshot (NeuralNetwork.cu:85-104, util.cuh:291-303):
- * path/weight{i} and path/bias{i} (FC) as space-separated reconstructed doubles
+ * path/weight{i} and path/bias{i} (FC), path/gamma{i} and path/beta{i} (LN) as
+ * space-separated reconstructed doubles
  * in the default stream format; creates the directory */
 int pir_net_save_snapshot(pir_net *net, const char *path);
 /* NeuralNetwork::loadSnapshot (NeuralNetwork.cu:74-83, util.cuh:271-289):

@@ -100,8 +100,12 @@ _SIGS = {
     "ref_averagepool_expand_delta": [_u64p, _sz, _u64p, _sz, _i, _i],
     "ref_avgpool_forward": [_u64p, _u64p] + [_i] * 6,
     "ref_avgpool_backward": [_u64p, _sz, _u64p, _sz, _i, _i],
+    "ref_ln_forward": [_u64p] * 6 + [_sz, _i, _i],
+    "ref_ln_backward": [_u64p] * 6 + [_sz, _i, _i, _i],
+    "ref_inv_sqrt_fixed": [_u64, _i],
 }
 _RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_averagepool_im2row": _sz,
+        "ref_inv_sqrt_fixed": _u64,
         "ref_scalar_divide": _u64, "ref_version": _i}
 
 _lib = None
@@ -465,6 +469,33 @@ def avgpool_backward(delta, batch, H, W, C, pool):
     return deltas
 
 
+def ln_forward(x, gamma, beta, num_batches, fp):
+    """LNLayer::forward (LNLayer.cu:66-162).  Returns (activations, xhat, invSigma)."""
+    x, gamma, beta = _u(x), _u(gamma), _u(beta)
+    n = x.shape[-1]
+    act = np.zeros((3, 2, n), np.uint64)
+    xhat = np.zeros((3, 2, n), np.uint64)
+    inv_sigma = np.zeros((3, 2, num_batches), np.uint64)
+    lib().ref_ln_forward(_p(x), _p(gamma), _p(beta), _p(xhat), _p(inv_sigma), _p(act), n, num_batches, fp)
+    return act, xhat, inv_sigma
+
+
+def ln_backward(delta, xhat, inv_sigma, gamma, beta, num_batches, fp, lr):
+    """LNLayer::backward (LNLayer.cu:166-284).  Returns (deltas, xhat after the
+    pass, updated gamma, updated beta)."""
+    delta, inv_sigma = _u(delta), _u(inv_sigma)
+    xhat, gamma, beta = _u(xhat).copy(), _u(gamma).copy(), _u(beta).copy()
+    n = delta.shape[-1]
+    deltas = np.zeros((3, 2, n), np.uint64)
+    lib().ref_ln_backward(_p(delta), _p(xhat), _p(inv_sigma), _p(gamma), _p(beta), _p(deltas), n, num_batches, fp, lr)
+    return deltas, xhat, gamma, beta
+
+
+def inv_sqrt_fixed(x, fp):
+    """inv_sqrt_fixed_point_functor (functors.h:92-103) of one ring element."""
+    return lib().ref_inv_sqrt_fixed(int(x) & 0xFFFFFFFFFFFFFFFF, fp)
+
+
 def net_forward(layers, batch, fp, params, x_shares):
     """NeuralNetwork::forward (NeuralNetwork.cu:108-133) over the oracle layers.
 
@@ -495,6 +526,10 @@ def net_forward(layers, batch, fp, params, x_shares):
         elif kind == "avgpool":
             H, Wd, C, pool, s = l[1:]
             h = avgpool_forward(h, batch, H, Wd, C, pool, s)
+            out.append((h, None))
+        elif kind == "ln":
+            gamma, beta = params[i]
+            h, _, _ = ln_forward(h, gamma, beta, batch, fp)
             out.append((h, None))
         else:
             raise ValueError(kind)
@@ -644,6 +679,11 @@ def net_backward(layers, batch, fp, params, x_shares, fwd, labels, lr, out_delta
         elif kind == "avgpool":
             H, Wd, C, pool, s = layers[i][1:]
             d = avgpool_backward(delta, batch, H, Wd, C, pool)
+        elif kind == "ln":
+            gamma, beta = params[i]
+            _, xhat, inv_sigma = ln_forward(inp, gamma, beta, batch, fp)  # the forward's saved state
+            d, _, g2, b2 = ln_backward(delta, xhat, inv_sigma, gamma, beta, batch, fp, lr)
+            new_params[i] = (g2, b2)
         else:
             raise ValueError(kind)
         deltas[i] = d

@@ -1282,6 +1282,139 @@ void avgpool_backward(Rs<u64> delta, Rs<u64> deltas, int C, int pool) {
     divide_public(deltas, (u64)pool * pool);
 }
 
+// gpu::elementVectorSubtract, matrix.cuh:97-107: a[row*cols + col] -= b[rowwise ? col : row].
+template <typename T>
+void element_vector_subtract(T *a, const T *b, bool rowwise, int rows, int cols) {
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) a[(size_t)r * cols + c] -= b[rowwise ? c : r];
+}
+
+// inv_sqrt_fixed_point_functor, functors.h:92-103: v = (double)(int64)x / 2^fp,
+// v = 1 / sqrt(v), back with (uint64)(v * 2^fp).  sqrt and the division are
+// correctly rounded IEEE operations on any conforming unit; the reference
+// converts on its GPU, which clamps to the uint64 range (negative -> 0,
+// +inf -> 2^64 - 1) and maps NaN to 0.
+inline u64 inv_sqrt_fixed(u64 x, int fp) {
+    double v = (double)(s64)x / (double)(1 << fp);
+    v = 1 / std::sqrt(v);
+    double f = v * (double)(1 << fp);
+    if (!(f > 0.0)) return 0;
+    if (f >= 18446744073709551616.0) return ~(u64)0;
+    return (u64)f;
+}
+
+template <typename T>
+T *ptr(View<T> v) {
+    return &v[0];
+}
+
+// LNLayer::forward, LNLayer.cu:66-162 (the revealed-variance branch, :124-132).
+// Rows are the numBatches images, each of batchSize = input.size() / numBatches
+// features; gamma and beta have batchSize entries and are expanded with
+// vectorExpand by a factor of numBatches, as the reference does.
+void ln_forward(Rs<u64> input, Rs<u64> gamma, Rs<u64> beta, Rs<u64> xhat, Rs<u64> invSigma, Rs<u64> act,
+                int numBatches, int fp) {
+    r_zero(act);
+    r_zero(xhat);
+    r_zero(invSigma);
+    const size_t n = input.size();
+    const int batchSize = (int)(n / numBatches);
+    RsBuf<u64> mu(numBatches);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) reduce_sum(ptr(input.s[q][i]), ptr(mu.v().s[q][i]), false, numBatches, batchSize);
+    divide_public(mu.v(), (u64)batchSize);
+    r_op_r(xhat, input, ADD);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            element_vector_subtract(ptr(xhat.s[q][i]), ptr(mu.v().s[q][i]), false, numBatches, batchSize);
+    RsBuf<u64> std2(n);
+    r_op_r(std2.v(), xhat, ADD);
+    r_mult(std2.v(), xhat);
+    divide_public(std2.v(), (u64)1 << fp);
+    RsBuf<u64> variance(numBatches);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            reduce_sum(ptr(std2.v().s[q][i]), ptr(variance.v().s[q][i]), false, numBatches, batchSize);
+    divide_public(variance.v(), (u64)batchSize);
+    r_op_s(variance.v(), (u64)(1 << (fp - 8)), ADD);  // epsilon
+    LocBuf<u64> revealed(numBatches);
+    reconstruct(variance.v(), revealed.v());
+    for (int q = 0; q < 3; q++)
+        for (auto &v : revealed.d[q]) v = inv_sqrt_fixed(v, fp);
+    r_op_l(invSigma, revealed.v(), ADD);
+    RsBuf<u64> expInv(n);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) vector_expand(invSigma.s[q][i], expInv.v().s[q][i], n / numBatches);
+    r_mult(xhat, expInv.v());
+    divide_public(xhat, (u64)1 << fp);
+    RsBuf<u64> expGamma(n), expBeta(n);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) {
+            vector_expand(gamma.s[q][i], expGamma.v().s[q][i], n / gamma.size());
+            vector_expand(beta.s[q][i], expBeta.v().s[q][i], n / gamma.size());
+        }
+    r_op_r(act, xhat, ADD);
+    r_mult(act, expGamma.v());
+    divide_public(act, (u64)1 << fp);
+    r_op_r(act, expBeta.v(), ADD);
+}
+
+// LNLayer::backward, LNLayer.cu:166-284: dBeta, dGamma, the delta, then
+// gamma -= dGamma and beta -= dBeta.  xhat is overwritten (xhat *= green), as
+// in the reference.
+void ln_backward(Rs<u64> delta, Rs<u64> xhat, Rs<u64> invSigma, Rs<u64> gamma, Rs<u64> beta, Rs<u64> deltas,
+                 int numBatches, int fp, int lr) {
+    r_zero(deltas);
+    const size_t n = delta.size();
+    const int batchSize = (int)(n / numBatches);
+    auto per_share = [](auto f) {
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 2; i++) f(q, i);
+    };
+    RsBuf<u64> dBeta(beta.size());
+    per_share([&](int q, int i) { reduce_sum(ptr(delta.s[q][i]), ptr(dBeta.v().s[q][i]), true, numBatches, batchSize); });
+    divide_public(dBeta.v(), (u64)1 << lr);
+
+    RsBuf<u64> dGamma(gamma.size()), temp(n);
+    r_op_r(temp.v(), delta, ADD);
+    r_mult(temp.v(), xhat);
+    per_share([&](int q, int i) { reduce_sum(ptr(temp.v().s[q][i]), ptr(dGamma.v().s[q][i]), true, numBatches, batchSize); });
+    divide_public(dGamma.v(), (u64)1 << (lr + fp));
+
+    RsBuf<u64> dxhat(n), expGamma(n);
+    r_op_r(dxhat.v(), delta, ADD);
+    per_share([&](int q, int i) { vector_expand(gamma.s[q][i], expGamma.v().s[q][i], n / gamma.size()); });
+    r_mult(dxhat.v(), expGamma.v());
+    divide_public(dxhat.v(), (u64)1 << fp);
+
+    RsBuf<u64> green(n), greenTemp(numBatches);
+    r_op_r(green.v(), dxhat.v(), ADD);
+    r_mult(green.v(), xhat);
+    per_share([&](int q, int i) {
+        reduce_sum(ptr(green.v().s[q][i]), ptr(greenTemp.v().s[q][i]), false, numBatches, batchSize);
+    });
+    divide_public(greenTemp.v(), (u64)1 << fp);
+    per_share([&](int q, int i) { vector_expand(greenTemp.v().s[q][i], green.v().s[q][i], n / numBatches); });
+    r_mult(xhat, green.v());
+    divide_public(xhat, (u64)1 << fp);
+
+    RsBuf<u64> blue(numBatches);
+    per_share([&](int q, int i) { reduce_sum(ptr(dxhat.v().s[q][i]), ptr(blue.v().s[q][i]), false, numBatches, batchSize); });
+    r_op_r(deltas, dxhat.v(), ADD);
+    r_op_s(deltas, (u64)batchSize, MUL);
+    RsBuf<u64> expBlue(n), expInv(n);
+    per_share([&](int q, int i) {
+        vector_expand(blue.v().s[q][i], expBlue.v().s[q][i], n / numBatches);
+        vector_expand(invSigma.s[q][i], expInv.v().s[q][i], n / numBatches);
+    });
+    r_op_r(deltas, expBlue.v(), SUB);
+    r_op_r(deltas, xhat, SUB);
+    r_mult(deltas, expInv.v());
+    divide_public(deltas, ((u64)1 << fp) * (u64)batchSize);
+    r_op_r(gamma, dGamma.v(), SUB);
+    r_op_r(beta, dBeta.v(), SUB);
+}
+
 // NeuralNetwork::_reveal_softmax_grad, NeuralNetwork.cu:207-267.  Every party
 // opens the logits, computes softmax in FP64 on the host (std::exp, summed in
 // element order per row), converts back with to_fixed_functor (functors.h:116-123)
@@ -1617,6 +1750,19 @@ void ref_avgpool_forward(const u64 *input, u64 *act, int batch, int H, int W, in
 void ref_avgpool_backward(const u64 *delta, size_t n_in, u64 *deltas, size_t n_out, int C, int pool) {
     avgpool_backward(wrapRs((u64 *)delta, n_in), wrapRs(deltas, n_out), C, pool);
 }
+void ref_ln_forward(const u64 *input, const u64 *gamma, const u64 *beta, u64 *xhat, u64 *inv_sigma, u64 *act,
+                    size_t n, int num_batches, int fp) {
+    size_t g = n / num_batches;
+    ln_forward(wrapRs((u64 *)input, n), wrapRs((u64 *)gamma, g), wrapRs((u64 *)beta, g), wrapRs(xhat, n),
+               wrapRs(inv_sigma, (size_t)num_batches), wrapRs(act, n), num_batches, fp);
+}
+void ref_ln_backward(const u64 *delta, u64 *xhat, const u64 *inv_sigma, u64 *gamma, u64 *beta, u64 *deltas, size_t n,
+                     int num_batches, int fp, int lr) {
+    size_t g = n / num_batches;
+    ln_backward(wrapRs((u64 *)delta, n), wrapRs(xhat, n), wrapRs((u64 *)inv_sigma, (size_t)num_batches),
+                wrapRs(gamma, g), wrapRs(beta, g), wrapRs(deltas, n), num_batches, fp, lr);
+}
+u64 ref_inv_sqrt_fixed(u64 x, int fp) { return inv_sqrt_fixed(x, fp); }
 void ref_reveal_softmax_grad(const u64 *act, const double *labels, size_t n, u64 *deltas, int nClasses, int fp,
                              int logMiniBatch) {
     reveal_softmax_grad(wrapRs((u64 *)act, n), labels, wrapRs(deltas, n), nClasses, fp, logMiniBatch);

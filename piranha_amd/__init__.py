@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libpir_hip.so")
 
 ALGO_PLAIN, ALGO_LIMB = 0, 1
 MODE_FUSED, MODE_PLAIN = 0, 1
-LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL = 0, 1, 2, 3, 4
+LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL, LAYER_LN = 0, 1, 2, 3, 4, 5
 CONV_FPROP, CONV_DGRAD, CONV_WGRAD = 0, 1, 2
 LIMB_WIDE, LIMB_VARIANTS = 8, 16  # kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
 OP_ADD, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV = range(8)

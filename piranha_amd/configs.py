@@ -4,13 +4,14 @@ format (src/util/model.cu:24-115, files/models/*.json).
 
 Layer specs: ("fc", in, out) | ("cnn", H, W, Cin, Cout, f, stride, pad) |
 ("relu", n_per_image) | ("maxpool", H, W, C, pool, stride) |
-("avgpool", H, W, C, pool, stride).
+("avgpool", H, W, C, pool, stride) | ("ln", n_per_image).
 """
 import json
 
-from . import LAYER_AVGPOOL, LAYER_CNN, LAYER_FC, LAYER_MAXPOOL, LAYER_RELU
+from . import LAYER_AVGPOOL, LAYER_CNN, LAYER_FC, LAYER_LN, LAYER_MAXPOOL, LAYER_RELU
 
-_TYPES = {"fc": LAYER_FC, "cnn": LAYER_CNN, "relu": LAYER_RELU, "maxpool": LAYER_MAXPOOL, "avgpool": LAYER_AVGPOOL}
+_TYPES = {"fc": LAYER_FC, "cnn": LAYER_CNN, "relu": LAYER_RELU, "maxpool": LAYER_MAXPOOL, "avgpool": LAYER_AVGPOOL,
+          "ln": LAYER_LN}
 
 
 def to_spec(layers):
@@ -21,7 +22,7 @@ def to_spec(layers):
 def input_size(layers):
     """Per-image input element count of the first layer."""
     l = layers[0]
-    if l[0] in ("fc", "relu"):
+    if l[0] in ("fc", "relu", "ln"):
         return l[1]
     return l[1] * l[2] * l[3]
 
@@ -31,7 +32,7 @@ def output_size(layers):
     l = layers[-1]
     if l[0] == "fc":
         return l[2]
-    if l[0] == "relu":
+    if l[0] in ("relu", "ln"):
         return l[1]
     if l[0] == "cnn":
         H, W, _, Cout, f, s, p = l[1:]
@@ -54,6 +55,8 @@ def from_model_json(doc, avgpool_as_maxpool=False):
             out.append(("fc", l["input_dim"], l["output_dim"]))
         elif t == "relu":
             out.append(("relu", l["input_dim"]))
+        elif t == "ln":
+            out.append(("ln", l["input_dim"]))
         elif t == "cnn":
             out.append(("cnn", l["input_hw"][0], l["input_hw"][1], l["in_channels"], l["out_channels"],
                         l["filter_hw"][0], l["stride"], l["padding"]))

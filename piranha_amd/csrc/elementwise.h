@@ -30,6 +30,8 @@ int bitexpand(const u64 *a, size_t n, void *bits, int elem_bytes, hipStream_t s)
 int set_carry_out_msb(void *rbits, void *abits, void *msb, size_t n, int bw, int xr, int eb, hipStream_t s);
 int expand_compare(const void *b, const void *inv, void *out, size_t n, size_t out_n, int eb, hipStream_t s);
 int element_vector_add(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s);
+// gpu::elementVectorSubtract (matrix.cuh:97-107): a[row*cols + col] -= b[rowwise ? col : row]
+int element_vector_subtract(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s);
 int truncate_cols(const void *a, size_t a_n, void *b, int sc, int ec, int eb, hipStream_t s);
 int stride_pad(const u64 *in, u64 *out, size_t n_in, int stride, int pad, u64 pv, hipStream_t s);
 size_t maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);

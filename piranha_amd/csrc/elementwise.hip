@@ -186,6 +186,23 @@ int element_vector_add(u64 *a, const u64 *b, int rowwise, int rows, int cols, hi
     return OK;
 }
 
+// matrix.cuh:97-107 (elementVectorSubtract): a[row*cols + col] -= b[rowwise ? col : row]
+__global__ __launch_bounds__(BLK) void evsub_kernel(u64 *a, const u64 *b, int rowwise, int rows, int cols) {
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)rows * cols) return;
+    size_t row = t / (size_t)cols, col = t % (size_t)cols;
+    a[t] -= b[rowwise ? col : row];
+}
+
+int element_vector_subtract(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s) {
+    PIR_REQUIRE(rows >= 0 && cols >= 0, "elementVectorSubtract: negative dimension");
+    size_t n = (size_t)rows * cols;
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(evsub_kernel, dim3(g1(n)), dim3(BLK), 0, s, a, b, rowwise, rows, cols);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 // matrix.cuh:142-155
 template <typename T>
 __global__ __launch_bounds__(BLK) void truncate_cols_kernel(const T *a, T *b, size_t rows, int sc, int ec) {

@@ -1,6 +1,7 @@
 // layers.cpp -- FC / CNN / ReLU / Maxpool layers (forward and backward) and the network.
 #include "layers.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -10,6 +11,15 @@
 #include "fused_rss.h"
 
 namespace pir {
+
+// (uint64)f as the reference's GPU converts: clamped to the uint64 range,
+// NaN to 0 (to_fixed_functor, inv_sqrt_fixed_point_functor).
+static u64 to_u64_clamped(double f) {
+    if (!(f > 0.0)) return 0;  // NaN, zero or negative
+    if (f >= 18446744073709551616.0) return ~0ull;
+    return (u64)f;
+}
+
 
 static bool fused_limb() { return ctx().mode == MODE_FUSED && ctx().algo == ALGO_LIMB; }
 
@@ -448,6 +458,164 @@ void AveragepoolLayer::backward(const RSS64 &delta, const RSS64 &) {
 }
 
 // ---------------------------------------------------------------------------
+// LNLayer.cu
+
+LNLayer::LNLayer(const LNConfig &c, int n, int)
+    : Layer(n), conf(c), gamma(c.inputSize), beta(c.inputSize), xhat(c.numBatches * c.inputSize),
+      invSigma(c.numBatches), activations(c.inputSize * c.numBatches), revealed_(c.numBatches),
+      invVals_(c.numBatches) {
+    // LNLayer.cu:28-35
+    gamma.setPublic(std::vector<double>(c.inputSize, 1.0), ctx().frac_bits);
+    beta.setPublic(std::vector<double>(c.inputSize, 0.0), ctx().frac_bits);
+    invSqrt_.prepare(3 * c.numBatches);
+}
+
+void LNLayer::printLayer() {
+    printf("----------------------------------------------\n(%d) LN Layer\t\t  %zu x %zu\n", layerNum + 1,
+           conf.inputSize, conf.numBatches);
+}
+
+// v = (double)(int64)x / 2^fp; v = 1 / sqrt(v); (uint64)(v * 2^fp) with the
+// clamping conversion of the reference's GPU.  sqrt and the division are
+// correctly rounded IEEE-754 operations on the host as on any conforming unit.
+void LNLayer::InvSqrtStage::compute(const u64 *in, u64 *out, size_t n) const {
+    const double scale = (double)(1 << fp);
+    for (size_t i = 0; i < n; i++) {
+        double v = (double)(s64)in[i] / scale;
+        v = 1 / std::sqrt(v);
+        out[i] = to_u64_clamped(v * scale);
+    }
+}
+
+// LNLayer.cu:66-162
+void LNLayer::forward(const RSS64 &input) {
+    const int fp = ctx().frac_bits;
+    const int nb = (int)conf.numBatches;
+    const size_t n = input.size();
+    const int bs = (int)(n / (size_t)nb);
+    hipStream_t s = ctx().stream;
+    activations.zero();
+    xhat.zero();
+    invSigma.zero();
+
+    RSS64 mu(nb);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(reduce_sum(input.getShare(q, i)->data(), mu.getShare(q, i)->data(), 0, nb, bs, s));
+    dividePublic(mu, (u64)bs);
+    xhat += input;
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(element_vector_subtract(xhat.getShare(q, i)->data(), mu.getShare(q, i)->data(), 0, nb, bs, s));
+
+    RSS64 std2(n);
+    std2 += xhat;
+    std2 *= xhat;
+    dividePublic(std2, (u64)1 << fp);
+    RSS64 variance(nb);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(reduce_sum(std2.getShare(q, i)->data(), variance.getShare(q, i)->data(), 0, nb, bs, s));
+    dividePublic(variance, (u64)bs);
+    variance += (u64)(1 << (fp - 8));  // epsilon
+
+    reconstruct(variance, revealed_);
+    invSqrt_.fp = fp;
+    invSqrt_.enqueue(revealed_.base(), invVals_.base(), s);
+    invSigma += invVals_;
+
+    RSS64 expInv(n);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            check(vector_expand(invSigma.getShare(q, i)->data(), invSigma.size(), expInv.getShare(q, i)->data(),
+                                (int)(n / invSigma.size()), s));
+    xhat *= expInv;
+    dividePublic(xhat, (u64)1 << fp);
+
+    RSS64 expGamma(n), expBeta(n);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) {
+            check(vector_expand(gamma.getShare(q, i)->data(), gamma.size(), expGamma.getShare(q, i)->data(),
+                                (int)(n / gamma.size()), s));
+            check(vector_expand(beta.getShare(q, i)->data(), beta.size(), expBeta.getShare(q, i)->data(),
+                                (int)(n / gamma.size()), s));
+        }
+    activations += xhat;
+    activations *= expGamma;
+    dividePublic(activations, (u64)1 << fp);
+    activations += expBeta;
+}
+
+// LNLayer.cu:166-284 (xhat is overwritten, as in the reference)
+void LNLayer::backward(const RSS64 &delta, const RSS64 &) {
+    ensureDeltas();
+    const int fp = ctx().frac_bits, lr = ctx().log_lr;
+    const int nb = (int)conf.numBatches;
+    const size_t n = delta.size();
+    const int bs = (int)(n / (size_t)nb);
+    hipStream_t s = ctx().stream;
+    auto per_share = [](auto f) {
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 2; i++) f(q, i);
+    };
+    auto expand = [&](const RSS64 &a, RSS64 &b, int q, int i) {
+        check(vector_expand(a.getShare(q, i)->data(), a.size(), b.getShare(q, i)->data(), (int)(b.size() / a.size()),
+                            s));
+    };
+    deltas.zero();
+
+    RSS64 dBeta(beta.size());
+    per_share([&](int q, int i) {
+        check(reduce_sum(delta.getShare(q, i)->data(), dBeta.getShare(q, i)->data(), 1, nb, bs, s));
+    });
+    dividePublic(dBeta, (u64)1 << lr);
+
+    RSS64 dGamma(gamma.size()), temp(n);
+    temp += delta;
+    temp *= xhat;
+    per_share([&](int q, int i) {
+        check(reduce_sum(temp.getShare(q, i)->data(), dGamma.getShare(q, i)->data(), 1, nb, bs, s));
+    });
+    dividePublic(dGamma, (u64)1 << (lr + fp));
+
+    RSS64 dxhat(n), expGamma(n);
+    dxhat += delta;
+    per_share([&](int q, int i) { expand(gamma, expGamma, q, i); });
+    dxhat *= expGamma;
+    dividePublic(dxhat, (u64)1 << fp);
+
+    RSS64 green(n), greenTemp(nb);
+    green += dxhat;
+    green *= xhat;
+    per_share([&](int q, int i) {
+        check(reduce_sum(green.getShare(q, i)->data(), greenTemp.getShare(q, i)->data(), 0, nb, bs, s));
+    });
+    dividePublic(greenTemp, (u64)1 << fp);
+    per_share([&](int q, int i) { expand(greenTemp, green, q, i); });
+    xhat *= green;
+    dividePublic(xhat, (u64)1 << fp);
+
+    RSS64 blue(nb);
+    per_share([&](int q, int i) {
+        check(reduce_sum(dxhat.getShare(q, i)->data(), blue.getShare(q, i)->data(), 0, nb, bs, s));
+    });
+    deltas += dxhat;
+    deltas *= (u64)bs;
+    RSS64 expBlue(n), expInv(n);
+    per_share([&](int q, int i) {
+        expand(blue, expBlue, q, i);
+        expand(invSigma, expInv, q, i);
+    });
+    deltas -= expBlue;
+    deltas -= xhat;
+    deltas *= expInv;
+    dividePublic(deltas, ((u64)1 << fp) * (u64)bs);
+
+    gamma -= dGamma;
+    beta -= dBeta;
+}
+
+// ---------------------------------------------------------------------------
 // Network (NeuralNetwork.cu:32-133)
 
 // Geometry of one layer spec row: every size positive, windows inside the
@@ -459,6 +627,7 @@ static bool spec_valid(const int *v) {
             return v[1] > 0 && v[2] > 0 && v[3] > 0 && v[4] > 0 && v[5] > 0 && v[6] > 0 && v[7] >= 0 &&
                    v[1] + 2 * v[7] >= v[5] && v[2] + 2 * v[7] >= v[5];
         case 2: return v[1] > 0;
+        case 5: return v[1] > 0;
         case 3:
         case 4: return v[1] > 0 && v[2] > 0 && v[3] > 0 && v[4] > 0 && v[5] > 0 && v[1] >= v[4] && v[2] >= v[4];
         default: return true;  // rejected below
@@ -481,8 +650,9 @@ NeuralNetwork::NeuralNetwork(const std::vector<LayerSpec> &spec, int b, int seed
             case 4:
                 layers.emplace_back(new AveragepoolLayer(AveragepoolConfig(v[1], v[2], v[3], v[4], v[5], b), (int)i, s));
                 break;
+            case 5: layers.emplace_back(new LNLayer(LNConfig(v[1], b), (int)i, s)); break;
             default:
-                set_error("network: only FC, CNN, ReLU, Maxpool and Averagepool layers are supported");
+                set_error("network: only FC, CNN, ReLU, Maxpool, Averagepool and LN layers are supported");
                 throw PirError(ERR_UNSUPPORTED);
         }
     }
@@ -547,59 +717,58 @@ void NeuralNetwork::prepareBackward() {
     _prepare_softmax();
 }
 
-NeuralNetwork::HostSoftmax::~HostSoftmax() {
+HostStage::~HostStage() {
     if (in_) (void)hipHostFree(in_);
     if (out_) (void)hipHostFree(out_);
 }
 
-void NeuralNetwork::HostSoftmax::prepare(size_t n, int views, int rows, int nc, int fp) {
-    if (n != n_ || views != views_) {
-        if (in_) hip_check(hipHostFree(in_));
-        if (out_) hip_check(hipHostFree(out_));
-        in_ = out_ = nullptr;
-        n_ = 0;
-        hip_check(hipHostMalloc((void **)&in_, views * n * sizeof(u64), hipHostMallocDefault));
-        hip_check(hipHostMalloc((void **)&out_, views * n * sizeof(u64), hipHostMallocDefault));
-        n_ = n;
-        views_ = views;
-    }
-    rows_ = rows;
-    nc_ = nc;
-    fp_ = fp;
+void HostStage::prepare(size_t n) {
+    if (n == n_ && in_) return;
+    if (in_) hip_check(hipHostFree(in_));
+    if (out_) hip_check(hipHostFree(out_));
+    in_ = out_ = nullptr;
+    n_ = 0;
+    hip_check(hipHostMalloc((void **)&in_, std::max<size_t>(n, 1) * sizeof(u64), hipHostMallocDefault));
+    hip_check(hipHostMalloc((void **)&out_, std::max<size_t>(n, 1) * sizeof(u64), hipHostMallocDefault));
+    n_ = n;
 }
 
-void NeuralNetwork::HostSoftmax::enqueue(const u64 *dev_in, u64 *dev_out, hipStream_t s) {
-    const size_t bytes = (size_t)views_ * n_ * sizeof(u64);
+void HostStage::enqueue(const u64 *dev_in, u64 *dev_out, hipStream_t s) {
+    if (!in_) {
+        set_error("host stage used before prepare()");
+        throw PirError(ERR_STATE);
+    }
+    if (n_ == 0) return;
+    const size_t bytes = n_ * sizeof(u64);
     hip_check(hipMemcpyAsync(in_, dev_in, bytes, hipMemcpyDeviceToHost, s));
-    hip_check(hipLaunchHostFunc(s, &HostSoftmax::run, this));
+    hip_check(hipLaunchHostFunc(s, &HostStage::run, this));
     hip_check(hipMemcpyAsync(dev_out, out_, bytes, hipMemcpyHostToDevice, s));
+}
+
+void HostStage::run(void *self) {
+    const HostStage &h = *(const HostStage *)self;
+    h.compute(h.in_, h.out_, h.n_);
 }
 
 // NeuralNetwork.cu:220-257 for each view: x = (double)(int64)v / 2^fp
 // (to_double_functor, functors.h:105-114), exp summed in element order per row,
 // each value divided by its row's sum, back with (uint64)(x * 2^fp)
-// (to_fixed_functor, functors.h:116-123; the reference converts on its GPU,
-// which clamps to the uint64 range and maps NaN to 0).
-void NeuralNetwork::HostSoftmax::run(void *self) {
-    const HostSoftmax &h = *(const HostSoftmax *)self;
-    const double scale = (double)(1 << h.fp_);
-    std::vector<double> e((size_t)h.nc_);
-    for (int v = 0; v < h.views_; v++)
-        for (int r = 0; r < h.rows_; r++) {
-            const size_t base = (size_t)v * h.n_ + (size_t)r * h.nc_;
+// (to_fixed_functor, functors.h:116-123).
+void NeuralNetwork::SoftmaxStage::compute(const u64 *in, u64 *out, size_t n) const {
+    const double scale = (double)(1 << fp);
+    const size_t per_view = n / (size_t)views;
+    std::vector<double> e((size_t)nc);
+    for (int v = 0; v < views; v++)
+        for (int r = 0; r < rows; r++) {
+            const size_t base = (size_t)v * per_view + (size_t)r * nc;
             double sum = 0.0;
-            for (int i = 0; i < h.nc_; i++) {
-                e[i] = std::exp((double)(s64)h.in_[base + i] / scale);
+            for (int i = 0; i < nc; i++) {
+                e[i] = std::exp((double)(s64)in[base + i] / scale);
                 sum += e[i];
             }
-            for (int i = 0; i < h.nc_; i++) {
+            for (int i = 0; i < nc; i++) {
                 double q = e[i] / sum;
-                double f = q * scale;
-                u64 fx;
-                if (!(f > 0.0)) fx = 0;  // NaN, zero or negative
-                else if (f >= 18446744073709551616.0) fx = ~0ull;
-                else fx = (u64)f;
-                h.out_[base + i] = fx;
+                out[base + i] = to_u64_clamped(q * scale);
             }
         }
 }
@@ -612,8 +781,13 @@ void NeuralNetwork::_prepare_softmax() {
         softmaxOpen_.resize(n);
         softmaxVals_.resize(n);
     }
-    softmaxAC_.prepare(n, 2, batch, nc, fp);
-    softmaxAll_.prepare(n, 3, batch, nc, fp);
+    for (SoftmaxStage *st : {&softmaxAC_, &softmaxAll_}) {
+        st->views = st == &softmaxAC_ ? 2 : 3;
+        st->rows = batch;
+        st->nc = nc;
+        st->fp = fp;
+        st->prepare((size_t)st->views * n);
+    }
 }
 
 void NeuralNetwork::_reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas) {

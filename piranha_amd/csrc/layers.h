@@ -42,6 +42,35 @@ struct AveragepoolConfig {
         : imageHeight(h), imageWidth(w), features(c), poolSize(pool), stride(s), batchSize(b) {}
 };
 
+// A host computation in stream order between two device buffers: the device
+// values are copied to pinned host memory, a host function maps them and the
+// results are copied back; a captured graph records the function as a host
+// node.  Used where the reference computes on revealed (public) values on the
+// host.  prepare() allocates the pinned buffers and must run outside any
+// capture; they keep their addresses until the stage is destroyed.
+class HostStage {
+  public:
+    HostStage() = default;
+    HostStage(const HostStage &) = delete;
+    HostStage &operator=(const HostStage &) = delete;
+    virtual ~HostStage();
+    void prepare(size_t n);  // n u64 values in and out
+    void enqueue(const u64 *dev_in, u64 *dev_out, hipStream_t s);
+
+  protected:
+    virtual void compute(const u64 *in, u64 *out, size_t n) const = 0;
+
+  private:
+    static void run(void *self);
+    u64 *in_ = nullptr, *out_ = nullptr;
+    size_t n_ = 0;
+};
+
+struct LNConfig {
+    size_t inputSize, numBatches;
+    LNConfig(size_t in, size_t nb) : inputSize(in), numBatches(nb) {}
+};
+
 class ReLULayer;
 
 class Layer {
@@ -59,6 +88,14 @@ class Layer {
     RSS64 *getDelta() { return &deltas; }
     virtual RSS64 *getWeights() { return nullptr; }
     virtual RSS64 *getBiases() { return nullptr; }
+    // the parameter tensors of saveSnapshot / loadSnapshot with their file
+    // names (weight / bias; gamma / beta for LN)
+    virtual std::vector<std::pair<std::string, RSS64 *>> snapshotTensors() {
+        std::vector<std::pair<std::string, RSS64 *>> t;
+        if (getWeights()) t.emplace_back("weight", getWeights());
+        if (getBiases()) t.emplace_back("bias", getBiases());
+        return t;
+    }
     // reluPrime / maxPrime of ReLU and Maxpool layers (Share<uint8_t>)
     virtual RSS<u8> *getDerivative() { return nullptr; }
     virtual size_t inputSize() const = 0;
@@ -194,6 +231,39 @@ class AveragepoolLayer : public Layer {
     MaxpoolGeom geom() const;
 };
 
+// LNLayer.cu: layer normalisation over each image's inputSize features, with
+// the variance revealed and its inverse square root taken on the host
+// (LNLayer.cu:124-132), as a composition of the protocol operations.
+class LNLayer : public Layer {
+  public:
+    LNLayer(const LNConfig &conf, int layerNum, int seed);
+    void printLayer() override;
+    void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
+    RSS64 *getActivation() override { return &activations; }
+    RSS64 *getWeights() override { return &gamma; }  // LNLayer.h:40-41
+    RSS64 *getBiases() override { return &beta; }
+    std::vector<std::pair<std::string, RSS64 *>> snapshotTensors() override {
+        return {{"gamma", &gamma}, {"beta", &beta}};
+    }
+    size_t inputSize() const override { return conf.inputSize * conf.numBatches; }
+
+    LNConfig conf;
+    RSS64 gamma, beta, xhat, invSigma, activations;
+
+  private:
+    // inv_sqrt_fixed_point_functor (functors.h:92-103) on revealed values
+    class InvSqrtStage : public HostStage {
+      public:
+        int fp = 0;
+
+      protected:
+        void compute(const u64 *in, u64 *out, size_t n) const override;
+    };
+    InvSqrtStage invSqrt_;
+    PartyData<u64> revealed_, invVals_;
+};
+
 // Layer description used by the C ABI: {type, a0..a6} (see include/pir.h).
 struct LayerSpec {
     int v[8];
@@ -233,25 +303,15 @@ class NeuralNetwork {
   private:
     // The reveal-softmax is computed on the host, as in the reference, whose
     // softmax loop reads the opened logits one by one into host doubles
-    // (NeuralNetwork.cu:220-257): the opened logits go to pinned host memory,
-    // a host function computes the fixed-point softmax per row and the values
-    // come back, all in stream order (a captured training step records the
-    // host function as a graph node).  Buffers are allocated by prepare(),
-    // outside any capture, and keep their addresses for the life of the network.
-    class HostSoftmax {
+    // (NeuralNetwork.cu:220-257).
+    class SoftmaxStage : public HostStage {
       public:
-        ~HostSoftmax();
-        void prepare(size_t n, int views, int rows, int nc, int fp);
-        // dev_in [views][n] opened logits -> dev_out [views][n] fixed-point softmax
-        void enqueue(const u64 *dev_in, u64 *dev_out, hipStream_t s);
-        static void run(void *self);
+        int views = 0, rows = 0, nc = 0, fp = 0;
 
-      private:
-        u64 *in_ = nullptr, *out_ = nullptr;  // pinned host [views][n]
-        size_t n_ = 0;
-        int views_ = 0, rows_ = 0, nc_ = 0, fp_ = 0;
+      protected:
+        void compute(const u64 *in, u64 *out, size_t n) const override;
     };
-    HostSoftmax softmaxAC_, softmaxAll_;  // fused form (views A, C), op-by-op form (3 views)
+    SoftmaxStage softmaxAC_, softmaxAll_;  // fused form (views A, C), op-by-op form (3 views)
     PartyData<u64> softmaxOpen_, softmaxVals_;
     void _prepare_softmax();
     void _reveal_softmax_grad(const RSS64 &labels, RSS64 &deltas);  // NeuralNetwork.cu:207-267

@@ -203,8 +203,8 @@ const JValue &member(const JValue &o, const char *key, const std::string &where)
 }
 
 int int_of(const JValue &v, const std::string &what) {
-    if (v.kind != JValue::Number || v.num != (double)(long long)v.num || v.num < -2147483648.0 ||
-        v.num > 2147483647.0)
+    if (v.kind != JValue::Number || !(v.num >= -2147483648.0 && v.num <= 2147483647.0) ||
+        v.num != (double)(long long)v.num)
         model_error(what + " must be an integer");
     return (int)v.num;
 }
@@ -250,8 +250,8 @@ ModelFile load_model_file(const std::string &filename) {
             s.v[0] = 0;
             s.v[1] = int_member(l, "input_dim", where);
             s.v[2] = int_member(l, "output_dim", where);
-        } else if (t.str == "relu") {
-            s.v[0] = 2;
+        } else if (t.str == "relu" || t.str == "ln") {
+            s.v[0] = t.str == "relu" ? 2 : 5;
             s.v[1] = int_member(l, "input_dim", where);
         } else if (t.str == "cnn") {
             s.v[0] = 1;
@@ -269,7 +269,7 @@ ModelFile load_model_file(const std::string &filename) {
             s.v[3] = int_member(l, "in_channels", where);
             s.v[4] = pair_entry(l, "pool_hw", 0, where);
             s.v[5] = int_member(l, "stride", where);
-        } else if (t.str == "ln" || t.str == "res") {
+        } else if (t.str == "res") {
             model_error(where + ": \"" + t.str + "\" layers are not built by this library", ERR_UNSUPPORTED);
         } else {
             model_error(where + ": unknown layer type \"" + t.str + "\"");
@@ -337,20 +337,16 @@ void save_snapshot(NeuralNetwork &net, const std::string &path) {
         throw PirError(ERR_ARG);
     }
     const int fp = ctx().frac_bits;
-    for (auto &l : net.layers) {
-        if (!l->getWeights()) continue;
-        save_share(path + "/weight" + std::to_string(l->layerNum), *l->getWeights(), fp);
-        if (l->getBiases()) save_share(path + "/bias" + std::to_string(l->layerNum), *l->getBiases(), fp);
-    }
+    for (auto &l : net.layers)
+        for (auto &t : l->snapshotTensors()) save_share(path + "/" + t.first + std::to_string(l->layerNum), *t.second, fp);
 }
 
 void load_snapshot(NeuralNetwork &net, const std::string &path) {
     const int fp = ctx().frac_bits;
     for (auto &l : net.layers) {
-        if (!l->getWeights()) continue;
-        load_share(path + "/weight" + std::to_string(l->layerNum), *l->getWeights(), fp);
-        if (l->getBiases()) load_share(path + "/bias" + std::to_string(l->layerNum), *l->getBiases(), fp);
-        l->weightsChanged();
+        auto tensors = l->snapshotTensors();
+        for (auto &t : tensors) load_share(path + "/" + t.first + std::to_string(l->layerNum), *t.second, fp);
+        if (!tensors.empty()) l->weightsChanged();
     }
 }
 

@@ -22,10 +22,11 @@ struct ModelFile {
 // fields, ERR_UNSUPPORTED for layer types this library does not build).
 ModelFile load_model_file(const std::string &filename);
 
-// Writes path/weight{i} (and path/bias{i} for FC layers) for every layer with
-// weights: the reconstructed fixed-point values as doubles, each followed by a
-// space, in the C++ stream's default format (6 significant digits).  Creates
-// the directory when missing.
+// Writes path/weight{i} (and path/bias{i} for FC layers; path/gamma{i} and
+// path/beta{i} for LN layers; FCLayer.cu:51-69, CNNLayer.cu:53-64,
+// LNLayer.cu:47-63) for every layer with parameters: the reconstructed
+// fixed-point values as doubles, each followed by a space, in the C++ stream's
+// default format (6 significant digits).  Creates the directory when missing.
 void save_snapshot(NeuralNetwork &net, const std::string &path);
 // Reads the same files back into public shares (setPublic); a file with fewer
 // values than the layer holds leaves the rest at zero, a missing file gives

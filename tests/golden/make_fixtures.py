@@ -5,9 +5,9 @@ doubles the reference's loadShareFromFile parses, util.cuh:271-289).
 Run in a container that has the reference checkout:
     python tests/golden/make_fixtures.py /root/reference
 Only data files (the fixtures and the model descriptions of files/models/,
-copied as they are to tests/golden/models/) and the numeric brace lists of two
-GPU.cu known-answer tests are read; no reference code is imported, compiled or
-run.
+copied as they are to tests/golden/models/) and the numeric brace lists of a
+few GPU.cu / Layer.cu known-answer tests are read; no reference code is
+imported, compiled or run.
 """
 import os
 import sys
@@ -67,18 +67,25 @@ FILES = {
 # Known-answer vectors written inline in the reference's src/test/GPU.cu: the
 # brace lists of the named test are read as numbers (no code is run).
 KATS = {
-    "CutlassConvDgrad2": {"delta": "dgrad2_delta", "filters": "dgrad2_filters", "expected": "dgrad2_expected"},
-    "CutlassConvWgrad2": {"delta": "wgrad2_delta", "input": "wgrad2_input", "expected": "wgrad2_expected"},
+    ("GPU.cu", "GPUTest", "CutlassConvDgrad2"): {"delta": "dgrad2_delta", "filters": "dgrad2_filters",
+                                                 "expected": "dgrad2_expected"},
+    ("GPU.cu", "GPUTest", "CutlassConvWgrad2"): {"delta": "wgrad2_delta", "input": "wgrad2_input",
+                                                 "expected": "wgrad2_expected"},
+    # LayerNorm known answers (Layer.cu:887-1000): 2 rows of 48 features
+    ("Layer.cu", "LayerTest", "LNForward"): {"input": "lnfwd_input", "expected": "lnfwd_expected"},
+    ("Layer.cu", "LayerTest", "LNBackward"): {"input": "lnbwd_input", "delta": "lnbwd_delta",
+                                              "expected": "lnbwd_expected", "expected_gamma": "lnbwd_gamma",
+                                              "expected_beta": "lnbwd_beta"},
 }
 
 
 def kat_arrays(ref):
     import re
 
-    text = open(os.path.join(ref, "src", "test", "GPU.cu")).read()
     out = {}
-    for test, names in KATS.items():
-        start = text.index("TYPED_TEST(GPUTest, %s)" % test)
+    for (fname, fixture, test), names in KATS.items():
+        text = open(os.path.join(ref, "src", "test", fname)).read()
+        start = text.index("TYPED_TEST(%s, %s)" % (fixture, test))
         end = text.find("TYPED_TEST(", start + 1)
         block = text[start:end if end > 0 else len(text)]
         for var, key in names.items():

@@ -113,7 +113,7 @@ def test_load_short_and_missing_files(tmp_path):
 
 def test_load_model_errors(tmp_path):
     with pytest.raises(P.PirError, match="not built by this library"):
-        P.Network.from_model_file(os.path.join(MODELS, "lenet-ln-norelu.json"), FP)
+        P.Network.from_model_file(os.path.join(MODELS, "resnet18-cifar10.json"), FP)
     bad = tmp_path / "bad.json"
     bad.write_text('{"input_size": 4, "num_classes": 3, "batch_size": 1, "dataset": "x", "model": ['
                    '{"layer": "fc", "input_dim": 4, "output_dim": 3}, {"layer": "relu", "input_dim": 4}]}')

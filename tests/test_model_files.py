@@ -12,7 +12,7 @@ import piranha_amd as P
 from piranha_amd import configs as C
 
 MODELS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models")
-UNSUPPORTED = {"ln", "res"}
+UNSUPPORTED = {"res"}
 
 
 def model_names():

@@ -192,7 +192,7 @@ int pir_fpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
 /* ---- layers and network (src/nn/FCLayer.cu etc., NeuralNetwork.cu) -------------- */
 typedef struct pir_net pir_net;
 enum { PIR_LAYER_FC = 0, PIR_LAYER_CNN = 1, PIR_LAYER_RELU = 2, PIR_LAYER_MAXPOOL = 3, PIR_LAYER_AVGPOOL = 4,
-       PIR_LAYER_LN = 5 };
+       PIR_LAYER_LN = 5, PIR_LAYER_RES = 6 };
 enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
 /* spec: per layer 8 ints {type, a0..a6}:
  *   FC      {0, inputDim, outputDim}                                (FCConfig.h:16)
@@ -201,6 +201,7 @@ enum { PIR_MODE_FUSED = 0, PIR_MODE_PLAIN = 1 };
  *   Maxpool {3, H, W, C, pool, stride}                              (MaxpoolConfig.h:16)
  *   Avgpool {4, H, W, C, pool, stride}                              (AveragepoolConfig.h:15)
  *   LN      {5, inputDim}                                           (LNConfig.h:13)
+ *   Res     {6, H, W, in_planes, planes, num_blocks, stride, expansion} (ResLayerConfig.h:19; expansion 1)
  * Each layer's input size must equal the previous layer's output size.
  * Weights are initialised as in FCLayer.cu:31-48 / CNNLayer.cu:36-50 with
  * std::default_random_engine(seed + layer) unless set with pir_net_set_weights. */
@@ -212,7 +213,7 @@ int pir_net_num_layers(const pir_net *net);
 /* loadModel (model.cu:24-115) without building the network: the file's
  * input_size, num_classes, batch_size, the layer count (*num_layers) and up to
  * max_layers layer specs (8 ints each, the pir_net_create form).  Layer types
- * fc, relu, cnn, maxpool, averagepool, ln; res gives PIR_ERR_UNSUPPORTED. */
+ * fc, relu, cnn, maxpool, averagepool, ln, res. */
 int pir_model_read(const char *network_filename, int *input_size, int *num_classes, int *batch_size, int *num_layers,
                    int *spec, int max_layers);
 /* loadModel + the network constructor: batch <= 0 takes the file's batch_size */
@@ -248,6 +249,14 @@ int pir_net_weights(pir_net *net, int layer, const uint64_t **shares, size_t *n,
                     size_t *n_bias);
 /* weights (and FC biases) from host share arrays [3][2][n] */
 int pir_net_set_weight_shares(pir_net *net, int layer, const uint64_t *weight_shares, const uint64_t *bias_shares);
+/* Every parameter tensor of a layer, in the order and with the names of its
+ * snapshot files (weight{n} / bias{n}, gamma{n} / beta{n} for LN, a Res layer's
+ * inner layers block by block, ResLayer.cu:134-167): the count, one tensor's
+ * device shares [3][2][n] and name, and setting one from host shares. */
+int pir_net_num_params(pir_net *net, int layer);
+int pir_net_param(pir_net *net, int layer, int index, const uint64_t **shares, size_t *n);
+int pir_net_param_name(pir_net *net, int layer, int index, char *name, size_t capacity);
+int pir_net_set_param_shares(pir_net *net, int layer, int index, const uint64_t *shares);
 /* hipGraph capture of the whole forward (input from pir_net_forward_shares' buffer).
  * A weight change (set_weights, a backward pass) drops the graph; capture again. */
 int pir_net_capture(pir_net *net, const uint64_t *input_shares);

@@ -496,6 +496,147 @@ def inv_sqrt_fixed(x, fp):
     return lib().ref_inv_sqrt_fixed(int(x) & 0xFFFFFFFFFFFFFFFF, fp)
 
 
+# --- ResLayer (ResLayer.cu) as a composition of the oracle's layers ---------
+
+def res_structure(H, W, in_planes, planes, num_blocks, stride, expansion):
+    """ResLayer's inner layers (ResLayer.cu:23-103): per block a (main,
+    shortcut) pair of symbolic layer lists; the shortcut is empty when the
+    block input is added directly."""
+    def block(ih, iw, inp, s):
+        oh, ow = (ih + 2 - 3) // s + 1, (iw + 2 - 3) // s + 1
+        n = oh * ow * planes
+        main = [("cnn", ih, iw, inp, planes, 3, s, 1), ("ln", n), ("relu", n),
+                ("cnn", oh, ow, planes, planes, 3, 1, 1), ("ln", n), ("relu", n)]
+        sc = []
+        if s != 1 or inp != expansion * planes:
+            sc = [("cnn", ih, iw, inp, expansion * planes, 1, s, 0), ("ln", oh * ow * expansion * planes)]
+        return main, sc, oh, ow
+
+    out = []
+    m, sc, ih, iw = block(H, W, in_planes, stride)
+    out.append((m, sc))
+    for _ in range(1, num_blocks):
+        m, sc, ih, iw = block(ih, iw, planes * expansion, 1)
+        out.append((m, sc))
+    return out
+
+
+def _n_params(l):
+    return {"cnn": 1, "ln": 2}.get(l[0], 0)
+
+
+def _split_params(structure, flat):
+    """The flat snapshot-order list (ResLayer.cu:134-167: each block's layers,
+    then its shortcut) -> per block (main, shortcut) lists of per-layer lists."""
+    flat = list(flat)
+    out, k = [], 0
+    for main, sc in structure:
+        parts = []
+        for layers in (main, sc):
+            per = []
+            for l in layers:
+                per.append(flat[k:k + _n_params(l)])
+                k += _n_params(l)
+            parts.append(per)
+        out.append(tuple(parts))
+    assert k == len(flat), "parameter count does not match the ResLayer"
+    return out
+
+
+def _sub_forward(l, h, p, batch, fp):
+    if l[0] == "cnn":
+        return cnn_forward(h, p[0], batch, *l[1:], fp), None
+    if l[0] == "ln":
+        act, xhat, inv = ln_forward(h, p[0], p[1], batch, fp)
+        return act, (xhat, inv)
+    return relu_forward(h)
+
+
+def _sub_backward(l, delta, inp, p, extra, batch, fp, lr):
+    if l[0] == "cnn":
+        d, W2 = cnn_backward(delta, inp, p[0], batch, *l[1:], fp, lr)
+        return d, [W2]
+    if l[0] == "ln":
+        d, _, g2, b2 = ln_backward(delta, extra[0], extra[1], p[0], p[1], batch, fp, lr)
+        return d, [g2, b2]
+    return relu_backward(delta, extra), []
+
+
+def _block_forward(layers, params, inp, residual, batch, fp):
+    acts, extras, h = [], [], inp
+    for l, p in zip(layers[:-1], params[:-1]):
+        a, e = _sub_forward(l, h, p, batch, fp)
+        acts.append(a)
+        extras.append(e)
+        h = a
+    lli = acts[-1].copy() if residual is None else acts[-1] + residual  # share-wise, mod 2^64
+    a, e = _sub_forward(layers[-1], lli, params[-1], batch, fp)
+    acts.append(a)
+    extras.append(e)
+    return acts, extras, lli
+
+
+def res_forward(x, batch, fp, spec, flat_params):
+    """ResLayer::forward (ResLayer.cu:196-236).  Returns (output, state) with
+    the state backward needs."""
+    structure = res_structure(*spec)
+    params = _split_params(structure, flat_params)
+    state, h = [], _u(x)
+    for (main, sc), (pm, ps) in zip(structure, params):
+        if sc:
+            sc_acts, sc_extras, sc_lli = _block_forward(sc, ps, h, None, batch, fp)
+            residual = sc_acts[-1]
+        else:
+            sc_acts = sc_extras = sc_lli = None
+            residual = h
+        acts, extras, lli = _block_forward(main, pm, h, residual, batch, fp)
+        state.append((h, acts, extras, lli, sc_acts, sc_extras, sc_lli))
+        h = acts[-1]
+    return h, state
+
+
+def _block_backward(layers, params, acts, extras, lli, delta, forward_input, batch, fp, lr):
+    last = len(layers) - 1
+    dl, newp = [None] * len(layers), [None] * len(layers)
+    dl[last], newp[last] = _sub_backward(layers[last], delta, lli, params[last], extras[last], batch, fp, lr)
+    for i in range(last - 1, 0, -1):
+        dl[i], newp[i] = _sub_backward(layers[i], dl[i + 1], acts[i - 1], params[i], extras[i], batch, fp, lr)
+    dl[0], newp[0] = _sub_backward(layers[0], dl[1], forward_input, params[0], extras[0], batch, fp, lr)
+    return dl, newp
+
+
+def res_backward(delta, x, batch, fp, lr, spec, flat_params):
+    """ResLayer::backward (ResLayer.cu:250-307), with every inner weight update.
+    Returns (dL/d input, updated parameters in snapshot order).  The block
+    without a shortcut adds the scalar 1 to its first layer's delta, as the
+    reference does."""
+    structure = res_structure(*spec)
+    params = _split_params(structure, flat_params)
+    _, state = res_forward(x, batch, fp, spec, flat_params)
+    new = [None] * len(structure)
+    d = _u(delta)
+    for b in range(len(structure) - 1, -1, -1):
+        main, sc = structure[b]
+        pm, ps = params[b]
+        inp, acts, extras, lli, sc_acts, sc_extras, sc_lli = state[b]
+        dl, npm = _block_backward(main, pm, acts, extras, lli, d, inp, batch, fp, lr)
+        d0 = dl[0].copy()
+        nps = []
+        if sc:
+            sdl, nps = _block_backward(sc, ps, sc_acts, sc_extras, sc_lli, dl[-1], inp, batch, fp, lr)
+            d0 = d0 + sdl[0]
+        else:
+            d0[0, 0] += np.uint64(1)  # RSS += 1: A.sA and C.sB
+            d0[2, 1] += np.uint64(1)
+        new[b] = (npm, nps)
+        d = d0
+    flat = []
+    for npm, nps in new:
+        for per in list(npm) + list(nps):
+            flat.extend(per)
+    return d, flat
+
+
 def net_forward(layers, batch, fp, params, x_shares):
     """NeuralNetwork::forward (NeuralNetwork.cu:108-133) over the oracle layers.
 
@@ -530,6 +671,9 @@ def net_forward(layers, batch, fp, params, x_shares):
         elif kind == "ln":
             gamma, beta = params[i]
             h, _, _ = ln_forward(h, gamma, beta, batch, fp)
+            out.append((h, None))
+        elif kind == "res":
+            h, _ = res_forward(h, batch, fp, l[1:], params[i])
             out.append((h, None))
         else:
             raise ValueError(kind)
@@ -684,6 +828,8 @@ def net_backward(layers, batch, fp, params, x_shares, fwd, labels, lr, out_delta
             _, xhat, inv_sigma = ln_forward(inp, gamma, beta, batch, fp)  # the forward's saved state
             d, _, g2, b2 = ln_backward(delta, xhat, inv_sigma, gamma, beta, batch, fp, lr)
             new_params[i] = (g2, b2)
+        elif kind == "res":
+            d, new_params[i] = res_backward(delta, inp, batch, fp, lr, layers[i][1:], params[i])
         else:
             raise ValueError(kind)
         deltas[i] = d

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libpir_hip.so")
 
 ALGO_PLAIN, ALGO_LIMB = 0, 1
 MODE_FUSED, MODE_PLAIN = 0, 1
-LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL, LAYER_LN = 0, 1, 2, 3, 4, 5
+LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL, LAYER_LN, LAYER_RES = 0, 1, 2, 3, 4, 5, 6
 CONV_FPROP, CONV_DGRAD, CONV_WGRAD = 0, 1, 2
 LIMB_WIDE, LIMB_VARIANTS = 8, 16  # kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
 OP_ADD, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV = range(8)
@@ -90,6 +90,10 @@ _SIGS = {
     "pir_net_layer_spec": ([_vp, _i, _vp], _i),
     "pir_net_save_snapshot": ([_vp, ctypes.c_char_p], _i),
     "pir_net_load_snapshot": ([_vp, ctypes.c_char_p], _i),
+    "pir_net_num_params": ([_vp, _i], _i),
+    "pir_net_param": ([_vp, _i, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)], _i),
+    "pir_net_param_name": ([_vp, _i, _i, ctypes.c_char_p, _sz], _i),
+    "pir_net_set_param_shares": ([_vp, _i, _i, _vp], _i),
     "pir_net_set_weights": ([_vp, _i, _vp, _sz, _vp, _sz], _i),
     "pir_net_forward": ([_vp, _vp, _sz], _i),
     "pir_net_forward_shares": ([_vp, _vp], _i),
@@ -259,6 +263,14 @@ class Network:
         check(lib().pir_net_layer_spec(self.h, layer, v.ctypes.data))
         return tuple(int(x) for x in v)
 
+    def params(self, layer):
+        """[(snapshot name, shares (3, 2, n) uint64)] of every parameter tensor of a layer."""
+        return _params_of(self, layer)
+
+    def set_param_shares(self, layer, index, shares):
+        shares = np.ascontiguousarray(shares, np.uint64)
+        check(lib().pir_net_set_param_shares(self.h, layer, index, shares.ctypes.data))
+
     def save_snapshot(self, path):
         check(lib().pir_net_save_snapshot(self.h, os.fsencode(path)))
 
@@ -418,6 +430,20 @@ class Network:
             b = np.empty((3, 2, bn.value), np.uint64)
             check(lib().pir_memcpy_d2h(b.ctypes.data, bp.value, b.nbytes, None))
         return w, b
+
+
+def _params_of(net, layer):
+    out = []
+    for k in range(lib().pir_net_num_params(net.h, layer)):
+        p, n = _vp(), _sz()
+        check(lib().pir_net_param(net.h, layer, k, ctypes.byref(p), ctypes.byref(n)))
+        name = ctypes.create_string_buffer(64)
+        check(lib().pir_net_param_name(net.h, layer, k, name, 64))
+        net.synchronize()
+        a = np.empty((3, 2, n.value), np.uint64)
+        check(lib().pir_memcpy_d2h(a.ctypes.data, p.value, a.nbytes, None))
+        out.append((name.value.decode(), a))
+    return out
 
 
 def autotune_entries():

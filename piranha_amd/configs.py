@@ -4,14 +4,15 @@ format (src/util/model.cu:24-115, files/models/*.json).
 
 Layer specs: ("fc", in, out) | ("cnn", H, W, Cin, Cout, f, stride, pad) |
 ("relu", n_per_image) | ("maxpool", H, W, C, pool, stride) |
-("avgpool", H, W, C, pool, stride) | ("ln", n_per_image).
+("avgpool", H, W, C, pool, stride) | ("ln", n_per_image) |
+("res", H, W, in_planes, planes, num_blocks, stride, expansion).
 """
 import json
 
-from . import LAYER_AVGPOOL, LAYER_CNN, LAYER_FC, LAYER_LN, LAYER_MAXPOOL, LAYER_RELU
+from . import LAYER_AVGPOOL, LAYER_CNN, LAYER_FC, LAYER_LN, LAYER_MAXPOOL, LAYER_RELU, LAYER_RES
 
 _TYPES = {"fc": LAYER_FC, "cnn": LAYER_CNN, "relu": LAYER_RELU, "maxpool": LAYER_MAXPOOL, "avgpool": LAYER_AVGPOOL,
-          "ln": LAYER_LN}
+          "ln": LAYER_LN, "res": LAYER_RES}
 
 
 def to_spec(layers):
@@ -34,6 +35,9 @@ def output_size(layers):
         return l[2]
     if l[0] in ("relu", "ln"):
         return l[1]
+    if l[0] == "res":
+        H, W, _, planes, _, s, _ = l[1:]
+        return ((H + 2 - 3) // s + 1) * ((W + 2 - 3) // s + 1) * planes
     if l[0] == "cnn":
         H, W, _, Cout, f, s, p = l[1:]
         return ((H + 2 * p - f) // s + 1) * ((W + 2 * p - f) // s + 1) * Cout
@@ -57,6 +61,9 @@ def from_model_json(doc, avgpool_as_maxpool=False):
             out.append(("relu", l["input_dim"]))
         elif t == "ln":
             out.append(("ln", l["input_dim"]))
+        elif t == "res":
+            out.append(("res", l["input_hw"][0], l["input_hw"][1], l["in_planes"], l["out_planes"], l["num_blocks"],
+                        l["stride"], l["expansion"]))
         elif t == "cnn":
             out.append(("cnn", l["input_hw"][0], l["input_hw"][1], l["in_channels"], l["out_channels"],
                         l["filter_hw"][0], l["stride"], l["padding"]))

@@ -937,6 +937,52 @@ int pir_net_weights(pir_net *n, int layer, const uint64_t **shares, size_t *coun
     return OK;
 }
 
+namespace {
+RSS64 *net_param(pir_net *n, int layer, int index, std::string *name) {
+    if (!n || layer < 0 || layer >= pir_net_num_layers(n)) return nullptr;
+    auto t = n->net->layers[layer]->snapshotTensors();
+    if (index < 0 || index >= (int)t.size()) return nullptr;
+    if (name) *name = t[index].first;
+    return t[index].second;
+}
+}  // namespace
+
+int pir_net_num_params(pir_net *n, int layer) {
+    if (!n || layer < 0 || layer >= pir_net_num_layers(n)) return 0;
+    return (int)n->net->layers[layer]->snapshotTensors().size();
+}
+
+int pir_net_param(pir_net *n, int layer, int index, const uint64_t **shares, size_t *count) {
+    PIR_REQUIRE(shares && count, "pir_net_param: invalid arguments");
+    RSS64 *t = net_param(n, layer, index, nullptr);
+    PIR_REQUIRE(t, "pir_net_param: no such parameter");
+    *shares = t->base();
+    *count = t->size();
+    return OK;
+}
+
+int pir_net_param_name(pir_net *n, int layer, int index, char *name, size_t capacity) {
+    std::string s;
+    PIR_REQUIRE(name && capacity > 0 && net_param(n, layer, index, &s), "pir_net_param_name: no such parameter");
+    PIR_REQUIRE(s.size() < capacity, "pir_net_param_name: buffer too small");
+    std::copy(s.begin(), s.end(), name);
+    name[s.size()] = 0;
+    return OK;
+}
+
+int pir_net_set_param_shares(pir_net *n, int layer, int index, const uint64_t *shares) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(shares, "pir_net_set_param_shares: invalid arguments");
+        RSS64 *t = net_param(n, layer, index, nullptr);
+        PIR_REQUIRE(t, "pir_net_set_param_shares: no such parameter");
+        NetScope ns(n);
+        t->copy_from_host(shares);
+        n->net->layers[layer]->weightsChanged();
+        drop_graph(n);
+        return (int)OK;
+    });
+}
+
 int pir_net_capture(pir_net *n, const uint64_t *shares) {
     return guarded([&]() -> int {
         PIR_REQUIRE(n, "pir_net_capture: invalid arguments");

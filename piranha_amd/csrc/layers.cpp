@@ -616,6 +616,141 @@ void LNLayer::backward(const RSS64 &delta, const RSS64 &) {
 }
 
 // ---------------------------------------------------------------------------
+// ResLayer.cu
+
+ResLayer::ResLayer(const ResLayerConfig &c, int n, int) : Layer(n), conf(c) {
+    if (c.expansion != 1) {
+        // the block output has `planes` channels and a shortcut expansion * planes:
+        // the residual sum needs them equal
+        set_error("ResLayer: only expansion 1 is supported");
+        throw PirError(ERR_UNSUPPORTED);
+    }
+    size_t ih = c.imageHeight, iw = c.imageWidth;
+    int layer_ctr = n * 1000;
+    appendBlock(ih, iw, c.in_planes, c.planes, c.stride, layer_ctr);
+    for (size_t i = 1; i < c.num_blocks; i++) appendBlock(ih, iw, c.planes * c.expansion, c.planes, 1, layer_ctr);
+}
+
+// ResLayer.cu:42-103
+void ResLayer::appendBlock(size_t &ih, size_t &iw, size_t in_planes, size_t planes, size_t stride, int &layer_ctr) {
+    const int base_seed = layerNum * 1000 + (int)blocks.size() * 10;
+    const size_t b = conf.batchSize;
+    Block block, shortcut;
+    block.emplace_back(new CNNLayer(CNNConfig(ih, iw, in_planes, planes, 3, stride, 1, b), layer_ctr++, base_seed + 0));
+    if (stride != 1 || in_planes != conf.expansion * planes) {
+        shortcut.emplace_back(
+            new CNNLayer(CNNConfig(ih, iw, in_planes, conf.expansion * planes, 1, stride, 0, b), layer_ctr++, base_seed + 2));
+        size_t oh = (ih + 2 - 3) / stride + 1, ow = (iw + 2 - 3) / stride + 1;
+        shortcut.emplace_back(new LNLayer(LNConfig(oh * ow * conf.expansion * planes, b), layer_ctr++, 0));
+    }
+    ih = (ih + 2 - 3) / stride + 1;
+    iw = (iw + 2 - 3) / stride + 1;
+    block.emplace_back(new LNLayer(LNConfig(iw * ih * planes, b), layer_ctr++, 0));
+    block.emplace_back(new ReLULayer(ReLUConfig(iw * ih * planes, b), layer_ctr++, 0));
+    block.emplace_back(new CNNLayer(CNNConfig(ih, iw, planes, planes, 3, 1, 1, b), layer_ctr++, base_seed + 1));
+    block.emplace_back(new LNLayer(LNConfig(iw * ih * planes, b), layer_ctr++, 0));
+    block.emplace_back(new ReLULayer(ReLUConfig(iw * ih * planes, b), layer_ctr++, 0));
+    blocks.push_back(std::move(block));
+    shortcuts.push_back(std::move(shortcut));
+}
+
+void ResLayer::printLayer() {
+    printf("----------------------------------------------\n(%d) Res Layer\n\n\t\t\t  %zu\t\t(BatchSize)\t\t%zu\t\t(Num "
+           "Blocks)\n\t\t\t  %zu\t\t(In Planes)\t\t%zu\t\t(Planes)\n\t\t\t  %zu\t\t(Stride)\t\t%zu\t\t(Expansion)\n",
+           layerNum + 1, conf.batchSize, conf.num_blocks, conf.in_planes, conf.planes, conf.stride, conf.expansion);
+    for (size_t i = 0; i < blocks.size(); i++) {
+        for (size_t k = 0; k + 1 < blocks[i].size(); k++) blocks[i][k]->printLayer();
+        for (auto &l : shortcuts[i]) l->printLayer();
+        blocks[i].back()->printLayer();
+    }
+}
+
+// ResLayer.cu:196-219: the block's layers in order; the last one takes the
+// penultimate activation plus the shortcut's output (or the block input).
+void ResLayer::forwardBlock(size_t idx, Block &b, const RSS64 *input, bool isShortcut) {
+    b[0]->forward(*input);
+    for (size_t i = 1; i + 1 < b.size(); i++) b[i]->forward(*b[i - 1]->getActivation());
+    const RSS64 *pen = b[b.size() - 2]->getActivation();
+    RSS64 lastLayerInput(pen->size());
+    lastLayerInput += *pen;
+    if (!isShortcut) {
+        if (!shortcuts[idx].empty()) {
+            forwardBlock(0, shortcuts[idx], input, true);
+            lastLayerInput += *shortcuts[idx].back()->getActivation();
+        } else {
+            lastLayerInput += *input;
+        }
+    }
+    b.back()->forward(lastLayerInput);
+}
+
+// ResLayer.cu:227-236
+void ResLayer::forward(const RSS64 &input) {
+    forwardBlock(0, blocks[0], &input, false);
+    for (size_t i = 1; i < blocks.size(); i++) forwardBlock(i, blocks[i], blocks[i - 1].back()->getActivation(), false);
+}
+
+// ResLayer.cu:250-286.  Without a shortcut the reference adds the scalar 1 to
+// the first layer's delta (RSS += 1: A.sA and C.sB), which is kept.
+void ResLayer::backwardBlock(size_t idx, Block &b, const RSS64 *deltas, const RSS64 *forwardInput, bool isShortcut) {
+    const size_t last = b.size() - 1;
+    const RSS64 *pen = b[last - 1]->getActivation();
+    RSS64 lastLayerInput(pen->size());
+    lastLayerInput += *pen;
+    if (!isShortcut) {
+        if (!shortcuts[idx].empty()) lastLayerInput += *shortcuts[idx].back()->getActivation();
+        else lastLayerInput += *forwardInput;
+    }
+    b[last]->backward(*deltas, lastLayerInput);
+    for (size_t i = last - 1; i > 0; i--) b[i]->backward(*b[i + 1]->getDelta(), *b[i - 1]->getActivation());
+    if (b.size() > 1) b[0]->backward(*b[1]->getDelta(), *forwardInput);
+    if (!isShortcut) {
+        if (!shortcuts[idx].empty()) {
+            backwardBlock(0, shortcuts[idx], b[last]->getDelta(), forwardInput, true);
+            *b[0]->getDelta() += *shortcuts[idx][0]->getDelta();
+        } else {
+            *b[0]->getDelta() += (u64)1;
+        }
+    }
+}
+
+// ResLayer.cu:288-307
+void ResLayer::backward(const RSS64 &deltas, const RSS64 &forwardInput) {
+    const size_t L = blocks.size() - 1;
+    if (blocks.size() > 1) backwardBlock(L, blocks[L], &deltas, blocks[L - 1].back()->getActivation(), false);
+    else backwardBlock(L, blocks[L], &deltas, &forwardInput, false);
+    for (size_t i = blocks.size() - 2; blocks.size() > 2 && i > 0; i--)
+        backwardBlock(i, blocks[i], blocks[i + 1].front()->getDelta(), blocks[i - 1].back()->getActivation(), false);
+    if (blocks.size() > 1) backwardBlock(0, blocks[0], blocks[1].front()->getDelta(), &forwardInput, false);
+}
+
+// ResLayer.cu:134-167 (saveSnapshotBlock order: block layers, then the shortcut)
+std::vector<std::pair<std::string, RSS64 *>> ResLayer::snapshotTensors() {
+    std::vector<std::pair<std::string, RSS64 *>> t;
+    for (size_t i = 0; i < blocks.size(); i++) {
+        for (auto &l : blocks[i])
+            for (auto &e : l->snapshotTensors()) t.push_back(e);
+        for (auto &l : shortcuts[i])
+            for (auto &e : l->snapshotTensors()) t.push_back(e);
+    }
+    return t;
+}
+
+void ResLayer::allocDelta() {
+    for (size_t i = 0; i < blocks.size(); i++) {
+        for (auto &l : blocks[i]) l->allocDelta();
+        for (auto &l : shortcuts[i]) l->allocDelta();
+    }
+}
+
+void ResLayer::weightsChanged() {
+    for (size_t i = 0; i < blocks.size(); i++) {
+        for (auto &l : blocks[i]) l->weightsChanged();
+        for (auto &l : shortcuts[i]) l->weightsChanged();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Network (NeuralNetwork.cu:32-133)
 
 // Geometry of one layer spec row: every size positive, windows inside the
@@ -628,6 +763,9 @@ static bool spec_valid(const int *v) {
                    v[1] + 2 * v[7] >= v[5] && v[2] + 2 * v[7] >= v[5];
         case 2: return v[1] > 0;
         case 5: return v[1] > 0;
+        case 6:
+            return v[1] > 0 && v[2] > 0 && v[3] > 0 && v[4] > 0 && v[5] > 0 && v[6] > 0 && v[7] > 0 && v[1] + 2 >= 3 &&
+                   v[2] + 2 >= 3;
         case 3:
         case 4: return v[1] > 0 && v[2] > 0 && v[3] > 0 && v[4] > 0 && v[5] > 0 && v[1] >= v[4] && v[2] >= v[4];
         default: return true;  // rejected below
@@ -651,8 +789,12 @@ NeuralNetwork::NeuralNetwork(const std::vector<LayerSpec> &spec, int b, int seed
                 layers.emplace_back(new AveragepoolLayer(AveragepoolConfig(v[1], v[2], v[3], v[4], v[5], b), (int)i, s));
                 break;
             case 5: layers.emplace_back(new LNLayer(LNConfig(v[1], b), (int)i, s)); break;
+            case 6:
+                layers.emplace_back(
+                    new ResLayer(ResLayerConfig(b, v[1], v[2], v[3], v[4], v[5], v[6], v[7]), (int)i, s));
+                break;
             default:
-                set_error("network: only FC, CNN, ReLU, Maxpool, Averagepool and LN layers are supported");
+                set_error("network: only FC, CNN, ReLU, Maxpool, Averagepool, LN and Res layers are supported");
                 throw PirError(ERR_UNSUPPORTED);
         }
     }

@@ -85,15 +85,16 @@ class Layer {
     // the weight update (learning rate 2^-log_lr of the context).
     virtual void backward(const RSS64 &delta, const RSS64 &forwardInput) = 0;
     virtual RSS64 *getActivation() = 0;
-    RSS64 *getDelta() { return &deltas; }
+    virtual RSS64 *getDelta() { return &deltas; }
     virtual RSS64 *getWeights() { return nullptr; }
     virtual RSS64 *getBiases() { return nullptr; }
     // the parameter tensors of saveSnapshot / loadSnapshot with their file
-    // names (weight / bias; gamma / beta for LN)
+    // names (weight{n} / bias{n}; gamma{n} / beta{n} for LN; a ResLayer lists
+    // those of its inner layers, numbered layerNum * 1000 + k)
     virtual std::vector<std::pair<std::string, RSS64 *>> snapshotTensors() {
         std::vector<std::pair<std::string, RSS64 *>> t;
-        if (getWeights()) t.emplace_back("weight", getWeights());
-        if (getBiases()) t.emplace_back("bias", getBiases());
+        if (getWeights()) t.emplace_back("weight" + std::to_string(layerNum), getWeights());
+        if (getBiases()) t.emplace_back("bias" + std::to_string(layerNum), getBiases());
         return t;
     }
     // reluPrime / maxPrime of ReLU and Maxpool layers (Share<uint8_t>)
@@ -109,7 +110,7 @@ class Layer {
     // precedes this one (its input is that ReLU's activation), in one pass; the
     // ReLU layer's deltas receive exactly what its own backward would write.
     virtual bool backwardWithReLU(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer &relu) { return false; }
-    void allocDelta() { ensureDeltas(); }
+    virtual void allocDelta() { ensureDeltas(); }
 
   protected:
     // allocated on the first backward pass (inference never needs it)
@@ -244,7 +245,7 @@ class LNLayer : public Layer {
     RSS64 *getWeights() override { return &gamma; }  // LNLayer.h:40-41
     RSS64 *getBiases() override { return &beta; }
     std::vector<std::pair<std::string, RSS64 *>> snapshotTensors() override {
-        return {{"gamma", &gamma}, {"beta", &beta}};
+        return {{"gamma" + std::to_string(layerNum), &gamma}, {"beta" + std::to_string(layerNum), &beta}};
     }
     size_t inputSize() const override { return conf.inputSize * conf.numBatches; }
 
@@ -262,6 +263,43 @@ class LNLayer : public Layer {
     };
     InvSqrtStage invSqrt_;
     PartyData<u64> revealed_, invVals_;
+};
+
+struct ResLayerConfig {
+    size_t batchSize, imageHeight, imageWidth, in_planes, planes, num_blocks, stride, expansion;
+    ResLayerConfig(size_t b, size_t h, size_t w, size_t in, size_t p, size_t nb, size_t s, size_t e)
+        : batchSize(b), imageHeight(h), imageWidth(w), in_planes(in), planes(p), num_blocks(nb), stride(s),
+          expansion(e) {}
+};
+
+// ResLayer.cu: num_blocks residual blocks, each CNN 3x3 (stride) - LN - ReLU -
+// CNN 3x3 - LN - ReLU, the last ReLU applied to the sum of its input and the
+// block input (or, when the shape changes, of a 1x1 CNN - LN shortcut).  Inner
+// layers are numbered layerNum * 1000 + k and seeded layerNum * 1000 +
+// 10 * block + {0, 1, 2} (ResLayer.cu:23-103).
+class ResLayer : public Layer {
+  public:
+    ResLayer(const ResLayerConfig &conf, int layerNum, int seed);
+    void printLayer() override;
+    void forward(const RSS64 &input) override;
+    void backward(const RSS64 &delta, const RSS64 &forwardInput) override;
+    RSS64 *getActivation() override { return blocks.back().back()->getActivation(); }
+    RSS64 *getDelta() override { return blocks.front().front()->getDelta(); }
+    size_t inputSize() const override {
+        return conf.batchSize * conf.imageHeight * conf.imageWidth * conf.in_planes;
+    }
+    std::vector<std::pair<std::string, RSS64 *>> snapshotTensors() override;
+    void allocDelta() override;
+    void weightsChanged() override;
+
+    using Block = std::vector<std::unique_ptr<Layer>>;
+    ResLayerConfig conf;
+    std::vector<Block> blocks, shortcuts;  // an empty shortcut: the block input is added
+
+  private:
+    void appendBlock(size_t &ih, size_t &iw, size_t in_planes, size_t planes, size_t stride, int &layer_ctr);
+    void forwardBlock(size_t idx, Block &b, const RSS64 *input, bool isShortcut);
+    void backwardBlock(size_t idx, Block &b, const RSS64 *deltas, const RSS64 *forwardInput, bool isShortcut);
 };
 
 // Layer description used by the C ABI: {type, a0..a6} (see include/pir.h).

@@ -270,7 +270,14 @@ ModelFile load_model_file(const std::string &filename) {
             s.v[4] = pair_entry(l, "pool_hw", 0, where);
             s.v[5] = int_member(l, "stride", where);
         } else if (t.str == "res") {
-            model_error(where + ": \"" + t.str + "\" layers are not built by this library", ERR_UNSUPPORTED);
+            s.v[0] = 6;
+            s.v[1] = pair_entry(l, "input_hw", 0, where);
+            s.v[2] = pair_entry(l, "input_hw", 1, where);
+            s.v[3] = int_member(l, "in_planes", where);
+            s.v[4] = int_member(l, "out_planes", where);
+            s.v[5] = int_member(l, "num_blocks", where);
+            s.v[6] = int_member(l, "stride", where);
+            s.v[7] = int_member(l, "expansion", where);
         } else {
             model_error(where + ": unknown layer type \"" + t.str + "\"");
         }
@@ -338,14 +345,14 @@ void save_snapshot(NeuralNetwork &net, const std::string &path) {
     }
     const int fp = ctx().frac_bits;
     for (auto &l : net.layers)
-        for (auto &t : l->snapshotTensors()) save_share(path + "/" + t.first + std::to_string(l->layerNum), *t.second, fp);
+        for (auto &t : l->snapshotTensors()) save_share(path + "/" + t.first, *t.second, fp);
 }
 
 void load_snapshot(NeuralNetwork &net, const std::string &path) {
     const int fp = ctx().frac_bits;
     for (auto &l : net.layers) {
         auto tensors = l->snapshotTensors();
-        for (auto &t : tensors) load_share(path + "/" + t.first + std::to_string(l->layerNum), *t.second, fp);
+        for (auto &t : tensors) load_share(path + "/" + t.first, *t.second, fp);
         if (!tensors.empty()) l->weightsChanged();
     }
 }

@@ -18,8 +18,8 @@ struct ModelFile {
     std::vector<LayerSpec> layers;
 };
 
-// Parses a model file; throws PirError (ERR_ARG for malformed JSON or missing
-// fields, ERR_UNSUPPORTED for layer types this library does not build).
+// Parses a model file; throws PirError (ERR_ARG for malformed JSON, missing
+// fields or an unknown layer type).
 ModelFile load_model_file(const std::string &filename);
 
 // Writes path/weight{i} (and path/bias{i} for FC layers; path/gamma{i} and

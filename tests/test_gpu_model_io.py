@@ -112,8 +112,12 @@ def test_load_short_and_missing_files(tmp_path):
 
 
 def test_load_model_errors(tmp_path):
-    with pytest.raises(P.PirError, match="not built by this library"):
-        P.Network.from_model_file(os.path.join(MODELS, "resnet18-cifar10.json"), FP)
+    bottleneck = tmp_path / "res.json"  # ResLayer blocks with expansion 4 (the residual sum needs 1)
+    bottleneck.write_text('{"input_size": 48, "num_classes": 8, "batch_size": 1, "dataset": "x", "model": ['
+                          '{"layer": "res", "input_hw": [4, 4], "in_planes": 3, "out_planes": 2, "num_blocks": 1, '
+                          '"stride": 1, "expansion": 4}]}')
+    with pytest.raises(P.PirError, match="expansion"):
+        P.Network.from_model_file(str(bottleneck), FP)
     bad = tmp_path / "bad.json"
     bad.write_text('{"input_size": 4, "num_classes": 3, "batch_size": 1, "dataset": "x", "model": ['
                    '{"layer": "fc", "input_dim": 4, "output_dim": 3}, {"layer": "relu", "input_dim": 4}]}')

@@ -2,7 +2,7 @@
 the C ABI's parser (pir_model_read, host-only), checked against Python's own
 JSON reader on every model description the reference ships
 (tests/golden/models/, copied from files/models/), plus the error cases:
-malformed JSON, missing fields, and layer types this library does not build."""
+malformed JSON, missing fields and unknown layer types."""
 import json
 import os
 
@@ -12,7 +12,6 @@ import piranha_amd as P
 from piranha_amd import configs as C
 
 MODELS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models")
-UNSUPPORTED = {"res"}
 
 
 def model_names():
@@ -31,11 +30,6 @@ def test_reads_reference_model(name):
         doc = python_doc(name)
     except json.JSONDecodeError:
         with pytest.raises(P.PirError, match="model file"):  # files/models/resnet18-imagenet.json is not JSON
-            P.read_model(path)
-        return
-    kinds = {l["layer"] for l in doc["model"]}
-    if kinds & UNSUPPORTED:
-        with pytest.raises(P.PirError, match="not built by this library"):
             P.read_model(path)
         return
     ins, ncl, bs, spec = P.read_model(path)

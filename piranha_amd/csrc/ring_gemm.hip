@@ -443,6 +443,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     constexpr int KG = LIMB_BK / KPT;  // threads per row for staging
     constexpr int PLANE = TM * LDS_ROW;
     constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
+    constexpr bool PRE_A = AMODE >= 2 && AMODE <= 5;  // measured: faster for fprop / dgrad, slower otherwise
     __shared__ __attribute__((aligned(16))) int8_t As2[2][8 * PLANE];
     int cur = 0;
 
@@ -488,6 +489,17 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int ks = ks_begin; ks < ks_end; ++ks) {
+        // fprop / dgrad forms: all eight A limb planes of this K-step are read
+        // from LDS before the next step's global loads issue, so the MFMA run
+        // below waits on no LDS read (the dense and wgrad forms stream them one
+        // plane ahead, which measures faster for them)
+        v4i ap[8];
+        if (PRE_A) {
+            const int8_t *As = As2[cur];
+#pragma unroll
+            for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         // prefetch the next K-step into registers (the last step re-reads itself)
         const bool more = ks + 1 < ks_end;
         if (more) L.advance();
@@ -496,20 +508,30 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
 #pragma unroll
         for (int j = 0; j < 8; j++) bn[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
 
-        // acc[i+j] += A_i . B_j for i + j < 8, A planes streamed from LDS one at a time
-        const int8_t *As = As2[cur];
-        v4i av = *(const v4i *)(As + a_off);
-        __builtin_amdgcn_s_setprio(1);
+        // acc[i+j] += A_i . B_j for i + j < 8
+        if (PRE_A) {
+            __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            v4i an = av;
-            if (i < 7) an = *(const v4i *)(As + (i + 1) * PLANE + a_off);
+            for (int i = 0; i < 8; i++)
 #pragma unroll
-            for (int j = 0; j < 8 - i; j++)
-                acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv[j], acc[i + j], 0, 0, 0);
-            av = an;
+                for (int j = 0; j < 8 - i; j++)
+                    acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bv[j], acc[i + j], 0, 0, 0);
+            __builtin_amdgcn_s_setprio(0);
+        } else {  // A planes streamed from LDS one at a time
+            const int8_t *As = As2[cur];
+            v4i av = *(const v4i *)(As + a_off);
+            __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                v4i an = av;
+                if (i < 7) an = *(const v4i *)(As + (i + 1) * PLANE + a_off);
+#pragma unroll
+                for (int j = 0; j < 8 - i; j++)
+                    acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv[j], acc[i + j], 0, 0, 0);
+                av = an;
+            }
+            __builtin_amdgcn_s_setprio(0);
         }
-        __builtin_amdgcn_s_setprio(0);
         // the other buffer was last read in the previous step, before the barrier
         stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, w);
         cur ^= 1;

@@ -110,6 +110,14 @@ int pir_averagepool_im2row(const uint64_t *im, uint64_t *out, int image_width, i
  * out_capacity (entries of out) must be at least n_in * pool_size */
 int pir_averagepool_expand_delta(const uint64_t *in, size_t n_in, uint64_t *out, size_t out_capacity, int Din,
                                  int pool_size, void *stream);
+/* gpu::transpose, matrix.cuh:43-59, 202-221: b[row*cols + col] = a[col*rows + row];
+ * elem_bytes 8 or 1 */
+int pir_transpose(const void *a, void *b, int rows, int cols, int elem_bytes, void *stream);
+/* gpu::getMSBs, bitwise.cuh:23-30, 126-152: msbs[i] = bits[i*bit_width + bit_width - 1]
+ * for i < n_msbs; gpu::setMSBs, bitwise.cuh:32-39, 154-182: that element of each of
+ * the n_bits / bit_width groups = val.  elem_bytes 8 or 1 */
+int pir_get_msbs(const void *bits, void *msbs, size_t n_msbs, size_t bit_width, int elem_bytes, void *stream);
+int pir_set_msbs(void *bits, size_t n_bits, uint64_t val, size_t bit_width, int elem_bytes, void *stream);
 /* gpu::reduceSum, matrix.cuh:109-126, 331-351: b[j] += sum_i (reduce_rows ?
  * a[cols*i + j] (j < cols) : a[cols*j + i] (j < rows)) */
 int pir_reduce_sum(const uint64_t *a, uint64_t *b, int reduce_rows, int rows, int cols, void *stream);

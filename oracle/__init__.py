@@ -103,6 +103,9 @@ _SIGS = {
     "ref_ln_forward": [_u64p] * 6 + [_sz, _i, _i],
     "ref_ln_backward": [_u64p] * 6 + [_sz, _i, _i, _i],
     "ref_inv_sqrt_fixed": [_u64, _i],
+    "ref_transpose_u64": [_u64p, _u64p, _i, _i],
+    "ref_get_msbs_u64": [_u64p, _u64p, _sz, _sz],
+    "ref_set_msbs_u64": [_u64p, _sz, _u64, _sz],
 }
 _RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_averagepool_im2row": _sz,
         "ref_inv_sqrt_fixed": _u64,
@@ -489,6 +492,27 @@ def ln_backward(delta, xhat, inv_sigma, gamma, beta, num_batches, fp, lr):
     deltas = np.zeros((3, 2, n), np.uint64)
     lib().ref_ln_backward(_p(delta), _p(xhat), _p(inv_sigma), _p(gamma), _p(beta), _p(deltas), n, num_batches, fp, lr)
     return deltas, xhat, gamma, beta
+
+
+def transpose(a, rows, cols):
+    """gpu::transpose (matrix.cuh:43-59) of one array."""
+    a = np.ascontiguousarray(a, np.uint64)
+    b = np.zeros(rows * cols, np.uint64)
+    lib().ref_transpose_u64(_p(a), _p(b), rows, cols)
+    return b
+
+
+def get_msbs(bits, n, bit_width):
+    bits = np.ascontiguousarray(bits, np.uint64)
+    out = np.zeros(n, np.uint64)
+    lib().ref_get_msbs_u64(_p(bits), _p(out), n, bit_width)
+    return out
+
+
+def set_msbs(bits, val, bit_width):
+    bits = np.ascontiguousarray(bits, np.uint64).copy()
+    lib().ref_set_msbs_u64(_p(bits), bits.size, int(val), bit_width)
+    return bits
 
 
 def inv_sqrt_fixed(x, fp):

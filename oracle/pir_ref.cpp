@@ -1282,6 +1282,23 @@ void avgpool_backward(Rs<u64> delta, Rs<u64> deltas, int C, int pool) {
     divide_public(deltas, (u64)pool * pool);
 }
 
+// gpu::transpose kernel, matrix.cuh:43-59: b[ROW*cols + COL] = a[COL*rows + ROW].
+template <typename T>
+void transpose_mat(const T *a, T *b, int rows, int cols) {
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) b[(size_t)r * cols + c] = a[(size_t)c * rows + r];
+}
+
+// gpu::getMSBs / setMSBs kernels, bitwise.cuh:23-39.
+template <typename T>
+void get_msbs(const T *bits, T *msbs, size_t n, size_t bw) {
+    for (size_t i = 0; i < n; i++) msbs[i] = bits[i * bw + (bw - 1)];
+}
+template <typename T>
+void set_msbs(T *bits, size_t n_bits, T v, size_t bw) {
+    for (size_t i = 0; i < n_bits / bw; i++) bits[i * bw + (bw - 1)] = v;
+}
+
 // gpu::elementVectorSubtract, matrix.cuh:97-107: a[row*cols + col] -= b[rowwise ? col : row].
 template <typename T>
 void element_vector_subtract(T *a, const T *b, bool rowwise, int rows, int cols) {
@@ -1763,6 +1780,9 @@ void ref_ln_backward(const u64 *delta, u64 *xhat, const u64 *inv_sigma, u64 *gam
                 wrapRs(gamma, g), wrapRs(beta, g), wrapRs(deltas, n), num_batches, fp, lr);
 }
 u64 ref_inv_sqrt_fixed(u64 x, int fp) { return inv_sqrt_fixed(x, fp); }
+void ref_transpose_u64(const u64 *a, u64 *b, int rows, int cols) { transpose_mat(a, b, rows, cols); }
+void ref_get_msbs_u64(const u64 *bits, u64 *msbs, size_t n, size_t bw) { get_msbs(bits, msbs, n, bw); }
+void ref_set_msbs_u64(u64 *bits, size_t n_bits, u64 v, size_t bw) { set_msbs(bits, n_bits, v, bw); }
 void ref_reveal_softmax_grad(const u64 *act, const double *labels, size_t n, u64 *deltas, int nClasses, int fp,
                              int logMiniBatch) {
     reveal_softmax_grad(wrapRs((u64 *)act, n), labels, wrapRs(deltas, n), nClasses, fp, logMiniBatch);

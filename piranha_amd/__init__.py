@@ -65,6 +65,9 @@ _SIGS = {
     "pir_maxpool_im2row_size": ([_i] * 7, _sz),
     "pir_maxpool_im2row": ([_vp, _vp] + [_i] * 7 + [_u64, _vp], _i),
     "pir_averagepool_im2row_size": ([_i] * 7, _sz),
+    "pir_transpose": ([_vp, _vp, _i, _i, _i, _vp], _i),
+    "pir_get_msbs": ([_vp, _vp, _sz, _sz, _i, _vp], _i),
+    "pir_set_msbs": ([_vp, _sz, _u64, _sz, _i, _vp], _i),
     "pir_averagepool_im2row": ([_vp, _vp] + [_i] * 7 + [_vp], _i),
     "pir_averagepool_expand_delta": ([_vp, _sz, _vp, _sz, _i, _i, _vp], _i),
     "pir_dd_op": ([_i, _i, _vp, _ll, _vp, _ll, _u64, _sz, _vp], _i),

@@ -345,6 +345,15 @@ int pir_maxpool_im2row(const uint64_t *im, uint64_t *out, int H, int W, int f, i
                        int padding, uint64_t pv, void *stream) {
     return maxpool_im2row(im, out, H, W, f, Din, batch, stride, padding, pv, (hipStream_t)stream);
 }
+int pir_transpose(const void *a, void *b, int rows, int cols, int elem_bytes, void *stream) {
+    return transpose(a, b, rows, cols, elem_bytes, (hipStream_t)stream);
+}
+int pir_get_msbs(const void *bits, void *msbs, size_t n_msbs, size_t bit_width, int elem_bytes, void *stream) {
+    return get_msbs(bits, msbs, n_msbs, bit_width, elem_bytes, (hipStream_t)stream);
+}
+int pir_set_msbs(void *bits, size_t n_bits, uint64_t val, size_t bit_width, int elem_bytes, void *stream) {
+    return set_msbs(bits, n_bits, val, bit_width, elem_bytes, (hipStream_t)stream);
+}
 size_t pir_averagepool_im2row_size(int image_width, int image_height, int f, int Din, int batch, int stride,
                                    int padding) {
     return averagepool_im2row_size(image_width, image_height, f, Din, batch, stride, padding);

@@ -30,6 +30,12 @@ int bitexpand(const u64 *a, size_t n, void *bits, int elem_bytes, hipStream_t s)
 int set_carry_out_msb(void *rbits, void *abits, void *msb, size_t n, int bw, int xr, int eb, hipStream_t s);
 int expand_compare(const void *b, const void *inv, void *out, size_t n, size_t out_n, int eb, hipStream_t s);
 int element_vector_add(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s);
+// gpu::transpose (matrix.cuh:43-59, 202-221): b[row*cols + col] = a[col*rows + row]
+int transpose(const void *a, void *b, int rows, int cols, int elem_bytes, hipStream_t s);
+// gpu::getMSBs / setMSBs (bitwise.cuh:23-39, 126-182): element bit_width - 1 of
+// each group of bit_width elements
+int get_msbs(const void *bits, void *msbs, size_t n_msbs, size_t bit_width, int elem_bytes, hipStream_t s);
+int set_msbs(void *bits, size_t n_bits, u64 val, size_t bit_width, int elem_bytes, hipStream_t s);
 // gpu::elementVectorSubtract (matrix.cuh:97-107): a[row*cols + col] -= b[rowwise ? col : row]
 int element_vector_subtract(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s);
 int truncate_cols(const void *a, size_t a_n, void *b, int sc, int ec, int eb, hipStream_t s);

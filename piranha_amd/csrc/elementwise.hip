@@ -186,6 +186,58 @@ int element_vector_add(u64 *a, const u64 *b, int rowwise, int rows, int cols, hi
     return OK;
 }
 
+// matrix.cuh:43-59 (transpose): b[row*cols + col] = a[col*rows + row]; one
+// thread per output element, so the writes are contiguous
+template <typename T>
+__global__ __launch_bounds__(BLK) void transpose_kernel(const T *a, T *b, int rows, int cols) {
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)rows * cols) return;
+    size_t row = t / (size_t)cols, col = t % (size_t)cols;
+    b[t] = a[col * (size_t)rows + row];
+}
+
+int transpose(const void *a, void *b, int rows, int cols, int eb, hipStream_t s) {
+    PIR_REQUIRE(rows >= 0 && cols >= 0 && (eb == 1 || eb == 8), "transpose: invalid arguments");
+    size_t n = (size_t)rows * cols;
+    if (n == 0) return OK;
+    if (eb == 8) hipLaunchKernelGGL(transpose_kernel<u64>, dim3(g1(n)), dim3(BLK), 0, s, (const u64 *)a, (u64 *)b, rows, cols);
+    else hipLaunchKernelGGL(transpose_kernel<u8>, dim3(g1(n)), dim3(BLK), 0, s, (const u8 *)a, (u8 *)b, rows, cols);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// bitwise.cuh:23-39 (getMSBs / setMSBs): element bitWidth - 1 of every group
+// of bitWidth is the most significant bit of one value
+template <typename T>
+__global__ __launch_bounds__(BLK) void get_msbs_kernel(const T *bits, T *msbs, size_t n, size_t bw) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) msbs[i] = bits[i * bw + (bw - 1)];
+}
+template <typename T>
+__global__ __launch_bounds__(BLK) void set_msbs_kernel(T *bits, T v, size_t n, size_t bw) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) bits[i * bw + (bw - 1)] = v;
+}
+
+int get_msbs(const void *bits, void *msbs, size_t n_msbs, size_t bit_width, int eb, hipStream_t s) {
+    PIR_REQUIRE(bit_width >= 1 && (eb == 1 || eb == 8), "getMSBs: invalid arguments");
+    if (n_msbs == 0) return OK;
+    if (eb == 8) hipLaunchKernelGGL(get_msbs_kernel<u64>, dim3(g1(n_msbs)), dim3(BLK), 0, s, (const u64 *)bits, (u64 *)msbs, n_msbs, bit_width);
+    else hipLaunchKernelGGL(get_msbs_kernel<u8>, dim3(g1(n_msbs)), dim3(BLK), 0, s, (const u8 *)bits, (u8 *)msbs, n_msbs, bit_width);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+int set_msbs(void *bits, size_t n_bits, u64 val, size_t bit_width, int eb, hipStream_t s) {
+    PIR_REQUIRE(bit_width >= 1 && (eb == 1 || eb == 8), "setMSBs: invalid arguments");
+    size_t n = n_bits / bit_width;  // bits.size() / bitWidth, bitwise.cuh:156
+    if (n == 0) return OK;
+    if (eb == 8) hipLaunchKernelGGL(set_msbs_kernel<u64>, dim3(g1(n)), dim3(BLK), 0, s, (u64 *)bits, (u64)val, n, bit_width);
+    else hipLaunchKernelGGL(set_msbs_kernel<u8>, dim3(g1(n)), dim3(BLK), 0, s, (u8 *)bits, (u8)val, n, bit_width);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 // matrix.cuh:97-107 (elementVectorSubtract): a[row*cols + col] -= b[rowwise ? col : row]
 __global__ __launch_bounds__(BLK) void evsub_kernel(u64 *a, const u64 *b, int rowwise, int rows, int cols) {
     size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;

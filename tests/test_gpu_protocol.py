@@ -205,3 +205,39 @@ def test_maxpool(rng, mode, kind, k, dt):
     assert np.array_equal(dr.numpy(), r)
     assert np.array_equal(dd.numpy(), d)
     assert np.array_equal(dp.numpy(), p_after)
+
+
+# gpu::transpose (matrix.cuh:43-59) and getMSBs / setMSBs (bitwise.cuh:23-39)
+
+def test_transpose_kat():
+    a = P.to_dev(np.array([1, 4, 2, 5, 3, 6], np.uint64))
+    b = P.to_dev(np.zeros(6, np.uint64))
+    P.check(P.lib().pir_transpose(a.ptr, b.ptr, 2, 3, 8, None))
+    assert b.numpy().tolist() == [1, 2, 3, 4, 5, 6]  # GPU.cu:247-261
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 7), (5, 9), (300, 257)])
+@pytest.mark.parametrize("eb", [8, 1])
+def test_transpose(rng, rows, cols, eb):
+    a = rand_u64(rng, rows * cols)
+    if eb == 1:
+        a = (a & np.uint64(0xFF)).astype(np.uint8)
+    da, db = P.to_dev(a), P.to_dev(np.zeros_like(a))
+    P.check(P.lib().pir_transpose(da.ptr, db.ptr, rows, cols, eb, None))
+    want = O.transpose(a.astype(np.uint64), rows, cols)
+    assert np.array_equal(db.numpy().astype(np.uint64), want)
+
+
+@pytest.mark.parametrize("bw", [1, 8, 64])
+@pytest.mark.parametrize("eb", [8, 1])
+def test_get_set_msbs(rng, bw, eb):
+    n = 1001
+    bits = rand_u64(rng, n * bw + 3)
+    if eb == 1:
+        bits = (bits & np.uint64(1)).astype(np.uint8)
+    dt = np.uint64 if eb == 8 else np.uint8
+    db, dm = P.to_dev(bits), P.to_dev(np.zeros(n, dt))
+    P.check(P.lib().pir_get_msbs(db.ptr, dm.ptr, n, bw, eb, None))
+    assert np.array_equal(dm.numpy().astype(np.uint64), O.get_msbs(bits.astype(np.uint64), n, bw))
+    P.check(P.lib().pir_set_msbs(db.ptr, bits.size, 1, bw, eb, None))
+    assert np.array_equal(db.numpy().astype(np.uint64), O.set_msbs(bits.astype(np.uint64), 1, bw))

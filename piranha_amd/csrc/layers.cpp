@@ -896,20 +896,19 @@ void HostStage::run(void *self) {
 // (to_double_functor, functors.h:105-114), exp summed in element order per row,
 // each value divided by its row's sum, back with (uint64)(x * 2^fp)
 // (to_fixed_functor, functors.h:116-123).
+// Runs on the runtime's callback thread, so it allocates nothing: exp is
+// evaluated twice per element (row-sum pass, output pass), the same function
+// of the same input both times.
 void NeuralNetwork::SoftmaxStage::compute(const u64 *in, u64 *out, size_t n) const {
     const double scale = (double)(1 << fp);
     const size_t per_view = n / (size_t)views;
-    std::vector<double> e((size_t)nc);
     for (int v = 0; v < views; v++)
         for (int r = 0; r < rows; r++) {
             const size_t base = (size_t)v * per_view + (size_t)r * nc;
             double sum = 0.0;
+            for (int i = 0; i < nc; i++) sum += std::exp((double)(s64)in[base + i] / scale);
             for (int i = 0; i < nc; i++) {
-                e[i] = std::exp((double)(s64)in[base + i] / scale);
-                sum += e[i];
-            }
-            for (int i = 0; i < nc; i++) {
-                double q = e[i] / sum;
+                double q = std::exp((double)(s64)in[base + i] / scale) / sum;
                 out[base + i] = to_u64_clamped(q * scale);
             }
         }

@@ -490,6 +490,10 @@ void LNLayer::InvSqrtStage::compute(const u64 *in, u64 *out, size_t n) const {
 // LNLayer.cu:66-162
 void LNLayer::forward(const RSS64 &input) {
     const int fp = ctx().frac_bits;
+    if (fp < 8) {  // epsilon = 1 << (FLOAT_PRECISION - 8), LNLayer.cu:119
+        set_error("LN layer: needs at least 8 fractional bits");
+        throw PirError(ERR_UNSUPPORTED);
+    }
     const int nb = (int)conf.numBatches;
     const size_t n = input.size();
     const int bs = (int)(n / (size_t)nb);

@@ -141,6 +141,11 @@ class JsonReader {
     JValue value() {
         space();
         if (i_ >= s_.size()) fail("unexpected end of the document");
+        if (++depth_ > kMaxDepth) fail("nesting deeper than 64 levels");
+        struct Leave {
+            int &d;
+            ~Leave() { --d; }
+        } leave{depth_};
         JValue v;
         char c = s_[i_];
         if (c == '{') {
@@ -187,8 +192,10 @@ class JsonReader {
         return v;
     }
 
+    static constexpr int kMaxDepth = 64;
     const std::string &s_;
     size_t i_ = 0;
+    int depth_ = 0;
 };
 
 [[noreturn]] void model_error(const std::string &msg, int code = ERR_ARG) {

@@ -174,3 +174,10 @@ def test_snapshot_gamma_beta(tmp_path, rng):
     for k, name in enumerate(("gamma0", "beta0")):
         vals = np.array([float(t) for t in (tmp_path / name).read_text().split()])
         assert np.array_equal(fresh.weights(0)[k], O.set_public(vals, fp))
+
+
+def test_needs_eight_fractional_bits():
+    # epsilon = 1 << (FLOAT_PRECISION - 8) (LNLayer.cu:119) needs FP >= 8
+    net = P.Network([(P.LAYER_LN, 8)], 1, 5, seed=1)
+    with pytest.raises(P.PirError, match="8 fractional bits"):
+        net.forward(np.zeros(8))

@@ -62,6 +62,7 @@ def test_formatting_and_escapes(tmp_path):
      "input_dim"),
     ('{"input_size": 1, "num_classes": 1, "batch_size": 1, "dataset": "x", "model": [{"layer": "bn"}]}',
      "unknown layer type"),
+    ('{"input_size": 1, "model": ' + "[" * 100 + "]" * 100 + "}", "nesting"),
     ('{"input_size": 1, "num_classes": 1, "batch_size": 1, "dataset": "x", "model": [{"layer": "cnn", '
      '"input_hw": [4], "in_channels": 1, "out_channels": 1, "filter_hw": [3, 3], "stride": 1, "padding": 0}]}',
      "not a pair"),

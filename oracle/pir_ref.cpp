@@ -166,8 +166,11 @@ inline T apply_op(Op op, T x, T y) {
         case MUL: return (T)(x * y);
         case XOR: return (T)(x ^ y);
         case AND: return (T)(x & y);
-        case SHR: return (T)((S)x >> y);  // arithmetic shift, functors.cuh:46-59
-        case SHL: return (T)(x << y);
+        // arithmetic shift, functors.cuh:46-59 and :86-92, as the reference's GPU
+        // executes them: an amount >= the width clamps (>> fills with the sign,
+        // << gives 0), where x86 would take it mod 64
+        case SHR: return y >= (T)(8 * sizeof(T)) ? (T)((S)x < 0 ? ~(T)0 : (T)0) : (T)((S)x >> y);
+        case SHL: return y >= (T)(8 * sizeof(T)) ? (T)0 : (T)(x << y);
         case DIV: return sdiv<T>(x, y);
     }
     return x;

@@ -23,8 +23,11 @@ __device__ __forceinline__ T ew_apply(int op, T x, T y) {
         case OP_MUL: return (T)(x * y);
         case OP_XOR: return (T)(x ^ y);
         case OP_AND: return (T)(x & y);
-        case OP_SHR: return (T)((S)x >> y);
-        case OP_SHL: return (T)(x << y);
+        // functors.cuh:46-92 evaluated on the reference's GPU, whose shifts
+        // clamp the amount to the register width: >> by >= width fills with the
+        // sign, << by >= width gives 0 (this unit masks the amount instead)
+        case OP_SHR: return y >= (T)(8 * sizeof(T)) ? (T)((S)x < 0 ? ~(T)0 : (T)0) : (T)((S)x >> y);
+        case OP_SHL: return y >= (T)(8 * sizeof(T)) ? (T)0 : (T)(x << y);
         case OP_DIV: {
             // functors.cuh:33-45 / 94-105: (S)((double)(S)x / (double)(S)y)
             if (sizeof(T) == 8) return (T)signed_div_u64((u64)x, (u64)y);

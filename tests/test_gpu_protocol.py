@@ -241,3 +241,16 @@ def test_get_set_msbs(rng, bw, eb):
     assert np.array_equal(dm.numpy().astype(np.uint64), O.get_msbs(bits.astype(np.uint64), n, bw))
     P.check(P.lib().pir_set_msbs(db.ptr, bits.size, 1, bw, eb, None))
     assert np.array_equal(db.numpy().astype(np.uint64), O.set_msbs(bits.astype(np.uint64), 1, bw))
+
+
+@pytest.mark.parametrize("op", [P.OP_SHL, P.OP_SHR])
+def test_dd_op_shift_clamps(rng, op):
+    # amounts >= 64 clamp as on the reference's GPU (the oracle's rule; pinned there)
+    amounts = np.array([0, 1, 7, 31, 32, 63, 64, 65, 127, 200, 1 << 32, 1 << 63, (1 << 64) - 1], np.uint64)
+    n = amounts.size * 50
+    x = rand_u64(rng, n)
+    y = np.tile(amounts, 50)
+    dx, dy = P.to_dev(x), P.to_dev(y)
+    P.check(P.lib().pir_dd_op(op, 8, dx.ptr, 1, dy.ptr, 1, 0, n, None))
+    want = O.op_local(np.broadcast_to(x, (3, 2, n)), np.broadcast_to(y, (3, n)), 6 if op == P.OP_SHL else 5)[0, 0]
+    assert np.array_equal(dx.numpy(), want)

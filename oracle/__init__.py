@@ -104,6 +104,8 @@ _SIGS = {
     "ref_ln_backward": [_u64p] * 6 + [_sz, _i, _i, _i],
     "ref_inv_sqrt_fixed": [_u64, _i],
     "ref_transpose_u64": [_u64p, _u64p, _i, _i],
+    "ref_get_powers": [_u64p, _sz, _u64p],
+    "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
     "ref_get_msbs_u64": [_u64p, _u64p, _sz, _sz],
     "ref_set_msbs_u64": [_u64p, _sz, _u64, _sz],
 }
@@ -513,6 +515,26 @@ def set_msbs(bits, val, bit_width):
     bits = np.ascontiguousarray(bits, np.uint64).copy()
     lib().ref_set_msbs_u64(_p(bits), bits.size, int(val), bit_width)
     return bits
+
+
+def get_powers(x):
+    """getPowers (RSS.inl:1211-1235): opened [3][n] floor(log2) of each value."""
+    x = _u(x)
+    n = x.shape[-1]
+    pw = np.zeros((3, n), np.uint64)
+    lib().ref_get_powers(_p(x), n, _p(pw))
+    return pw
+
+
+TAYLOR_SQRT, TAYLOR_INVERSE, TAYLOR_SIGMOID = 0, 1, 2
+
+
+def taylor(x, fn, fp):
+    """sqrt / inverse / sigmoid of RSS shares (RSS.inl:570-612, taylorSeries :1237-1320)."""
+    x = _u(x)
+    out = np.zeros_like(x)
+    lib().ref_taylor(_p(x), x.shape[-1], _p(out), fn, fp)
+    return out
 
 
 def inv_sqrt_fixed(x, fp):

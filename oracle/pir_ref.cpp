@@ -724,6 +724,105 @@ void select_share(Rs<u64> x, Rs<u64> y, Rs<U> b, Rs<u64> z) {
     r_op_r(z, result.v(), ADD);
 }
 
+// getPowers, RSS.inl:1211-1235: for every bit b, powers takes the value b
+// where in - 2^b >= 0 (dReLU, selectShare); the result is opened.
+void get_powers(Rs<u64> in, Loc<u64> pow) {
+    const size_t n = in.size();
+    RsBuf<u64> powers(pow.size()), cur(n), diff(n);
+    RsBuf<uint8_t> cmp(n);
+    for (int bit = 0; bit < 64; bit++) {
+        r_fill(cur.v(), (u64)bit);
+        r_zero(diff.v());
+        r_op_r(diff.v(), in, ADD);
+        r_op_s(diff.v(), (u64)1 << bit, SUB);
+        r_zero(cmp.v());
+        drelu(diff.v(), cmp.v());
+        select_share(powers.v(), cur.v(), cmp.v(), powers.v());
+    }
+    reconstruct(powers.v(), pow);
+}
+
+// The coefficient functors of taylorSeries as the reference's GPU evaluates
+// them.  `1 << x` is a 32-bit shift whose amount clamps at 32 (so 1 << 31 is
+// INT_MIN and larger amounts give 0); double / float -> integer conversions
+// clamp to the destination range with NaN -> 0.
+inline int int_pow2(u64 x) { return x >= 32 ? 0 : (int)((unsigned)1 << x); }
+inline u64 clamp_s64(double v) {
+    if (v != v) return 0;
+    if (v >= 9223372036854775808.0) return (u64)INT64_MAX;
+    if (v < -9223372036854775808.0) return (u64)INT64_MIN;
+    return (u64)(s64)v;
+}
+inline u64 clamp_u64(float v) {
+    if (!(v > 0.0f)) return 0;
+    if (v >= 18446744073709551616.0f) return ~(u64)0;
+    return (u64)v;
+}
+// tofixed_variable_precision_functor, functors.cuh:107-116: (T)(S)(a * (1 << x))
+inline u64 tofixed_vp(double a, u64 x) { return clamp_s64(a * (double)int_pow2(x)); }
+// calc_fn, functors.h:22-31: val = 2^(x - FP) in float, (T)(fn(val) * (1 << x));
+// fn 0 = sqrt, 1 = 1/x, 2 = sigmoid 1 / (1 + exp(-x)), all in float
+inline u64 calc_fn(int fn, u64 x, int fp) {
+    // pow(2, float(x) - FP): an exact power of two, +infinity past the float range
+    float val = x > (u64)(fp + 127) ? INFINITY : std::ldexp(1.0f, (int)x - fp);
+    float f = fn == 0 ? std::sqrt(val) : fn == 1 ? 1.0f / val : 1.0f / (1.0f + std::exp(-val));
+    return clamp_u64(f * (float)int_pow2(x));
+}
+
+// taylorSeries, RSS.inl:1237-1320, with every party using its own opened powers.
+void taylor_series(Rs<u64> in, Rs<u64> out, double a0, double a1, double a2, int fn, int fp) {
+    const size_t n = in.size();
+    r_zero(out);
+    RsBuf<u64> scratch(n);
+    LocBuf<u64> pow(n), ones(n), coef(n);
+    get_powers(in, pow.v());
+    for (int q = 0; q < 3; q++)
+        for (auto &v : pow.d[q]) v += 1;
+    auto fill_ones = [&](const LocBuf<u64> &sh) {
+        for (int q = 0; q < 3; q++)
+            for (size_t i = 0; i < n; i++) ones.d[q][i] = apply_op<u64>(SHL, 1, sh.d[q][i]);
+    };
+    fill_ones(pow);
+    auto coeffs = [&](double a) {
+        for (int q = 0; q < 3; q++)
+            for (size_t i = 0; i < n; i++) coef.d[q][i] = tofixed_vp(a, pow.d[q][i]);
+    };
+    if (a2 != 0.0) {
+        coeffs(a2);
+        r_zero(scratch.v());
+        r_op_r(scratch.v(), in, ADD);
+        r_mult(scratch.v(), in);
+        divide_public_vec(scratch.v(), ones.v());
+        r_op_l(scratch.v(), coef.v(), MUL);
+        divide_public_vec(scratch.v(), ones.v());
+        r_op_r(out, scratch.v(), ADD);
+    }
+    if (a1 != 0.0) {
+        coeffs(a1);
+        r_zero(scratch.v());
+        r_op_r(scratch.v(), in, ADD);
+        r_op_l(scratch.v(), coef.v(), MUL);
+        divide_public_vec(scratch.v(), ones.v());
+        r_op_r(out, scratch.v(), ADD);
+    }
+    coeffs(a0);
+    r_op_l(out, coef.v(), ADD);
+    for (int q = 0; q < 3; q++)
+        for (size_t i = 0; i < n; i++) coef.d[q][i] = calc_fn(fn, pow.d[q][i], fp);
+    r_op_l(out, coef.v(), MUL);
+    divide_public_vec(out, ones.v());
+    LocBuf<u64> posPow(n), negPow(n);
+    for (int q = 0; q < 3; q++)
+        for (size_t i = 0; i < n; i++) {
+            s64 pw = (s64)(pow.d[q][i] - (u64)fp);
+            posPow.d[q][i] = pw >= 0 ? (u64)pw : 0;
+            negPow.d[q][i] = pw < 0 ? (u64)(-pw) : 0;
+        }
+    fill_ones(posPow);
+    divide_public_vec(out, ones.v());
+    r_op_l(out, negPow.v(), SHL);  // both shares, lshift_functor
+}
+
 // ReLU, RSS.inl:854-865
 template <typename U>
 void relu(Rs<u64> input, Rs<u64> result, Rs<U> dresult) {
@@ -1783,6 +1882,17 @@ void ref_ln_backward(const u64 *delta, u64 *xhat, const u64 *inv_sigma, u64 *gam
                 wrapRs(gamma, g), wrapRs(beta, g), wrapRs(deltas, n), num_batches, fp, lr);
 }
 u64 ref_inv_sqrt_fixed(u64 x, int fp) { return inv_sqrt_fixed(x, fp); }
+void ref_get_powers(const u64 *in, size_t n, u64 *pow) {
+    u64 *pp[3] = {pow, pow + n, pow + 2 * n};
+    Loc<u64> l;
+    for (int q = 0; q < 3; q++) l.p[q] = View<u64>{pp[q], 0, 1, n};
+    get_powers(wrapRs((u64 *)in, n), l);
+}
+// fn: 0 sqrt (RSS.inl:570-593), 1 inverse (:595-603), 2 sigmoid (:605-612)
+void ref_taylor(const u64 *in, size_t n, u64 *out, int fn, int fp) {
+    static const double c[3][3] = {{0.424, 0.584, 0.0}, {4.245, -5.857, 2.630}, {0.494286, 0.275589, -0.038751}};
+    taylor_series(wrapRs((u64 *)in, n), wrapRs(out, n), c[fn][0], c[fn][1], c[fn][2], fn, fp);
+}
 void ref_transpose_u64(const u64 *a, u64 *b, int rows, int cols) { transpose_mat(a, b, rows, cols); }
 void ref_get_msbs_u64(const u64 *bits, u64 *msbs, size_t n, size_t bw) { get_msbs(bits, msbs, n, bw); }
 void ref_set_msbs_u64(u64 *bits, size_t n_bits, u64 v, size_t bw) { set_msbs(bits, n_bits, v, bw); }

@@ -171,6 +171,19 @@ int pir_rss_select_share(const uint64_t *x, const uint64_t *y, const void *b, ui
                          void *stream);                                                    /* RSS.inl:542-569 */
 /* maxpool over gathered pools [n/k windows][k] (modified in place like the reference), RSS.inl:867-1017 */
 int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
+/* getPowers, RSS.inl:1211-1235: each party's opened power of x [3][2][n] -> pow [3][n] */
+int pir_rss_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream);
+/* taylorSeries, RSS.inl:1237-1320: a0 + a1 x + a2 x^2 on the range-reduced x,
+ * rescaled by fn (PIR_TAYLOR_SQRT / _INVERSE / _SIGMOID, the functors of
+ * functors.h:33-49) at fp fractional bits (the reference's FLOAT_PRECISION);
+ * out [3][2][n] must not overlap x */
+enum { PIR_TAYLOR_SQRT = 0, PIR_TAYLOR_INVERSE = 1, PIR_TAYLOR_SIGMOID = 2 };
+int pir_rss_taylor_series(const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn,
+                          int fp, void *stream);
+/* sqrt / inverse / sigmoid, RSS.inl:570-612: taylorSeries with the reference's coefficients */
+int pir_rss_sqrt(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_rss_inverse(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_rss_sigmoid(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
 
 /* ---- TPC (2 parties) and FPC (4 parties), src/mpc/TPC.inl, FPC.inl ------- */
 /* TPC value: [2 parties][n] (one share each).  FPC value: [4 parties][3 shares][n],

@@ -106,6 +106,7 @@ _SIGS = {
     "ref_transpose_u64": [_u64p, _u64p, _i, _i],
     "ref_get_powers": [_u64p, _sz, _u64p],
     "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
+    "ref_taylor_series": [_u64p, _sz, _u64p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _i],
     "ref_get_msbs_u64": [_u64p, _u64p, _sz, _sz],
     "ref_set_msbs_u64": [_u64p, _sz, _u64, _sz],
 }
@@ -527,6 +528,14 @@ def get_powers(x):
 
 
 TAYLOR_SQRT, TAYLOR_INVERSE, TAYLOR_SIGMOID = 0, 1, 2
+
+
+def taylor_series(x, a0, a1, a2, fn, fp):
+    """taylorSeries (RSS.inl:1237-1320) with explicit coefficients."""
+    x = _u(x)
+    out = np.zeros_like(x)
+    lib().ref_taylor_series(_p(x), x.shape[-1], _p(out), a0, a1, a2, fn, fp)
+    return out
 
 
 def taylor(x, fn, fp):

@@ -1889,6 +1889,9 @@ void ref_get_powers(const u64 *in, size_t n, u64 *pow) {
     get_powers(wrapRs((u64 *)in, n), l);
 }
 // fn: 0 sqrt (RSS.inl:570-593), 1 inverse (:595-603), 2 sigmoid (:605-612)
+void ref_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
+    taylor_series(wrapRs((u64 *)in, n), wrapRs(out, n), a0, a1, a2, fn, fp);
+}
 void ref_taylor(const u64 *in, size_t n, u64 *out, int fn, int fp) {
     static const double c[3][3] = {{0.424, 0.584, 0.0}, {4.245, -5.857, 2.630}, {0.494286, 0.275589, -0.038751}};
     taylor_series(wrapRs((u64 *)in, n), wrapRs(out, n), c[fn][0], c[fn][1], c[fn][2], fn, fp);

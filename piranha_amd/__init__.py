@@ -23,12 +23,14 @@ LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL, LAYER_LN, LAYER_R
 CONV_FPROP, CONV_DGRAD, CONV_WGRAD = 0, 1, 2
 LIMB_WIDE, LIMB_VARIANTS = 8, 16  # kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
 OP_ADD, OP_SUB, OP_MUL, OP_XOR, OP_AND, OP_SHR, OP_SHL, OP_DIV = range(8)
+TAYLOR_SQRT, TAYLOR_INVERSE, TAYLOR_SIGMOID = range(3)
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _sz = ctypes.c_size_t
 _u64 = ctypes.c_uint64
 _ll = ctypes.c_longlong
+_d = ctypes.c_double
 
 _SIGS = {
     "pir_version": ([], _i),
@@ -81,6 +83,11 @@ _SIGS = {
     "pir_rss_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
     "pir_rss_convolution_op": ([_i, _vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
     "pir_rss_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_rss_get_powers": ([_vp, _sz, _vp, _vp], _i),
+    "pir_rss_taylor_series": ([_vp, _sz, _vp, _d, _d, _d, _i, _i, _vp], _i),
+    "pir_rss_sqrt": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_rss_inverse": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_rss_sigmoid": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_rss_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),
     "pir_rss_select_share": ([_vp, _vp, _vp, _vp, _sz, _i, _vp], _i),
     "pir_rss_maxpool": ([_vp, _sz, _vp, _vp, _i, _i, _vp], _i),

@@ -670,6 +670,61 @@ int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, 
     });
 }
 
+namespace {
+bool overlaps(const void *a, size_t abytes, const void *b, size_t bbytes) {
+    const char *x = (const char *)a, *y = (const char *)b;
+    return x < y + bbytes && y < x + abytes;
+}
+}  // namespace
+
+int pir_rss_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(!overlaps(x, 6 * n * 8, pow, 3 * n * 8), "pir_rss_get_powers: pow must not overlap x");
+        RSS<u64> X = RSS<u64>::wrap((u64 *)x, n);
+        PartyData<u64> pw = PartyData<u64>::wrap(pow, n);
+        getPowers(X, pw);
+        return (int)OK;
+    });
+}
+
+int pir_rss_taylor_series(const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn,
+                          int fp, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(!overlaps(x, 6 * n * 8, out, 6 * n * 8), "pir_rss_taylor_series: out must not overlap x");
+        RSS<u64> X = RSS<u64>::wrap((u64 *)x, n), Y = RSS<u64>::wrap(out, n);
+        taylorSeries(X, Y, a0, a1, a2, fn, fp);
+        return (int)OK;
+    });
+}
+
+namespace {
+int taylor_named(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream, const char *what,
+                 void (*f)(const RSS<u64> &, RSS<u64> &, int)) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        if (overlaps(x, 6 * n * 8, out, 6 * n * 8)) {
+            set_error(std::string(what) + ": out must not overlap x");
+            return (int)ERR_ARG;
+        }
+        RSS<u64> X = RSS<u64>::wrap((u64 *)x, n), Y = RSS<u64>::wrap(out, n);
+        f(X, Y, fp);
+        return (int)OK;
+    });
+}
+}  // namespace
+
+int pir_rss_sqrt(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream) {
+    return taylor_named(x, n, out, fp, stream, "pir_rss_sqrt", rss_sqrt);
+}
+int pir_rss_inverse(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream) {
+    return taylor_named(x, n, out, fp, stream, "pir_rss_inverse", inverse);
+}
+int pir_rss_sigmoid(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream) {
+    return taylor_named(x, n, out, fp, stream, "pir_rss_sigmoid", sigmoid);
+}
+
 // ---- network -------------------------------------------------------------------
 
 namespace {

@@ -72,4 +72,24 @@ struct FastDiv {
     }
 };
 
+// 64-bit left shift with the amount clamped as the reference's GPU shifts
+// clamp it: x << y for y >= 64 is 0.
+__host__ __device__ __forceinline__ u64 shl_clamped(u64 x, u64 y) { return y >= 64 ? 0 : x << y; }
+
+// The coefficient functors of taylorSeries (RSS.inl:1237-1320) tabulated over
+// their argument x, the opened power plus one.  Both functors scale by the
+// 32-bit `1 << x`, which is 0 for every x >= 32, so entry TAYLOR_TAB - 1 holds
+// the value of every x >= 32 and a lookup clamps its index there.
+constexpr int TAYLOR_TAB = 33;
+struct TaylorTab {
+    u64 v[TAYLOR_TAB];
+    __host__ __device__ __forceinline__ u64 at(u64 x) const { return v[x < TAYLOR_TAB - 1 ? x : TAYLOR_TAB - 1]; }
+};
+struct TaylorTables {
+    TaylorTab c2, c1, c0;  // tofixed_variable_precision_functor(a2 / a1 / a0)
+    TaylorTab f;           // calc_fn(fn)
+    int fp;                // FLOAT_PRECISION
+    int use_a2, use_a1;    // a2 != 0, a1 != 0
+};
+
 }  // namespace pir

@@ -176,16 +176,26 @@ class PartyData {
         p_[2] = c;
         n_ = a.size();
     }
+    // View over an external contiguous [3][n] array.
+    static PartyData wrap(T *base, size_t n) {
+        PartyData r;
+        r.n_ = n;
+        for (int q = 0; q < 3; q++) r.p_[q] = DeviceData<T>(base + q * n, n);
+        r.ext_base_ = base;
+        return r;
+    }
     void resize(size_t n) {
         n_ = n;
         buf_ = DevBuf(3 * n * sizeof(T));
+        ext_base_ = nullptr;
         for (int q = 0; q < 3; q++) p_[q] = DeviceData<T>((T *)buf_.get() + q * n, n);
         zero();
     }
     size_t size() const { return n_; }
     DeviceData<T> &operator[](int q) { return p_[q]; }
     const DeviceData<T> &operator[](int q) const { return p_[q]; }
-    T *base() const { return buf_.get() ? (T *)buf_.get() : nullptr; }
+    // [3][n] base pointer when the three arrays are contiguous, else null.
+    T *base() const { return buf_.get() ? (T *)buf_.get() : ext_base_; }
     void zero() { fill((T)0); }
     void fill(T v) {
         MultiLaunch l(OP_SET);
@@ -214,6 +224,7 @@ class PartyData {
     DevBuf buf_;
     DeviceData<T> p_[3];
     size_t n_ = 0;
+    T *ext_base_ = nullptr;
 };
 
 // Fixed-point encode (RSS.inl:126, util.cuh:28-33): (T)(v * (1 << FP)); on

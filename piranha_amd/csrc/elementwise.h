@@ -38,6 +38,12 @@ int get_msbs(const void *bits, void *msbs, size_t n_msbs, size_t bit_width, int 
 int set_msbs(void *bits, size_t n_bits, u64 val, size_t bit_width, int elem_bytes, hipStream_t s);
 // gpu::elementVectorSubtract (matrix.cuh:97-107): a[row*cols + col] -= b[rowwise ? col : row]
 int element_vector_subtract(u64 *a, const u64 *b, int rowwise, int rows, int cols, hipStream_t s);
+// out[i] = tab.at(x[i]): a tabulated coefficient functor of taylorSeries
+// (functors.cuh:107-116, functors.h:22-31) applied through thrust::transform.
+int taylor_lookup(const u64 *x, size_t n, const TaylorTab &tab, u64 *out, hipStream_t s);
+// filter_positive_powers / filter_negative_powers (functors.h:7-20) on one array:
+// pos[i] = (S)x >= 0 ? x : 0, neg[i] = (S)x < 0 ? -(S)x : 0.
+int filter_powers(const u64 *x, size_t n, u64 *pos, u64 *neg, hipStream_t s);
 int truncate_cols(const void *a, size_t a_n, void *b, int sc, int ec, int eb, hipStream_t s);
 int stride_pad(const u64 *in, u64 *out, size_t n_in, int stride, int pad, u64 pv, hipStream_t s);
 size_t maxpool_im2row_size(int H, int W, int f, int Din, int batch, int stride, int padding);

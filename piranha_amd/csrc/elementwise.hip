@@ -241,6 +241,35 @@ int set_msbs(void *bits, size_t n_bits, u64 val, size_t bit_width, int eb, hipSt
     return OK;
 }
 
+__global__ __launch_bounds__(BLK) void taylor_lookup_kernel(const u64 *__restrict__ x, size_t n, const TaylorTab tab,
+                                                             u64 *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) out[i] = tab.at(x[i]);
+}
+
+int taylor_lookup(const u64 *x, size_t n, const TaylorTab &tab, u64 *out, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(taylor_lookup_kernel, dim3(g1(n)), dim3(BLK), 0, s, x, n, tab, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+__global__ __launch_bounds__(BLK) void filter_powers_kernel(const u64 *__restrict__ x, size_t n, u64 *__restrict__ pos,
+                                                            u64 *__restrict__ neg) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const u64 v = x[i];
+    pos[i] = (s64)v >= 0 ? v : 0;
+    neg[i] = (s64)v < 0 ? (u64)0 - v : 0;
+}
+
+int filter_powers(const u64 *x, size_t n, u64 *pos, u64 *neg, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(filter_powers_kernel, dim3(g1(n)), dim3(BLK), 0, s, x, n, pos, neg);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 // matrix.cuh:97-107 (elementVectorSubtract): a[row*cols + col] -= b[rowwise ? col : row]
 __global__ __launch_bounds__(BLK) void evsub_kernel(u64 *a, const u64 *b, int rowwise, int rows, int cols) {
     size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;

@@ -37,6 +37,11 @@ int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s);
 template <typename U>
 int fused_convex_comb(u64 *d, const u64 *c, const U *e, size_t n, hipStream_t s);
 int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s);
+// getPowers (RSS.inl:1211-1235): in [3][2][n] -> each party's opened power [3][n].
+int fused_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s);
+// taylorSeries (RSS.inl:1237-1320) in one pass: in [3][2][n] -> out [3][2][n]
+// (out must not overlap in); t holds the tabulated coefficient functors.
+int fused_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s);
 
 // ---- backward (training) path ----
 // ReLULayer::backward (ReLULayer.cu:91-123): out = selectShare(0, delta, reluPrime).

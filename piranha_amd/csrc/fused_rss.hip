@@ -576,19 +576,9 @@ __global__ __launch_bounds__(EW_BLOCK) void divide_public_kernel(u64 *__restrict
     u64 dd[3];
 #pragma unroll
     for (int p = 0; p < 3; p++) dd[p] = dv ? dv[(size_t)p * n + i] : d;
-    // rPrime: scalar -> fill(d) (A.sA = C.sB = d); vector -> 0 + d on A.sA and C.sB
-    v.s[PARTY_A][0] -= dd[PARTY_A];
-    v.s[PARTY_C][1] -= dd[PARTY_C];
-    u64 t[3];
-#pragma unroll
-    for (int p = 0; p < 3; p++) t[p] = v.s[p][0] + v.s[p][1] + v.s[prev_party(p)][0];
-    Sh3<u64> o;
-#pragma unroll
-    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
-    // only A (shareA) and C (shareB) add the opened quotient
-    o.s[PARTY_A][0] = 1 + signed_div_u64(t[PARTY_A], dd[PARTY_A]);
-    o.s[PARTY_C][1] = 1 + signed_div_u64(t[PARTY_C], dd[PARTY_C]);
-    store3(a, n, i, o);
+    // rPrime: scalar -> fill(d) (A.sA = C.sB = d); vector -> 0 + d on A.sA and C.sB,
+    // the same shares either way
+    store3(a, n, i, divide_public_local_(v, dd));
 }
 
 int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s) {
@@ -617,6 +607,99 @@ int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s) {
     PIR_REQUIRE(k >= 0 && k <= 62, "dividePublic3out3: denominator must be 2^k, 0 <= k <= 62");
     if (n == 0) return OK;
     hipLaunchKernelGGL(divide_3out3_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, k, out);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// getPowers (RSS.inl:1211-1235): the 64 compare-and-select rounds on one
+// element's shares in registers, then each party's opened power -> pow [3][n].
+__global__ __launch_bounds__(EW_BLOCK) void get_powers_kernel(const u64 *__restrict__ in, size_t n,
+                                                              u64 *__restrict__ pow) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Sh3<u64> pw = get_powers_(load3(in, n, i));
+#pragma unroll
+    for (int p = 0; p < 3; p++) pow[(size_t)p * n + i] = pw.s[p][0] + pw.s[p][1] + pw.s[prev_party(p)][0];
+}
+
+int fused_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(get_powers_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, in, n, pow);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+// taylorSeries (RSS.inl:1237-1320) for one element, every party with its own
+// opened power pow_p = getPowers_p + 1 and ones_p = 1 << pow_p:
+//   out = [(in * in / ones) * c2 / ones] + [in * c1 / ones] + c0
+//   out = out * f / ones / (1 << max(pow - FP, 0)) << max(FP - pow, 0)
+// where each division is dividePublic by the party's public denominator.
+__global__ __launch_bounds__(EW_BLOCK) void taylor_kernel(const u64 *__restrict__ in, size_t n,
+                                                          u64 *__restrict__ out, const TaylorTables t) {
+    size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const Sh3<u64> x = load3(in, n, i);
+    const Sh3<u64> pw = get_powers_(x);
+    u64 pow[3], ones[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        pow[p] = pw.s[p][0] + pw.s[p][1] + pw.s[prev_party(p)][0] + 1;
+        ones[p] = shl_clamped(1, pow[p]);
+    }
+    Sh3<u64> o;
+#pragma unroll
+    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
+    // scratch *= c (RSS *= DeviceData: both shares); out += scratch (RSS += RSS)
+    auto scale_add = [&](Sh3<u64> sc, const TaylorTab &c, bool square) {
+        if (square) {
+            rss_mult_(sc, x);
+            sc = divide_public_local_(sc, ones);
+        }
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const u64 cp = c.at(pow[p]);
+            sc.s[p][0] *= cp;
+            sc.s[p][1] *= cp;
+        }
+        sc = divide_public_local_(sc, ones);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            o.s[p][0] += sc.s[p][0];
+            o.s[p][1] += sc.s[p][1];
+        }
+    };
+    if (t.use_a2) scale_add(x, t.c2, true);
+    if (t.use_a1) scale_add(x, t.c1, false);
+    // out += a0Coeff (RSS += DeviceData: A.sA and C.sB)
+    o.s[PARTY_A][0] += t.c0.at(pow[PARTY_A]);
+    o.s[PARTY_C][1] += t.c0.at(pow[PARTY_C]);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const u64 fp_ = t.f.at(pow[p]);
+        o.s[p][0] *= fp_;
+        o.s[p][1] *= fp_;
+    }
+    o = divide_public_local_(o, ones);
+    // pow -= FP; the non-negative part divides, the negative part shifts left
+    u64 pos_ones[3], neg[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const u64 e = pow[p] - (u64)t.fp;
+        pos_ones[p] = shl_clamped(1, (s64)e >= 0 ? e : 0);
+        neg[p] = (s64)e < 0 ? (u64)0 - e : 0;
+    }
+    o = divide_public_local_(o, pos_ones);
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        o.s[p][0] = shl_clamped(o.s[p][0], neg[p]);
+        o.s[p][1] = shl_clamped(o.s[p][1], neg[p]);
+    }
+    store3(out, n, i, o);
+}
+
+int fused_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(taylor_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, in, n, out, t);
     PIR_LAUNCH_CHECK();
     return OK;
 }

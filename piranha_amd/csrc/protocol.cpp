@@ -2,6 +2,8 @@
 // parties, in the fused and the op-by-op execution modes (see protocol.h).
 #include "protocol.h"
 
+#include <cmath>
+
 #include "fused_rss.h"
 
 namespace pir {
@@ -587,6 +589,139 @@ void maxpool(RSS64 &input, RSS64 &result, RSS<U> &dresult, int k) {
         k /= 2;
     }
     round(true);
+}
+
+// ---------------------------------------------------------------------------
+// getPowers and the Taylor-series functions
+
+// RSS.inl:1211-1235
+void getPowers(const RSS64 &in, PartyData<u64> &pow) {
+    if (fused_mode() && in.base() && pow.base()) {
+        check(fused_get_powers(in.base(), in.size(), pow.base(), S()));
+        return;
+    }
+    size_t n = in.size();
+    RSS64 powers(pow.size()), cur(n), diff(n);
+    RSS<u8> cmp(n);
+    for (int bit = 0; bit < 64; bit++) {
+        cur.fill((u64)bit);
+        diff.zero();
+        diff += in;
+        diff -= (u64)1 << bit;
+        cmp.zero();
+        dReLU(diff, cmp);
+        selectShare(powers, cur, cmp, powers);
+    }
+    reconstruct(powers, pow);
+}
+
+namespace {
+// The reference's GPU: `1 << x` is a 32-bit int shift whose amount the
+// hardware clamps (1 << 31 is INT_MIN, 0 from 32 on); floating-point ->
+// integer conversions saturate to the destination range with NaN -> 0.
+int int_shift_pow2(u64 x) { return x >= 32 ? 0 : (int)(1u << x); }
+u64 saturate_s64(double v) {
+    if (v != v) return 0;
+    if (v >= 9223372036854775808.0) return (u64)INT64_MAX;
+    if (v < -9223372036854775808.0) return (u64)INT64_MIN;
+    return (u64)(s64)v;
+}
+u64 saturate_u64(float v) {
+    if (!(v > 0.0f)) return 0;
+    if (v >= 18446744073709551616.0f) return ~(u64)0;
+    return (u64)v;
+}
+// tofixed_variable_precision_functor (functors.cuh:107-116): (T)(S)(a * (1 << x))
+u64 tofixed_variable_precision(double a, u64 x) { return saturate_s64(a * (double)int_shift_pow2(x)); }
+// calc_fn (functors.h:22-31) with sqrt_lambda / inv_lambda / sigmoid_lambda
+// (functors.h:33-49): val = 2^(x - FP) in float, (T)(fn(val) * (1 << x)).
+u64 calc_fn(int fn, u64 x, int fp) {
+    // pow(2, float(x) - FP) is an exact power of two, +infinity past the float range
+    float val = x > (u64)(fp + 127) ? INFINITY : std::ldexp(1.0f, (int)x - fp);
+    float f = fn == TAYLOR_SQRT ? std::sqrt(val) : fn == TAYLOR_INVERSE ? 1.0f / val : 1.0f / (1.0f + std::exp(-val));
+    return saturate_u64(f * (float)int_shift_pow2(x));
+}
+}  // namespace
+
+TaylorTables taylor_tables(double a0, double a1, double a2, int fn, int fp) {
+    auto fail = [](int code, const char *msg) {
+        set_error(msg);
+        throw PirError(code);
+    };
+    if (fn < TAYLOR_SQRT || fn > TAYLOR_SIGMOID) fail(ERR_ARG, "taylorSeries: unknown function");
+    if (fp < 0 || fp > 62) fail(ERR_ARG, "taylorSeries: precision must be in [0, 62]");
+    TaylorTables t{};
+    auto fill = [&](TaylorTab &tab, auto f) {
+        for (int x = 0; x < TAYLOR_TAB; x++) tab.v[x] = f((u64)x);
+        // the last entry stands for every larger argument
+        for (u64 x : {(u64)TAYLOR_TAB, (u64)63, (u64)64, (u64)fp + 128, ~(u64)0})
+            if (f(x) != tab.v[TAYLOR_TAB - 1]) fail(ERR_STATE, "taylorSeries: coefficient table range");
+    };
+    fill(t.c2, [&](u64 x) { return tofixed_variable_precision(a2, x); });
+    fill(t.c1, [&](u64 x) { return tofixed_variable_precision(a1, x); });
+    fill(t.c0, [&](u64 x) { return tofixed_variable_precision(a0, x); });
+    fill(t.f, [&](u64 x) { return calc_fn(fn, x, fp); });
+    t.fp = fp;
+    t.use_a2 = a2 != 0.0;
+    t.use_a1 = a1 != 0.0;
+    return t;
+}
+
+// RSS.inl:1237-1320.  The coefficient functors are applied by table lookup
+// (their argument, the opened power plus one, takes at most 33 distinct values).
+void taylorSeries(const RSS64 &in, RSS64 &out, double a0, double a1, double a2, int fn, int fp) {
+    const TaylorTables t = taylor_tables(a0, a1, a2, fn, fp);
+    const size_t n = out.size();
+    if (fused_mode() && in.base() && out.base()) {
+        check(fused_taylor(in.base(), n, out.base(), t, S()));
+        return;
+    }
+    out.zero();
+    RSS64 scratch(n);
+    PartyData<u64> pow(n), ones(n), coeff(n);
+    getPowers(in, pow);
+    pow += 1;
+    ones.fill(1);
+    ones <<= pow;
+    auto lookup = [&](const TaylorTab &tab) { check(taylor_lookup(pow.base(), 3 * n, tab, coeff.base(), S())); };
+    if (t.use_a2) {
+        lookup(t.c2);
+        scratch.zero();
+        scratch += in;
+        scratch *= in;
+        dividePublic(scratch, ones);
+        scratch *= coeff;
+        dividePublic(scratch, ones);
+        out += scratch;
+    }
+    if (t.use_a1) {
+        lookup(t.c1);
+        scratch.zero();
+        scratch += in;
+        scratch *= coeff;
+        dividePublic(scratch, ones);
+        out += scratch;
+    }
+    lookup(t.c0);
+    out += coeff;
+    lookup(t.f);
+    out *= coeff;
+    dividePublic(out, ones);
+    // back to FP fractional bits
+    pow += (u64)0 - (u64)fp;
+    PartyData<u64> positivePow(n), negativePow(n);
+    check(filter_powers(pow.base(), 3 * n, positivePow.base(), negativePow.base(), S()));
+    ones.fill(1);
+    ones <<= positivePow;
+    dividePublic(out, ones);
+    out <<= negativePow;  // both shares (lshift_functor)
+}
+
+// RSS.inl:570-612
+void rss_sqrt(const RSS64 &in, RSS64 &out, int fp) { taylorSeries(in, out, 0.424, 0.584, 0.0, TAYLOR_SQRT, fp); }
+void inverse(const RSS64 &in, RSS64 &out, int fp) { taylorSeries(in, out, 4.245, -5.857, 2.630, TAYLOR_INVERSE, fp); }
+void sigmoid(const RSS64 &in, RSS64 &out, int fp) {
+    taylorSeries(in, out, 0.494286, 0.275589, -0.038751, TAYLOR_SIGMOID, fp);
 }
 
 // Explicit instantiations

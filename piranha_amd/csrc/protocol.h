@@ -58,6 +58,21 @@ void maxpool(RSS64 &input, RSS64 &result, RSS<U> &dresult, int k);
 template <typename T>
 void carryOut(RSS<T> &p, RSS<T> &g, int k, RSS<T> &out);
 
+// getPowers (RSS.inl:1211-1235): each party's opened power, the largest bit b
+// with in - 2^b >= 0 as a signed value (0 when there is none).
+void getPowers(const RSS64 &in, PartyData<u64> &pow);
+// taylorSeries (RSS.inl:1237-1320): a0 + a1 x + a2 x^2 on x range-reduced by
+// each element's power, rescaled by the functor fn at FP = fp fractional bits.
+// `out` must not overlap `in`.
+enum TaylorFn { TAYLOR_SQRT = 0, TAYLOR_INVERSE = 1, TAYLOR_SIGMOID = 2 };
+TaylorTables taylor_tables(double a0, double a1, double a2, int fn, int fp);
+void taylorSeries(const RSS64 &in, RSS64 &out, double a0, double a1, double a2, int fn, int fp);
+// sqrt / inverse / sigmoid (RSS.inl:570-612).  The reference's sqrt is
+// rss_sqrt here so that ::sqrt stays visible inside namespace pir.
+void rss_sqrt(const RSS64 &in, RSS64 &out, int fp);
+void inverse(const RSS64 &in, RSS64 &out, int fp);
+void sigmoid(const RSS64 &in, RSS64 &out, int fp);
+
 template <typename T>
 void rss_mult_impl(RSS<T> &x, const RSS<T> &y);
 template <typename T>

@@ -194,6 +194,46 @@ __device__ __forceinline__ Sh3<u64> divide_public_(Sh3<u64> v, u64 d) {
     return o;
 }
 
+// dividePublic by per-party public denominators d[p] (RSS.inl:438-455 with
+// r = 1, r' = 0 + d): a -= r' (A.sA -= d_A, C.sB -= d_C); t_p = reconstruct(a)_p;
+// a = r + (A.sA, C.sB += t_p / d_p).
+__device__ __forceinline__ Sh3<u64> divide_public_local_(Sh3<u64> v, const u64 d[3]) {
+    v.s[PARTY_A][0] -= d[PARTY_A];
+    v.s[PARTY_C][1] -= d[PARTY_C];
+    u64 t[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) t[p] = v.s[p][0] + v.s[p][1] + v.s[prev_party(p)][0];
+    Sh3<u64> o;
+#pragma unroll
+    for (int p = 0; p < 3; p++) o.s[p][0] = o.s[p][1] = 0;
+    o.s[PARTY_A][0] = 1 + signed_div_u64(t[PARTY_A], d[PARTY_A]);
+    o.s[PARTY_C][1] = 1 + signed_div_u64(t[PARTY_C], d[PARTY_C]);
+    return o;
+}
+
+// getPowers (RSS.inl:1211-1235) before its final reconstruct: for bit = 0..63,
+// diff = in - 2^bit (A.sA, C.sB), dReLU(diff) in uint8 shares and
+// powers = selectShare(powers, fill(bit), dReLU), so that powers ends at the
+// largest bit with in - 2^bit >= 0.
+__device__ __forceinline__ Sh3<u64> get_powers_(const Sh3<u64> &in) {
+    Sh3<u64> powers;
+#pragma unroll
+    for (int p = 0; p < 3; p++) powers.s[p][0] = powers.s[p][1] = 0;
+#pragma unroll 1
+    for (int bit = 0; bit < 64; bit++) {
+        Sh3<u64> diff = in, cur;
+        diff.s[PARTY_A][0] -= (u64)1 << bit;
+        diff.s[PARTY_C][1] -= (u64)1 << bit;
+        Sh3<u8> cmp = drelu_<u8>(diff);
+#pragma unroll
+        for (int p = 0; p < 3; p++) cur.s[p][0] = cur.s[p][1] = 0;
+        cur.s[PARTY_A][0] = (u64)bit;
+        cur.s[PARTY_C][1] = (u64)bit;
+        powers = select_(powers, cur, cmp);
+    }
+    return powers;
+}
+
 template <typename T>
 __device__ __forceinline__ Sh3<T> load3(const T *__restrict__ base, size_t n, size_t i) {
     Sh3<T> v;

@@ -106,6 +106,12 @@ _SIGS = {
     "ref_transpose_u64": [_u64p, _u64p, _i, _i],
     "ref_get_powers": [_u64p, _sz, _u64p],
     "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
+    "ref_tpc_mult": [_u64p, _u64p, _sz],
+    "ref_tpc_and": [ctypes.c_void_p, ctypes.c_void_p, _sz, _i],
+    "ref_tpc_drelu": [_u64p, _sz, ctypes.c_void_p, _i],
+    "ref_tpc_select_share": [_u64p, _u64p, ctypes.c_void_p, _u64p, _sz, _i],
+    "ref_tpc_relu": [_u64p, _sz, _u64p, ctypes.c_void_p, _i],
+    "ref_tpc_maxpool": [_u64p, _sz, _u64p, ctypes.c_void_p, _i, _i],
     "ref_taylor_series": [_u64p, _sz, _u64p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _i],
     "ref_get_msbs_u64": [_u64p, _u64p, _sz, _sz],
     "ref_set_msbs_u64": [_u64p, _sz, _u64, _sz],
@@ -986,3 +992,57 @@ def tpc_convolution(A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
 def fpc_convolution(A, B, batch, H, W, f, Din, Dout, stride, pad, trunc):
     """convolution(kFprop) (FPC.inl:744-782, 861-927)."""
     return _scheme_conv("fpc", A, B, batch, H, W, f, Din, Dout, stride, pad, trunc)
+
+
+def _eb(dtype):
+    return np.dtype(dtype).itemsize
+
+
+def tpc_mult(x, y):
+    """TPC *= TPC (TPC.inl:242-272, Beaver triples x = y = z = fill(1))."""
+    x = _u(x).copy()
+    lib().ref_tpc_mult(_p(x), _p(_u(y)), x.shape[-1])
+    return x
+
+
+def tpc_and(x, y):
+    """TPC &= TPC (TPC.inl:281-312, boolean triples x = y = z = fill(1))."""
+    x = np.ascontiguousarray(x).copy()
+    y = np.ascontiguousarray(y, x.dtype)
+    lib().ref_tpc_and(x.ctypes.data, y.ctypes.data, x.shape[-1], _eb(x.dtype))
+    return x
+
+
+def tpc_drelu(x, out_dtype=np.uint8):
+    """dReLU (TPC.inl:828-868): (2, n) shares of [x >= 0]."""
+    x = _u(x)
+    out = np.zeros((2, x.shape[-1]), out_dtype)
+    lib().ref_tpc_drelu(_p(x), x.shape[-1], out.ctypes.data, _eb(out_dtype))
+    return out
+
+
+def tpc_select_share(x, y, b):
+    """selectShare (TPC.inl:585-613): z = b ? y : x."""
+    x, y, b = _u(x), _u(y), np.ascontiguousarray(b)
+    z = np.zeros_like(x)
+    lib().ref_tpc_select_share(_p(x), _p(y), b.ctypes.data, _p(z), x.shape[-1], _eb(b.dtype))
+    return z
+
+
+def tpc_relu(x, out_dtype=np.uint8):
+    """ReLU (TPC.inl:870-884): (result, dresult)."""
+    x = _u(x)
+    y = np.zeros_like(x)
+    d = np.zeros((2, x.shape[-1]), out_dtype)
+    lib().ref_tpc_relu(_p(x), x.shape[-1], _p(y), d.ctypes.data, _eb(out_dtype))
+    return y, d
+
+
+def tpc_maxpool(pools, k, out_dtype=np.uint8):
+    """maxpool over gathered pools (TPC.inl:886-1030); returns (result, dresult, pools_after)."""
+    pools = _u(pools).copy()
+    n = pools.shape[-1]
+    result = np.zeros((2, n // k), np.uint64)
+    d = np.zeros((2, n), out_dtype)
+    lib().ref_tpc_maxpool(_p(pools), n, _p(result), d.ctypes.data, k, _eb(out_dtype))
+    return result, d, pools

@@ -992,6 +992,263 @@ void t_convolution(Tp A, Tp B, Tp C, int batch, int H, int W, int f, int Din, in
     t_divide_public(C, (u64)1 << trunc);
 }
 
+// TPC comparison protocols, element type T (u64 values, U bits): the
+// reference's op sequences over strided views, every party in turn.
+template <typename T>
+struct Tv {
+    View<T> p[2];
+    size_t size() const { return p[0].n; }
+};
+template <typename T>
+struct TBuf {
+    std::vector<T> d[2];
+    explicit TBuf(size_t n) {
+        for (auto &v : d) v.assign(n, 0);
+    }
+    Tv<T> v() {
+        Tv<T> r;
+        for (int q = 0; q < 2; q++) r.p[q] = View<T>{d[q].data(), 0, 1, d[q].size()};
+        return r;
+    }
+};
+template <typename T>
+Tv<T> wrapTv(T *a, size_t n) {
+    Tv<T> r;
+    for (int q = 0; q < 2; q++) r.p[q] = View<T>{a + q * n, 0, 1, n};
+    return r;
+}
+template <typename T>
+Tv<T> stridedTv(const Tv<T> &x, size_t first, size_t stride) {
+    Tv<T> r;
+    for (int q = 0; q < 2; q++) r.p[q] = strided(x.p[q], first, stride);
+    return r;
+}
+// TPCBase::zero / fill, TPC.inl:98-106
+template <typename T>
+void tv_zero(Tv<T> x) {
+    for (int q = 0; q < 2; q++) dd_fill(x.p[q], (T)0);
+}
+template <typename T>
+void tv_fill(Tv<T> x, T v) {
+    dd_fill(x.p[TA], v);
+    dd_fill(x.p[TB], (T)0);
+}
+// TPC op= TPC share-wise (TPC.inl:228-242, 274-279)
+template <typename T>
+void tv_op_t(Tv<T> x, Tv<T> y, Op op) {
+    for (int q = 0; q < 2; q++) dd_op(x.p[q], y.p[q], op);
+}
+// TPC op= DeviceData (TPC.inl:173-226): + - ^ on party A only, & * >> << on both
+template <typename T>
+void tv_op_l(Tv<T> x, const View<T> y[2], Op op) {
+    bool both = (op == MUL || op == AND || op == SHR || op == SHL);
+    for (int q = 0; q < 2; q++)
+        if (both || q == TA) dd_op(x.p[q], y[q], op);
+}
+// reconstruct, TPC.inl:551-572, in the element type
+template <typename T>
+void tv_reconstruct(Tv<T> in, View<T> out[2]) {
+    for (int q = 0; q < 2; q++)
+        for (size_t k = 0; k < in.size(); k++) out[q][k] = (T)(in.p[q][k] + in.p[1 - q][k]);
+}
+template <typename T>
+struct LocPair {
+    std::vector<T> d[2];
+    View<T> v[2];
+    explicit LocPair(size_t n) {
+        for (int q = 0; q < 2; q++) {
+            d[q].assign(n, 0);
+            v[q] = View<T>{d[q].data(), 0, 1, n};
+        }
+    }
+};
+// TPC *= TPC, TPC.inl:242-272, with getBeaverTriples x = y = z = fill(1)
+template <typename T>
+void tv_mult(Tv<T> self, Tv<T> rhs) {
+    size_t n = rhs.size();
+    TBuf<T> x(n), y(n), z(n);
+    tv_fill(x.v(), (T)1);
+    tv_fill(y.v(), (T)1);
+    tv_fill(z.v(), (T)1);
+    LocPair<T> e(n), f(n), temp(n);
+    dd_op(x.v().p[TA], self.p[TA], ADD), dd_op(x.v().p[TB], self.p[TB], ADD);
+    dd_op(y.v().p[TA], rhs.p[TA], ADD), dd_op(y.v().p[TB], rhs.p[TB], ADD);
+    tv_reconstruct(x.v(), e.v);
+    tv_reconstruct(y.v(), f.v);
+    for (int q = 0; q < 2; q++) {
+        dd_op(x.v().p[q], self.p[q], SUB);
+        dd_op(y.v().p[q], rhs.p[q], SUB);
+    }
+    tv_zero(self);
+    tv_op_t(self, z.v(), ADD);
+    for (int q = 0; q < 2; q++) {
+        dd_fill(temp.v[q], (T)0);
+        dd_op(temp.v[q], f.v[q], ADD);
+        dd_op(temp.v[q], y.v().p[q], SUB);
+        dd_op(temp.v[q], e.v[q], MUL);
+    }
+    tv_op_l(self, temp.v, ADD);
+    for (int q = 0; q < 2; q++) {
+        dd_fill(temp.v[q], (T)0);
+        dd_op(temp.v[q], x.v().p[q], SUB);
+        dd_op(temp.v[q], f.v[q], MUL);
+    }
+    tv_op_l(self, temp.v, ADD);
+}
+// TPC &= TPC, TPC.inl:281-312, with getBooleanBeaverTriples x = y = z = fill(1)
+template <typename T>
+void tv_and(Tv<T> self, Tv<T> rhs) {
+    size_t n = rhs.size();
+    TBuf<T> x(n), y(n), z(n);
+    tv_fill(x.v(), (T)1);
+    tv_fill(y.v(), (T)1);
+    tv_fill(z.v(), (T)1);
+    LocPair<T> e(n), f(n), temp(n);
+    for (int q = 0; q < 2; q++) {
+        dd_op(x.v().p[q], self.p[q], XOR);
+        dd_op(y.v().p[q], rhs.p[q], XOR);
+    }
+    tv_reconstruct(x.v(), e.v);
+    tv_reconstruct(y.v(), f.v);
+    for (int q = 0; q < 2; q++) {
+        dd_op(x.v().p[q], self.p[q], XOR);
+        dd_op(y.v().p[q], rhs.p[q], XOR);
+    }
+    tv_zero(self);
+    tv_op_t(self, z.v(), XOR);
+    for (int q = 0; q < 2; q++) {
+        dd_fill(temp.v[q], (T)0);
+        dd_op(temp.v[q], f.v[q], XOR);
+        dd_op(temp.v[q], y.v().p[q], XOR);
+        dd_op(temp.v[q], e.v[q], AND);
+    }
+    tv_op_l(self, temp.v, XOR);
+    for (int q = 0; q < 2; q++) {
+        dd_fill(temp.v[q], (T)0);
+        dd_op(temp.v[q], x.v().p[q], XOR);
+        dd_op(temp.v[q], f.v[q], AND);
+    }
+    tv_op_l(self, temp.v, XOR);
+}
+// carryOut, TPC.inl:1092-1200 (the RSS tree over the TPC AND)
+template <typename U>
+void tv_carry_out(Tv<U> p, Tv<U> g, int k, Tv<U> out) {
+    int stride = 2, offset = 1;
+    Tv<U> pEven = stridedTv(p, 0, stride), pOdd = stridedTv(p, offset, stride);
+    Tv<U> gEven = stridedTv(g, 0, stride), gOdd = stridedTv(g, offset, stride);
+    while (k > 1) {
+        tv_and(gEven, pOdd);
+        tv_and(pEven, pOdd);
+        tv_op_t(gOdd, gEven, XOR);
+        gEven = stridedTv(g, offset, stride * 2);
+        offset += stride;
+        gOdd = stridedTv(g, offset, stride * 2);
+        stride *= 2;
+        pEven = stridedTv(p, 0, stride);
+        pOdd = stridedTv(p, stride / 2, stride);
+        k /= 2;
+    }
+    Tv<U> outEven = stridedTv(out, 0, 2), outOdd = stridedTv(out, 1, 2);
+    for (int q = 0; q < 2; q++) {
+        for (size_t k2 = 0; k2 < gEven.p[q].n; k2++) outEven.p[q][k2] = gEven.p[q][k2];
+        for (size_t k2 = 0; k2 < gOdd.p[q].n; k2++) outOdd.p[q][k2] = gOdd.p[q][k2];
+    }
+}
+// dReLU, TPC.inl:828-868
+template <typename U>
+void tv_drelu(Tv<u64> input, Tv<U> result) {
+    const int bw = 64;
+    size_t n = input.size();
+    TBuf<u64> r(n);
+    TBuf<U> rbits(n * bw);
+    tv_fill(rbits.v(), (U)1);
+    LocPair<u64> a(n);
+    tv_op_t(r.v(), input, ADD);
+    tv_reconstruct(r.v(), a.v);
+    LocPair<U> abits(n * bw);
+    for (int q = 0; q < 2; q++) {
+        dd_op_s(a.v[q], (u64)1, ADD);
+        bitexpand(a.v[q], abits.v[q]);
+    }
+    TBuf<U> msb(n);
+    for (int q = 0; q < 2; q++) set_carry_out_msb(rbits.v().p[q], abits.v[q], msb.v().p[q], bw, q == TA);
+    TBuf<U> g(n * bw), pp(n * bw);
+    tv_op_t(g.v(), rbits.v(), ADD);
+    tv_op_l(g.v(), abits.v, AND);
+    tv_op_t(pp.v(), rbits.v(), ADD);
+    tv_op_l(pp.v(), abits.v, XOR);
+    TBuf<U> pre(result.size());
+    tv_carry_out(pp.v(), g.v(), bw, pre.v());
+    tv_op_t(pre.v(), msb.v(), XOR);
+    tv_fill(result, (U)1);
+    tv_op_t(result, pre.v(), SUB);
+}
+// selectShare, TPC.inl:585-613, with tpc_convex_comb_functor (:32-53)
+template <typename U>
+void tv_select_share(Tv<u64> x, Tv<u64> y, Tv<U> b, Tv<u64> z) {
+    size_t n = x.size();
+    TBuf<u64> c(n);
+    TBuf<U> cbits(b.size());
+    tv_op_t(cbits.v(), b, XOR);
+    LocPair<U> e(cbits.d[0].size());
+    tv_reconstruct(cbits.v(), e.v);
+    TBuf<u64> d(n);
+    for (int q = 0; q < 2; q++)
+        for (size_t k = 0; k < n; k++) {
+            u64 cv = c.d[q][k];
+            d.d[q][k] = e.v[q][k] == 1 ? (q == TA ? 1 - cv : (u64)0 - cv) : cv;
+        }
+    TBuf<u64> res(n);
+    tv_op_t(res.v(), y, ADD);
+    tv_op_t(res.v(), x, SUB);
+    tv_mult(res.v(), d.v());
+    tv_op_t(res.v(), x, ADD);
+    tv_zero(z);
+    tv_op_t(z, res.v(), ADD);
+}
+// ReLU, TPC.inl:870-884
+template <typename U>
+void tv_relu(Tv<u64> input, Tv<u64> result, Tv<U> dresult) {
+    tv_drelu(input, dresult);
+    TBuf<u64> zeros(input.size());
+    tv_select_share(zeros.v(), input, dresult, result);
+}
+// maxpool, TPC.inl:886-1030
+template <typename U>
+void tv_maxpool(Tv<u64> input, Tv<u64> result, Tv<U> dresult, int k) {
+    tv_fill(dresult, (U)1);
+    int stride = 2, offset = 1;
+    Tv<u64> even = stridedTv(input, 0, stride), odd = stridedTv(input, offset, stride);
+    auto round = [&](bool last) {
+        TBuf<u64> diff(even.size());
+        tv_op_t(diff.v(), even, ADD);
+        tv_op_t(diff.v(), odd, SUB);
+        TBuf<U> b(even.size());
+        tv_drelu(diff.v(), b.v());
+        tv_select_share(odd, even, b.v(), even);
+        if (!last) {
+            stride *= 2;
+            even = stridedTv(input, 0, stride);
+            odd = stridedTv(input, stride / 2, stride);
+        } else {
+            tv_zero(result);
+            tv_op_t(result, even, ADD);
+        }
+        TBuf<U> negated(b.d[0].size());
+        tv_fill(negated.v(), (U)1);
+        tv_op_t(negated.v(), b.v(), SUB);
+        TBuf<U> expandedB(input.size());
+        expand_compare(b.v().p[TA], negated.v().p[TA], expandedB.v().p[TA]);
+        expand_compare(b.v().p[TB], negated.v().p[TB], expandedB.v().p[TB]);
+        tv_and(dresult, expandedB.v());
+    };
+    while (k > 2) {
+        round(false);
+        k /= 2;
+    }
+    round(true);
+}
+
 // ---------------------------------------------------------------------------
 // FPC: four parties, three shares each (src/mpc/FPC.h, FPC.inl).  A value is
 // [4 parties][3 shares][n].  Party q's share s holds component (q + s + 1) % 4
@@ -1889,6 +2146,28 @@ void ref_get_powers(const u64 *in, size_t n, u64 *pow) {
     get_powers(wrapRs((u64 *)in, n), l);
 }
 // fn: 0 sqrt (RSS.inl:570-593), 1 inverse (:595-603), 2 sigmoid (:605-612)
+// TPC comparison protocols; eb = 1 (uint8 bits) or 8 (uint64 bits)
+void ref_tpc_mult(u64 *x, const u64 *y, size_t n) { tv_mult(wrapTv(x, n), wrapTv((u64 *)y, n)); }
+void ref_tpc_and(void *x, const void *y, size_t n, int eb) {
+    if (eb == 1) tv_and(wrapTv((uint8_t *)x, n), wrapTv((uint8_t *)y, n));
+    else tv_and(wrapTv((u64 *)x, n), wrapTv((u64 *)y, n));
+}
+void ref_tpc_drelu(const u64 *x, size_t n, void *res, int eb) {
+    if (eb == 1) tv_drelu(wrapTv((u64 *)x, n), wrapTv((uint8_t *)res, n));
+    else tv_drelu(wrapTv((u64 *)x, n), wrapTv((u64 *)res, n));
+}
+void ref_tpc_select_share(const u64 *x, const u64 *y, const void *b, u64 *z, size_t n, int eb) {
+    if (eb == 1) tv_select_share(wrapTv((u64 *)x, n), wrapTv((u64 *)y, n), wrapTv((uint8_t *)b, n), wrapTv(z, n));
+    else tv_select_share(wrapTv((u64 *)x, n), wrapTv((u64 *)y, n), wrapTv((u64 *)b, n), wrapTv(z, n));
+}
+void ref_tpc_relu(const u64 *x, size_t n, u64 *y, void *d, int eb) {
+    if (eb == 1) tv_relu(wrapTv((u64 *)x, n), wrapTv(y, n), wrapTv((uint8_t *)d, n));
+    else tv_relu(wrapTv((u64 *)x, n), wrapTv(y, n), wrapTv((u64 *)d, n));
+}
+void ref_tpc_maxpool(u64 *pools, size_t n, u64 *result, void *d, int k, int eb) {
+    if (eb == 1) tv_maxpool(wrapTv(pools, n), wrapTv(result, n / k), wrapTv((uint8_t *)d, n), k);
+    else tv_maxpool(wrapTv(pools, n), wrapTv(result, n / k), wrapTv((u64 *)d, n), k);
+}
 void ref_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
     taylor_series(wrapRs((u64 *)in, n), wrapRs(out, n), a0, a1, a2, fn, fp);
 }

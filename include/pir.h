@@ -193,6 +193,17 @@ int pir_rss_sigmoid(const uint64_t *x, size_t n, uint64_t *out, int fp, void *st
  * reference's op-by-op form; algo picks the GEMM of the op-by-op form. */
 int pir_tpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream); /* TPC.inl:551-572, out [2][n] */
 int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream);       /* TPC.inl:370-384 */
+/* TPC element-wise protocols, both parties' shares [2][n]; bit values uint8
+ * (elem_bytes 1) or uint64 (8); Beaver triples x = y = z = fill(1) as the
+ * reference's placeholder precompute (Precompute.h:93-105) */
+int pir_tpc_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream);        /* TPC *= TPC, TPC.inl:242-272 */
+int pir_tpc_and(void *x, const void *y, size_t n, int elem_bytes, void *stream); /* TPC &= TPC, TPC.inl:281-312 */
+int pir_tpc_drelu(const uint64_t *x, size_t n, void *result, int elem_bytes, void *stream); /* TPC.inl:828-868 */
+int pir_tpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int elem_bytes, void *stream); /* :870-884 */
+int pir_tpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int elem_bytes,
+                         void *stream);                                          /* TPC.inl:585-613 */
+/* maxpool over gathered pools [n/k windows][k], k in {2, 4, 8, 16} (pools modified like the reference), TPC.inl:886-1030 */
+int pir_tpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
 /* matmul, TPC.inl:574-583 + localMatMul :1059-1090 (transpose flags as in pir_gemm) */
 int pir_tpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
                    int trunc, int algo, void *stream);

@@ -140,6 +140,12 @@ _SIGS = {
     "pir_autotune_load": ([ctypes.c_char_p], _i),
     "pir_tpc_reconstruct": ([_vp, _sz, _vp, _vp], _i),
     "pir_tpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
+    "pir_tpc_mult": ([_vp, _vp, _sz, _vp], _i),
+    "pir_tpc_and": ([_vp, _vp, _sz, _i, _vp], _i),
+    "pir_tpc_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_tpc_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),
+    "pir_tpc_select_share": ([_vp, _vp, _vp, _vp, _sz, _i, _vp], _i),
+    "pir_tpc_maxpool": ([_vp, _sz, _vp, _vp, _i, _i, _vp], _i),
     "pir_tpc_matmul": ([_vp, _vp, _vp] + [_i] * 8 + [_vp], _i),
     "pir_tpc_convolution": ([_vp, _vp, _vp] + [_i] * 10 + [_vp], _i),
     "pir_fpc_reconstruct": ([_vp, _sz, _vp, _vp], _i),

@@ -9,6 +9,7 @@
 #include "model_io.h"
 #include "protocol.h"
 #include "ring_gemm.h"
+#include "tpc_compare.h"
 #include "tpc_fpc.h"
 
 using namespace pir;
@@ -19,6 +20,11 @@ static_assert(PIR_CONV_FPROP == kFprop && PIR_CONV_DGRAD == kDgrad && PIR_CONV_W
               "convolution operator numbering");
 
 namespace {
+
+bool overlaps(const void *a, size_t abytes, const void *b, size_t bbytes) {
+    const char *x = (const char *)a, *y = (const char *)b;
+    return x < y + bbytes && y < x + abytes;
+}
 
 // Settings of the default context used by the free-standing pir_rss_* calls.
 int g_mode = MODE_FUSED;
@@ -528,6 +534,50 @@ int pir_tpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream
     });
 }
 
+int pir_tpc_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return tpc_mult(x, y, n, (hipStream_t)stream);
+    });
+}
+
+int pir_tpc_and(void *x, const void *y, size_t n, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return tpc_and(x, y, n, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_tpc_drelu(const uint64_t *x, size_t n, void *result, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return tpc_relu(x, n, result, nullptr, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_tpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(!overlaps(x, 16 * n, y, 16 * n), "pir_tpc_relu: y must not overlap x");
+        return tpc_relu(x, n, dresult, y, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_tpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int eb,
+                         void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return tpc_select(x, y, b, z, n, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_tpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return tpc_maxpool(pools, n, result, dresult, k, eb, (hipStream_t)stream);
+    });
+}
+
 int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
     return guarded([&]() -> int {
         Scope sc(stream);
@@ -669,13 +719,6 @@ int pir_rss_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, 
         return (int)OK;
     });
 }
-
-namespace {
-bool overlaps(const void *a, size_t abytes, const void *b, size_t bbytes) {
-    const char *x = (const char *)a, *y = (const char *)b;
-    return x < y + bbytes && y < x + abytes;
-}
-}  // namespace
 
 int pir_rss_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream) {
     return guarded([&]() -> int {

@@ -107,6 +107,12 @@ _SIGS = {
     "ref_get_powers": [_u64p, _sz, _u64p],
     "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
     "ref_tpc_mult": [_u64p, _u64p, _sz],
+    "ref_fpc_mult": [_u64p, _u64p, _sz],
+    "ref_fpc_and": [ctypes.c_void_p, ctypes.c_void_p, _sz, _i],
+    "ref_fpc_drelu": [_u64p, _sz, ctypes.c_void_p, _i],
+    "ref_fpc_select_share": [_u64p, _u64p, ctypes.c_void_p, _u64p, _sz, _i],
+    "ref_fpc_relu": [_u64p, _sz, _u64p, ctypes.c_void_p, _i],
+    "ref_fpc_maxpool": [_u64p, _sz, _u64p, ctypes.c_void_p, _i, _i],
     "ref_tpc_and": [ctypes.c_void_p, ctypes.c_void_p, _sz, _i],
     "ref_tpc_drelu": [_u64p, _sz, ctypes.c_void_p, _i],
     "ref_tpc_select_share": [_u64p, _u64p, ctypes.c_void_p, _u64p, _sz, _i],
@@ -1045,4 +1051,54 @@ def tpc_maxpool(pools, k, out_dtype=np.uint8):
     result = np.zeros((2, n // k), np.uint64)
     d = np.zeros((2, n), out_dtype)
     lib().ref_tpc_maxpool(_p(pools), n, _p(result), d.ctypes.data, k, _eb(out_dtype))
+    return result, d, pools
+
+
+def fpc_mult(x, y):
+    """FPC *= FPC (FPC.inl:321-362): cross terms, share-wise product, six reshareFPC."""
+    x = _u(x).copy()
+    lib().ref_fpc_mult(_p(x), _p(_u(y)), x.shape[-1])
+    return x
+
+
+def fpc_and(x, y):
+    """FPC &= FPC (FPC.inl:374-398), the reshare adding in the element type."""
+    x = np.ascontiguousarray(x).copy()
+    y = np.ascontiguousarray(y, x.dtype)
+    lib().ref_fpc_and(x.ctypes.data, y.ctypes.data, x.shape[-1], _eb(x.dtype))
+    return x
+
+
+def fpc_drelu(x, out_dtype=np.uint8):
+    """dReLU (FPC.inl:929-989): (4, 3, n) shares of [x >= 0]."""
+    x = _u(x)
+    out = np.zeros((4, 3, x.shape[-1]), out_dtype)
+    lib().ref_fpc_drelu(_p(x), x.shape[-1], out.ctypes.data, _eb(out_dtype))
+    return out
+
+
+def fpc_select_share(x, y, b):
+    """selectShare (FPC.inl:671-712): z = b ? y : x."""
+    x, y, b = _u(x), _u(y), np.ascontiguousarray(b)
+    z = np.zeros_like(x)
+    lib().ref_fpc_select_share(_p(x), _p(y), b.ctypes.data, _p(z), x.shape[-1], _eb(b.dtype))
+    return z
+
+
+def fpc_relu(x, out_dtype=np.uint8):
+    """ReLU (FPC.inl:991-1002): (result, dresult)."""
+    x = _u(x)
+    y = np.zeros_like(x)
+    d = np.zeros((4, 3, x.shape[-1]), out_dtype)
+    lib().ref_fpc_relu(_p(x), x.shape[-1], _p(y), d.ctypes.data, _eb(out_dtype))
+    return y, d
+
+
+def fpc_maxpool(pools, k, out_dtype=np.uint8):
+    """maxpool over gathered pools (FPC.inl:1004-1181); returns (result, dresult, pools_after)."""
+    pools = _u(pools).copy()
+    n = pools.shape[-1]
+    result = np.zeros((4, 3, n // k), np.uint64)
+    d = np.zeros((4, 3, n), out_dtype)
+    lib().ref_fpc_maxpool(_p(pools), n, _p(result), d.ctypes.data, k, _eb(out_dtype))
     return result, d, pools

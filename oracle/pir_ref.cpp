@@ -1828,6 +1828,251 @@ void reveal_softmax_grad(Rs<u64> act, const double *labels, Rs<u64> deltas, int 
     if (logMiniBatch > 1) divide_public(deltas, (u64)1 << logMiniBatch);
 }
 
+// FPC comparison protocols, element type T: the reference's op sequences
+// (FPC.inl) over strided views, every party in turn.
+template <typename T>
+struct Fv {
+    View<T> s[4][3];
+    size_t size() const { return s[0][0].n; }
+};
+template <typename T>
+struct FBuf {
+    std::vector<T> d[4][3];
+    explicit FBuf(size_t n) {
+        for (auto &pp : d)
+            for (auto &v : pp) v.assign(n, 0);
+    }
+    Fv<T> v() {
+        Fv<T> r;
+        for (int q = 0; q < 4; q++)
+            for (int i = 0; i < 3; i++) r.s[q][i] = View<T>{d[q][i].data(), 0, 1, d[q][i].size()};
+        return r;
+    }
+};
+template <typename T>
+Fv<T> wrapFv(T *a, size_t n) {
+    Fv<T> r;
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) r.s[q][i] = View<T>{a + (3 * q + i) * n, 0, 1, n};
+    return r;
+}
+template <typename T>
+Fv<T> stridedFv(const Fv<T> &x, size_t first, size_t stride) {
+    Fv<T> r;
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) r.s[q][i] = strided(x.s[q][i], first, stride);
+    return r;
+}
+template <typename T>
+struct Loc4 {
+    std::vector<T> d[4];
+    View<T> v[4];
+    explicit Loc4(size_t n) {
+        for (int q = 0; q < 4; q++) {
+            d[q].assign(n, 0);
+            v[q] = View<T>{d[q].data(), 0, 1, n};
+        }
+    }
+};
+// FPCBase::zero / fill, FPC.inl:112-124
+template <typename T>
+void fv_zero(Fv<T> x) {
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) dd_fill(x.s[q][i], (T)0);
+}
+template <typename T>
+void fv_fill(Fv<T> x, T v) {
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) dd_fill(x.s[q][i], i == f_public_slot(q) ? v : (T)0);
+}
+// FPC op= FPC share-wise (FPC.inl:302-373)
+template <typename T>
+void fv_op_f(Fv<T> x, Fv<T> y, Op op) {
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) dd_op(x.s[q][i], y.s[q][i], op);
+}
+// FPC op= DeviceData (FPC.inl:222-300): + - ^ on the public slot, & * >> << on all
+template <typename T>
+void fv_op_l(Fv<T> x, const View<T> y[4], Op op) {
+    bool all = (op == MUL || op == AND || op == SHR || op == SHL);
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++)
+            if (all || i == f_public_slot(q)) dd_op(x.s[q][i], y[q], op);
+}
+// reconstruct, FPC.inl:582-604, in the element type
+template <typename T>
+void fv_reconstruct(Fv<T> in, View<T> out[4]) {
+    for (int q = 0; q < 4; q++)
+        for (size_t k = 0; k < in.size(); k++)
+            out[q][k] = (T)(in.s[q][0][k] + in.s[q][1][k] + in.s[q][2][k] + in.s[f_prev(q)][0][k]);
+}
+// reshareFPC, FPC.inl:1183-1260 (zero PRGs), added in the element type:
+// parties i, j and G add a mask into their slot of component H, G's mask
+// being the one party i sends.
+template <typename T>
+void fv_reshare(const View<T> z[4], const View<T> zp[4], int i, int j, Fv<T> out) {
+    const int H = f_partyH(i, j);
+    const bool opp = f_opposites(i, j);
+    for (int q = 0; q < 4; q++) {
+        if (q == H) continue;
+        const int src = (q == i || q == j) ? q : i;
+        const View<T> &mask = opp ? zp[src] : z[src];
+        View<T> slot = out.s[q][f_shareH(i, j, q)];
+        for (size_t k = 0; k < out.size(); k++) slot[k] = (T)(slot[k] + mask[k]);
+    }
+}
+// FPC *= FPC (FPC.inl:321-362) and FPC &= FPC (:374-398): the cross terms
+// z = A.y1 + B.y0 and z' = A.y2 + C.y0 of every party, the share-wise
+// product, then reshareFPC over the six party pairs.
+template <typename T>
+void fv_product(Fv<T> self, Fv<T> rhs, bool boolean) {
+    size_t n = rhs.size();
+    const Op mul = boolean ? AND : MUL, add = boolean ? XOR : ADD;
+    Loc4<T> z(n), zp(n);
+    for (int q = 0; q < 4; q++)
+        for (size_t k = 0; k < n; k++) {
+            const T a = self.s[q][0][k], b = self.s[q][1][k], c = self.s[q][2][k];
+            const T y0 = rhs.s[q][0][k], y1 = rhs.s[q][1][k], y2 = rhs.s[q][2][k];
+            z.v[q][k] = apply_op<T>(add, apply_op<T>(mul, a, y1), apply_op<T>(mul, b, y0));
+            zp.v[q][k] = apply_op<T>(add, apply_op<T>(mul, a, y2), apply_op<T>(mul, c, y0));
+        }
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) dd_op(self.s[q][i], rhs.s[q][i], mul);
+    const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    for (auto &pr : pairs) fv_reshare(z.v, zp.v, pr[0], pr[1], self);
+}
+// carryOut, FPC.inl:1297-1416 (the RSS tree over the FPC AND)
+template <typename U>
+void fv_carry_out(Fv<U> p, Fv<U> g, int k, Fv<U> out) {
+    int stride = 2, offset = 1;
+    Fv<U> pEven = stridedFv(p, 0, stride), pOdd = stridedFv(p, offset, stride);
+    Fv<U> gEven = stridedFv(g, 0, stride), gOdd = stridedFv(g, offset, stride);
+    while (k > 1) {
+        fv_product(gEven, pOdd, true);
+        fv_product(pEven, pOdd, true);
+        fv_op_f(gOdd, gEven, XOR);
+        gEven = stridedFv(g, offset, stride * 2);
+        offset += stride;
+        gOdd = stridedFv(g, offset, stride * 2);
+        stride *= 2;
+        pEven = stridedFv(p, 0, stride);
+        pOdd = stridedFv(p, stride / 2, stride);
+        k /= 2;
+    }
+    Fv<U> outEven = stridedFv(out, 0, 2), outOdd = stridedFv(out, 1, 2);
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++) {
+            for (size_t k2 = 0; k2 < gEven.s[q][i].n; k2++) outEven.s[q][i][k2] = gEven.s[q][i][k2];
+            for (size_t k2 = 0; k2 < gOdd.s[q][i].n; k2++) outOdd.s[q][i][k2] = gOdd.s[q][i][k2];
+        }
+}
+// dReLU, FPC.inl:929-989, with getMSBs / setMSBs (bitwise.cuh:126-182)
+template <typename U>
+void fv_drelu(Fv<u64> input, Fv<U> result) {
+    const int bw = 64;
+    size_t n = input.size();
+    FBuf<u64> r(n);
+    FBuf<U> rbits(n * bw);
+    fv_fill(rbits.v(), (U)1);
+    Loc4<u64> a(n);
+    fv_op_f(r.v(), input, ADD);
+    fv_reconstruct(r.v(), a.v);
+    Loc4<U> abits(n * bw), abits_msb(n);
+    for (int q = 0; q < 4; q++) {
+        dd_op_s(a.v[q], (u64)1, ADD);
+        bitexpand(a.v[q], abits.v[q]);
+    }
+    FBuf<U> msb(n);
+    Fv<U> rb = rbits.v(), mb = msb.v();
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++)
+            for (size_t k = 0; k < n; k++) mb.s[q][i][k] = rb.s[q][i][k * bw + bw - 1];
+    for (int q = 0; q < 4; q++)
+        for (size_t k = 0; k < n; k++) abits_msb.v[q][k] = abits.v[q][k * bw + bw - 1];
+    fv_op_l(mb, abits_msb.v, XOR);
+    for (int q = 0; q < 4; q++)
+        for (size_t k = 0; k < n; k++) {
+            abits.v[q][k * bw + bw - 1] = 1;
+            for (int i = 0; i < 3; i++) rb.s[q][i][k * bw + bw - 1] = 0;
+        }
+    FBuf<U> g(n * bw), pp(n * bw);
+    fv_op_f(g.v(), rb, ADD);
+    fv_op_l(g.v(), abits.v, AND);
+    fv_op_f(pp.v(), rb, ADD);
+    fv_op_l(pp.v(), abits.v, XOR);
+    FBuf<U> pre(result.size());
+    fv_carry_out(pp.v(), g.v(), bw, pre.v());
+    fv_op_f(pre.v(), mb, XOR);
+    fv_fill(result, (U)1);
+    fv_op_f(result, pre.v(), SUB);
+}
+// selectShare, FPC.inl:671-712, with FPC_convex_comb_functor (:30-66)
+template <typename U>
+void fv_select_share(Fv<u64> x, Fv<u64> y, Fv<U> b, Fv<u64> z) {
+    size_t n = x.size();
+    FBuf<u64> c(n);
+    FBuf<U> cbits(b.size());
+    fv_op_f(cbits.v(), b, XOR);
+    Loc4<U> e(b.size());
+    fv_reconstruct(cbits.v(), e.v);
+    FBuf<u64> d(n);
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < 3; i++)
+            for (size_t k = 0; k < n; k++) {
+                u64 cv = c.d[q][i][k];
+                d.d[q][i][k] = e.v[q][k] == 1 ? (i == f_public_slot(q) ? 1 - cv : (u64)0 - cv) : cv;
+            }
+    FBuf<u64> res(n);
+    fv_op_f(res.v(), y, ADD);
+    fv_op_f(res.v(), x, SUB);
+    fv_product(res.v(), d.v(), false);
+    fv_op_f(res.v(), x, ADD);
+    fv_zero(z);
+    fv_op_f(z, res.v(), ADD);
+}
+// ReLU, FPC.inl:991-1002
+template <typename U>
+void fv_relu(Fv<u64> input, Fv<u64> result, Fv<U> dresult) {
+    fv_drelu(input, dresult);
+    FBuf<u64> zeros(input.size());
+    fv_select_share(zeros.v(), input, dresult, result);
+}
+// maxpool, FPC.inl:1004-1181
+template <typename U>
+void fv_maxpool(Fv<u64> input, Fv<u64> result, Fv<U> dresult, int k) {
+    fv_fill(dresult, (U)1);
+    int stride = 2, offset = 1;
+    Fv<u64> even = stridedFv(input, 0, stride), odd = stridedFv(input, offset, stride);
+    auto round = [&](bool last) {
+        FBuf<u64> diff(even.size());
+        fv_op_f(diff.v(), even, ADD);
+        fv_op_f(diff.v(), odd, SUB);
+        FBuf<U> b(even.size());
+        fv_drelu(diff.v(), b.v());
+        fv_select_share(odd, even, b.v(), even);
+        if (!last) {
+            stride *= 2;
+            even = stridedFv(input, 0, stride);
+            odd = stridedFv(input, stride / 2, stride);
+        } else {
+            fv_zero(result);
+            fv_op_f(result, even, ADD);
+        }
+        FBuf<U> negated(b.d[0][0].size());
+        fv_fill(negated.v(), (U)1);
+        fv_op_f(negated.v(), b.v(), SUB);
+        FBuf<U> expandedB(input.size());
+        for (int q = 0; q < 4; q++)
+            for (int i = 0; i < 3; i++) expand_compare(b.v().s[q][i], negated.v().s[q][i], expandedB.v().s[q][i]);
+        fv_product(dresult, expandedB.v(), true);
+    };
+    while (k > 2) {
+        round(false);
+        k /= 2;
+    }
+    round(true);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2167,6 +2412,28 @@ void ref_tpc_relu(const u64 *x, size_t n, u64 *y, void *d, int eb) {
 void ref_tpc_maxpool(u64 *pools, size_t n, u64 *result, void *d, int k, int eb) {
     if (eb == 1) tv_maxpool(wrapTv(pools, n), wrapTv(result, n / k), wrapTv((uint8_t *)d, n), k);
     else tv_maxpool(wrapTv(pools, n), wrapTv(result, n / k), wrapTv((u64 *)d, n), k);
+}
+// FPC comparison protocols; eb = 1 (uint8 bits) or 8 (uint64 bits)
+void ref_fpc_mult(u64 *x, const u64 *y, size_t n) { fv_product(wrapFv(x, n), wrapFv((u64 *)y, n), false); }
+void ref_fpc_and(void *x, const void *y, size_t n, int eb) {
+    if (eb == 1) fv_product(wrapFv((uint8_t *)x, n), wrapFv((uint8_t *)y, n), true);
+    else fv_product(wrapFv((u64 *)x, n), wrapFv((u64 *)y, n), true);
+}
+void ref_fpc_drelu(const u64 *x, size_t n, void *res, int eb) {
+    if (eb == 1) fv_drelu(wrapFv((u64 *)x, n), wrapFv((uint8_t *)res, n));
+    else fv_drelu(wrapFv((u64 *)x, n), wrapFv((u64 *)res, n));
+}
+void ref_fpc_select_share(const u64 *x, const u64 *y, const void *b, u64 *z, size_t n, int eb) {
+    if (eb == 1) fv_select_share(wrapFv((u64 *)x, n), wrapFv((u64 *)y, n), wrapFv((uint8_t *)b, n), wrapFv(z, n));
+    else fv_select_share(wrapFv((u64 *)x, n), wrapFv((u64 *)y, n), wrapFv((u64 *)b, n), wrapFv(z, n));
+}
+void ref_fpc_relu(const u64 *x, size_t n, u64 *y, void *d, int eb) {
+    if (eb == 1) fv_relu(wrapFv((u64 *)x, n), wrapFv(y, n), wrapFv((uint8_t *)d, n));
+    else fv_relu(wrapFv((u64 *)x, n), wrapFv(y, n), wrapFv((u64 *)d, n));
+}
+void ref_fpc_maxpool(u64 *pools, size_t n, u64 *result, void *d, int k, int eb) {
+    if (eb == 1) fv_maxpool(wrapFv(pools, n), wrapFv(result, n / k), wrapFv((uint8_t *)d, n), k);
+    else fv_maxpool(wrapFv(pools, n), wrapFv(result, n / k), wrapFv((u64 *)d, n), k);
 }
 void ref_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
     taylor_series(wrapRs((u64 *)in, n), wrapRs(out, n), a0, a1, a2, fn, fp);

@@ -212,6 +212,16 @@ int pir_tpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
                         int Dout, int stride, int pad, int trunc, int algo, void *stream);
 int pir_fpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream); /* FPC.inl:582-604, out [4][n] */
 int pir_fpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream);       /* FPC.inl:566-580 */
+/* FPC element-wise protocols on [4][3][n] values, bit values uint8 (elem_bytes 1)
+ * or uint64 (8); reshareFPC with zero PRGs as the reference's placeholder */
+int pir_fpc_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream);        /* FPC *= FPC, FPC.inl:321-362 */
+int pir_fpc_and(void *x, const void *y, size_t n, int elem_bytes, void *stream); /* FPC &= FPC, FPC.inl:374-398 */
+int pir_fpc_drelu(const uint64_t *x, size_t n, void *result, int elem_bytes, void *stream); /* FPC.inl:929-989 */
+int pir_fpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int elem_bytes, void *stream); /* :991-1002 */
+int pir_fpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int elem_bytes,
+                         void *stream);                                          /* FPC.inl:671-712 */
+/* maxpool over gathered pools [n/k windows][k], k in {2, 4, 8, 16} (pools modified like the reference), FPC.inl:1004-1181 */
+int pir_fpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
 /* reshareFPC, FPC.inl:1183-1260: z, zp [4][n] per party, added into out [4][3][n]; 0 <= i < j < 4 */
 int pir_fpc_reshare(const uint64_t *z, const uint64_t *zp, size_t n, int i, int j, uint64_t *out, void *stream);
 /* matmul, FPC.inl:635-666 */

@@ -141,6 +141,12 @@ _SIGS = {
     "pir_tpc_reconstruct": ([_vp, _sz, _vp, _vp], _i),
     "pir_tpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
     "pir_tpc_mult": ([_vp, _vp, _sz, _vp], _i),
+    "pir_fpc_mult": ([_vp, _vp, _sz, _vp], _i),
+    "pir_fpc_and": ([_vp, _vp, _sz, _i, _vp], _i),
+    "pir_fpc_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_fpc_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),
+    "pir_fpc_select_share": ([_vp, _vp, _vp, _vp, _sz, _i, _vp], _i),
+    "pir_fpc_maxpool": ([_vp, _sz, _vp, _vp, _i, _i, _vp], _i),
     "pir_tpc_and": ([_vp, _vp, _sz, _i, _vp], _i),
     "pir_tpc_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_tpc_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),

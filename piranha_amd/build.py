@@ -19,7 +19,7 @@ SOURCES = [
     "ring_gemm.hip",
     "fused_rss.hip",
     "elementwise.hip",
-    "tpc_compare.hip",
+    "tpc_fpc_compare.hip",
     "runtime.cpp",
     "protocol.cpp",
     "tpc_fpc.cpp",

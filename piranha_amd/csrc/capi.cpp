@@ -9,7 +9,7 @@
 #include "model_io.h"
 #include "protocol.h"
 #include "ring_gemm.h"
-#include "tpc_compare.h"
+#include "tpc_fpc_compare.h"
 #include "tpc_fpc.h"
 
 using namespace pir;
@@ -575,6 +575,50 @@ int pir_tpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, 
     return guarded([&]() -> int {
         Scope sc(stream);
         return tpc_maxpool(pools, n, result, dresult, k, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_fpc_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return fpc_mult(x, y, n, (hipStream_t)stream);
+    });
+}
+
+int pir_fpc_and(void *x, const void *y, size_t n, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return fpc_and(x, y, n, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_fpc_drelu(const uint64_t *x, size_t n, void *result, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return fpc_relu(x, n, result, nullptr, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_fpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        PIR_REQUIRE(!overlaps(x, 96 * n, y, 96 * n), "pir_fpc_relu: y must not overlap x");
+        return fpc_relu(x, n, dresult, y, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_fpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int eb,
+                         void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return fpc_select(x, y, b, z, n, eb, (hipStream_t)stream);
+    });
+}
+
+int pir_fpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int eb, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return fpc_maxpool(pools, n, result, dresult, k, eb, (hipStream_t)stream);
     });
 }
 

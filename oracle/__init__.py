@@ -107,6 +107,10 @@ _SIGS = {
     "ref_get_powers": [_u64p, _sz, _u64p],
     "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
     "ref_tpc_mult": [_u64p, _u64p, _sz],
+    "ref_tpc_get_powers": [_u64p, _sz, _u64p],
+    "ref_fpc_get_powers": [_u64p, _sz, _u64p],
+    "ref_tpc_taylor_series": [_u64p, _sz, _u64p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _i],
+    "ref_fpc_taylor_series": [_u64p, _sz, _u64p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _i],
     "ref_fpc_mult": [_u64p, _u64p, _sz],
     "ref_fpc_and": [ctypes.c_void_p, ctypes.c_void_p, _sz, _i],
     "ref_fpc_drelu": [_u64p, _sz, ctypes.c_void_p, _i],
@@ -1102,3 +1106,29 @@ def fpc_maxpool(pools, k, out_dtype=np.uint8):
     d = np.zeros((4, 3, n), out_dtype)
     lib().ref_fpc_maxpool(_p(pools), n, _p(result), d.ctypes.data, k, _eb(out_dtype))
     return result, d, pools
+
+
+TAYLOR_COEFFS = {TAYLOR_SQRT: (0.424, 0.584, 0.0), TAYLOR_INVERSE: (4.245, -5.857, 2.630),
+                 TAYLOR_SIGMOID: (0.494286, 0.275589, -0.038751)}  # RSS.inl / TPC.inl / FPC.inl sqrt, inverse, sigmoid
+
+
+def scheme_get_powers(kind, x):
+    """getPowers of a TPC (2, n) or FPC (4, 3, n) value: each party's opened power."""
+    x = _u(x)
+    n = x.shape[-1]
+    pw = np.zeros((_NSH[kind][0], n), np.uint64)
+    getattr(lib(), f"ref_{kind}_get_powers")(_p(x), n, _p(pw))
+    return pw
+
+
+def scheme_taylor_series(kind, x, a0, a1, a2, fn, fp):
+    """taylorSeries of a TPC or FPC value (TPC.inl:1228-1290, FPC.inl:1443-1526)."""
+    x = _u(x)
+    out = np.zeros_like(x)
+    getattr(lib(), f"ref_{kind}_taylor_series")(_p(x), x.shape[-1], _p(out), a0, a1, a2, fn, fp)
+    return out
+
+
+def scheme_taylor(kind, x, fn, fp):
+    """sqrt / inverse / sigmoid of a TPC or FPC value with the reference's coefficients."""
+    return scheme_taylor_series(kind, x, *TAYLOR_COEFFS[fn], fn, fp)

@@ -2073,6 +2073,165 @@ void fv_maxpool(Fv<u64> input, Fv<u64> result, Fv<U> dresult, int k) {
     round(true);
 }
 
+// getPowers and taylorSeries of the TPC and FPC schemes (TPC.inl:1202-1290,
+// FPC.inl:1418-1526): the same op sequence as the RSS versions above, on each
+// scheme's share algebra through a small adapter.
+struct TpcAlg {
+    static constexpr int P = 2;
+    using V = Tv<u64>;
+    using VB = Tv<uint8_t>;
+    using Buf = TBuf<u64>;
+    using BufB = TBuf<uint8_t>;
+    template <typename T>
+    static void zero(Tv<T> x) {
+        tv_zero(x);
+    }
+    static void fill(V x, u64 v) { tv_fill(x, v); }
+    static void op(V x, V y, Op o) { tv_op_t(x, y, o); }
+    static void op_l(V x, const View<u64> y[P], Op o) { tv_op_l(x, y, o); }
+    // TPC -= T: party A only (TPC.inl:153-159)
+    static void sub_scalar(V x, u64 v) {
+        for (size_t k = 0; k < x.size(); k++) x.p[TA][k] -= v;
+    }
+    static void mult(V x, V y) { tv_mult(x, y); }
+    static void drelu(V x, VB r) { tv_drelu(x, r); }
+    static void select(V x, V y, VB b, V z) { tv_select_share(x, y, b, z); }
+    static void reconstruct(V x, View<u64> out[P]) { tv_reconstruct(x, out); }
+    static void shl_all(V x, const View<u64> y[P]) {
+        for (int q = 0; q < P; q++) dd_op(x.p[q], y[q], SHL);
+    }
+};
+struct FpcAlg {
+    static constexpr int P = 4;
+    using V = Fv<u64>;
+    using VB = Fv<uint8_t>;
+    using Buf = FBuf<u64>;
+    using BufB = FBuf<uint8_t>;
+    template <typename T>
+    static void zero(Fv<T> x) {
+        fv_zero(x);
+    }
+    static void fill(V x, u64 v) { fv_fill(x, v); }
+    static void op(V x, V y, Op o) { fv_op_f(x, y, o); }
+    static void op_l(V x, const View<u64> y[P], Op o) { fv_op_l(x, y, o); }
+    // FPC -= T: the public slot (FPC.inl:197-208)
+    static void sub_scalar(V x, u64 v) {
+        for (int q = 0; q < 4; q++) {
+            int slot = f_public_slot(q);
+            if (slot >= 0)
+                for (size_t k = 0; k < x.size(); k++) x.s[q][slot][k] -= v;
+        }
+    }
+    static void mult(V x, V y) { fv_product(x, y, false); }
+    static void drelu(V x, VB r) { fv_drelu(x, r); }
+    static void select(V x, V y, VB b, V z) { fv_select_share(x, y, b, z); }
+    static void reconstruct(V x, View<u64> out[P]) { fv_reconstruct(x, out); }
+    static void shl_all(V x, const View<u64> y[P]) {
+        for (int q = 0; q < P; q++)
+            for (int i = 0; i < 3; i++) dd_op(x.s[q][i], y[q], SHL);
+    }
+};
+
+template <typename A>
+struct PartyArrays {
+    std::vector<u64> d[A::P];
+    View<u64> v[A::P];
+    explicit PartyArrays(size_t n) {
+        for (int q = 0; q < A::P; q++) {
+            d[q].assign(n, 0);
+            v[q] = View<u64>{d[q].data(), 0, 1, n};
+        }
+    }
+};
+
+// getPowers (TPC.inl:1202-1226, FPC.inl:1418-1441)
+template <typename A>
+void scheme_get_powers(typename A::V in, View<u64> pow[A::P]) {
+    const size_t n = in.size();
+    typename A::Buf powers(n), cur(n), diff(n);
+    typename A::BufB cmp(n);
+    for (int bit = 0; bit < 64; bit++) {
+        A::fill(cur.v(), (u64)bit);
+        A::zero(diff.v());
+        A::op(diff.v(), in, ADD);
+        A::sub_scalar(diff.v(), (u64)1 << bit);
+        A::zero(cmp.v());
+        A::drelu(diff.v(), cmp.v());
+        A::select(powers.v(), cur.v(), cmp.v(), powers.v());
+    }
+    A::reconstruct(powers.v(), pow);
+}
+
+// dividePublic(Share, DeviceData) (TPC.inl:386-407, FPC.inl:566-580 with the
+// vector getDividedShares, Precompute.h:121-134): r = fill(1), r' = 0 + d.
+template <typename A>
+void scheme_divide_public_vec(typename A::V a, const View<u64> d[A::P]) {
+    const size_t n = a.size();
+    typename A::Buf r(n), rPrime(n);
+    A::op_l(rPrime.v(), d, ADD);
+    A::fill(r.v(), 1);
+    A::op(a, rPrime.v(), SUB);
+    PartyArrays<A> rec(n);
+    A::reconstruct(a, rec.v);
+    for (int q = 0; q < A::P; q++)
+        for (size_t k = 0; k < n; k++) rec.v[q][k] = sdiv<u64>(rec.v[q][k], d[q][k]);
+    A::zero(a);
+    A::op(a, r.v(), ADD);
+    A::op_l(a, rec.v, ADD);
+}
+
+// taylorSeries (TPC.inl:1228-1290, FPC.inl:1443-1526)
+template <typename A>
+void scheme_taylor_series(typename A::V in, typename A::V out, double a0, double a1, double a2, int fn, int fp) {
+    const size_t n = in.size();
+    A::zero(out);
+    typename A::Buf scratch(n);
+    PartyArrays<A> pow(n), ones(n), coef(n), posPow(n), negPow(n);
+    scheme_get_powers<A>(in, pow.v);
+    for (int q = 0; q < A::P; q++)
+        for (size_t k = 0; k < n; k++) {
+            pow.v[q][k] += 1;
+            ones.v[q][k] = apply_op<u64>(SHL, 1, pow.v[q][k]);
+        }
+    auto coeffs = [&](double a) {
+        for (int q = 0; q < A::P; q++)
+            for (size_t k = 0; k < n; k++) coef.v[q][k] = tofixed_vp(a, pow.v[q][k]);
+    };
+    if (a2 != 0.0) {
+        coeffs(a2);
+        A::zero(scratch.v());
+        A::op(scratch.v(), in, ADD);
+        A::mult(scratch.v(), in);
+        scheme_divide_public_vec<A>(scratch.v(), ones.v);
+        A::op_l(scratch.v(), coef.v, MUL);
+        scheme_divide_public_vec<A>(scratch.v(), ones.v);
+        A::op(out, scratch.v(), ADD);
+    }
+    if (a1 != 0.0) {
+        coeffs(a1);
+        A::zero(scratch.v());
+        A::op(scratch.v(), in, ADD);
+        A::op_l(scratch.v(), coef.v, MUL);
+        scheme_divide_public_vec<A>(scratch.v(), ones.v);
+        A::op(out, scratch.v(), ADD);
+    }
+    coeffs(a0);
+    A::op_l(out, coef.v, ADD);
+    for (int q = 0; q < A::P; q++)
+        for (size_t k = 0; k < n; k++) coef.v[q][k] = calc_fn(fn, pow.v[q][k], fp);
+    A::op_l(out, coef.v, MUL);
+    scheme_divide_public_vec<A>(out, ones.v);
+    for (int q = 0; q < A::P; q++)
+        for (size_t k = 0; k < n; k++) {
+            s64 pw = (s64)(pow.v[q][k] - (u64)fp);
+            posPow.v[q][k] = pw >= 0 ? (u64)pw : 0;
+            negPow.v[q][k] = pw < 0 ? (u64)0 - (u64)pw : 0;
+            ones.v[q][k] = apply_op<u64>(SHL, 1, posPow.v[q][k]);
+        }
+    scheme_divide_public_vec<A>(out, ones.v);
+    A::shl_all(out, negPow.v);  // every share (lshift_functor)
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2434,6 +2593,22 @@ void ref_fpc_relu(const u64 *x, size_t n, u64 *y, void *d, int eb) {
 void ref_fpc_maxpool(u64 *pools, size_t n, u64 *result, void *d, int k, int eb) {
     if (eb == 1) fv_maxpool(wrapFv(pools, n), wrapFv(result, n / k), wrapFv((uint8_t *)d, n), k);
     else fv_maxpool(wrapFv(pools, n), wrapFv(result, n / k), wrapFv((u64 *)d, n), k);
+}
+// getPowers / taylorSeries of TPC ([2][n]) and FPC ([4][3][n]); pow is [P][n]
+void ref_tpc_get_powers(const u64 *in, size_t n, u64 *pow) {
+    View<u64> pv[2] = {View<u64>{pow, 0, 1, n}, View<u64>{pow + n, 0, 1, n}};
+    scheme_get_powers<TpcAlg>(wrapTv((u64 *)in, n), pv);
+}
+void ref_fpc_get_powers(const u64 *in, size_t n, u64 *pow) {
+    View<u64> pv[4];
+    for (int q = 0; q < 4; q++) pv[q] = View<u64>{pow + q * n, 0, 1, n};
+    scheme_get_powers<FpcAlg>(wrapFv((u64 *)in, n), pv);
+}
+void ref_tpc_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
+    scheme_taylor_series<TpcAlg>(wrapTv((u64 *)in, n), wrapTv(out, n), a0, a1, a2, fn, fp);
+}
+void ref_fpc_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
+    scheme_taylor_series<FpcAlg>(wrapFv((u64 *)in, n), wrapFv(out, n), a0, a1, a2, fn, fp);
 }
 void ref_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
     taylor_series(wrapRs((u64 *)in, n), wrapRs(out, n), a0, a1, a2, fn, fp);

@@ -222,6 +222,21 @@ int pir_fpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, ui
                          void *stream);                                          /* FPC.inl:671-712 */
 /* maxpool over gathered pools [n/k windows][k], k in {2, 4, 8, 16} (pools modified like the reference), FPC.inl:1004-1181 */
 int pir_fpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
+/* getPowers (TPC.inl:1202-1226, FPC.inl:1418-1441): each party's opened power,
+ * pow [2][n] / [4][n]; taylorSeries (TPC.inl:1228-1290, FPC.inl:1443-1526) and
+ * sqrt / inverse / sigmoid (TPC.inl:615-642, FPC.inl:714-741) as pir_rss_taylor_series */
+int pir_tpc_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream);
+int pir_fpc_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream);
+int pir_tpc_taylor_series(const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn,
+                          int fp, void *stream);
+int pir_fpc_taylor_series(const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn,
+                          int fp, void *stream);
+int pir_tpc_sqrt(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_tpc_inverse(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_tpc_sigmoid(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_fpc_sqrt(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_fpc_inverse(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
+int pir_fpc_sigmoid(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
 /* reshareFPC, FPC.inl:1183-1260: z, zp [4][n] per party, added into out [4][3][n]; 0 <= i < j < 4 */
 int pir_fpc_reshare(const uint64_t *z, const uint64_t *zp, size_t n, int i, int j, uint64_t *out, void *stream);
 /* matmul, FPC.inl:635-666 */

@@ -142,6 +142,16 @@ _SIGS = {
     "pir_tpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
     "pir_tpc_mult": ([_vp, _vp, _sz, _vp], _i),
     "pir_fpc_mult": ([_vp, _vp, _sz, _vp], _i),
+    "pir_tpc_get_powers": ([_vp, _sz, _vp, _vp], _i),
+    "pir_fpc_get_powers": ([_vp, _sz, _vp, _vp], _i),
+    "pir_tpc_taylor_series": ([_vp, _sz, _vp, _d, _d, _d, _i, _i, _vp], _i),
+    "pir_fpc_taylor_series": ([_vp, _sz, _vp, _d, _d, _d, _i, _i, _vp], _i),
+    "pir_tpc_sqrt": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_tpc_inverse": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_tpc_sigmoid": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_fpc_sqrt": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_fpc_inverse": ([_vp, _sz, _vp, _i, _vp], _i),
+    "pir_fpc_sigmoid": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_fpc_and": ([_vp, _vp, _sz, _i, _vp], _i),
     "pir_fpc_drelu": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_fpc_relu": ([_vp, _sz, _vp, _vp, _i, _vp], _i),

@@ -622,6 +622,64 @@ int pir_fpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, 
     });
 }
 
+// getPowers / taylorSeries of TPC (P = 2, S = 1) and FPC (P = 4, S = 3) values [P][S][n]
+namespace {
+int scheme_get_powers_call(int P, int S, const uint64_t *x, size_t n, uint64_t *pow, void *stream, const char *what,
+                           int (*f)(const u64 *, size_t, u64 *, hipStream_t)) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        if (overlaps(x, (size_t)P * S * 8 * n, pow, (size_t)P * 8 * n)) {
+            set_error(std::string(what) + ": pow must not overlap x");
+            return (int)ERR_ARG;
+        }
+        return f(x, n, pow, (hipStream_t)stream);
+    });
+}
+int scheme_taylor_call(int P, int S, const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn, int fp,
+                       void *stream, const char *what,
+                       int (*f)(const u64 *, size_t, u64 *, const TaylorTables &, hipStream_t)) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        if (overlaps(x, (size_t)P * S * 8 * n, out, (size_t)P * S * 8 * n)) {
+            set_error(std::string(what) + ": out must not overlap x");
+            return (int)ERR_ARG;
+        }
+        const TaylorTables t = taylor_tables(a0, a1, a2, fn, fp);
+        return f(x, n, out, t, (hipStream_t)stream);
+    });
+}
+// RSS.inl:570-612, TPC.inl:615-642, FPC.inl:714-741
+const double kTaylorCoeffs[3][3] = {{0.424, 0.584, 0.0}, {4.245, -5.857, 2.630}, {0.494286, 0.275589, -0.038751}};
+}  // namespace
+
+int pir_tpc_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream) {
+    return scheme_get_powers_call(2, 1, x, n, pow, stream, "pir_tpc_get_powers", tpc_get_powers);
+}
+int pir_fpc_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream) {
+    return scheme_get_powers_call(4, 3, x, n, pow, stream, "pir_fpc_get_powers", fpc_get_powers);
+}
+int pir_tpc_taylor_series(const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn, int fp,
+                          void *stream) {
+    return scheme_taylor_call(2, 1, x, n, out, a0, a1, a2, fn, fp, stream, "pir_tpc_taylor_series", tpc_taylor);
+}
+int pir_fpc_taylor_series(const uint64_t *x, size_t n, uint64_t *out, double a0, double a1, double a2, int fn, int fp,
+                          void *stream) {
+    return scheme_taylor_call(4, 3, x, n, out, a0, a1, a2, fn, fp, stream, "pir_fpc_taylor_series", fpc_taylor);
+}
+#define PIR_SCHEME_TAYLOR(scheme, P, S, name, FN)                                                             \
+    int pir_##scheme##_##name(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream) {           \
+        const double *c = kTaylorCoeffs[FN];                                                                 \
+        return scheme_taylor_call(P, S, x, n, out, c[0], c[1], c[2], FN, fp, stream, "pir_" #scheme "_" #name, \
+                                        scheme##_taylor);                                                    \
+    }
+PIR_SCHEME_TAYLOR(tpc, 2, 1, sqrt, TAYLOR_SQRT)
+PIR_SCHEME_TAYLOR(tpc, 2, 1, inverse, TAYLOR_INVERSE)
+PIR_SCHEME_TAYLOR(tpc, 2, 1, sigmoid, TAYLOR_SIGMOID)
+PIR_SCHEME_TAYLOR(fpc, 4, 3, sqrt, TAYLOR_SQRT)
+PIR_SCHEME_TAYLOR(fpc, 4, 3, inverse, TAYLOR_INVERSE)
+PIR_SCHEME_TAYLOR(fpc, 4, 3, sigmoid, TAYLOR_SIGMOID)
+#undef PIR_SCHEME_TAYLOR
+
 int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
     return guarded([&]() -> int {
         Scope sc(stream);

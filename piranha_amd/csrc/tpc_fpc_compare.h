@@ -38,4 +38,13 @@ int fpc_select(const u64 *x, const u64 *y, const void *b, u64 *z, size_t n, int 
 // {2, 4, 8, 16}; result [4][3][n / k], d [4][3][n]; pools modified in place
 int fpc_maxpool(u64 *pools, size_t n, u64 *result, void *d, int k, int eb, hipStream_t s);
 
+// getPowers (TPC.inl:1202-1226, FPC.inl:1418-1441): each party's opened
+// power, pow [2][n] (TPC) or [4][n] (FPC)
+int tpc_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s);
+int fpc_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s);
+// taylorSeries (TPC.inl:1228-1290, FPC.inl:1443-1526) with the tabulated
+// coefficient functors of taylor_tables(); out must not overlap in
+int tpc_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s);
+int fpc_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s);
+
 }  // namespace pir

@@ -355,7 +355,194 @@ __global__ __launch_bounds__(BLK) void fpc_maxpool_kernel(u64 *__restrict__ pool
     fstore(result, windows, w, fload(pools, n, base));
 }
 
+// ---------------------------------------------------------------------------
+// getPowers and taylorSeries (TPC.inl:1202-1290, FPC.inl:1418-1526) through a
+// scheme policy: the share algebra of one element for all parties.
+
+struct TpcPolicy {
+    static constexpr int P = 2;
+    using V = Tp2<u64>;
+    using B = Tp2<u8>;
+    static __device__ __forceinline__ V load(const u64 *b, size_t n, size_t i) { return tload(b, n, i); }
+    static __device__ __forceinline__ void store(u64 *b, size_t n, size_t i, const V &v) { tstore(b, n, i, v); }
+    static __device__ __forceinline__ V fill(u64 v) { return V{{v, 0}}; }  // TPC.inl:103-106
+    static __device__ __forceinline__ u64 open(const V &x, int) { return x.s[0] + x.s[1]; }
+    // TPC -= T and += DeviceData: party A only (TPC.inl:145-189)
+    static __device__ __forceinline__ void sub_public(V &x, u64 v) { x.s[0] -= v; }
+    static __device__ __forceinline__ void add_public(V &x, const u64 c[P]) { x.s[0] += c[0]; }
+    // *= DeviceData, <<= DeviceData: every share (TPC.inl:191-226)
+    static __device__ __forceinline__ void scale(V &x, const u64 c[P]) {
+        for (int q = 0; q < P; q++) x.s[q] *= c[q];
+    }
+    static __device__ __forceinline__ void shl(V &x, const u64 c[P]) {
+        for (int q = 0; q < P; q++) x.s[q] = shl_clamped(x.s[q], c[q]);
+    }
+    static __device__ __forceinline__ void add(V &x, const V &y) {
+        for (int q = 0; q < P; q++) x.s[q] += y.s[q];
+    }
+    static __device__ __forceinline__ V mult(const V &x, const V &y) { return tpc_mult_(x, y); }
+    static __device__ __forceinline__ B drelu(const V &x) { return tpc_drelu_<u8>(x); }
+    static __device__ __forceinline__ V select(const V &x, const V &y, const B &b) { return tpc_select_<u8>(x, y, b); }
+    // dividePublic(TPC, DeviceData) (TPC.inl:386-407): r = fill(1), r' = 0 + d
+    // (party A), a -= r'; each party opens a and divides by its d; a = r + opened (party A)
+    static __device__ __forceinline__ V divide_public(V a, const u64 d[P]) {
+        a.s[0] -= d[0];
+        return V{{1 + signed_div_u64(open(a, 0), d[0]), 0}};
+    }
+};
+
+struct FpcPolicy {
+    static constexpr int P = 4;
+    using V = Fp4<u64>;
+    using B = Fp4<u8>;
+    static __device__ __forceinline__ V load(const u64 *b, size_t n, size_t i) { return fload(b, n, i); }
+    static __device__ __forceinline__ void store(u64 *b, size_t n, size_t i, const V &v) { fstore(b, n, i, v); }
+    static __device__ __forceinline__ V fill(u64 v) {  // FPC.inl:119-124
+        V x;
+        for (int q = 0; q < P; q++)
+            for (int t = 0; t < 3; t++) x.s[q][t] = t == fpc_public_slot(q) ? v : 0;
+        return x;
+    }
+    static __device__ __forceinline__ u64 open(const V &x, int q) {  // FPC.inl:582-604
+        return x.s[q][0] + x.s[q][1] + x.s[q][2] + x.s[fpc_prev(q)][0];
+    }
+    // FPC -= T and += DeviceData: the public slot (FPC.inl:184-250)
+    static __device__ __forceinline__ void sub_public(V &x, u64 v) {
+        for (int q = 1; q < P; q++) x.s[q][fpc_public_slot(q)] -= v;
+    }
+    static __device__ __forceinline__ void add_public(V &x, const u64 c[P]) {
+        for (int q = 1; q < P; q++) x.s[q][fpc_public_slot(q)] += c[q];
+    }
+    // *= DeviceData, <<= DeviceData: every share (FPC.inl:252-300)
+    static __device__ __forceinline__ void scale(V &x, const u64 c[P]) {
+        for (int q = 0; q < P; q++)
+            for (int t = 0; t < 3; t++) x.s[q][t] *= c[q];
+    }
+    static __device__ __forceinline__ void shl(V &x, const u64 c[P]) {
+        for (int q = 0; q < P; q++)
+            for (int t = 0; t < 3; t++) x.s[q][t] = shl_clamped(x.s[q][t], c[q]);
+    }
+    static __device__ __forceinline__ void add(V &x, const V &y) {
+        for (int q = 0; q < P; q++)
+            for (int t = 0; t < 3; t++) x.s[q][t] += y.s[q][t];
+    }
+    static __device__ __forceinline__ V mult(V x, const V &y) {
+        fpc_product_<u64, false>(x, y);
+        return x;
+    }
+    static __device__ __forceinline__ B drelu(const V &x) { return fpc_drelu_<u8>(x); }
+    static __device__ __forceinline__ V select(const V &x, const V &y, const B &b) { return fpc_select_<u8>(x, y, b); }
+    // dividePublic(FPC, DeviceData) (FPC.inl:566-580 with the vector getDividedShares)
+    static __device__ __forceinline__ V divide_public(V a, const u64 d[P]) {
+        for (int q = 1; q < P; q++) a.s[q][fpc_public_slot(q)] -= d[q];
+        V o = fill(0);
+        for (int q = 1; q < P; q++) o.s[q][fpc_public_slot(q)] = 1 + signed_div_u64(open(a, q), d[q]);
+        return o;
+    }
+};
+
+// getPowers: 64 rounds of diff = in - 2^bit, dReLU(diff) in uint8 shares,
+// powers = selectShare(powers, fill(bit), dReLU).
+template <class S>
+__device__ __forceinline__ typename S::V scheme_get_powers_(const typename S::V &in) {
+    typename S::V powers = S::fill(0);
+#pragma unroll 1
+    for (int bit = 0; bit < 64; bit++) {
+        typename S::V diff = in;
+        S::sub_public(diff, (u64)1 << bit);
+        powers = S::select(powers, S::fill((u64)bit), S::drelu(diff));
+    }
+    return powers;
+}
+
+template <class S>
+__global__ __launch_bounds__(BLK) void scheme_get_powers_kernel(const u64 *__restrict__ in, size_t n,
+                                                                u64 *__restrict__ pow) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const typename S::V pw = scheme_get_powers_<S>(S::load(in, n, i));
+#pragma unroll
+    for (int q = 0; q < S::P; q++) pow[(size_t)q * n + i] = S::open(pw, q);
+}
+
+// taylorSeries for one element, each party with its own opened power
+// pow_q = getPowers_q + 1 and ones_q = 1 << pow_q (see the RSS taylor_kernel).
+template <class S>
+__global__ __launch_bounds__(BLK) void scheme_taylor_kernel(const u64 *__restrict__ in, size_t n,
+                                                            u64 *__restrict__ out, const TaylorTables t) {
+    using V = typename S::V;
+    constexpr int P = S::P;
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const V x = S::load(in, n, i);
+    const V pw = scheme_get_powers_<S>(x);
+    u64 pow[P], ones[P], c[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+        pow[q] = S::open(pw, q) + 1;
+        ones[q] = shl_clamped(1, pow[q]);
+    }
+    V o = S::fill(0);
+    auto lookup = [&](const TaylorTab &tab) {
+#pragma unroll
+        for (int q = 0; q < P; q++) c[q] = tab.at(pow[q]);
+    };
+    if (t.use_a2) {  // scratch = in * in / ones * a2Coeff / ones
+        V sc = S::divide_public(S::mult(x, x), ones);
+        lookup(t.c2);
+        S::scale(sc, c);
+        S::add(o, S::divide_public(sc, ones));
+    }
+    if (t.use_a1) {  // scratch = in * a1Coeff / ones
+        V sc = x;
+        lookup(t.c1);
+        S::scale(sc, c);
+        S::add(o, S::divide_public(sc, ones));
+    }
+    lookup(t.c0);
+    S::add_public(o, c);
+    lookup(t.f);
+    S::scale(o, c);
+    o = S::divide_public(o, ones);
+    // pow -= FP: the non-negative part divides, the negative part shifts every share left
+    u64 neg[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+        const u64 e = pow[q] - (u64)t.fp;
+        ones[q] = shl_clamped(1, (s64)e >= 0 ? e : 0);
+        neg[q] = (s64)e < 0 ? (u64)0 - e : 0;
+    }
+    o = S::divide_public(o, ones);
+    S::shl(o, neg);
+    S::store(out, n, i, o);
+}
+
 }  // namespace
+
+template <class S>
+static int scheme_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(scheme_get_powers_kernel<S>, dim3(grid(n)), dim3(BLK), 0, s, in, n, pow);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+template <class S>
+static int scheme_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(scheme_taylor_kernel<S>, dim3(grid(n)), dim3(BLK), 0, s, in, n, out, t);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
+int tpc_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s) { return scheme_get_powers<TpcPolicy>(in, n, pow, s); }
+int fpc_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s) { return scheme_get_powers<FpcPolicy>(in, n, pow, s); }
+int tpc_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s) {
+    return scheme_taylor<TpcPolicy>(in, n, out, t, s);
+}
+int fpc_taylor(const u64 *in, size_t n, u64 *out, const TaylorTables &t, hipStream_t s) {
+    return scheme_taylor<FpcPolicy>(in, n, out, t, s);
+}
 
 int fpc_mult(u64 *x, const u64 *y, size_t n, hipStream_t s) {
     if (n == 0) return OK;

@@ -222,6 +222,10 @@ int pir_fpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, ui
                          void *stream);                                          /* FPC.inl:671-712 */
 /* maxpool over gathered pools [n/k windows][k], k in {2, 4, 8, 16} (pools modified like the reference), FPC.inl:1004-1181 */
 int pir_fpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int elem_bytes, void *stream);
+/* dividePublic(Share, DeviceData) (TPC.inl:386-407, FPC.inl:566-580): per-party
+ * public denominators d_local [2][n] (TPC) or [4][n] (FPC) */
+int pir_tpc_divide_public_local(uint64_t *a, size_t n, const uint64_t *d_local, void *stream);
+int pir_fpc_divide_public_local(uint64_t *a, size_t n, const uint64_t *d_local, void *stream);
 /* getPowers (TPC.inl:1202-1226, FPC.inl:1418-1441): each party's opened power,
  * pow [2][n] / [4][n]; taylorSeries (TPC.inl:1228-1290, FPC.inl:1443-1526) and
  * sqrt / inverse / sigmoid (TPC.inl:615-642, FPC.inl:714-741) as pir_rss_taylor_series */

@@ -107,6 +107,8 @@ _SIGS = {
     "ref_get_powers": [_u64p, _sz, _u64p],
     "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
     "ref_tpc_mult": [_u64p, _u64p, _sz],
+    "ref_tpc_divide_public_vec": [_u64p, _sz, _u64p],
+    "ref_fpc_divide_public_vec": [_u64p, _sz, _u64p],
     "ref_tpc_get_powers": [_u64p, _sz, _u64p],
     "ref_fpc_get_powers": [_u64p, _sz, _u64p],
     "ref_tpc_taylor_series": [_u64p, _sz, _u64p, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _i],
@@ -1132,3 +1134,10 @@ def scheme_taylor_series(kind, x, a0, a1, a2, fn, fp):
 def scheme_taylor(kind, x, fn, fp):
     """sqrt / inverse / sigmoid of a TPC or FPC value with the reference's coefficients."""
     return scheme_taylor_series(kind, x, *TAYLOR_COEFFS[fn], fn, fp)
+
+
+def scheme_divide_public_vec(kind, a, d):
+    """dividePublic(Share, DeviceData) of a TPC or FPC value by per-party denominators d (P, n)."""
+    a = _u(a).copy()
+    getattr(lib(), f"ref_{kind}_divide_public_vec")(_p(a), a.shape[-1], _p(_u(d)))
+    return a

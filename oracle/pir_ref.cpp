@@ -2604,6 +2604,15 @@ void ref_fpc_get_powers(const u64 *in, size_t n, u64 *pow) {
     for (int q = 0; q < 4; q++) pv[q] = View<u64>{pow + q * n, 0, 1, n};
     scheme_get_powers<FpcAlg>(wrapFv((u64 *)in, n), pv);
 }
+void ref_tpc_divide_public_vec(u64 *a, size_t n, const u64 *d) {
+    View<u64> dv[2] = {View<u64>{(u64 *)d, 0, 1, n}, View<u64>{(u64 *)d + n, 0, 1, n}};
+    scheme_divide_public_vec<TpcAlg>(wrapTv(a, n), dv);
+}
+void ref_fpc_divide_public_vec(u64 *a, size_t n, const u64 *d) {
+    View<u64> dv[4];
+    for (int q = 0; q < 4; q++) dv[q] = View<u64>{(u64 *)d + q * n, 0, 1, n};
+    scheme_divide_public_vec<FpcAlg>(wrapFv(a, n), dv);
+}
 void ref_tpc_taylor_series(const u64 *in, size_t n, u64 *out, double a0, double a1, double a2, int fn, int fp) {
     scheme_taylor_series<TpcAlg>(wrapTv((u64 *)in, n), wrapTv(out, n), a0, a1, a2, fn, fp);
 }

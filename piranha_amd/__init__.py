@@ -142,6 +142,8 @@ _SIGS = {
     "pir_tpc_divide_public": ([_vp, _sz, _u64, _vp], _i),
     "pir_tpc_mult": ([_vp, _vp, _sz, _vp], _i),
     "pir_fpc_mult": ([_vp, _vp, _sz, _vp], _i),
+    "pir_tpc_divide_public_local": ([_vp, _sz, _vp, _vp], _i),
+    "pir_fpc_divide_public_local": ([_vp, _sz, _vp, _vp], _i),
     "pir_tpc_get_powers": ([_vp, _sz, _vp, _vp], _i),
     "pir_fpc_get_powers": ([_vp, _sz, _vp, _vp], _i),
     "pir_tpc_taylor_series": ([_vp, _sz, _vp, _d, _d, _d, _i, _i, _vp], _i),

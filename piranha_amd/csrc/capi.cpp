@@ -652,6 +652,19 @@ int scheme_taylor_call(int P, int S, const uint64_t *x, size_t n, uint64_t *out,
 const double kTaylorCoeffs[3][3] = {{0.424, 0.584, 0.0}, {4.245, -5.857, 2.630}, {0.494286, 0.275589, -0.038751}};
 }  // namespace
 
+int pir_tpc_divide_public_local(uint64_t *a, size_t n, const uint64_t *d_local, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return tpc_divide_public_vec(a, n, d_local, (hipStream_t)stream);
+    });
+}
+int pir_fpc_divide_public_local(uint64_t *a, size_t n, const uint64_t *d_local, void *stream) {
+    return guarded([&]() -> int {
+        Scope sc(stream);
+        return fpc_divide_public_vec(a, n, d_local, (hipStream_t)stream);
+    });
+}
+
 int pir_tpc_get_powers(const uint64_t *x, size_t n, uint64_t *pow, void *stream) {
     return scheme_get_powers_call(2, 1, x, n, pow, stream, "pir_tpc_get_powers", tpc_get_powers);
 }

@@ -38,6 +38,10 @@ int fpc_select(const u64 *x, const u64 *y, const void *b, u64 *z, size_t n, int 
 // {2, 4, 8, 16}; result [4][3][n / k], d [4][3][n]; pools modified in place
 int fpc_maxpool(u64 *pools, size_t n, u64 *result, void *d, int k, int eb, hipStream_t s);
 
+// dividePublic(Share, DeviceData) (TPC.inl:386-407, FPC.inl:566-580 with the
+// vector getDividedShares, Precompute.h:121-134): d [2][n] / [4][n] per party
+int tpc_divide_public_vec(u64 *a, size_t n, const u64 *d, hipStream_t s);
+int fpc_divide_public_vec(u64 *a, size_t n, const u64 *d, hipStream_t s);
 // getPowers (TPC.inl:1202-1226, FPC.inl:1418-1441): each party's opened
 // power, pow [2][n] (TPC) or [4][n] (FPC)
 int tpc_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s);

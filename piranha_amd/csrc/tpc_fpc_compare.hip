@@ -517,7 +517,33 @@ __global__ __launch_bounds__(BLK) void scheme_taylor_kernel(const u64 *__restric
     S::store(out, n, i, o);
 }
 
+// dividePublic(Share, DeviceData): d [P][n], each party's public denominators
+template <class S>
+__global__ __launch_bounds__(BLK) void scheme_divide_public_kernel(u64 *__restrict__ a, size_t n,
+                                                                   const u64 *__restrict__ d) {
+    size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    u64 dq[S::P];
+#pragma unroll
+    for (int q = 0; q < S::P; q++) dq[q] = d[(size_t)q * n + i];
+    S::store(a, n, i, S::divide_public(S::load(a, n, i), dq));
+}
+
 }  // namespace
+
+template <class S>
+static int scheme_divide_public(u64 *a, size_t n, const u64 *d, hipStream_t s) {
+    if (n == 0) return OK;
+    hipLaunchKernelGGL(scheme_divide_public_kernel<S>, dim3(grid(n)), dim3(BLK), 0, s, a, n, d);
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+int tpc_divide_public_vec(u64 *a, size_t n, const u64 *d, hipStream_t s) {
+    return scheme_divide_public<TpcPolicy>(a, n, d, s);
+}
+int fpc_divide_public_vec(u64 *a, size_t n, const u64 *d, hipStream_t s) {
+    return scheme_divide_public<FpcPolicy>(a, n, d, s);
+}
 
 template <class S>
 static int scheme_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s) {

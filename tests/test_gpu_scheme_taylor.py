@@ -105,3 +105,15 @@ def test_argument_errors(rng, kind):
     with pytest.raises(P.PirError, match="precision"):
         P.check(getattr(lib, f"pir_{kind}_inverse")(dx.ptr, 16, dy.ptr, 63, None))
     P.check(getattr(lib, f"pir_{kind}_sigmoid")(dx.ptr, 0, dy.ptr, 20, None))  # empty
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("general", [False, True], ids=["init", "general"])
+def test_divide_public_local(rng, kind, general):
+    n = 400
+    a = make(rng, kind, n, 20, general)
+    d = rng.integers(1, 1 << 20, (SHAPE[kind][0], n)).astype(np.uint64)
+    d[:, :8] = [1, 2, 3, 0, 7, 1 << 20, 1 << 40, (1 << 64) - 1]  # including 0 and a "negative" denominator
+    da, dd = P.to_dev(a), P.to_dev(d)
+    P.check(getattr(P.lib(), f"pir_{kind}_divide_public_local")(da.ptr, n, dd.ptr, None))
+    assert np.array_equal(da.numpy(), O.scheme_divide_public_vec(kind, a, d))

@@ -58,3 +58,22 @@ def test_matches_rss(rng, kind, fn, fp):
     r = O.scheme_taylor(kind, INIT[kind](v, fp), fn, fp)
     want = O.reconstruct(O.taylor(O.init_list(v, fp), fn, fp))[0]
     assert np.array_equal(opened(kind, r), want)
+
+
+@pytest.mark.parametrize("kind", ["tpc", "fpc"])
+def test_truncate_vec_kat(kind):
+    # Func.cu:133-154 (TruncateVec)
+    a = INIT[kind]([1 << 3, 2 << 1, 3 << 0, -3 << 5, 2515014.0], FP)
+    d = np.array([1 << 3, 1 << 1, 1, 1 << 5, 1 << 2], np.uint64)
+    r = O.scheme_divide_public_vec(kind, a, np.tile(d, (a.shape[0], 1)))
+    got = opened(kind, r).view(np.int64) / 2.0**FP
+    assert np.allclose(got, [1, 2, 3, -3, 628753.5], atol=1e-3)
+
+
+@pytest.mark.parametrize("kind", ["tpc", "fpc"])
+def test_divide_public_vec_matches_scalar(rng, kind):
+    a = INIT[kind](rng.uniform(-100, 100, 64), FP)
+    for d in (1, 3, 1 << 13, 1000003):
+        dv = np.full((a.shape[0], 64), d, np.uint64)
+        scalar = O.tpc_divide_public(a, d) if kind == "tpc" else O.fpc_divide_public(a, d)
+        assert np.array_equal(O.scheme_divide_public_vec(kind, a, dv), scalar)

@@ -134,3 +134,23 @@ def test_argument_errors(rng):
     with pytest.raises(P.PirError, match="overlap"):
         P.check(P.lib().pir_fpc_relu(dx.ptr, 32, dx.ptr, dd.ptr, 8, None))
     P.check(P.lib().pir_fpc_drelu(dx.ptr, 0, dd.ptr, 1, None))  # empty
+
+
+def test_relu_full_size_closed_form(rng):
+    # 4 M elements in the public component: [v >= 0] in the public slots and
+    # max(v, 0) opened alike by every party
+    n = 1 << 22
+    v = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    v[:3] = [0, -1, 1]
+    x = np.zeros((4, 3, n), np.uint64)
+    for q, s in ((1, 2), (2, 1), (3, 0)):
+        x[q, s] = v.view(np.uint64)
+    dx, dy, dd = P.to_dev(x), P.Buffer((4, 3, n), np.uint64), P.Buffer((4, 3, n), np.uint8)
+    P.check(P.lib().pir_fpc_relu(dx.ptr, n, dy.ptr, dd.ptr, 1, None))
+    d, y = dd.numpy(), dy.numpy()
+    bit = (v >= 0).astype(np.uint8)
+    for q in range(4):
+        for s in range(3):
+            assert np.array_equal(d[q, s], bit if (q, s) in ((1, 2), (2, 1), (3, 0)) else np.zeros(n, np.uint8))
+    opened = O.fpc_reconstruct(y)
+    assert all(np.array_equal(opened[q].view(np.int64), np.maximum(v, 0)) for q in range(4))

@@ -136,3 +136,19 @@ def test_argument_errors(rng):
     with pytest.raises(P.PirError, match="overlap"):
         P.check(P.lib().pir_tpc_relu(dx.ptr, 8, dx.ptr, dd.ptr, 8, None))
     P.check(P.lib().pir_tpc_drelu(dx.ptr, 0, dd.ptr, 1, None))  # empty
+
+
+def test_relu_full_size_closed_form(rng):
+    # 4 M elements (a VGG16 b64 conv1 activation): [v >= 0] and max(v, 0) on
+    # every element, with party B's shares zero
+    n = 1 << 22
+    v = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    v[:3] = [0, -1, 1]
+    x = np.zeros((2, n), np.uint64)
+    r = rng.integers(0, 1 << 62, n).astype(np.uint64)
+    x[0], x[1] = v.view(np.uint64) - r, r
+    dx, dy, dd = P.to_dev(x), P.Buffer((2, n), np.uint64), P.Buffer((2, n), np.uint8)
+    P.check(P.lib().pir_tpc_relu(dx.ptr, n, dy.ptr, dd.ptr, 1, None))
+    d, y = dd.numpy(), dy.numpy()
+    assert np.array_equal(d[0], (v >= 0).astype(np.uint8)) and not d[1].any()
+    assert np.array_equal((y[0] + y[1]).view(np.int64), np.maximum(v, 0))

@@ -11,7 +11,8 @@ Also reported on the same line:
     VALU GEMMs, one launch per DeviceData / RSS op, uint8-per-bit dReLU);
   * ring_gemm: the 3-term RSS local product A0B0+A0B1+A1B0 at M=N=K in
     {512..8192}, Gop/s = 6MNK/t, plain VALU vs int8-limb MFMA;
-  * other_configs: SecureML MLP, LeNet and the uneven-batch AlexNet workload;
+  * other_configs: SecureML MLP, LeNet and the uneven-batch AlexNet workload,
+    and the headline forward on uniformly random sharings (below);
   * roofline of the dominant kernel (the limb implicit-GEMM convolution),
     timed per launch with HIP events; its HBM traffic per launch comes from
     the committed rocprofv3 counter summary under profiles/;
@@ -201,6 +202,43 @@ def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True, autotune=True)
     if capture:
         net.capture()
     return net
+
+
+def random_sharing(shares, rng):
+    """A uniformly random consistent RSS sharing of the values that party A
+    opens from `shares` (3, 2, n): x1, x2 random, x0 = v - x1 - x2, party p
+    holding (x_p, x_p+1)."""
+    v = shares[0, 0] + shares[0, 1] + shares[2, 0]
+    x1 = rng.integers(0, 2**64 - 1, v.size, dtype=np.uint64, endpoint=True)
+    x2 = rng.integers(0, 2**64 - 1, v.size, dtype=np.uint64, endpoint=True)
+    x0 = v - x1 - x2
+    return np.ascontiguousarray(np.stack([np.stack([x0, x1]), np.stack([x1, x2]), np.stack([x2, x0])]))
+
+
+def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True):
+    """The headline forward with every weight and input share re-dealt as a
+    uniformly random sharing of the same values: the same GEMMs and kernels
+    on full-entropy digit planes, where the reference's placeholder precompute
+    leaves two of a value's three components at zero."""
+    net = make_net(P, C, layers, batch, fp, P.MODE_FUSED, capture=False, autotune=False)
+    rng = np.random.default_rng(7)
+    for li in range(net.num_layers):
+        for k, (_, sh) in enumerate(net.params(li)):
+            net.set_param_shares(li, k, random_sharing(sh, rng))
+    ptr, n = net.input_ptr()
+    host = np.zeros((3, 2, n), np.uint64)
+    P.check(P.lib().pir_memcpy_d2h(host.ctypes.data, ptr, host.nbytes, None))
+    host = random_sharing(host, rng)
+    P.check(P.lib().pir_memcpy_h2d(ptr, host.ctypes.data, host.nbytes, None))
+    net.forward_shares(None)
+    net.synchronize()
+    if autotune:
+        net.autotune()
+    net.capture()
+    t = time_graph(P, net, steps, warmup)
+    del net
+    return {"images_per_s": batch / t, "ms_per_forward": t * 1e3,
+            "sharing": "uniformly random consistent RSS sharings of the same weights and inputs"}
 
 
 def bench_training(P, C, layers, batch, fp, steps, warmup, autotune=True):
@@ -433,6 +471,9 @@ def main():
                                             "grouping": "queries concatenated along the batch axis into one graph"}
         log(f"alexnet uneven: {total} images, {total / tt:.1f} images/s")
         del n
+        rs = bench_random_shares(P, C, layers, batch, fp, args.steps, 20, autotune=not args.no_autotune)
+        other["vgg16_b64_fp26_random_sharings"] = rs
+        log(f"VGG16 on random sharings: {rs['images_per_s']:.1f} images/s")
         result["other_configs"] = other
 
         log("CPU baseline (oracle)")

@@ -1,8 +1,6 @@
 // runtime.cpp -- context, caching allocator, error state, RSS host helpers.
 #include "runtime.h"
 
-#include <mutex>
-
 #include "device_data.h"
 
 namespace pir {
@@ -12,7 +10,9 @@ static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 const char *last_error() { return g_error.c_str(); }
 
-static Context g_default_ctx;
+// The free-standing calls run in a per-thread default context, so threads
+// calling the library at once share no stream setting or cached block.
+static thread_local Context g_default_ctx;
 static thread_local Context *g_ctx = nullptr;
 
 Context &ctx() { return g_ctx ? *g_ctx : g_default_ctx; }
@@ -23,9 +23,9 @@ Allocator::~Allocator() {
     for (auto &kv : used_) (void)hipFree(kv.first);
 }
 
-void *Allocator::alloc(size_t bytes) {
+void *Allocator::alloc(size_t bytes, hipStream_t stream) {
     size_t sz = (bytes + 255) & ~(size_t)255;
-    auto it = free_.find(sz);
+    auto it = free_.find({stream, sz});
     if (it != free_.end()) {
         void *p = it->second;
         free_.erase(it);
@@ -39,7 +39,8 @@ void *Allocator::alloc(size_t bytes) {
     }
     void *p = nullptr;
     if (hipMalloc(&p, sz) != hipSuccess) {
-        // release cached blocks and retry once
+        // release the cached blocks of every stream once their work is done, and retry once
+        (void)hipDeviceSynchronize();
         for (auto &kv : free_) (void)hipFree(kv.second);
         reserved_ = 0;
         for (auto &kv : used_) reserved_ += kv.second;
@@ -54,10 +55,10 @@ void *Allocator::alloc(size_t bytes) {
     return p;
 }
 
-void Allocator::release(void *p) {
+void Allocator::release(void *p, hipStream_t stream) {
     auto it = used_.find(p);
     if (it == used_.end()) return;
-    free_.emplace(it->second, p);
+    free_.emplace(std::make_pair(stream, it->second), p);
     used_.erase(it);
 }
 

@@ -19,16 +19,19 @@ namespace pir {
 enum ExecMode { MODE_FUSED = 0, MODE_PLAIN = 1 };
 enum GemmAlgo { ALGO_PLAIN = 0, ALGO_LIMB = 1 };
 
+// Freed blocks are cached per stream and handed out again only to requests on
+// the same stream, whose order then keeps the block's earlier kernels ahead of
+// its new ones; no block moves between streams while work may be pending.
 class Allocator {
   public:
     ~Allocator();
-    void *alloc(size_t bytes);
-    void release(void *p);
+    void *alloc(size_t bytes, hipStream_t stream);
+    void release(void *p, hipStream_t stream);
     size_t bytes_reserved() const { return reserved_; }
     bool capturing = false;  // set while a hipGraph capture is open: no new hipMalloc allowed
 
   private:
-    std::multimap<size_t, void *> free_;
+    std::multimap<std::pair<hipStream_t, size_t>, void *> free_;
     std::unordered_map<void *, size_t> used_;
     size_t reserved_ = 0;
 };
@@ -68,10 +71,10 @@ class DevBuf {
   public:
     DevBuf() = default;
     explicit DevBuf(size_t bytes) : bytes_(bytes) {
-        if (bytes) p_ = ctx().alloc.alloc(bytes);
+        if (bytes) p_ = ctx().alloc.alloc(bytes, ctx().stream);
     }
     ~DevBuf() {
-        if (p_) ctx().alloc.release(p_);
+        if (p_) ctx().alloc.release(p_, ctx().stream);
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -81,7 +84,7 @@ class DevBuf {
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
-            if (p_) ctx().alloc.release(p_);
+            if (p_) ctx().alloc.release(p_, ctx().stream);
             p_ = o.p_;
             bytes_ = o.bytes_;
             o.p_ = nullptr;

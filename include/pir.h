@@ -34,9 +34,8 @@ int pir_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream); /* s
 int pir_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 int pir_fill_u64(uint64_t *dst, uint64_t value, size_t n, void *stream); /* DeviceData::fill, DeviceData.h:83-85 */
 int pir_fill_u8(uint8_t *dst, uint8_t value, size_t n, void *stream);
-/* Streams and threads: every call queues its work on the stream it is given
- * (NULL: the null stream) and returns without waiting, except the copies
- * marked synchronous.  Temporary device buffers are cached per stream and
+/* Streams and threads: a call queues its device work on the stream it is
+ * given (NULL: the null stream).  Temporary device buffers are cached per stream and
  * reused only on the same stream, so successive calls may alternate streams
  * without synchronising.  The free-standing calls keep their state per
  * calling thread; a pir_net belongs to one thread at a time.  pir_set_mode

@@ -364,6 +364,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="heuristic GEMM plans only")
     ap.add_argument("--train-only", action="store_true", help="VGG16 training step only (for profiling runs)")
+    ap.add_argument("--random-sharings-only", action="store_true",
+                    help="the VGG16 forward on uniformly random sharings only (for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -382,6 +384,13 @@ def main():
                             autotune=not args.no_autotune)
         log(f"VGG16 training: {tr['ms_per_step']:.2f} ms/step, {tr['images_per_s']:.1f} images/s")
         print(json.dumps({"training": tr}), flush=True)
+        return
+
+    if args.random_sharings_only:
+        rs = bench_random_shares(P, C, C.VGG16_CIFAR, args.batch, 26, args.steps, args.warmup,
+                                 autotune=not args.no_autotune)
+        log(f"VGG16 on random sharings: {rs['ms_per_forward']:.3f} ms/forward, {rs['images_per_s']:.1f} images/s")
+        print(json.dumps({"random_sharings": rs}), flush=True)
         return
 
     layers, batch, fp = C.VGG16_CIFAR, args.batch, 26

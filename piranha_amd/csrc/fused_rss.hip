@@ -219,6 +219,19 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
         store3(relu_out, outN, o, select_<u8>(zero, e[0], rb));
     }
     const size_t mpN = outN * f2;
+    if (sizeof(U) == 1 && K == 4 && f2 == 4) {
+        // the window's four derivative bytes of each share array in one 4-byte store
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int sh = 0; sh < 2; sh++) {
+                u32 v = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) v |= (u32)(u8)d[j].s[p][sh] << (8 * j);
+                *(u32 *)(mprime + (size_t)(2 * p + sh) * mpN + o * 4) = v;
+            }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < K; j++)
         if (j < f2) store3(mprime, mpN, o * f2 + j, d[j]);

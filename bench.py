@@ -192,8 +192,13 @@ def time_eager(net, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True, autotune=True):
+def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True, autotune=True, sigma=None):
+    """The network with the layers' default (He) weights, or N(0, sigma)
+    weights and 0.01 FC biases when `sigma` is given."""
     net = P.Network(C.to_spec(layers), batch, fp, seed=seed, mode=mode)
+    if sigma is not None:
+        for i, (w, b) in C.normal_weights(layers, sigma, seed).items():
+            net.set_weights(i, w, b)
     rng = np.random.default_rng(seed)
     net.forward(rng.uniform(0.0, 1.0, batch * C.input_size(layers)))  # setPublic of U[0,1) inputs
     net.synchronize()
@@ -465,14 +470,13 @@ def main():
 
         other = {}
         for name, L, b, f in [("secureml_b128_fp13", C.SECUREML, 128, 13), ("lenet_b128_fp13", C.LENET, 128, 13)]:
-            n = make_net(P, C, L, b, f, P.MODE_FUSED)
+            n = make_net(P, C, L, b, f, P.MODE_FUSED, sigma=0.1)  # N(0, 0.1) weights
             tt = time_graph(P, n, max(5, args.steps // 2), 2)
             other[name] = {"images_per_s": b / tt, "ms_per_forward": tt * 1e3}
             log(f"{name}: {b / tt:.1f} images/s")
             del n
-        qrng = np.random.default_rng(20)
-        qb = qrng.integers(1, 33, 64)
-        total = int(qb.sum())
+        qb = C.uneven_query_batches()
+        total = int(sum(qb))
         n = make_net(P, C, C.ALEXNET_CIFAR, total, 20, P.MODE_FUSED)
         tt = time_graph(P, n, max(5, args.steps // 2), 2)
         other["alexnet_uneven_64q_fp20"] = {"images_per_s": total / tt, "ms_per_forward": tt * 1e3,

@@ -37,11 +37,18 @@ int pir_fill_u8(uint8_t *dst, uint8_t value, size_t n, void *stream);
 /* Streams and threads: a call queues its device work on the stream it is
  * given (NULL: the null stream).  Temporary device buffers are cached per stream and
  * reused only on the same stream, so successive calls may alternate streams
- * without synchronising.  The free-standing calls keep their state per
- * calling thread; a pir_net belongs to one thread at a time.  pir_set_mode
- * is process-wide. */
+ * without synchronising.  The free-standing calls keep their state (and their
+ * cache of temporaries) per calling thread; a pir_net belongs to one thread at
+ * a time and keeps its own cache until pir_net_destroy.  pir_set_mode is
+ * process-wide.  pir_stream_destroy synchronises the stream and returns the
+ * calling thread's cached temporaries of that stream to HIP. */
 int pir_stream_create(void **stream);
 int pir_stream_destroy(void *stream);
+/* No counterpart in the reference (thrust frees every temporary at once):
+ * synchronise the device and return all of the calling thread's cached
+ * temporaries to HIP; the bytes the calling thread's cache holds. */
+int pir_release_cached(void);
+size_t pir_cached_bytes(void);
 int pir_stream_synchronize(void *stream);
 int pir_device_synchronize(void);
 

@@ -44,6 +44,8 @@ _SIGS = {
     "pir_fill_u8": ([_vp, ctypes.c_uint8, _sz, _vp], _i),
     "pir_stream_create": ([ctypes.POINTER(_vp)], _i),
     "pir_stream_destroy": ([_vp], _i),
+    "pir_release_cached": ([], _i),
+    "pir_cached_bytes": ([], _sz),
     "pir_stream_synchronize": ([_vp], _i),
     "pir_device_synchronize": ([], _i),
     "pir_set_mode": ([_i], _i),

@@ -120,3 +120,29 @@ ALEXNET_CIFAR = [
 ]
 
 MODELS = {"secureml": SECUREML, "lenet": LENET, "vgg16_cifar": VGG16_CIFAR, "alexnet_cifar": ALEXNET_CIFAR}
+
+
+def uneven_query_batches(queries=64, seed=20):
+    """Per-query batch sizes of the uneven-batch workload: `queries` concurrent
+    AlexNet-CIFAR inferences, each with a batch drawn uniformly from 1..32 by
+    numpy's default_rng(seed) (64 queries, 986 images for seed 20)."""
+    import numpy as np
+
+    return [int(b) for b in np.random.default_rng(seed).integers(1, 33, queries)]
+
+
+def normal_weights(layers, sigma, seed, bias=0.01):
+    """{layer: (weights, biases)} as doubles for every FC / CNN layer, weights
+    drawn from N(0, sigma) by numpy's default_rng(seed) in layer order, FC
+    biases `bias` (the reference's default, FCLayer.cu:46), CNN layers without
+    bias; for pir_net_set_weights (layouts [in][out] and [Cout][f][f][Cin])."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = {}
+    for i, l in enumerate(layers):
+        if l[0] == "fc":
+            out[i] = (rng.normal(0.0, sigma, l[1] * l[2]), np.full(l[2], bias))
+        elif l[0] == "cnn":
+            out[i] = (rng.normal(0.0, sigma, l[3] * l[4] * l[5] * l[5]), None)
+    return out

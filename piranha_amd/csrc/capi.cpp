@@ -1,6 +1,7 @@
 // capi.cpp -- extern "C" entry points of libpir_hip.so (declared in include/pir.h).
 #include "../../include/pir.h"
 
+#include <atomic>
 #include <memory>
 
 #include "elementwise.h"
@@ -26,14 +27,15 @@ bool overlaps(const void *a, size_t abytes, const void *b, size_t bbytes) {
     return x < y + bbytes && y < x + abytes;
 }
 
-// Settings of the default context used by the free-standing pir_rss_* calls.
-int g_mode = MODE_FUSED;
+// Execution form of the free-standing pir_rss_* calls, process-wide (read by
+// every call on any thread, written by pir_set_mode).
+std::atomic<int> g_mode{MODE_FUSED};
 
 struct Scope {
     explicit Scope(void *stream, int algo = ALGO_LIMB) {
         Context &c = ctx();
         c.stream = (hipStream_t)stream;
-        c.mode = g_mode;
+        c.mode = g_mode.load(std::memory_order_relaxed);
         c.algo = algo;
     }
 };
@@ -170,9 +172,19 @@ int pir_stream_create(void **stream) {
     return OK;
 }
 int pir_stream_destroy(void *stream) {
+    // the calling thread's cached temporaries of this stream can never be
+    // handed out again: return them to HIP once the stream's work is done
+    PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    ctx().alloc.trim_stream((hipStream_t)stream);
     PIR_HIP(hipStreamDestroy((hipStream_t)stream));
     return OK;
 }
+int pir_release_cached(void) {
+    PIR_HIP(hipDeviceSynchronize());
+    ctx().alloc.trim_all();
+    return OK;
+}
+size_t pir_cached_bytes(void) { return ctx().alloc.bytes_cached(); }
 int pir_stream_synchronize(void *stream) {
     PIR_HIP(hipStreamSynchronize((hipStream_t)stream));
     return OK;
@@ -183,7 +195,7 @@ int pir_device_synchronize(void) {
 }
 int pir_set_mode(int mode) {
     PIR_REQUIRE(mode == MODE_FUSED || mode == MODE_PLAIN, "pir_set_mode: unknown mode");
-    g_mode = mode;
+    g_mode.store(mode, std::memory_order_relaxed);
     return OK;
 }
 

@@ -30,15 +30,18 @@ __device__ __forceinline__ T ew_apply(int op, T x, T y) {
         case OP_SHL: return y >= (T)(8 * sizeof(T)) ? (T)0 : (T)(x << y);
         case OP_DIV: {
             // functors.cuh:33-45 / 94-105: (S)((double)(S)x / (double)(S)y)
-            if (sizeof(T) == 8) return (T)signed_div_u64((u64)x, (u64)y);
-            // narrow types: the double quotient of two small integers truncates
-            // to the integer quotient (C division truncates toward zero too),
-            // clamped to the type's range as the 64-bit form is
-            const int lo = -(1 << (sizeof(T) * 8 - 1)), hi = (1 << (sizeof(T) * 8 - 1)) - 1;
-            int xs = (int)(S)x, ys = (int)(S)y;
-            if (ys == 0) return (T)(S)(xs == 0 ? 0 : (xs < 0 ? lo : hi));
-            int q = xs / ys;
-            return (T)(S)(q > hi ? hi : q);
+            if constexpr (sizeof(T) == 8) {
+                return (T)signed_div_u64((u64)x, (u64)y);
+            } else {
+                // narrow types: the double quotient of two small integers truncates
+                // to the integer quotient (C division truncates toward zero too),
+                // clamped to the type's range as the 64-bit form is
+                const int lo = -(1 << (sizeof(T) * 8 - 1)), hi = (1 << (sizeof(T) * 8 - 1)) - 1;
+                int xs = (int)(S)x, ys = (int)(S)y;
+                if (ys == 0) return (T)(S)(xs == 0 ? 0 : (xs < 0 ? lo : hi));
+                int q = xs / ys;
+                return (T)(S)(q > hi ? hi : q);
+            }
         }
         case OP_SET: return y;
     }

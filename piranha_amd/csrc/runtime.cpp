@@ -41,10 +41,7 @@ void *Allocator::alloc(size_t bytes, hipStream_t stream) {
     if (hipMalloc(&p, sz) != hipSuccess) {
         // release the cached blocks of every stream once their work is done, and retry once
         (void)hipDeviceSynchronize();
-        for (auto &kv : free_) (void)hipFree(kv.second);
-        reserved_ = 0;
-        for (auto &kv : used_) reserved_ += kv.second;
-        free_.clear();
+        trim_all();
         if (hipMalloc(&p, sz) != hipSuccess) {
             set_error("hipMalloc failed for " + std::to_string(sz) + " bytes");
             throw PirError(ERR_HIP);
@@ -53,6 +50,32 @@ void *Allocator::alloc(size_t bytes, hipStream_t stream) {
     reserved_ += sz;
     used_[p] = sz;
     return p;
+}
+
+void Allocator::trim_stream(hipStream_t stream) {
+    for (auto it = free_.begin(); it != free_.end();) {
+        if (it->first.first == stream) {
+            (void)hipFree(it->second);
+            reserved_ -= it->first.second;
+            it = free_.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+void Allocator::trim_all() {
+    for (auto &kv : free_) {
+        (void)hipFree(kv.second);
+        reserved_ -= kv.first.second;
+    }
+    free_.clear();
+}
+
+size_t Allocator::bytes_cached() const {
+    size_t b = 0;
+    for (auto &kv : free_) b += kv.first.second;
+    return b;
 }
 
 void Allocator::release(void *p, hipStream_t stream) {

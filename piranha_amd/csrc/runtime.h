@@ -27,7 +27,13 @@ class Allocator {
     ~Allocator();
     void *alloc(size_t bytes, hipStream_t stream);
     void release(void *p, hipStream_t stream);
+    // Returns the cached free blocks of `stream` to HIP (the caller has
+    // synchronised the stream, or is about to destroy it after doing so).
+    void trim_stream(hipStream_t stream);
+    // Returns every cached free block to HIP (after a device synchronisation).
+    void trim_all();
     size_t bytes_reserved() const { return reserved_; }
+    size_t bytes_cached() const;
     bool capturing = false;  // set while a hipGraph capture is open: no new hipMalloc allowed
 
   private:
@@ -66,29 +72,37 @@ struct Context {
 Context &ctx();
 void set_ctx(Context *c);
 
-// RAII device buffer from the context allocator.
+// RAII device buffer from the context allocator.  The buffer remembers the
+// allocator and stream it came from and returns the block to exactly those,
+// whichever context is current when it is destroyed.
 class DevBuf {
   public:
     DevBuf() = default;
     explicit DevBuf(size_t bytes) : bytes_(bytes) {
-        if (bytes) p_ = ctx().alloc.alloc(bytes, ctx().stream);
+        if (bytes) {
+            alloc_ = &ctx().alloc;
+            stream_ = ctx().stream;
+            p_ = alloc_->alloc(bytes, stream_);
+        }
     }
-    ~DevBuf() {
-        if (p_) ctx().alloc.release(p_, ctx().stream);
-    }
+    ~DevBuf() { reset(); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), bytes_(o.bytes_), alloc_(o.alloc_), stream_(o.stream_) {
         o.p_ = nullptr;
         o.bytes_ = 0;
+        o.alloc_ = nullptr;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
-            if (p_) ctx().alloc.release(p_, ctx().stream);
+            reset();
             p_ = o.p_;
             bytes_ = o.bytes_;
+            alloc_ = o.alloc_;
+            stream_ = o.stream_;
             o.p_ = nullptr;
             o.bytes_ = 0;
+            o.alloc_ = nullptr;
         }
         return *this;
     }
@@ -96,8 +110,16 @@ class DevBuf {
     size_t bytes() const { return bytes_; }
 
   private:
+    void reset() {
+        if (p_ && alloc_) alloc_->release(p_, stream_);
+        p_ = nullptr;
+        bytes_ = 0;
+        alloc_ = nullptr;
+    }
     void *p_ = nullptr;
     size_t bytes_ = 0;
+    Allocator *alloc_ = nullptr;
+    hipStream_t stream_ = nullptr;
 };
 
 // Host-side failure inside the C++ protocol API (which mirrors the reference's

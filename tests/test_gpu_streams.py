@@ -2,7 +2,8 @@
 back-to-back products alternating between two streams with no
 synchronisation between them (temporary buffers must not move between streams
 while work is pending), and two threads calling at once, each on its own
-stream; every result against the oracle."""
+stream; every result against the oracle.  Destroying a stream returns the
+calling thread's cached temporaries of that stream."""
 import ctypes
 import threading
 
@@ -23,10 +24,13 @@ def new_stream():
     return s.value
 
 
-def cases(rng, count):
+def cases(rng, count, same_shape=False):
     out = []
     for i in range(count):
-        M, N, K = 64 + 8 * (i % 5), 96 + 16 * (i % 3), 160 + 32 * (i % 4)
+        if same_shape:
+            M, N, K = 64, 96, 160
+        else:
+            M, N, K = 64 + 8 * (i % 5), 96 + 16 * (i % 3), 160 + 32 * (i % 4)
         a = O.init_list(rng.uniform(-4, 4, M * K), FP)
         b = O.init_list(rng.uniform(-4, 4, K * N), FP)
         out.append((M, N, K, a, b, O.matmul(a, b, M, N, K, False, False, False, FP)))
@@ -46,9 +50,13 @@ def launch(case, bufs, stream):
 
 
 def test_alternating_streams(rng):
+    # one shape throughout: every call asks for temporaries of the same sizes,
+    # so a cache keyed by size alone would hand the block the previous call on
+    # the other stream had just released to a call whose stream has not yet
+    # waited for that work
     streams = [new_stream(), new_stream()]
     try:
-        cs = cases(rng, 16)
+        cs = cases(rng, 16, same_shape=True)
         live = upload(cs)
         for i, (c, b) in enumerate(zip(cs, live)):  # back to back, no synchronisation
             launch(c, b, streams[i % 2])
@@ -87,3 +95,17 @@ def test_two_threads(rng):
         t.join(timeout=100)
     assert not any(t.is_alive() for t in threads)
     assert errors == []
+
+
+def test_stream_destroy_releases_cache(rng):
+    L = P.lib()
+    P.check(L.pir_release_cached())
+    s = new_stream()
+    c = cases(rng, 1)[0]
+    b = upload([c])[0]
+    launch(c, b, s)
+    P.check(L.pir_stream_synchronize(s))
+    assert np.array_equal(b[2].numpy(), c[5])
+    assert L.pir_cached_bytes() > 0  # the product's temporaries, cached for stream s
+    P.check(L.pir_stream_destroy(s))
+    assert L.pir_cached_bytes() == 0

@@ -85,6 +85,19 @@ int pir_conv_wgrad(const uint64_t *dy, const uint64_t *x, uint64_t *dw, int b, i
 int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint64_t *A1, const uint64_t *B0,
                    const uint64_t *B1, uint64_t *C, void *stream);
 
+/* No counterpart in the reference: pir_ring_gemm3's limb form (algo 1) with
+ * nothing allocated, for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The packed operand and the split-K partials live in a fixed
+ * device array of PIR_NOALLOC_SCRATCH_BYTES on each GPU, held from the
+ * library's load on that GPU until the process ends; a shape that needs more
+ * (about 16 * K * N bytes, plus 8 * M * N per extra K split) fails with status
+ * 1 and a message.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (other streams or threads) race: the caller orders them. */
+#define PIR_NOALLOC_SCRATCH_BYTES (128u << 20)
+int pir_ring_gemm3_noalloc(int M, int N, int K, const uint64_t *A0, const uint64_t *A1, const uint64_t *B0,
+                           const uint64_t *B1, uint64_t *C, void *stream);
+
 /* The limb form with explicit operands, no counterpart in the reference (it is
  * how the CUTLASS gemm above is re-expressed): C = sum_s A_s . B_s over nseg
  * (1..12) K-segments, B_s(k,n) = b0[s] + (b1 ? b1[s] : 0) packed once (weights) by

@@ -57,6 +57,7 @@ _SIGS = {
     "pir_vector_expand": ([_vp, _sz, _vp, _i, _vp], _i),
     "pir_maxpool_row2im": ([_vp, _vp] + [_i] * 6 + [_vp], _i),
     "pir_ring_gemm3": ([_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "pir_ring_gemm3_noalloc": ([_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "pir_limb_packed_bytes": ([_i, _i, _i], _sz),
     "pir_limb_pack": ([_vp, _vp, _i, _i, _i, _i, _vp, _vp], _i),
     "pir_limb_gemm": ([_vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _i, _vp], _i),

@@ -20,6 +20,7 @@ SOURCES = [
     "fused_rss.hip",
     "elementwise.hip",
     "tpc_fpc_compare.hip",
+    "noalloc.hip",
     "runtime.cpp",
     "protocol.cpp",
     "tpc_fpc.cpp",

@@ -20,7 +20,10 @@ Also reported on the same line:
     and of the maxpool kernels (each with the ReLU fused after it);
   * fp64_error: the reconstructed VGG16 logits against an FP64 forward of the
     same network (weights decoded from the shares);
-  * cpu_baseline: the CPU oracle on VGG16 at a small batch, host cores.
+  * measured_peaks: HBM copy / read / write streams, the int8 MFMA rate and
+    the 64-bit multiply-add rate, measured beside the datasheet figures;
+  * cpu_baseline: the CPU oracle on a bounded sample of every configuration
+    (VGG16 is the headline entry), host cores and CPU model.
 """
 import argparse
 import json
@@ -158,16 +161,37 @@ def hbm_roofline(P, layers, batch, layer_ms):
     return out
 
 
-def copy_calibration(P, n):
-    """One x += y launch of the element-op kernel over n u64 (reads 16n bytes,
-    writes 8n): the known-size kernel that scales the HBM byte counters."""
-    L = P.lib()
-    x, y = P.Buffer((n,), np.uint64), P.Buffer((n,), np.uint64)
-    P.synchronize()
-    P.check(L.pir_dd_op(P.OP_ADD, 8, x.ptr, 1, y.ptr, 1, 0, n, None))
-    P.synchronize()
-    log(f"copy calibration: multi_op_kernel<unsigned long> over {n} u64, reads {16 * n} B, writes {8 * n} B")
-    del x, y
+CALIB_BYTES = 1 << 30
+
+
+def counter_calibration(P):
+    """Known-size kernels that scale the HBM byte counters: the read-only
+    stream (peak_read_kernel, reads CALIB_BYTES, writes nothing) and the
+    write-only fill (peak_write_kernel, writes CALIB_BYTES, reads nothing),
+    each launched twice (warm-up + one timed launch); the fill also runs once
+    to prepare the read kernel's source."""
+    for which, name in ((P.PEAK_HBM_READ, "peak_read_kernel"), (P.PEAK_HBM_WRITE, "peak_write_kernel")):
+        P.measure_peak(which, CALIB_BYTES, reps=1)
+        log(f"counter calibration: {name} over {CALIB_BYTES} B")
+
+
+def measured_peaks(P):
+    """Measured peaks beside the datasheet figures (include/pir.h
+    pir_measure_peak): HBM copy / read / write streams over 1 GiB buffers, the
+    int8 MFMA instruction rate, the 64-bit multiply-add rate of the VALUs."""
+    out = {}
+    for key, which, unit in (("hbm_copy", P.PEAK_HBM_COPY, "GB/s"), ("hbm_read", P.PEAK_HBM_READ, "GB/s"),
+                             ("hbm_write", P.PEAK_HBM_WRITE, "GB/s"), ("mfma_i8", P.PEAK_MFMA_I8, "TOP/s"),
+                             ("valu_u64_muladd", P.PEAK_VALU_U64, "G/s")):
+        best = 0.0
+        for _ in range(3):
+            r, _ms = P.measure_peak(which, 1 << 30, reps=10)
+            best = max(best, r)
+        out[key] = {"value": best, "unit": unit}
+    out["hbm_copy"]["datasheet"] = HBM_PEAK_GBS
+    out["mfma_i8"]["datasheet"] = INT8_PEAK_TOPS
+    log("measured peaks: " + ", ".join(f"{k} {v['value']:.0f} {v['unit']}" for k, v in out.items()))
+    return out
 
 
 def time_graph(P, net, steps, warmup):
@@ -331,27 +355,74 @@ def bench_ring_gemm(P, sizes, reps):
     return out
 
 
-def cpu_baseline(C, batch):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(C, vgg_batch):
+    """The oracle (the reference's arithmetic restated in C++, g++ -O3, OpenMP
+    over the host cores; plain uint64 GEMM loops, uint8-per-bit dReLU) on a
+    bounded sample of every configuration: SecureML b128, LeNet b32 and VGG16
+    b`vgg_batch` forwards, the uneven AlexNet workload's first 3 queries, and
+    the 3-term ring product at 512..2048 in full and at 4096 on 256 of its
+    rows.  The headline entry is VGG16."""
     import oracle as O
-    layers, fp = C.VGG16_CIFAR, 26
-    rng = np.random.default_rng(1)
-    params = {}
-    for i, l in enumerate(layers):
-        if l[0] == "fc":
-            params[i] = (O.set_public(rng.normal(0, np.sqrt(2 / l[1]), l[1] * l[2]), fp),
-                         O.set_public(np.full(l[2], 0.01), fp))
-        elif l[0] == "cnn":
-            fan = l[5] * l[5] * l[3]
-            params[i] = (O.set_public(rng.normal(0, np.sqrt(2 / fan), l[3] * l[4] * l[5] * l[5]), fp), None)
-    x = O.set_public(rng.uniform(0, 1, batch * C.input_size(layers)), fp)
-    O.net_forward(layers, 1, fp, {k: v for k, v in params.items()}, O.set_public(rng.uniform(0, 1, 3072), fp))
-    t0 = time.perf_counter()
-    O.net_forward(layers, batch, fp, params, x)
-    dt = time.perf_counter() - t0
+
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": batch / dt, "unit": "images/s", "cores": cores, "kind": "port",
-            "sample": f"VGG16-CIFAR 3PC forward, batch {batch}, FP26, CPU oracle (g++ -O3 OpenMP, "
-                      f"plain uint64 GEMM loops, uint8-per-bit dReLU), {dt:.1f} s"}
+    table = {}
+
+    def net_rate(name, layers, batch, fp, sigma, seed=1):
+        if sigma is None:  # He init N(0, sqrt(2 / fan_in)), the layers' default
+            rng = np.random.default_rng(seed)
+            w = {}
+            for i, l in enumerate(layers):
+                if l[0] == "fc":
+                    w[i] = (rng.normal(0, np.sqrt(2 / l[1]), l[1] * l[2]), np.full(l[2], 0.01))
+                elif l[0] == "cnn":
+                    w[i] = (rng.normal(0, np.sqrt(2 / (l[5] * l[5] * l[3])), l[3] * l[4] * l[5] * l[5]), None)
+        else:
+            w = C.normal_weights(layers, sigma, seed)
+        params = {i: (O.set_public(a, fp), None if b is None else O.set_public(b, fp)) for i, (a, b) in w.items()}
+        x = O.set_public(np.random.default_rng(seed).uniform(0, 1, batch * C.input_size(layers)), fp)
+        t0 = time.perf_counter()
+        O.net_forward(layers, batch, fp, params, x)
+        dt = time.perf_counter() - t0
+        table[name] = {"value": batch / dt, "unit": "images/s", "sample": f"batch {batch}, {dt:.2f} s"}
+        log(f"CPU {name}: {batch / dt:.3f} images/s ({dt:.1f} s)")
+
+    net_rate("secureml_b128_fp13", C.SECUREML, 128, 13, 0.1)
+    net_rate("lenet_fp13", C.LENET, 32, 13, 0.1)
+    qb = C.uneven_query_batches()[:3]
+    net_rate("alexnet_uneven_fp20", C.ALEXNET_CIFAR, int(sum(qb)), 20, None)
+    table["alexnet_uneven_fp20"]["sample"] += " (the first 3 queries concatenated)"
+    net_rate("vgg16_fp26", C.VGG16_CIFAR, vgg_batch, 26, None)
+
+    rng = np.random.default_rng(0)
+    for n in (512, 1024, 2048, 4096):
+        rows = n if n <= 2048 else 256
+        A0, A1 = (rng.integers(0, 2**64, rows * n, dtype=np.uint64) for _ in range(2))
+        B0, B1 = (rng.integers(0, 2**64, n * n, dtype=np.uint64) for _ in range(2))
+        t0 = time.perf_counter()
+        for A, B in ((A0, B0), (A0, B1), (A1, B0)):
+            O.gemm(rows, n, n, A, 1, B, 1, 1)
+        dt = time.perf_counter() - t0
+        gops = 6.0 * rows * n * n / dt / 1e9
+        table[f"ring_gemm_{n}"] = {"value": gops, "unit": "Gop/s (6MNK/t)",
+                                   "sample": f"{rows} of {n} rows, {dt:.2f} s" if rows < n else f"{dt:.2f} s"}
+        log(f"CPU ring gemm n={n}: {gops:.1f} Gop/s ({dt:.1f} s)")
+
+    head = table["vgg16_fp26"]
+    return {"value": head["value"], "unit": "images/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"VGG16-CIFAR 3PC forward, batch {vgg_batch}, FP26, CPU oracle (g++ -O3 OpenMP, "
+                      f"plain uint64 GEMM loops, uint8-per-bit dReLU), {head['sample'].split(', ')[1]}",
+            "table": table}
 
 
 def main():
@@ -363,9 +434,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--gemm-sizes", default="512,1024,2048,4096,8192")
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-batch", type=int, default=32)
-    ap.add_argument("--copy-calib", type=int, default=0,
-                    help="also run one x += y element-op launch over this many u64 (counter calibration)")
+    ap.add_argument("--cpu-batch", type=int, default=8)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run the read-only and write-only kernels of known size (counter calibration)")
     ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="heuristic GEMM plans only")
     ap.add_argument("--train-only", action="store_true", help="VGG16 training step only (for profiling runs)")
@@ -381,8 +452,8 @@ def main():
     import piranha_amd as P
     from piranha_amd import configs as C
 
-    if args.copy_calib:
-        copy_calibration(P, args.copy_calib)
+    if args.calib:
+        counter_calibration(P)
 
     if args.train_only:
         tr = bench_training(P, C, C.VGG16_CIFAR, args.batch, 26, args.steps, args.warmup,
@@ -427,8 +498,14 @@ def main():
                 "ops_per_launch": k_ops / k_n,
                 "share_of_forward": k_ms / float(layer_ms.sum())}
     hbm = hbm_roofline(P, layers, batch, layer_ms)
+    peaks = measured_peaks(P)
+    roofline["measured_peak"] = peaks["mfma_i8"]["value"]
+    roofline["frac_of_measured_peak"] = achieved / peaks["mfma_i8"]["value"]
     for k, v in hbm.items():
-        log(f"{k}: {v['achieved']:.0f} GB/s ({100 * v['frac']:.1f}% of HBM peak)")
+        v["measured_copy_peak"] = peaks["hbm_copy"]["value"]
+        v["frac_of_measured_copy"] = v["achieved"] / peaks["hbm_copy"]["value"]
+        log(f"{k}: {v['achieved']:.0f} GB/s ({100 * v['frac']:.1f}% of HBM peak, "
+            f"{100 * v['frac_of_measured_copy']:.1f}% of the measured copy)")
     xin = np.random.default_rng(1).uniform(0.0, 1.0, batch * C.input_size(layers))  # make_net's input
     acc = fp64_error(net, layers, batch, fp, xin)
     log(f"reconstructed logits vs FP64 forward: max abs err {acc['max_abs_err']:.3g} "
@@ -446,6 +523,7 @@ def main():
                    "global_batch": batch, "parallelism": "none (1 GPU, 3 simulated parties)"},
         "roofline": roofline,
         "hbm_roofline": hbm,
+        "measured_peaks": peaks,
         "fp64_error": acc,
     }
 
@@ -467,6 +545,10 @@ def main():
 
         sizes = [int(s) for s in args.gemm_sizes.split(",") if s]
         result["ring_gemm"] = bench_ring_gemm(P, sizes, reps=2)
+        for rec in result["ring_gemm"]:
+            # the plain form's 64-bit multiply-adds (3 n^3) against the measured VALU rate
+            rec["plain_valu_frac"] = rec["plain_gops"] / 2.0 / peaks["valu_u64_muladd"]["value"]
+            rec["limb_frac_of_measured_mfma"] = rec["limb_int8_tops"] / peaks["mfma_i8"]["value"]
 
         other = {}
         for name, L, b, f in [("secureml_b128_fp13", C.SECUREML, 128, 13), ("lenet_b128_fp13", C.LENET, 128, 13)]:

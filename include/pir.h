@@ -413,6 +413,21 @@ int pir_autotune_clear(void);
 int pir_autotune_save(const char *path); /* text, one shape per line */
 int pir_autotune_load(const char *path);
 
+/* ---- measured peaks ----------------------------------------------------------
+ * No counterpart in the reference (its GEMM profiling harness,
+ * src/test/MatMulProfiling.cu:192-529, times the CUTLASS kernels only): the
+ * measured denominators of the roofline.  which:
+ *   PIR_PEAK_HBM_COPY   GB/s (bytes read + written) of a 16-byte-per-lane copy of `bytes`
+ *   PIR_PEAK_HBM_READ   GB/s of a read-only stream over `bytes` (writes nothing)
+ *   PIR_PEAK_HBM_WRITE  GB/s of a write-only fill of `bytes`
+ *   PIR_PEAK_MFMA_I8    int8 TOP/s of back-to-back v_mfma_i32_32x32x32_i8 (bytes ignored)
+ *   PIR_PEAK_VALU_U64   G 64-bit multiply-adds per second on the vector ALUs (bytes ignored)
+ * rate = the measured rate, ms = average launch time over `reps` launches after
+ * one warm-up.  Allocates and frees its buffers; synchronises the host. */
+enum { PIR_PEAK_HBM_COPY = 0, PIR_PEAK_HBM_READ = 1, PIR_PEAK_HBM_WRITE = 2, PIR_PEAK_MFMA_I8 = 3,
+       PIR_PEAK_VALU_U64 = 4 };
+int pir_measure_peak(int which, size_t bytes, int reps, double *rate, double *ms, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,7 @@ _SIGS = {
     "pir_release_cached": ([], _i),
     "pir_cached_bytes": ([], _sz),
     "pir_stream_synchronize": ([_vp], _i),
+    "pir_measure_peak": ([_i, _sz, _i, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp], _i),
     "pir_device_synchronize": ([], _i),
     "pir_set_mode": ([_i], _i),
     "pir_gemm": ([_i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp], _i),
@@ -244,6 +245,16 @@ class Buffer:
 
 def to_dev(a, stream=None):
     return Buffer.from_numpy(a, stream)
+
+
+PEAK_HBM_COPY, PEAK_HBM_READ, PEAK_HBM_WRITE, PEAK_MFMA_I8, PEAK_VALU_U64 = range(5)
+
+
+def measure_peak(which, nbytes=1 << 30, reps=10):
+    """(rate, ms per launch) of one of the measured peaks (see include/pir.h)."""
+    r, ms = _d(), _d()
+    check(lib().pir_measure_peak(which, nbytes, reps, ctypes.byref(r), ctypes.byref(ms), None))
+    return r.value, ms.value
 
 
 def synchronize():

@@ -21,6 +21,7 @@ SOURCES = [
     "elementwise.hip",
     "tpc_fpc_compare.hip",
     "noalloc.hip",
+    "peaks.hip",
     "runtime.cpp",
     "protocol.cpp",
     "tpc_fpc.cpp",

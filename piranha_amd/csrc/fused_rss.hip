@@ -41,20 +41,105 @@ __global__ __launch_bounds__(EW_BLOCK) void relu_kernel(const u64 *__restrict__ 
     }
 }
 
-template <typename U>
-int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s) {
+// Wide form: EPT consecutive elements per thread, every share array read and
+// written with 16-byte accesses and the bit shares with one EPT-byte (u8) or
+// 16-byte (u64) store per array, so a wave keeps more bytes in flight per
+// memory instruction.  The host uses it when n % EPT == 0 and every array
+// base is aligned for those accesses.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+template <int EPT>
+__device__ __forceinline__ void load3v(const u64 *__restrict__ base, size_t n, size_t i0, Sh3<u64> (&v)[EPT]) {
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const u64x2 *src = (const u64x2 *)(base + (size_t)(2 * p + s) * n + i0);
+#pragma unroll
+            for (int j = 0; j < EPT / 2; j++) {
+                u64x2 t = src[j];
+                v[2 * j].s[p][s] = t.x;
+                v[2 * j + 1].s[p][s] = t.y;
+            }
+        }
+}
+
+template <int EPT>
+__device__ __forceinline__ void store3v(u64 *__restrict__ base, size_t n, size_t i0, const Sh3<u64> (&v)[EPT]) {
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            u64x2 *dst = (u64x2 *)(base + (size_t)(2 * p + s) * n + i0);
+#pragma unroll
+            for (int j = 0; j < EPT / 2; j++) dst[j] = (u64x2){v[2 * j].s[p][s], v[2 * j + 1].s[p][s]};
+        }
+}
+
+template <int EPT>
+__device__ __forceinline__ void store3v(u8 *__restrict__ base, size_t n, size_t i0, const Sh3<u8> (&v)[EPT]) {
+    static_assert(EPT == 2 || EPT == 4 || EPT == 8, "byte group of 2, 4 or 8");
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            u64 w = 0;
+#pragma unroll
+            for (int j = 0; j < EPT; j++) w |= (u64)v[j].s[p][s] << (8 * j);
+            u8 *dst = base + (size_t)(2 * p + s) * n + i0;
+            if (EPT == 2) *(uint16_t *)dst = (uint16_t)w;
+            else if (EPT == 4) *(u32 *)dst = (u32)w;
+            else *(u64 *)dst = w;
+        }
+}
+
+template <typename U, bool RELU, int EPT>
+__global__ __launch_bounds__(EW_BLOCK) void relu_wide_kernel(const u64 *__restrict__ x, size_t n,
+                                                             U *__restrict__ dres, u64 *__restrict__ y) {
+    size_t i0 = ((size_t)blockIdx.x * EW_BLOCK + threadIdx.x) * EPT;
+    if (i0 >= n) return;
+    Sh3<u64> in[EPT];
+    load3v<EPT>(x, n, i0, in);
+    Sh3<U> b[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) b[j] = drelu_<U>(in[j]);
+    store3v<EPT>(dres, n, i0, b);
+    if (RELU) {
+        Sh3<u64> zero, out[EPT];
+#pragma unroll
+        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+#pragma unroll
+        for (int j = 0; j < EPT; j++) out[j] = select_<U>(zero, in[j], b[j]);
+        store3v<EPT>(y, n, i0, out);
+    }
+}
+
+constexpr int RELU_EPT = 4;
+
+static bool aligned_to(const void *p, size_t a) { return p == nullptr || ((uintptr_t)p % a) == 0; }
+
+template <typename U, bool RELU>
+static int launch_relu(const u64 *x, size_t n, U *dres, u64 *y, hipStream_t s) {
     if (n == 0) return OK;
-    hipLaunchKernelGGL((relu_kernel<U, false>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, (u64 *)nullptr);
+    if (n % RELU_EPT == 0 && aligned_to(x, 16) && aligned_to(y, 16) &&
+        aligned_to(dres, sizeof(U) == 1 ? RELU_EPT : 16)) {
+        hipLaunchKernelGGL((relu_wide_kernel<U, RELU, RELU_EPT>), dim3(ew_grid(n / RELU_EPT)), dim3(EW_BLOCK), 0, s,
+                           x, n, dres, y);
+    } else {
+        hipLaunchKernelGGL((relu_kernel<U, RELU>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, y);
+    }
     PIR_LAUNCH_CHECK();
     return OK;
 }
 
 template <typename U>
+int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s) {
+    return launch_relu<U, false>(x, n, dres, (u64 *)nullptr, s);
+}
+
+template <typename U>
 int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s) {
-    if (n == 0) return OK;
-    hipLaunchKernelGGL((relu_kernel<U, true>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, y);
-    PIR_LAUNCH_CHECK();
-    return OK;
+    return launch_relu<U, true>(x, n, dres, y, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -237,6 +322,71 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
         if (j < f2) store3(mprime, mpN, o * f2 + j, d[j]);
 }
 
+// 2x2 pools with an even channel count: two adjacent channels per thread, so
+// every gathered window value of both channels is one 16-byte load per share
+// array, the pooled value and the fused ReLU's output one 16-byte store, the
+// ReLU's bit shares one 2-byte store and the eight derivative bytes of the two
+// windows one 8-byte store per share array.
+template <typename U>
+__global__ __launch_bounds__(EW_BLOCK) void maxpool2x2_pair_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
+                                                                   u64 *__restrict__ act, U *__restrict__ mprime,
+                                                                   u64 *__restrict__ relu_out,
+                                                                   u8 *__restrict__ relu_d) {
+    static_assert(sizeof(U) == 1, "byte derivative shares");
+    const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    size_t o = ((size_t)blockIdx.x * EW_BLOCK + threadIdx.x) * 2;
+    if (o >= outN) return;
+    const size_t inN = (size_t)g.batch * g.H * g.W * g.C;
+    int c = (int)(o % g.C);
+    size_t r = o / g.C;
+    int ow = (int)(r % g.OW);
+    r /= g.OW;
+    int oh = (int)(r % g.OH);
+    int b = (int)(r / g.OH);
+    Sh3<u64> e0[4], e1[4];
+    Sh3<U> d0[4], d1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int ih = oh * g.stride + (j >> 1), iw = ow * g.stride + (j & 1);
+        const size_t idx = (((size_t)b * g.H + ih) * g.W + iw) * g.C + c;
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int sh = 0; sh < 2; sh++) {
+                u64x2 t = *(const u64x2 *)(x + (size_t)(2 * p + sh) * inN + idx);
+                e0[j].s[p][sh] = t.x;
+                e1[j].s[p][sh] = t.y;
+            }
+    }
+    tournament<U, 4>(e0, d0);
+    tournament<U, 4>(e1, d1);
+    Sh3<u64> pooled[2] = {e0[0], e1[0]};
+    store3v<2>(act, outN, o, pooled);
+    if (relu_out) {
+        Sh3<u8> rb[2] = {drelu_<u8>(e0[0]), drelu_<u8>(e1[0])};
+        store3v<2>(relu_d, outN, o, rb);
+        Sh3<u64> zero, ro[2];
+#pragma unroll
+        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+        ro[0] = select_<u8>(zero, e0[0], rb[0]);
+        ro[1] = select_<u8>(zero, e1[0], rb[1]);
+        store3v<2>(relu_out, outN, o, ro);
+    }
+    const size_t mpN = outN * 4;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int sh = 0; sh < 2; sh++) {
+            u64 v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                v |= (u64)(u8)d0[j].s[p][sh] << (8 * j);
+                v |= (u64)(u8)d1[j].s[p][sh] << (8 * (j + 4));
+            }
+            *(u64 *)(mprime + (size_t)(2 * p + sh) * mpN + o * 4) = v;
+        }
+}
+
 template <typename U>
 int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s, u64 *relu_out,
                         u8 *relu_d) {
@@ -247,6 +397,15 @@ int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime,
     size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
     if (outN == 0) return OK;
     int k = expanded_pool_size(g.f);
+    if constexpr (sizeof(U) == 1) {
+        if (g.f == 2 && g.C % 2 == 0 && aligned_to(x, 16) && aligned_to(act, 16) && aligned_to(relu_out, 16) &&
+            aligned_to(relu_d, 2) && aligned_to(mprime, 8)) {
+            hipLaunchKernelGGL((maxpool2x2_pair_kernel<U>), dim3(ew_grid(outN / 2)), dim3(EW_BLOCK), 0, s, x, g,
+                               act, mprime, relu_out, relu_d);
+            PIR_LAUNCH_CHECK();
+            return OK;
+        }
+    }
     switch (k) {
         case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d); break;
         case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d); break;

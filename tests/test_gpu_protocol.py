@@ -170,9 +170,9 @@ def test_convolution(rng, mode, kind, algo, b, H, W, f, Din, Dout, stride, pad):
     assert np.array_equal(dc.numpy(), O.convolution(a, w, b, H, W, f, Din, Dout, stride, pad, FP))
 
 
+@pytest.mark.parametrize("n", [700, 701])  # the 4-element-per-thread form, and the per-element form
 @pytest.mark.parametrize("dt", [np.uint8, np.uint64])
-def test_drelu_relu(rng, mode, kind, dt):
-    n = 700
+def test_drelu_relu(rng, mode, kind, dt, n):
     x = make(rng, kind, n)
     x[:, :, :4] = 0
     eb = np.dtype(dt).itemsize

@@ -18,51 +18,29 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int PEAK_BLOCK = 256;
-constexpr int UNROLL = 4;
 
-// dst[i] = src[i] over n 16-byte items, UNROLL items in flight per thread per
-// round, grid-stride.
+// One 16-byte item per thread, one launch over the whole buffer (measured
+// faster on MI355X than grid-stride loops, which keep fewer bytes in flight per
+// channel); the copy uses non-temporal accesses, the fastest copy form measured.
 __global__ __launch_bounds__(PEAK_BLOCK) void peak_copy_kernel(const u64x2 *__restrict__ src,
                                                                u64x2 *__restrict__ dst, size_t n) {
-    const size_t stride = (size_t)gridDim.x * PEAK_BLOCK;
-    size_t i = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
-    for (; i + (UNROLL - 1) * stride < n; i += UNROLL * stride) {
-        u64x2 v[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) v[u] = src[i + u * stride];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) dst[i + u * stride] = v[u];
-    }
-    for (; i < n; i += stride) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
-// Reads n 16-byte items and folds them; the fold is stored only if it equals
-// a value the filled buffer cannot produce, so the kernel writes nothing.
+// Reads n 16-byte items; the fold is stored only if it equals a value the
+// filled buffer cannot produce, so the kernel writes nothing.
 __global__ __launch_bounds__(PEAK_BLOCK) void peak_read_kernel(const u64x2 *__restrict__ src, size_t n,
                                                                u64 *__restrict__ sink) {
-    const size_t stride = (size_t)gridDim.x * PEAK_BLOCK;
-    size_t i = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
-    u64 acc = 0;
-    for (; i + (UNROLL - 1) * stride < n; i += UNROLL * stride) {
-        u64x2 v[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) v[u] = src[i + u * stride];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) acc ^= v[u].x ^ v[u].y;
-    }
-    for (; i < n; i += stride) acc ^= src[i].x ^ src[i].y;
-    if (acc == 0x5a5a5a5a5a5a5a5aull) sink[0] = acc;
+    const size_t i = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const u64x2 v = src[i];
+    if ((v.x ^ v.y) == 0x5a5a5a5a5a5a5a5aull) sink[0] = v.x;
 }
 
 __global__ __launch_bounds__(PEAK_BLOCK) void peak_write_kernel(u64x2 *__restrict__ dst, size_t n, u64 value) {
-    const size_t stride = (size_t)gridDim.x * PEAK_BLOCK;
-    size_t i = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
-    const u64x2 v = {value, value ^ 0xffull};
-    for (; i + (UNROLL - 1) * stride < n; i += UNROLL * stride) {
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) dst[i + u * stride] = v;
-    }
-    for (; i < n; i += stride) dst[i] = v;
+    const size_t i = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
+    if (i < n) dst[i] = (u64x2){value, value ^ 0xffull};
 }
 
 // Each wave issues `iters` rounds of 8 independent v_mfma_i32_32x32x32_i8 (two
@@ -107,18 +85,18 @@ __global__ __launch_bounds__(PEAK_BLOCK) void peak_valu_kernel(int iters, u64 m,
     out[t] = s;
 }
 
-// One workgroup per (CU, workgroup slot): enough resident waves to fill the
-// chip, with grid-stride loops over the buffer.
-constexpr unsigned HBM_GRID = 256 * 8;
+// Compute peaks: eight workgroups per CU, enough resident waves to fill the chip.
 constexpr unsigned COMPUTE_GRID = 256 * 8;
 
 int measure(int which, size_t bytes, int reps, double *rate, double *ms_out, hipStream_t s) {
     PIR_REQUIRE(reps >= 1 && rate && ms_out, "pir_measure_peak: reps >= 1 and result pointers needed");
     const bool hbm = which == PIR_PEAK_HBM_COPY || which == PIR_PEAK_HBM_READ || which == PIR_PEAK_HBM_WRITE;
     PIR_REQUIRE(hbm || which == PIR_PEAK_MFMA_I8 || which == PIR_PEAK_VALU_U64, "pir_measure_peak: unknown peak");
-    PIR_REQUIRE(!hbm || (bytes >= 16 && bytes % 16 == 0), "pir_measure_peak: bytes must be a positive multiple of 16");
+    PIR_REQUIRE(!hbm || (bytes >= 16 && bytes % 16 == 0 && bytes / 16 / PEAK_BLOCK < (1ull << 31)),
+                "pir_measure_peak: bytes must be a positive multiple of 16 below 8 TiB");
     void *a = nullptr, *b = nullptr;
     const size_t n16 = bytes / 16;
+    const unsigned hbm_grid = (unsigned)((n16 + PEAK_BLOCK - 1) / PEAK_BLOCK);
     const int mfma_iters = 4096, valu_iters = 4096;
     size_t sink_bytes = (size_t)COMPUTE_GRID * PEAK_BLOCK * 8;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -132,8 +110,8 @@ int measure(int which, size_t bytes, int reps, double *rate, double *ms_out, hip
     fail(hipMalloc(&b, hbm ? bytes : sink_bytes), "hipMalloc");
     if (rc == OK && which == PIR_PEAK_HBM_COPY) fail(hipMalloc(&a, bytes), "hipMalloc");
     if (rc == OK && (which == PIR_PEAK_HBM_COPY || which == PIR_PEAK_HBM_READ)) {
-        // source contents: a fill of known nonzero words
-        hipLaunchKernelGGL(peak_write_kernel, dim3(HBM_GRID), dim3(PEAK_BLOCK), 0, s,
+        // source contents: a fill of known words
+        hipLaunchKernelGGL(peak_write_kernel, dim3(hbm_grid), dim3(PEAK_BLOCK), 0, s,
                            (u64x2 *)(which == PIR_PEAK_HBM_COPY ? a : b), n16, 0x0123456789abcdefull);
         fail(hipGetLastError(), "fill");
     }
@@ -142,15 +120,15 @@ int measure(int which, size_t bytes, int reps, double *rate, double *ms_out, hip
     auto launch = [&]() {
         switch (which) {
             case PIR_PEAK_HBM_COPY:
-                hipLaunchKernelGGL(peak_copy_kernel, dim3(HBM_GRID), dim3(PEAK_BLOCK), 0, s, (const u64x2 *)a,
+                hipLaunchKernelGGL(peak_copy_kernel, dim3(hbm_grid), dim3(PEAK_BLOCK), 0, s, (const u64x2 *)a,
                                    (u64x2 *)b, n16);
                 break;
             case PIR_PEAK_HBM_READ:
-                hipLaunchKernelGGL(peak_read_kernel, dim3(HBM_GRID), dim3(PEAK_BLOCK), 0, s, (const u64x2 *)b, n16,
+                hipLaunchKernelGGL(peak_read_kernel, dim3(hbm_grid), dim3(PEAK_BLOCK), 0, s, (const u64x2 *)b, n16,
                                    (u64 *)b);
                 break;
             case PIR_PEAK_HBM_WRITE:
-                hipLaunchKernelGGL(peak_write_kernel, dim3(HBM_GRID), dim3(PEAK_BLOCK), 0, s, (u64x2 *)b, n16,
+                hipLaunchKernelGGL(peak_write_kernel, dim3(hbm_grid), dim3(PEAK_BLOCK), 0, s, (u64x2 *)b, n16,
                                    0xfedcba9876543210ull);
                 break;
             case PIR_PEAK_MFMA_I8:

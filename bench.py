@@ -233,32 +233,18 @@ def make_net(P, C, layers, batch, fp, mode, seed=1, capture=True, autotune=True,
     return net
 
 
-def random_sharing(shares, rng):
-    """A uniformly random consistent RSS sharing of the values that party A
-    opens from `shares` (3, 2, n): x1, x2 random, x0 = v - x1 - x2, party p
-    holding (x_p, x_p+1)."""
-    v = shares[0, 0] + shares[0, 1] + shares[2, 0]
-    x1 = rng.integers(0, 2**64 - 1, v.size, dtype=np.uint64, endpoint=True)
-    x2 = rng.integers(0, 2**64 - 1, v.size, dtype=np.uint64, endpoint=True)
-    x0 = v - x1 - x2
-    return np.ascontiguousarray(np.stack([np.stack([x0, x1]), np.stack([x1, x2]), np.stack([x2, x0])]))
-
-
-def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True):
-    """The headline forward with every weight and input share re-dealt as a
-    uniformly random sharing of the same values: the same GEMMs and kernels
-    on full-entropy digit planes, where the reference's placeholder precompute
-    leaves two of a value's three components at zero."""
+def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True, masked=False):
+    """The headline forward with every weight and input share re-dealt on the
+    GPU as a uniformly random sharing of the same values (the Philox dealer,
+    pir_net_rerandomize): the same GEMMs and kernels on full-entropy digit
+    planes, where the reference's placeholder precompute leaves two of a
+    value's three components at zero.  masked: also the masked precompute
+    mode (random truncation pairs, reshare masks, dReLU / selectShare
+    randomness drawn in the kernels)."""
     net = make_net(P, C, layers, batch, fp, P.MODE_FUSED, capture=False, autotune=False)
-    rng = np.random.default_rng(7)
-    for li in range(net.num_layers):
-        for k, (_, sh) in enumerate(net.params(li)):
-            net.set_param_shares(li, k, random_sharing(sh, rng))
-    ptr, n = net.input_ptr()
-    host = np.zeros((3, 2, n), np.uint64)
-    P.check(P.lib().pir_memcpy_d2h(host.ctypes.data, ptr, host.nbytes, None))
-    host = random_sharing(host, rng)
-    P.check(P.lib().pir_memcpy_h2d(ptr, host.ctypes.data, host.nbytes, None))
+    net.rerandomize(7)
+    if masked:
+        net.set_precompute(P.PRECOMPUTE_MASKED, 0x5EED, 0)
     net.forward_shares(None)
     net.synchronize()
     if autotune:
@@ -267,7 +253,8 @@ def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True):
     t = time_graph(P, net, steps, warmup)
     del net
     return {"images_per_s": batch / t, "ms_per_forward": t * 1e3,
-            "sharing": "uniformly random consistent RSS sharings of the same weights and inputs"}
+            "sharing": "uniformly random consistent RSS sharings of the same weights and inputs, dealt on the GPU",
+            "precompute": "masked (Philox4x32-10)" if masked else "placeholder (the reference's constants)"}
 
 
 def bench_training(P, C, layers, batch, fp, steps, warmup, autotune=True):
@@ -442,6 +429,8 @@ def main():
     ap.add_argument("--train-only", action="store_true", help="VGG16 training step only (for profiling runs)")
     ap.add_argument("--random-sharings-only", action="store_true",
                     help="the VGG16 forward on uniformly random sharings only (for profiling runs)")
+    ap.add_argument("--masked-only", action="store_true",
+                    help="the VGG16 forward in the masked precompute mode only (for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -462,9 +451,9 @@ def main():
         print(json.dumps({"training": tr}), flush=True)
         return
 
-    if args.random_sharings_only:
+    if args.random_sharings_only or args.masked_only:
         rs = bench_random_shares(P, C, C.VGG16_CIFAR, args.batch, 26, args.steps, args.warmup,
-                                 autotune=not args.no_autotune)
+                                 autotune=not args.no_autotune, masked=args.masked_only)
         log(f"VGG16 on random sharings: {rs['ms_per_forward']:.3f} ms/forward, {rs['images_per_s']:.1f} images/s")
         print(json.dumps({"random_sharings": rs}), flush=True)
         return
@@ -569,6 +558,9 @@ def main():
         rs = bench_random_shares(P, C, layers, batch, fp, args.steps, 20, autotune=not args.no_autotune)
         other["vgg16_b64_fp26_random_sharings"] = rs
         log(f"VGG16 on random sharings: {rs['images_per_s']:.1f} images/s")
+        rm = bench_random_shares(P, C, layers, batch, fp, args.steps, 20, autotune=not args.no_autotune, masked=True)
+        other["vgg16_b64_fp26_masked_precompute"] = rm
+        log(f"VGG16 in the masked precompute mode: {rm['images_per_s']:.1f} images/s")
         result["other_configs"] = other
 
         log("CPU baseline (oracle)")

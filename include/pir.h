@@ -211,6 +211,39 @@ int pir_rss_sqrt(const uint64_t *x, size_t n, uint64_t *out, int fp, void *strea
 int pir_rss_inverse(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
 int pir_rss_sigmoid(const uint64_t *x, size_t n, uint64_t *out, int fp, void *stream);
 
+/* ---- masked precompute mode and the seeded PRG ---------------------------
+ * The reference's correlated randomness is placeholder constants
+ * (src/mpc/Precompute.h:73-134: r = 1, r' = d, Beaver triples of ones; no mask
+ * in reshare, RSS.inl:1052-1074; r = 0 and rbits = 1 in dReLU, RSS.inl:806-852;
+ * c = cbits = 0 in selectShare, RSS.inl:542-569), and its AES PRG
+ * (src/mpc/AESObject.cpp:11-90) is never instantiated (src/main.cu:96-104).
+ * PIR_PRECOMPUTE_MASKED deals that randomness from Philox4x32-10 (Salmon et
+ * al., SC'11) keyed by `seed`: every protocol call that uses randomness takes
+ * the next 32-bit call number, and random word j of element e of that call is
+ * half (j & 1) of Philox4x32-10(key = seed, counter = (e_lo, e_hi, call, j >> 1)).
+ * In that mode dReLU ends with XOR 1 instead of 1 - x, selectShare opens its
+ * bit by XOR, and truncation returns floor or ceil of t / 2^k (the reference's
+ * r' = r * d plan, Precompute.h:107).  It runs the RSS inference protocols in
+ * the fused form: reshare, mult, and, dividePublic (2^k), dividePublic3out3,
+ * matmul, convolution (fprop), dReLU, ReLU, selectShare, maxpool and the FC /
+ * CNN / ReLU / Maxpool layers; everything else refuses it (status 3). */
+enum { PIR_PRECOMPUTE_PLACEHOLDER = 0, PIR_PRECOMPUTE_MASKED = 1 };
+/* the calling thread's free-standing calls: mode, key and next call number */
+int pir_set_precompute(int mode, uint64_t seed, uint32_t call);
+int pir_get_prg_call(uint32_t *call);
+/* Philox4x32-10 blocks: out[4i..4i+3] of counter ctr[4i..4i+3], key key[2i..2i+1]
+ * (device arrays of n blocks) */
+int pir_prg_philox(const uint32_t *ctr, const uint32_t *key, size_t n, uint32_t *out, void *stream);
+/* out[2b + h] = word h of element b of call `call` under `seed` (n words) */
+int pir_prg_fill(uint64_t seed, uint32_t call, size_t n, uint64_t *out, void *stream);
+/* The dealer: a random consistent RSS sharing shares [3][2][n] of the ring
+ * values[n] (x1, x2 = words 0, 1 of element i of the call, x0 = v - x1 - x2);
+ * pir_rss_rerandomize re-deals the value party A opens from shares_in (may
+ * equal shares_out). */
+int pir_rss_deal(const uint64_t *values, size_t n, uint64_t seed, uint32_t call, uint64_t *shares, void *stream);
+int pir_rss_rerandomize(const uint64_t *shares_in, size_t n, uint64_t seed, uint32_t call, uint64_t *shares_out,
+                        void *stream);
+
 /* ---- TPC (2 parties) and FPC (4 parties), src/mpc/TPC.inl, FPC.inl ------- */
 /* TPC value: [2 parties][n] (one share each).  FPC value: [4 parties][3 shares][n],
  * party q's share s holding component (q + s + 1) % 4.  Placeholder precompute
@@ -398,6 +431,15 @@ enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_
        PIR_LIMB_DGRAD_C32, PIR_LIMB_WGRAD, PIR_LIMB_WIDE = 8, PIR_LIMB_VARIANTS = 16 };
 int pir_net_profile_gemm(pir_net *net, double *ms, double *ops, int *launches);
 int pir_net_set_mode(pir_net *net, int mode);
+/* The network's precompute mode (see pir_set_precompute): in the masked mode
+ * layer i of a forward uses call number base + i, and each forward advances
+ * the base by the layer count.  A captured forward replays the call numbers of
+ * its capture. */
+int pir_net_set_precompute(pir_net *net, int mode, uint64_t seed, uint32_t call);
+/* Re-deals every parameter tensor (pir_net_param order, call k for the k-th)
+ * and the input buffer (the next call) as random sharings of the values they
+ * hold, from `seed` (the dealer of pir_rss_rerandomize). */
+int pir_net_rerandomize(pir_net *net, uint64_t seed);
 
 /* ---- limb GEMM autotuner (no counterpart in the reference: CUTLASS tile
  * shapes are fixed at compile time there) ------------------------------------

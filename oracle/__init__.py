@@ -128,6 +128,34 @@ _SIGS = {
     "ref_get_msbs_u64": [_u64p, _u64p, _sz, _sz],
     "ref_set_msbs_u64": [_u64p, _sz, _u64, _sz],
 }
+_u32 = ctypes.c_uint32
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_M = [_u64, _u32]  # seed, call of the masked precompute mode
+_SIGS.update({
+    "ref_philox": [_u32p, _u32p, _sz, _u32p],
+    "ref_prg_fill": [_u64, _u32, _sz, _u64p],
+    "ref_deal": [_u64p, _sz, _u64, _u32, _u64p],
+    "ref_reshare_m": [_u64p, _sz, _u64p] + _M,
+    "ref_rss_mult_m": [_u64p, _u64p, _sz] + _M,
+    "ref_rss_and_m_u8": [_u8p, _u8p, _sz] + _M,
+    "ref_rss_and_m_u64": [_u64p, _u64p, _sz] + _M,
+    "ref_divide_public_m": [_u64p, _sz, _i] + _M,
+    "ref_divide_public_3out3_m": [_u64p, _sz, _i, _u64p] + _M,
+    "ref_matmul_m": [_u64p, _sz, _u64p, _sz, _u64p] + [_i] * 7 + _M,
+    "ref_convolution_m": [_u64p, _sz, _u64p, _sz, _u64p, _sz] + [_i] * 9 + _M,
+    "ref_drelu_m_u8": [_u64p, _sz, _u8p] + _M,
+    "ref_drelu_m_u64": [_u64p, _sz, _u64p] + _M,
+    "ref_select_share_m_u8": [_u64p, _u64p, _u8p, _u64p, _sz] + _M,
+    "ref_select_share_m_u64": [_u64p, _u64p, _u64p, _u64p, _sz] + _M,
+    "ref_relu_m_u8": [_u64p, _sz, _u64p, _u8p] + _M,
+    "ref_relu_m_u64": [_u64p, _sz, _u64p, _u64p] + _M,
+    "ref_maxpool_m_u8": [_u64p, _sz, _u64p, _u8p, _i] + _M,
+    "ref_maxpool_m_u64": [_u64p, _sz, _u64p, _u64p, _i] + _M,
+    "ref_fc_forward_m": [_u64p, _u64p, _u64p, _u64p, _i, _i, _i, _i] + _M,
+    "ref_cnn_forward_m": [_u64p, _u64p, _u64p] + [_i] * 9 + _M,
+    "ref_maxpool_forward_m": [_u64p, _u64p, _u8p] + [_i] * 7 + _M,
+})
+
 _RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_averagepool_im2row": _sz,
         "ref_inv_sqrt_fixed": _u64,
         "ref_scalar_divide": _u64, "ref_version": _i}
@@ -156,10 +184,12 @@ def lib():
 
 
 def _p(a):
-    """ctypes pointer to a C-contiguous numpy array (uint64, uint8 or float64)."""
+    """ctypes pointer to a C-contiguous numpy array (uint64, uint32, uint8 or float64)."""
     assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
     if a.dtype == np.uint64:
         return a.ctypes.data_as(_u64p)
+    if a.dtype == np.uint32:
+        return a.ctypes.data_as(_u32p)
     if a.dtype == np.uint8:
         return a.ctypes.data_as(_u8p)
     if a.dtype == np.float64:
@@ -1141,3 +1171,157 @@ def scheme_divide_public_vec(kind, a, d):
     a = _u(a).copy()
     getattr(lib(), f"ref_{kind}_divide_public_vec")(_p(a), a.shape[-1], _p(_u(d)))
     return a
+
+
+# --- masked precompute mode ----------------------------------------------------
+# The build's own mode (no reference counterpart: the reference deals only
+# placeholder constants, src/mpc/Precompute.h:73-134).  `seed` keys
+# Philox4x32-10, `call` is the protocol call's number; see pir_ref.cpp.
+
+def philox(ctr, key):
+    """Philox4x32-10 blocks: ctr (n, 4) and key (n, 2) uint32 -> (n, 4) uint32."""
+    ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    lib().ref_philox(_p(ctr), _p(key), ctr.shape[0], _p(out))
+    return out
+
+
+def prg_fill(seed, call, n):
+    out = np.zeros(n, np.uint64)
+    lib().ref_prg_fill(seed, call, n, _p(out))
+    return out
+
+
+def deal(values, seed, call):
+    values = _u(values)
+    out = np.zeros((3, 2, values.size), np.uint64)
+    lib().ref_deal(_p(values), values.size, seed, call, _p(out))
+    return out
+
+
+def reshare_m(c, seed, call):
+    c = _u(c).copy()
+    n = c.shape[-1]
+    out = np.zeros((3, 2, n), np.uint64)
+    lib().ref_reshare_m(_p(c), n, _p(out), seed, call)
+    return out
+
+
+def mult_m(x, y, seed, call):
+    x, y = _u(x).copy(), _u(y)
+    lib().ref_rss_mult_m(_p(x), _p(y), x.shape[-1], seed, call)
+    return x
+
+
+def rss_and_m(x, y, seed, call):
+    x = np.ascontiguousarray(x).copy()
+    y = np.ascontiguousarray(y, x.dtype)
+    (lib().ref_rss_and_m_u8 if x.dtype == np.uint8 else lib().ref_rss_and_m_u64)(_p(x), _p(y), x.shape[-1], seed, call)
+    return x
+
+
+def divide_public_m(a, k, seed, call):
+    a = _u(a).copy()
+    lib().ref_divide_public_m(_p(a), a.shape[-1], k, seed, call)
+    return a
+
+
+def divide_public_3out3_m(a, k, seed, call):
+    a = _u(a).copy()
+    n = a.shape[-1]
+    out = np.zeros((3, 2, n), np.uint64)
+    lib().ref_divide_public_3out3_m(_p(a), n, k, _p(out), seed, call)
+    return out
+
+
+def matmul_m(a, b, M, N, K, ta, tb, tc, trunc, seed, call):
+    a, b = _u(a), _u(b)
+    c = np.zeros((3, 2, M * N), np.uint64)
+    lib().ref_matmul_m(_p(a), a.shape[-1], _p(b), b.shape[-1], _p(c), M, N, K, int(ta), int(tb), int(tc), trunc,
+                       seed, call)
+    return c
+
+
+def convolution_m(A, B, batch, H, W, f, Din, Dout, stride, pad, trunc, seed, call):
+    A, B = _u(A), _u(B)
+    OH, OW = (H + 2 * pad - f) // stride + 1, (W + 2 * pad - f) // stride + 1
+    C = np.zeros((3, 2, batch * OH * OW * Dout), np.uint64)
+    lib().ref_convolution_m(_p(A), A.shape[-1], _p(B), B.shape[-1], _p(C), C.shape[-1], batch, H, W, f, Din, Dout,
+                            stride, pad, trunc, seed, call)
+    return C
+
+
+def drelu_m(x, seed, call, out_dtype=np.uint8):
+    x = _u(x)
+    n = x.shape[-1]
+    out = np.zeros((3, 2, n), out_dtype)
+    (lib().ref_drelu_m_u8 if out_dtype == np.uint8 else lib().ref_drelu_m_u64)(_p(x), n, _p(out), seed, call)
+    return out
+
+
+def select_share_m(x, y, b, seed, call):
+    x, y = _u(x), _u(y)
+    b = np.ascontiguousarray(b)
+    n = x.shape[-1]
+    z = np.zeros((3, 2, n), np.uint64)
+    (lib().ref_select_share_m_u8 if b.dtype == np.uint8 else lib().ref_select_share_m_u64)(
+        _p(x), _p(y), _p(b), _p(z), n, seed, call)
+    return z
+
+
+def relu_m(x, seed, call, out_dtype=np.uint8):
+    x = _u(x)
+    n = x.shape[-1]
+    out = np.zeros((3, 2, n), np.uint64)
+    dout = np.zeros((3, 2, n), out_dtype)
+    (lib().ref_relu_m_u8 if out_dtype == np.uint8 else lib().ref_relu_m_u64)(_p(x), n, _p(out), _p(dout), seed, call)
+    return out, dout
+
+
+def maxpool_m(pools, k, seed, call, out_dtype=np.uint8):
+    pools = _u(pools).copy()
+    n = pools.shape[-1]
+    result = np.zeros((3, 2, n // k), np.uint64)
+    dres = np.zeros((3, 2, n), out_dtype)
+    (lib().ref_maxpool_m_u8 if out_dtype == np.uint8 else lib().ref_maxpool_m_u64)(
+        _p(pools), n, _p(result), _p(dres), k, seed, call)
+    return result, dres, pools
+
+
+def net_forward_m(layers, batch, fp, params, x_shares, seed, call):
+    """NeuralNetwork::forward in the masked precompute mode: layer i uses call
+    number call + i (fc, cnn, relu, maxpool layers)."""
+    out = []
+    h = _u(x_shares)
+    for i, l in enumerate(layers):
+        kind, c = l[0], call + i
+        if kind == "fc":
+            W, b = _u(params[i][0]), _u(params[i][1])
+            act = np.zeros((3, 2, batch * l[2]), np.uint64)
+            lib().ref_fc_forward_m(_p(h), _p(W), _p(b), _p(act), batch, l[1], l[2], fp, seed, c)
+            h = act
+            out.append((h, None))
+        elif kind == "cnn":
+            W = _u(params[i][0])
+            H, Wd, Cin, Cout, f, s, p = l[1:]
+            OH, OW = (H - f + 2 * p) // s + 1, (Wd - f + 2 * p) // s + 1
+            act = np.zeros((3, 2, batch * OH * OW * Cout), np.uint64)
+            lib().ref_cnn_forward_m(_p(h), _p(W), _p(act), batch, H, Wd, Cin, Cout, f, s, p, fp, seed, c)
+            h = act
+            out.append((h, None))
+        elif kind == "relu":
+            h, d = relu_m(h, seed, c)
+            out.append((h, d))
+        elif kind == "maxpool":
+            H, Wd, C, pool, s = l[1:]
+            OH, OW = (H - pool) // s + 1, (Wd - pool) // s + 1
+            n = batch * C * OH * OW
+            act = np.zeros((3, 2, n), np.uint64)
+            mp = np.zeros((3, 2, n * pool * pool), np.uint8)
+            lib().ref_maxpool_forward_m(_p(h), _p(act), _p(mp), batch, H, Wd, C, pool, s, fp, seed, c)
+            h = act
+            out.append((h, mp))
+        else:
+            raise ValueError(f"{kind}: no masked form")
+    return out

@@ -37,6 +37,7 @@
 #include <vector>
 #include <type_traits>
 #include <algorithm>
+#include <functional>
 
 namespace {
 
@@ -2232,6 +2233,448 @@ void scheme_taylor_series(typename A::V in, typename A::V out, double a0, double
     A::shl_all(out, negPow.v);  // every share (lshift_functor)
 }
 
+// ---------------------------------------------------------------------------
+// Masked precompute mode.  The reference deals no randomness: its correlated
+// randomness is placeholder constants (src/mpc/Precompute.h:73-134) and its
+// AES PRG (src/mpc/AESObject.cpp:11-90) is never instantiated
+// (src/main.cu:96-104).  This section is the oracle of the build's masked
+// mode: the protocol steps of the functions above, op by op and per party,
+// with the randomness a trusted dealer would deal drawn from Philox4x32-10
+// (J. K. Salmon et al., "Parallel Random Numbers: As Easy as 1, 2, 3",
+// SC'11).  Word j of element e of call `call` is 64-bit half (j & 1) of the
+// block Philox4x32-10(key = (seed_lo, seed_hi), ctr = (e_lo, e_hi, call, j >> 1)).
+// Per-protocol word layout: reshare / mult / AND 0, 1; truncation 0 = R
+// (arithmetic shift by k + 2), 1, 2 = r_1, r_2, 3, 4 = r'_1, r'_2; dReLU 0 = R,
+// 1, 2 = arithmetic shares of R - 1, 3, 4 = binary shares of -R, 5 + 4L ..
+// 8 + 4L = the two ANDs of carry level L; selectShare 0 = bit c, 1, 2 its
+// binary shares, 3, 4 its arithmetic shares, 5, 6 the product's reshare.
+// The two steps that need x1 = x2 = 0: dReLU's `1 - preResult` becomes
+// preResult XOR 1 in the x0 slot, selectShare opens b ^ cbits by XOR.
+// Truncation: r' = r * 2^k (the plan of Precompute.h:107), the opened t - r'
+// divided exactly and truncated toward zero.
+// Pinned by: the Philox KAT vectors, and closed forms -- masked outputs
+// reconstruct to the placeholder outputs (exact ops) or to floor / ceil of the
+// exact quotient (truncation), and the shares are not placeholder shares.
+
+inline uint32_t mulhilo(uint32_t a, uint32_t b, uint32_t *hi) {
+    u64 p = (u64)a * b;
+    *hi = (uint32_t)(p >> 32);
+    return (uint32_t)p;
+}
+
+void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]}, k[2] = {key[0], key[1]};
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0, hi1;
+        uint32_t lo0 = mulhilo(0xD2511F53u, c[0], &hi0);
+        uint32_t lo1 = mulhilo(0xCD9E8D57u, c[2], &hi1);
+        uint32_t n[4] = {hi1 ^ c[1] ^ k[0], lo1, hi0 ^ c[3] ^ k[1], lo0};
+        for (int i = 0; i < 4; i++) c[i] = n[i];
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+    for (int i = 0; i < 4; i++) out[i] = c[i];
+}
+
+struct MPrg {
+    u64 seed;
+    uint32_t call;
+};
+
+u64 mword(const MPrg &g, u64 e, uint32_t j) {
+    const uint32_t ctr[4] = {(uint32_t)e, (uint32_t)(e >> 32), g.call, j >> 1};
+    const uint32_t key[2] = {(uint32_t)g.seed, (uint32_t)(g.seed >> 32)};
+    uint32_t o[4];
+    philox4x32_10(ctr, key, o);
+    return (j & 1) ? ((u64)o[2] | ((u64)o[3] << 32)) : ((u64)o[0] | ((u64)o[1] << 32));
+}
+
+// The zero sharing of entry k: a[0], a[1] from words j, j+1 of element elem(k)
+// (cut to `width`, or to bit `bit(k)` of the words), a[2] closing the sum (XOR
+// for binary shares).
+template <typename T>
+struct ZeroSharing {
+    MPrg g;
+    uint32_t j;
+    bool xr;
+    std::function<u64(size_t)> elem;
+    std::function<int(size_t)> bit;  // -1: full words cut to T
+    void masks(size_t k, T a[3]) const {
+        u64 e = elem(k);
+        u64 w0 = mword(g, e, j), w1 = mword(g, e, j + 1);
+        int b = bit ? bit(k) : -1;
+        if (b >= 0) {
+            a[0] = (T)((w0 >> b) & 1);
+            a[1] = (T)((w1 >> b) & 1);
+        } else {
+            a[0] = (T)w0;
+            a[1] = (T)w1;
+        }
+        a[2] = xr ? (T)(a[0] ^ a[1]) : (T)((T)0 - a[0] - a[1]);
+    }
+};
+
+// reshare (RSS.inl:1052-1074) with the zero sharing z: out_p = (c_p + a_p, c_next + a_next)
+template <typename T>
+void reshare_m(Loc<T> c, Rs<T> out, const ZeroSharing<T> &z) {
+    const size_t n = c.size();
+    std::vector<T> cs[3];
+    for (int q = 0; q < 3; q++) {
+        cs[q].resize(n);
+        for (size_t k = 0; k < n; k++) cs[q][k] = c.p[q][k];
+    }
+    for (size_t k = 0; k < n; k++) {
+        T a[3];
+        z.masks(k, a);
+        T v[3];
+        for (int q = 0; q < 3; q++) v[q] = z.xr ? (T)(cs[q][k] ^ a[q]) : (T)(cs[q][k] + a[q]);
+        for (int q = 0; q < 3; q++) {
+            out.s[q][0][k] = v[q];
+            out.s[q][1][k] = v[nextParty(q)];
+        }
+    }
+}
+
+// RSS *= and &= (RSS.inl:283-345) with a masked reshare
+template <typename T>
+void r_mult_m(Rs<T> x, Rs<T> y, const ZeroSharing<T> &z) {
+    LocBuf<T> c(x.size());
+    for (int q = 0; q < 3; q++)
+        for (size_t k = 0; k < x.size(); k++)
+            c.d[q][k] = (T)(x.s[q][0][k] * (T)(y.s[q][0][k] + y.s[q][1][k]) + x.s[q][1][k] * y.s[q][0][k]);
+    reshare_m(c.v(), x, z);
+}
+template <typename T>
+void r_and_m(Rs<T> x, Rs<T> y, const ZeroSharing<T> &z) {
+    LocBuf<T> c(x.size());
+    for (int q = 0; q < 3; q++)
+        for (size_t k = 0; k < x.size(); k++)
+            c.d[q][k] = (T)((x.s[q][0][k] & (T)(y.s[q][0][k] ^ y.s[q][1][k])) ^ (x.s[q][1][k] & y.s[q][0][k]));
+    reshare_m(c.v(), x, z);
+}
+
+inline u64 trunc_exact(u64 u, int k) {
+    s64 v = (s64)u;
+    u64 mag = v < 0 ? (u64)0 - u : u;
+    u64 q = mag >> k;
+    return v < 0 ? (u64)0 - q : q;
+}
+
+// The truncation pair of element e: R with |R 2^k| < 2^61 and r = (R - r1 - r2, r1, r2)
+void trunc_pair(const MPrg &g, u64 e, int k, u64 &R, u64 r[3]) {
+    R = (u64)((s64)mword(g, e, 0) >> std::min(k + 2, 63));
+    r[1] = mword(g, e, 1);
+    r[2] = mword(g, e, 2);
+    r[0] = R - r[1] - r[2];
+}
+
+// dividePublic3out3 (RSS.inl:1076-1094), getDividedShares with r' = r * 2^k
+// dealt as 3-of-3 shares (words 3, 4); `a` is modified as in the reference.
+void divide_public_3out3_m(Loc<u64> a, int k, Rs<u64> out, const MPrg &g) {
+    const size_t n = a.size();
+    std::vector<u64> r[3];
+    for (int q = 0; q < 3; q++) r[q].resize(n);
+    for (size_t i = 0; i < n; i++) {
+        u64 R, rr[3];
+        trunc_pair(g, i, k, R, rr);
+        for (int q = 0; q < 3; q++) r[q][i] = rr[q];
+        u64 rp1 = mword(g, i, 3), rp2 = mword(g, i, 4);
+        u64 rp[3] = {rp1, rp2, (R << k) - rp1 - rp2};
+        for (int q = 0; q < 3; q++) a.p[q][i] -= rp[q];
+    }
+    LocBuf<u64> rec(n);
+    reconstruct3out3(a, rec.v());
+    for (int q = 0; q < 3; q++)
+        for (size_t i = 0; i < n; i++) {
+            out.s[q][0][i] = r[q][i] + (q == PA ? trunc_exact(rec.d[q][i], k) : 0);
+            out.s[q][1][i] = r[nextParty(q)][i] + (q == PC ? trunc_exact(rec.d[q][i], k) : 0);
+        }
+}
+
+// dividePublic (RSS.inl:422-436) by 2^k with r' = r * 2^k dealt as RSS shares (words 3, 4)
+void divide_public_m(Rs<u64> a, int k, const MPrg &g) {
+    const size_t n = a.size();
+    std::vector<u64> r[3];
+    for (int q = 0; q < 3; q++) r[q].resize(n);
+    for (size_t i = 0; i < n; i++) {
+        u64 R, rr[3];
+        trunc_pair(g, i, k, R, rr);
+        for (int q = 0; q < 3; q++) r[q][i] = rr[q];
+        u64 rp1 = mword(g, i, 3), rp2 = mword(g, i, 4);
+        u64 rp[3] = {(R << k) - rp1 - rp2, rp1, rp2};
+        for (int q = 0; q < 3; q++) {
+            a.s[q][0][i] -= rp[q];
+            a.s[q][1][i] -= rp[nextParty(q)];
+        }
+    }
+    LocBuf<u64> rec(n);
+    reconstruct(a, rec.v());
+    for (int q = 0; q < 3; q++)
+        for (size_t i = 0; i < n; i++) {
+            a.s[q][0][i] = r[q][i] + (q == PA ? trunc_exact(rec.d[q][i], k) : 0);
+            a.s[q][1][i] = r[nextParty(q)][i] + (q == PC ? trunc_exact(rec.d[q][i], k) : 0);
+        }
+}
+
+void matmul_m(Rs<u64> a, Rs<u64> b, Rs<u64> c, int M, int N, int K, bool ta, bool tb, bool tc, int trunc,
+              const MPrg &g) {
+    LocBuf<u64> raw((size_t)M * N);
+    local_matmul(a, b, raw.v(), M, N, K, ta, tb, tc);
+    divide_public_3out3_m(raw.v(), trunc, c, g);
+}
+
+void convolution_m(Rs<u64> A, Rs<u64> B, Rs<u64> C, int batch, int H, int W, int f, int Din, int Dout, int stride,
+                   int pad, int trunc, const MPrg &g) {
+    int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
+    LocBuf<u64> local((size_t)batch * OH * OW * Dout);
+    const int terms[3][2] = {{0, 0}, {0, 1}, {1, 0}};
+    std::vector<u64> acc(local.size());
+    for (int q = 0; q < 3; q++)
+        for (auto &t : terms) {
+            conv_fprop(A.s[q][t[0]].base, B.s[q][t[1]].base, acc.data(), batch, H, W, Din, Dout, f, f, pad, pad,
+                       stride, 1);
+            for (size_t i = 0; i < acc.size(); i++) local.d[q][i] += acc[i];
+        }
+    divide_public_3out3_m(local.v(), trunc, C, g);
+}
+
+// dReLU (RSS.inl:806-852), masked: value i draws words j0 .. j0 + 28 of
+// element elem[i]; the 64 bits of value i are uint8 shares as in the reference.
+template <typename U>
+void drelu_m(Rs<u64> input, Rs<U> result, const MPrg &g, const std::vector<u64> &elem, uint32_t j0) {
+    const int bw = 64;
+    const size_t n = input.size();
+    RsBuf<u64> r(n);
+    RsBuf<U> rbits(n * bw);
+    for (size_t i = 0; i < n; i++) {
+        u64 e = elem[i];
+        u64 R = mword(g, e, j0), s1 = mword(g, e, j0 + 1), s2 = mword(g, e, j0 + 2);
+        u64 s[3] = {R - 1 - s1 - s2, s1, s2};
+        for (int q = 0; q < 3; q++) {
+            r.d[q][0][i] = s[q];
+            r.d[q][1][i] = s[nextParty(q)];
+        }
+        u64 b1 = mword(g, e, j0 + 3), b2 = mword(g, e, j0 + 4);
+        u64 b[3] = {((u64)0 - R) ^ b1 ^ b2, b1, b2};
+        for (int q = 0; q < 3; q++)
+            for (int t = 0; t < bw; t++) {
+                rbits.d[q][0][i * bw + t] = (U)((b[q] >> t) & 1);
+                rbits.d[q][1][i * bw + t] = (U)((b[nextParty(q)] >> t) & 1);
+            }
+    }
+    LocBuf<u64> a(n);
+    r_op_r(r.v(), input, ADD);
+    reconstruct(r.v(), a.v());
+    for (int q = 0; q < 3; q++) dd_op_s(a.v().p[q], (u64)1, ADD);
+    LocBuf<U> abits(n * bw);
+    for (int q = 0; q < 3; q++) bitexpand(a.v().p[q], abits.v().p[q]);
+    RsBuf<U> msb(n);
+    Rs<U> rb = rbits.v(), mb = msb.v();
+    Loc<U> ab = abits.v();
+    for (int q = 0; q < 3; q++) {
+        if (q != PC) {
+            set_carry_out_msb(rb.s[q][0], ab.p[q], mb.s[q][0], bw, q == PA);
+            set_carry_out_msb(rb.s[q][1], ab.p[q], mb.s[q][1], bw, q == PC);
+        } else {
+            set_carry_out_msb(rb.s[q][1], ab.p[q], mb.s[q][1], bw, q == PC);
+            set_carry_out_msb(rb.s[q][0], ab.p[q], mb.s[q][0], bw, q == PA);
+        }
+    }
+    RsBuf<U> gg(n * bw);
+    r_op_r(gg.v(), rb, ADD);
+    r_op_l(gg.v(), ab, AND);
+    RsBuf<U> pp(n * bw);
+    r_op_r(pp.v(), rb, ADD);
+    r_op_l(pp.v(), ab, XOR);
+    // carryOut (RSS.inl:1112-1209) with the level-L masks: bit (P % 64) of the
+    // words of value P / 64, P the view entry's position in the bit array
+    Rs<U> p = pp.v(), gv = gg.v();
+    int stride = 2, offset = 1, L = 0, k = bw;
+    Rs<U> pEven = stridedRs(p, 0, stride), pOdd = stridedRs(p, offset, stride);
+    Rs<U> gEven = stridedRs(gv, 0, stride), gOdd = stridedRs(gv, offset, stride);
+    auto zs = [&](const Rs<U> &view, uint32_t j) {
+        ZeroSharing<U> z{g, j0 + j, true, nullptr, nullptr};
+        const size_t off = view.s[0][0].off, st = view.s[0][0].stride;
+        z.elem = [&elem, off, st, bw](size_t kk) { return elem[(off + kk * st) / bw]; };
+        z.bit = [off, st, bw](size_t kk) { return (int)((off + kk * st) % bw); };
+        return z;
+    };
+    while (k > 1) {
+        r_and_m(gEven, pOdd, zs(gEven, 5 + 4 * L));
+        r_and_m(pEven, pOdd, zs(pEven, 7 + 4 * L));
+        r_op_r(gOdd, gEven, XOR);
+        gEven = stridedRs(gv, offset, stride * 2);
+        offset += stride;
+        gOdd = stridedRs(gv, offset, stride * 2);
+        stride *= 2;
+        pEven = stridedRs(p, 0, stride);
+        pOdd = stridedRs(p, stride / 2, stride);
+        k /= 2;
+        L++;
+    }
+    RsBuf<U> pre(n);
+    // carry_out's output copy (out[0::2] = G[63::128], out[1::2] = G[127::128]):
+    // value i's preResult is the carry G at bit 64i + 63
+    for (int q = 0; q < 3; q++)
+        for (int s = 0; s < 2; s++)
+            for (size_t i = 0; i < n; i++) pre.d[q][s][i] = gg.d[q][s][i * bw + bw - 1];
+    r_op_r(pre.v(), mb, XOR);
+    for (int q = 0; q < 3; q++)
+        for (int s = 0; s < 2; s++)
+            for (size_t i = 0; i < n; i++) {
+                U one = (U)((q == PA && s == 0) || (q == PC && s == 1) ? 1 : 0);
+                result.s[q][s][i] = (U)(pre.d[q][s][i] ^ one);
+            }
+}
+
+// selectShare (RSS.inl:542-569), masked: value i draws words j0 .. j0 + 6 of elem[i]
+template <typename U>
+void select_share_m(Rs<u64> x, Rs<u64> y, Rs<U> b, Rs<u64> z, const MPrg &g, const std::vector<u64> &elem,
+                    uint32_t j0) {
+    const size_t n = x.size();
+    RsBuf<u64> c(n);
+    RsBuf<U> cbits(n);
+    for (size_t i = 0; i < n; i++) {
+        u64 e = elem[i];
+        U cb = (U)(mword(g, e, j0) & 1), cb1 = (U)(mword(g, e, j0 + 1) & 1), cb2 = (U)(mword(g, e, j0 + 2) & 1);
+        U cbs[3] = {(U)(cb ^ cb1 ^ cb2), cb1, cb2};
+        u64 c1 = mword(g, e, j0 + 3), c2 = mword(g, e, j0 + 4);
+        u64 cs[3] = {(u64)cb - c1 - c2, c1, c2};
+        for (int q = 0; q < 3; q++) {
+            cbits.d[q][0][i] = cbs[q];
+            cbits.d[q][1][i] = cbs[nextParty(q)];
+            c.d[q][0][i] = cs[q];
+            c.d[q][1][i] = cs[nextParty(q)];
+        }
+    }
+    r_op_r(cbits.v(), b, XOR);
+    LocBuf<U> e(n);
+    for (int q = 0; q < 3; q++)
+        for (size_t i = 0; i < n; i++)
+            e.d[q][i] = (U)(cbits.d[q][0][i] ^ cbits.d[q][1][i] ^ cbits.d[prevParty(q)][0][i]);
+    RsBuf<u64> d(n);
+    convex_comb(d.v(), c.v(), e.v());
+    RsBuf<u64> result(n);
+    r_op_r(result.v(), y, ADD);
+    r_op_r(result.v(), x, SUB);
+    ZeroSharing<u64> zsh{g, j0 + 5, false, [&elem](size_t k) { return elem[k]; }, nullptr};
+    r_mult_m(result.v(), d.v(), zsh);
+    r_op_r(result.v(), x, ADD);
+    r_zero(z);
+    r_op_r(z, result.v(), ADD);
+}
+
+template <typename U>
+void relu_m(Rs<u64> input, Rs<u64> result, Rs<U> dresult, const MPrg &g) {
+    std::vector<u64> elem(input.size());
+    for (size_t i = 0; i < elem.size(); i++) elem[i] = i;
+    drelu_m(input, dresult, g, elem, 0);
+    RsBuf<u64> zeros(input.size());
+    select_share_m(zeros.v(), input, dresult, result, g, elem, 29);
+}
+
+// maxpool (RSS.inl:867-1017), masked: compare m of a window (in round order)
+// draws words 36m .. 36m + 35, the AND of round t at pool position pos
+// 36(k-1) + 2(tk + pos) and + 1; 1 - b becomes b ^ 1 in the x0 slot.
+template <typename U>
+void maxpool_m(Rs<u64> input, Rs<u64> result, Rs<U> dresult, int k, const MPrg &g) {
+    const size_t n = input.size();
+    r_fill(dresult, (U)1);
+    int stride = 2, offset = 1, t = 0;
+    uint32_t m_base = 0;
+    const int K = k;
+    Rs<u64> even = stridedRs(input, 0, stride), odd = stridedRs(input, offset, stride);
+    auto round = [&](bool last) {
+        const size_t cnt = even.size();
+        std::vector<u64> elem(cnt);
+        std::vector<uint32_t> jj(cnt);
+        for (size_t i = 0; i < cnt; i++) {
+            size_t pos = i * stride;
+            elem[i] = pos / K;
+            jj[i] = m_base + (uint32_t)((pos % K) / stride);
+        }
+        RsBuf<u64> diff(cnt);
+        r_op_r(diff.v(), even, ADD);
+        r_op_r(diff.v(), odd, SUB);
+        RsBuf<U> b(cnt);
+        // each compare has its own word offset: evaluate one compare index at a time
+        const uint32_t per_window = (uint32_t)(K / stride);
+        for (uint32_t j = 0; j < per_window; j++) {
+            std::vector<size_t> idx;
+            for (size_t i = 0; i < cnt; i++)
+                if (jj[i] == m_base + j) idx.push_back(i);
+            RsBuf<u64> dsub(idx.size()), ev(idx.size()), od(idx.size()), out(idx.size());
+            RsBuf<U> bsub(idx.size());
+            std::vector<u64> el(idx.size());
+            for (size_t a = 0; a < idx.size(); a++) {
+                el[a] = elem[idx[a]];
+                for (int q = 0; q < 3; q++)
+                    for (int s = 0; s < 2; s++) {
+                        dsub.d[q][s][a] = diff.d[q][s][idx[a]];
+                        ev.d[q][s][a] = even.s[q][s][idx[a]];
+                        od.d[q][s][a] = odd.s[q][s][idx[a]];
+                    }
+            }
+            const uint32_t m = m_base + j;
+            drelu_m(dsub.v(), bsub.v(), g, el, 36 * m);
+            select_share_m(od.v(), ev.v(), bsub.v(), out.v(), g, el, 36 * m + 29);
+            for (size_t a = 0; a < idx.size(); a++)
+                for (int q = 0; q < 3; q++)
+                    for (int s = 0; s < 2; s++) {
+                        b.d[q][s][idx[a]] = bsub.d[q][s][a];
+                        even.s[q][s][idx[a]] = out.d[q][s][a];
+                    }
+        }
+        m_base += per_window;
+        if (!last) {
+            stride *= 2;
+            even = stridedRs(input, 0, stride);
+            odd = stridedRs(input, stride / 2, stride);
+        } else {
+            r_zero(result);
+            r_op_r(result, even, ADD);
+        }
+        RsBuf<U> negated(cnt);
+        r_op_r(negated.v(), b.v(), ADD);
+        for (size_t i = 0; i < cnt; i++) {
+            negated.d[PA][0][i] ^= (U)1;
+            negated.d[PC][1][i] ^= (U)1;
+        }
+        RsBuf<U> expandedB(n);
+        for (int q = 0; q < 3; q++)
+            for (int s = 0; s < 2; s++)
+                expand_compare(b.v().s[q][s], negated.v().s[q][s], expandedB.v().s[q][s]);
+        const int tr = t;
+        // dresult &= expandedB with the zero sharing of words 36(k-1) + 2(tk + pos)
+        LocBuf<U> cz(n);
+        for (int q = 0; q < 3; q++)
+            for (size_t P = 0; P < n; P++)
+                cz.d[q][P] = (U)((dresult.s[q][0][P] & (U)(expandedB.d[q][0][P] ^ expandedB.d[q][1][P])) ^
+                                 (dresult.s[q][1][P] & expandedB.d[q][0][P]));
+        for (size_t P = 0; P < n; P++) {
+            uint32_t j = 36 * (uint32_t)(K - 1) + 2 * (uint32_t)(tr * K + (int)(P % K));
+            u64 e = P / K;
+            U a[3];
+            a[0] = (U)(mword(g, e, j) & 1);
+            a[1] = (U)(mword(g, e, j + 1) & 1);
+            a[2] = (U)(a[0] ^ a[1]);
+            U v[3];
+            for (int q = 0; q < 3; q++) v[q] = (U)(cz.d[q][P] ^ a[q]);
+            for (int q = 0; q < 3; q++) {
+                dresult.s[q][0][P] = v[q];
+                dresult.s[q][1][P] = v[nextParty(q)];
+            }
+        }
+        t++;
+    };
+    int kk = k;
+    while (kk > 2) {
+        round(false);
+        kk /= 2;
+    }
+    round(true);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2632,6 +3075,134 @@ void ref_set_msbs_u64(u64 *bits, size_t n_bits, u64 v, size_t bw) { set_msbs(bit
 void ref_reveal_softmax_grad(const u64 *act, const double *labels, size_t n, u64 *deltas, int nClasses, int fp,
                              int logMiniBatch) {
     reveal_softmax_grad(wrapRs((u64 *)act, n), labels, wrapRs(deltas, n), nClasses, fp, logMiniBatch);
+}
+
+
+// ---- masked precompute mode (the section "Masked precompute mode" above) ----
+void ref_philox(const uint32_t *ctr, const uint32_t *key, size_t n, uint32_t *out) {
+    for (size_t i = 0; i < n; i++) philox4x32_10(ctr + 4 * i, key + 2 * i, out + 4 * i);
+}
+void ref_prg_fill(u64 seed, uint32_t call, size_t n, u64 *out) {
+    MPrg g{seed, call};
+    for (size_t i = 0; i < n; i++) out[i] = mword(g, i / 2, (uint32_t)(i & 1));
+}
+// a random consistent sharing of values (x1, x2 = words 0, 1 of element i)
+void ref_deal(const u64 *values, size_t n, u64 seed, uint32_t call, u64 *shares) {
+    MPrg g{seed, call};
+    Rs<u64> S = wrapRs(shares, n);
+    for (size_t i = 0; i < n; i++) {
+        u64 x1 = mword(g, i, 0), x2 = mword(g, i, 1);
+        u64 x[3] = {values[i] - x1 - x2, x1, x2};
+        for (int q = 0; q < 3; q++) {
+            S.s[q][0][i] = x[q];
+            S.s[q][1][i] = x[nextParty(q)];
+        }
+    }
+}
+static std::vector<u64> iota_elems(size_t n) {
+    std::vector<u64> e(n);
+    for (size_t i = 0; i < n; i++) e[i] = i;
+    return e;
+}
+void ref_reshare_m(u64 *c, size_t n, u64 *out, u64 seed, uint32_t call) {
+    ZeroSharing<u64> z{MPrg{seed, call}, 0, false, [](size_t k) { return (u64)k; }, nullptr};
+    reshare_m(wrapLoc(c, n), wrapRs(out, n), z);
+}
+void ref_rss_mult_m(u64 *x, const u64 *y, size_t n, u64 seed, uint32_t call) {
+    ZeroSharing<u64> z{MPrg{seed, call}, 0, false, [](size_t k) { return (u64)k; }, nullptr};
+    r_mult_m(wrapRs(x, n), wrapRs((u64 *)y, n), z);
+}
+void ref_rss_and_m_u8(uint8_t *x, const uint8_t *y, size_t n, u64 seed, uint32_t call) {
+    ZeroSharing<uint8_t> z{MPrg{seed, call}, 0, true, [](size_t k) { return (u64)k; }, nullptr};
+    r_and_m(wrapRs(x, n), wrapRs((uint8_t *)y, n), z);
+}
+void ref_rss_and_m_u64(u64 *x, const u64 *y, size_t n, u64 seed, uint32_t call) {
+    ZeroSharing<u64> z{MPrg{seed, call}, 0, true, [](size_t k) { return (u64)k; }, nullptr};
+    r_and_m(wrapRs(x, n), wrapRs((u64 *)y, n), z);
+}
+void ref_divide_public_m(u64 *a, size_t n, int k, u64 seed, uint32_t call) {
+    divide_public_m(wrapRs(a, n), k, MPrg{seed, call});
+}
+void ref_divide_public_3out3_m(u64 *a, size_t n, int k, u64 *out, u64 seed, uint32_t call) {
+    divide_public_3out3_m(wrapLoc(a, n), k, wrapRs(out, n), MPrg{seed, call});
+}
+void ref_matmul_m(const u64 *a, size_t a_n, const u64 *b, size_t b_n, u64 *c, int M, int N, int K, int ta, int tb,
+                  int tc, int trunc, u64 seed, uint32_t call) {
+    matmul_m(wrapRs((u64 *)a, a_n), wrapRs((u64 *)b, b_n), wrapRs(c, (size_t)M * N), M, N, K, ta != 0, tb != 0,
+             tc != 0, trunc, MPrg{seed, call});
+}
+void ref_convolution_m(const u64 *A, size_t a_n, const u64 *B, size_t b_n, u64 *C, size_t c_n, int batch, int H,
+                       int W, int f, int Din, int Dout, int stride, int pad, int trunc, u64 seed, uint32_t call) {
+    convolution_m(wrapRs((u64 *)A, a_n), wrapRs((u64 *)B, b_n), wrapRs(C, c_n), batch, H, W, f, Din, Dout, stride,
+                  pad, trunc, MPrg{seed, call});
+}
+void ref_drelu_m_u8(const u64 *in, size_t n, uint8_t *out, u64 seed, uint32_t call) {
+    drelu_m(wrapRs((u64 *)in, n), wrapRs(out, n), MPrg{seed, call}, iota_elems(n), 0);
+}
+void ref_drelu_m_u64(const u64 *in, size_t n, u64 *out, u64 seed, uint32_t call) {
+    drelu_m(wrapRs((u64 *)in, n), wrapRs(out, n), MPrg{seed, call}, iota_elems(n), 0);
+}
+void ref_select_share_m_u8(const u64 *x, const u64 *y, const uint8_t *b, u64 *z, size_t n, u64 seed, uint32_t call) {
+    select_share_m(wrapRs((u64 *)x, n), wrapRs((u64 *)y, n), wrapRs((uint8_t *)b, n), wrapRs(z, n),
+                   MPrg{seed, call}, iota_elems(n), 0);
+}
+void ref_select_share_m_u64(const u64 *x, const u64 *y, const u64 *b, u64 *z, size_t n, u64 seed, uint32_t call) {
+    select_share_m(wrapRs((u64 *)x, n), wrapRs((u64 *)y, n), wrapRs((u64 *)b, n), wrapRs(z, n), MPrg{seed, call},
+                   iota_elems(n), 0);
+}
+void ref_relu_m_u8(const u64 *in, size_t n, u64 *out, uint8_t *dout, u64 seed, uint32_t call) {
+    relu_m(wrapRs((u64 *)in, n), wrapRs(out, n), wrapRs(dout, n), MPrg{seed, call});
+}
+void ref_relu_m_u64(const u64 *in, size_t n, u64 *out, u64 *dout, u64 seed, uint32_t call) {
+    relu_m(wrapRs((u64 *)in, n), wrapRs(out, n), wrapRs(dout, n), MPrg{seed, call});
+}
+void ref_maxpool_m_u8(u64 *pools, size_t n, u64 *result, uint8_t *dresult, int k, u64 seed, uint32_t call) {
+    maxpool_m(wrapRs(pools, n), wrapRs(result, n / k), wrapRs(dresult, n), k, MPrg{seed, call});
+}
+void ref_maxpool_m_u64(u64 *pools, size_t n, u64 *result, u64 *dresult, int k, u64 seed, uint32_t call) {
+    maxpool_m(wrapRs(pools, n), wrapRs(result, n / k), wrapRs(dresult, n), k, MPrg{seed, call});
+}
+// the layer forwards in the masked mode (FC: product, truncation, then the bias)
+void ref_fc_forward_m(const u64 *input, const u64 *W, const u64 *bias, u64 *act, int batch, int in, int out, int fp,
+                      u64 seed, uint32_t call) {
+    size_t actN = (size_t)batch * out;
+    Rs<u64> A = wrapRs(act, actN);
+    r_zero(A);
+    matmul_m(wrapRs((u64 *)input, (size_t)batch * in), wrapRs((u64 *)W, (size_t)in * out), A, batch, out, in, true,
+             true, true, fp, MPrg{seed, call});
+    Rs<u64> Bv = wrapRs((u64 *)bias, (size_t)out);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) element_vector_add(A.s[q][i], Bv.s[q][i], false, out, batch);
+}
+void ref_cnn_forward_m(const u64 *input, const u64 *W, u64 *act, int batch, int H, int Wd, int Din, int Dout, int f,
+                       int stride, int pad, int fp, u64 seed, uint32_t call) {
+    int OH = (H - f + 2 * pad) / stride + 1, OW = (Wd - f + 2 * pad) / stride + 1;
+    size_t actN = (size_t)batch * OH * OW * Dout;
+    Rs<u64> A = wrapRs(act, actN);
+    r_zero(A);
+    convolution_m(wrapRs((u64 *)input, (size_t)batch * H * Wd * Din), wrapRs((u64 *)W, (size_t)Dout * f * f * Din), A,
+                  batch, H, Wd, f, Din, Dout, stride, pad, fp, MPrg{seed, call});
+}
+void ref_maxpool_forward_m(const u64 *input, u64 *act, uint8_t *max_prime, int batch, int H, int W, int C, int pool,
+                           int stride, int fp, u64 seed, uint32_t call) {
+    int OH = (H - pool) / stride + 1, OW = (W - pool) / stride + 1;
+    size_t actN = (size_t)batch * C * OH * OW;
+    size_t mpN = actN * pool * pool;
+    Rs<u64> A = wrapRs(act, actN);
+    Rs<uint8_t> MP = wrapRs(max_prime, mpN);
+    r_zero(A);
+    r_zero(MP);
+    int eps = expanded_pool_size(pool);
+    u64 pad_value = (u64)(s64)(-10 * (1 << fp));
+    size_t inN = (size_t)batch * H * W * C;
+    RsBuf<u64> pools(0);
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++)
+            maxpool_im2row(input + (2 * q + i) * inN, pools.d[q][i], H, W, pool, C, batch, stride, 0, pad_value);
+    RsBuf<uint8_t> emp(pools.size());
+    maxpool_m(pools.v(), A, emp.v(), eps, MPrg{seed, call});
+    for (int q = 0; q < 3; q++)
+        for (int i = 0; i < 2; i++) truncate_cols(emp.v().s[q][i], MP.s[q][i], eps, pool * pool);
 }
 
 }  // extern "C"

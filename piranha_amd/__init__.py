@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libpir_hip.so")
 
 ALGO_PLAIN, ALGO_LIMB = 0, 1
 MODE_FUSED, MODE_PLAIN = 0, 1
+PRECOMPUTE_PLACEHOLDER, PRECOMPUTE_MASKED = 0, 1
 LAYER_FC, LAYER_CNN, LAYER_RELU, LAYER_MAXPOOL, LAYER_AVGPOOL, LAYER_LN, LAYER_RES = 0, 1, 2, 3, 4, 5, 6
 CONV_FPROP, CONV_DGRAD, CONV_WGRAD = 0, 1, 2
 LIMB_WIDE, LIMB_VARIANTS = 8, 16  # kernel variant = A-operand form (0..6) + 8 for the 32x128 tile
@@ -47,6 +48,14 @@ _SIGS = {
     "pir_release_cached": ([], _i),
     "pir_cached_bytes": ([], _sz),
     "pir_stream_synchronize": ([_vp], _i),
+    "pir_set_precompute": ([_i, _u64, ctypes.c_uint32], _i),
+    "pir_get_prg_call": ([ctypes.POINTER(ctypes.c_uint32)], _i),
+    "pir_prg_philox": ([_vp, _vp, _sz, _vp, _vp], _i),
+    "pir_prg_fill": ([_u64, ctypes.c_uint32, _sz, _vp, _vp], _i),
+    "pir_rss_deal": ([_vp, _sz, _u64, ctypes.c_uint32, _vp, _vp], _i),
+    "pir_rss_rerandomize": ([_vp, _sz, _u64, ctypes.c_uint32, _vp, _vp], _i),
+    "pir_net_set_precompute": ([_vp, _i, _u64, ctypes.c_uint32], _i),
+    "pir_net_rerandomize": ([_vp, _u64], _i),
     "pir_measure_peak": ([_i, _sz, _i, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp], _i),
     "pir_device_synchronize": ([], _i),
     "pir_set_mode": ([_i], _i),
@@ -265,6 +274,18 @@ def set_mode(mode):
     check(lib().pir_set_mode(mode))
 
 
+def set_precompute(mode, seed=0, call=0):
+    """The calling thread's precompute mode for the free-standing calls (the
+    masked mode draws call number `call`, then the next ones, from `seed`)."""
+    check(lib().pir_set_precompute(mode, seed, call))
+
+
+def prg_call():
+    c = ctypes.c_uint32()
+    check(lib().pir_get_prg_call(ctypes.byref(c)))
+    return c.value
+
+
 def _ptr_array(ptrs):
     arr = (_vp * len(ptrs))(*[p for p in ptrs])
     return ctypes.cast(arr, _vp), arr
@@ -336,6 +357,13 @@ class Network:
 
     def set_mode(self, mode):
         check(lib().pir_net_set_mode(self.h, mode))
+
+    def set_precompute(self, mode, seed=0, call=0):
+        check(lib().pir_net_set_precompute(self.h, mode, seed, call))
+
+    def rerandomize(self, seed):
+        """Every parameter and the input buffer re-dealt as random sharings."""
+        check(lib().pir_net_rerandomize(self.h, seed))
 
     def set_weights(self, layer, w, b=None):
         w = np.ascontiguousarray(w, np.float64).ravel()

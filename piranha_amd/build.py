@@ -22,6 +22,7 @@ SOURCES = [
     "tpc_fpc_compare.hip",
     "noalloc.hip",
     "peaks.hip",
+    "prg.hip",
     "runtime.cpp",
     "protocol.cpp",
     "tpc_fpc.cpp",

@@ -40,6 +40,14 @@ struct Scope {
     }
 };
 
+// The TPC and FPC protocols keep the reference's placeholder precompute: they
+// refuse the masked mode of the calling thread.
+struct SchemeScope : Scope {
+    explicit SchemeScope(void *stream, int algo = ALGO_LIMB) : Scope(stream, algo) {
+        require_placeholder("TPC / FPC protocols");
+    }
+};
+
 template <typename F>
 int guarded(F &&f) {
     try {
@@ -336,6 +344,36 @@ int pir_ring_gemm3(int algo, int M, int N, int K, const uint64_t *A0, const uint
     });
 }
 
+// ---- masked precompute mode and the seeded PRG --------------------------------
+
+int pir_set_precompute(int mode, uint64_t seed, uint32_t call) {
+    PIR_REQUIRE(mode == PRECOMPUTE_PLACEHOLDER || mode == PRECOMPUTE_MASKED, "pir_set_precompute: unknown mode");
+    Context &c = ctx();
+    c.precompute = mode;
+    c.prg_seed = seed;
+    c.prg_call = call;
+    return OK;
+}
+int pir_get_prg_call(uint32_t *call) {
+    PIR_REQUIRE(call, "pir_get_prg_call: null pointer");
+    *call = ctx().prg_call;
+    return OK;
+}
+int pir_prg_philox(const uint32_t *ctr, const uint32_t *key, size_t n, uint32_t *out, void *stream) {
+    return prg_philox(ctr, key, n, out, (hipStream_t)stream);
+}
+int pir_prg_fill(uint64_t seed, uint32_t call, size_t n, uint64_t *out, void *stream) {
+    return prg_fill(seed, call, n, out, (hipStream_t)stream);
+}
+int pir_rss_deal(const uint64_t *values, size_t n, uint64_t seed, uint32_t call, uint64_t *shares, void *stream) {
+    PIR_REQUIRE(values, "pir_rss_deal: null values");
+    return rss_deal(values, nullptr, n, seed, call, shares, (hipStream_t)stream);
+}
+int pir_rss_rerandomize(const uint64_t *in, size_t n, uint64_t seed, uint32_t call, uint64_t *out, void *stream) {
+    PIR_REQUIRE(in, "pir_rss_rerandomize: null shares");
+    return rss_deal(nullptr, in, n, seed, call, out, (hipStream_t)stream);
+}
+
 // ---- bit / layout kernels ----------------------------------------------------
 
 int pir_bitexpand(const uint64_t *a, size_t n, void *bits, int eb, void *stream) {
@@ -417,7 +455,7 @@ int pir_rss_reconstruct(const void *x, size_t n, void *out, int eb, void *stream
 int pir_rss_reshare(const uint64_t *c, size_t n, uint64_t *out, void *stream) {
     return guarded([&]() -> int {
         Scope sc(stream);
-        if (g_mode == MODE_FUSED) return fused_reshare(c, n, out, (hipStream_t)stream);
+        if (g_mode == MODE_FUSED) return fused_reshare(c, n, out, (hipStream_t)stream, call_prg("reshare"));
         PartyData<u64> cv(DeviceData<u64>((u64 *)c, n), DeviceData<u64>((u64 *)c + n, n),
                           DeviceData<u64>((u64 *)c + 2 * n, n));
         RSS<u64> o = RSS<u64>::wrap(out, n);
@@ -459,6 +497,7 @@ int pir_rss_divide_public(uint64_t *a, size_t n, uint64_t d, const uint64_t *d_l
                               DeviceData<u64>((u64 *)d_local + 2 * n, n));
             if (g_mode == MODE_FUSED) {
                 // the fused kernel reads the contiguous [3][n] denominator block
+                require_placeholder("dividePublic by per-party denominators");
                 PIR_TRY(fused_divide_public(a, n, 0, d_local, (hipStream_t)stream));
                 return (int)OK;
             }
@@ -474,7 +513,8 @@ int pir_rss_divide_public_3out3(const uint64_t *a, size_t n, int k, uint64_t *ou
     return guarded([&]() -> int {
         Scope sc(stream);
         PIR_REQUIRE(k >= 0 && k <= 62, "dividePublic3out3: k must be in [0, 62]");
-        if (g_mode == MODE_FUSED) return fused_divide_3out3(a, n, k, out, (hipStream_t)stream);
+        if (g_mode == MODE_FUSED)
+            return fused_divide_3out3(a, n, k, out, (hipStream_t)stream, call_prg("dividePublic3out3"));
         // the op-by-op form modifies its input (RSS.inl:1085): work on a copy
         PartyData<u64> av(n);
         PIR_HIP(hipMemcpyAsync(av.base(), a, 3 * n * 8, hipMemcpyDeviceToDevice, (hipStream_t)stream));
@@ -540,7 +580,7 @@ int pir_rss_convolution_op(int op, const uint64_t *a, const uint64_t *b, uint64_
 
 int pir_tpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         tpc_reconstruct(x, n, out);
         return (int)OK;
     });
@@ -548,28 +588,28 @@ int pir_tpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream
 
 int pir_tpc_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return tpc_mult(x, y, n, (hipStream_t)stream);
     });
 }
 
 int pir_tpc_and(void *x, const void *y, size_t n, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return tpc_and(x, y, n, eb, (hipStream_t)stream);
     });
 }
 
 int pir_tpc_drelu(const uint64_t *x, size_t n, void *result, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return tpc_relu(x, n, result, nullptr, eb, (hipStream_t)stream);
     });
 }
 
 int pir_tpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         PIR_REQUIRE(!overlaps(x, 16 * n, y, 16 * n), "pir_tpc_relu: y must not overlap x");
         return tpc_relu(x, n, dresult, y, eb, (hipStream_t)stream);
     });
@@ -578,42 +618,42 @@ int pir_tpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb
 int pir_tpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int eb,
                          void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return tpc_select(x, y, b, z, n, eb, (hipStream_t)stream);
     });
 }
 
 int pir_tpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return tpc_maxpool(pools, n, result, dresult, k, eb, (hipStream_t)stream);
     });
 }
 
 int pir_fpc_mult(uint64_t *x, const uint64_t *y, size_t n, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return fpc_mult(x, y, n, (hipStream_t)stream);
     });
 }
 
 int pir_fpc_and(void *x, const void *y, size_t n, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return fpc_and(x, y, n, eb, (hipStream_t)stream);
     });
 }
 
 int pir_fpc_drelu(const uint64_t *x, size_t n, void *result, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return fpc_relu(x, n, result, nullptr, eb, (hipStream_t)stream);
     });
 }
 
 int pir_fpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         PIR_REQUIRE(!overlaps(x, 96 * n, y, 96 * n), "pir_fpc_relu: y must not overlap x");
         return fpc_relu(x, n, dresult, y, eb, (hipStream_t)stream);
     });
@@ -622,14 +662,14 @@ int pir_fpc_relu(const uint64_t *x, size_t n, uint64_t *y, void *dresult, int eb
 int pir_fpc_select_share(const uint64_t *x, const uint64_t *y, const void *b, uint64_t *z, size_t n, int eb,
                          void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return fpc_select(x, y, b, z, n, eb, (hipStream_t)stream);
     });
 }
 
 int pir_fpc_maxpool(uint64_t *pools, size_t n, uint64_t *result, void *dresult, int k, int eb, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return fpc_maxpool(pools, n, result, dresult, k, eb, (hipStream_t)stream);
     });
 }
@@ -639,7 +679,7 @@ namespace {
 int scheme_get_powers_call(int P, int S, const uint64_t *x, size_t n, uint64_t *pow, void *stream, const char *what,
                            int (*f)(const u64 *, size_t, u64 *, hipStream_t)) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         if (overlaps(x, (size_t)P * S * 8 * n, pow, (size_t)P * 8 * n)) {
             set_error(std::string(what) + ": pow must not overlap x");
             return (int)ERR_ARG;
@@ -651,7 +691,7 @@ int scheme_taylor_call(int P, int S, const uint64_t *x, size_t n, uint64_t *out,
                        void *stream, const char *what,
                        int (*f)(const u64 *, size_t, u64 *, const TaylorTables &, hipStream_t)) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         if (overlaps(x, (size_t)P * S * 8 * n, out, (size_t)P * S * 8 * n)) {
             set_error(std::string(what) + ": out must not overlap x");
             return (int)ERR_ARG;
@@ -666,13 +706,13 @@ const double kTaylorCoeffs[3][3] = {{0.424, 0.584, 0.0}, {4.245, -5.857, 2.630},
 
 int pir_tpc_divide_public_local(uint64_t *a, size_t n, const uint64_t *d_local, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return tpc_divide_public_vec(a, n, d_local, (hipStream_t)stream);
     });
 }
 int pir_fpc_divide_public_local(uint64_t *a, size_t n, const uint64_t *d_local, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         return fpc_divide_public_vec(a, n, d_local, (hipStream_t)stream);
     });
 }
@@ -707,7 +747,7 @@ PIR_SCHEME_TAYLOR(fpc, 4, 3, sigmoid, TAYLOR_SIGMOID)
 
 int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         tpc_divide_public(a, n, d);
         return (int)OK;
     });
@@ -716,7 +756,7 @@ int pir_tpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
 int pir_tpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
                    int trunc, int algo, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        SchemeScope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
         PIR_REQUIRE(M > 0 && N > 0 && K > 0 && trunc >= 1 && trunc <= 62, "pir_tpc_matmul: invalid arguments");
         tpc_matmul(a, b, c, M, N, K, ta != 0, tb != 0, tc != 0, trunc);
         return (int)OK;
@@ -726,7 +766,7 @@ int pir_tpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int
 int pir_tpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
                         int Dout, int stride, int pad, int trunc, int algo, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        SchemeScope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
         PIR_REQUIRE(batch > 0 && H > 0 && W > 0 && f > 0 && Din > 0 && Dout > 0 && stride > 0 && pad >= 0 &&
                         trunc >= 1 && trunc <= 62 && (H + 2 * pad - f) >= 0 && (W + 2 * pad - f) >= 0,
                     "pir_tpc_convolution: invalid geometry");
@@ -737,7 +777,7 @@ int pir_tpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int b
 
 int pir_fpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         fpc_reconstruct(x, n, out);
         return (int)OK;
     });
@@ -745,7 +785,7 @@ int pir_fpc_reconstruct(const uint64_t *x, size_t n, uint64_t *out, void *stream
 
 int pir_fpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         fpc_divide_public(a, n, d);
         return (int)OK;
     });
@@ -753,7 +793,7 @@ int pir_fpc_divide_public(uint64_t *a, size_t n, uint64_t d, void *stream) {
 
 int pir_fpc_reshare(const uint64_t *z, const uint64_t *zp, size_t n, int i, int j, uint64_t *out, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream);
+        SchemeScope sc(stream);
         fpc_reshare(z, zp, n, i, j, out);
         return (int)OK;
     });
@@ -762,7 +802,7 @@ int pir_fpc_reshare(const uint64_t *z, const uint64_t *zp, size_t n, int i, int 
 int pir_fpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int N, int K, int ta, int tb, int tc,
                    int trunc, int algo, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        SchemeScope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
         PIR_REQUIRE(M > 0 && N > 0 && K > 0 && trunc >= 1 && trunc <= 62, "pir_fpc_matmul: invalid arguments");
         fpc_matmul(a, b, c, M, N, K, ta != 0, tb != 0, tc != 0, trunc);
         return (int)OK;
@@ -772,7 +812,7 @@ int pir_fpc_matmul(const uint64_t *a, const uint64_t *b, uint64_t *c, int M, int
 int pir_fpc_convolution(const uint64_t *a, const uint64_t *w, uint64_t *c, int batch, int H, int W, int f, int Din,
                         int Dout, int stride, int pad, int trunc, int algo, void *stream) {
     return guarded([&]() -> int {
-        Scope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
+        SchemeScope sc(stream, algo == PIR_ALGO_PLAIN ? ALGO_PLAIN : ALGO_LIMB);
         PIR_REQUIRE(batch > 0 && H > 0 && W > 0 && f > 0 && Din > 0 && Dout > 0 && stride > 0 && pad >= 0 &&
                         trunc >= 1 && trunc <= 62 && (H + 2 * pad - f) >= 0 && (W + 2 * pad - f) >= 0,
                     "pir_fpc_convolution: invalid geometry");
@@ -1023,6 +1063,16 @@ int pir_net_set_mode(pir_net *n, int mode) {
     return OK;
 }
 
+int pir_net_set_precompute(pir_net *n, int mode, uint64_t seed, uint32_t call) {
+    PIR_REQUIRE(n && (mode == PRECOMPUTE_PLACEHOLDER || mode == PRECOMPUTE_MASKED),
+                "pir_net_set_precompute: invalid arguments");
+    n->ctx.precompute = mode;
+    n->ctx.prg_seed = seed;
+    n->ctx.prg_call = call;
+    drop_graph(n);
+    return OK;
+}
+
 int pir_net_set_weights(pir_net *n, int layer, const double *w, size_t nw, const double *b, size_t nb) {
     return guarded([&]() -> int {
         PIR_REQUIRE(n && layer >= 0 && layer < pir_net_num_layers(n), "pir_net_set_weights: bad layer");
@@ -1181,6 +1231,26 @@ RSS64 *net_param(pir_net *n, int layer, int index, std::string *name) {
 }
 }  // namespace
 
+int pir_net_rerandomize(pir_net *n, uint64_t seed) {
+    return guarded([&]() -> int {
+        PIR_REQUIRE(n, "pir_net_rerandomize: invalid arguments");
+        NetScope ns(n);
+        u32 call = 0;
+        for (auto &L : n->net->layers) {
+            auto tensors = L->snapshotTensors();
+            for (auto &t : tensors) {
+                RSS64 *x = t.second;
+                PIR_TRY(rss_deal(nullptr, x->base(), x->size(), seed, call++, x->base(), n->ctx.stream));
+            }
+            if (!tensors.empty()) L->weightsChanged();
+        }
+        RSS64 &in = n->net->input;
+        PIR_TRY(rss_deal(nullptr, in.base(), in.size(), seed, call, in.base(), n->ctx.stream));
+        drop_graph(n);
+        return (int)OK;
+    });
+}
+
 int pir_net_num_params(pir_net *n, int layer) {
     if (!n || layer < 0 || layer >= pir_net_num_layers(n)) return 0;
     return (int)n->net->layers[layer]->snapshotTensors().size();
@@ -1225,10 +1295,14 @@ int pir_net_capture(pir_net *n, const uint64_t *shares) {
         if (shares && shares != n->net->input.base())
             PIR_HIP(hipMemcpyAsync(n->net->input.base(), shares, 6 * n->net->input.size() * 8,
                                    hipMemcpyDeviceToDevice, s));
-        // one eager pass packs the weights and warms the allocator cache
+        // one eager pass packs the weights and warms the allocator cache; the
+        // captured pass reuses its call numbers (masked precompute mode), so
+        // every replay computes what that eager pass computed
+        const u32 call_base = n->ctx.prg_call;
         n->net->run();
         PIR_HIP(hipStreamSynchronize(s));
         drop_graph(n);
+        n->ctx.prg_call = call_base;
         PIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         n->ctx.alloc.capturing = true;
         int rc = guarded([&]() -> int {

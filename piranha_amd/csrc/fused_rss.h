@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.h"
+#include "prg.h"
 
 namespace pir {
 
@@ -14,29 +15,36 @@ struct MaxpoolGeom {
 
 int expanded_pool_size(int f);
 
+// Launchers taking a PrgStream run the masked precompute mode when it is on
+// (rss_masked.h; element i of the call draws from its own word range) and the
+// reference's placeholder constants otherwise.
 template <typename U>
-int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s);
+int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s, const PrgStream &g = PrgStream{});
 template <typename U>
-int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s);
+int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s, const PrgStream &g = PrgStream{});
 template <typename U>
-int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s);
+int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s,
+                 const PrgStream &g = PrgStream{});
 template <typename U>
-int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s);
+int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s,
+                        const PrgStream &g = PrgStream{});
 // relu_out / relu_d (optional): the following ReLU layer computed on the pooled
-// values in the same pass (ReLU output shares and dReLU bits, [3][2][outN]).
+// values in the same pass (ReLU output shares and dReLU bits, [3][2][outN]);
+// gp is the pool's stream, gr the ReLU layer's.
 template <typename U>
 int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s,
-                        u64 *relu_out = nullptr, u8 *relu_d = nullptr);
+                        u64 *relu_out = nullptr, u8 *relu_d = nullptr, const PrgStream &gp = PrgStream{},
+                        const PrgStream &gr = PrgStream{});
 template <typename T>
 int fused_reconstruct(const T *x, size_t n, T *out, hipStream_t s);
-int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s);
-int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s);
+int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s, const PrgStream &g = PrgStream{});
+int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s, const PrgStream &g = PrgStream{});
 template <typename T>
-int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s);
-int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s);
+int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s, const PrgStream &g = PrgStream{});
+int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s, const PrgStream &g = PrgStream{});
 template <typename U>
 int fused_convex_comb(u64 *d, const u64 *c, const U *e, size_t n, hipStream_t s);
-int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s);
+int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s, const PrgStream &g = PrgStream{});
 // getPowers (RSS.inl:1211-1235): in [3][2][n] -> each party's opened power [3][n].
 int fused_get_powers(const u64 *in, size_t n, u64 *pow, hipStream_t s);
 // taylorSeries (RSS.inl:1237-1320) in one pass: in [3][2][n] -> out [3][2][n]

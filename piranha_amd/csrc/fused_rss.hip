@@ -12,6 +12,7 @@
 // (RSS.inl:422-455) and dividePublic3out3 (RSS.inl:1076-1094).
 #include "fused_rss.h"
 #include "rss_device.h"
+#include "rss_masked.h"
 
 namespace pir {
 
@@ -25,19 +26,21 @@ static unsigned ew_grid(size_t n) {
 // ---------------------------------------------------------------------------
 // dReLU / ReLU
 
-template <typename U, bool RELU>
+// MASKED: the masked precompute mode (rss_masked.h), element i drawing from
+// the call's word stream g.
+template <typename U, bool RELU, bool MASKED>
 __global__ __launch_bounds__(EW_BLOCK) void relu_kernel(const u64 *__restrict__ x, size_t n, U *__restrict__ dres,
-                                                        u64 *__restrict__ y) {
+                                                        u64 *__restrict__ y, PrgStream g) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     Sh3<u64> in = load3(x, n, i);
-    Sh3<U> b = drelu_<U>(in);
+    Sh3<U> b = MASKED ? drelu_m<U>(in, g, i, 0) : drelu_<U>(in);
     store3(dres, n, i, b);
     if (RELU) {
         Sh3<u64> zero;
 #pragma unroll
         for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
-        store3(y, n, i, select_<U>(zero, in, b));
+        store3(y, n, i, MASKED ? select_m<U>(zero, in, b, g, i, PRG_DRELU_WORDS) : select_<U>(zero, in, b));
     }
 }
 
@@ -93,23 +96,24 @@ __device__ __forceinline__ void store3v(u8 *__restrict__ base, size_t n, size_t 
         }
 }
 
-template <typename U, bool RELU, int EPT>
+template <typename U, bool RELU, int EPT, bool MASKED>
 __global__ __launch_bounds__(EW_BLOCK) void relu_wide_kernel(const u64 *__restrict__ x, size_t n,
-                                                             U *__restrict__ dres, u64 *__restrict__ y) {
+                                                             U *__restrict__ dres, u64 *__restrict__ y, PrgStream g) {
     size_t i0 = ((size_t)blockIdx.x * EW_BLOCK + threadIdx.x) * EPT;
     if (i0 >= n) return;
     Sh3<u64> in[EPT];
     load3v<EPT>(x, n, i0, in);
     Sh3<U> b[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) b[j] = drelu_<U>(in[j]);
+    for (int j = 0; j < EPT; j++) b[j] = MASKED ? drelu_m<U>(in[j], g, i0 + j, 0) : drelu_<U>(in[j]);
     store3v<EPT>(dres, n, i0, b);
     if (RELU) {
         Sh3<u64> zero, out[EPT];
 #pragma unroll
         for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
 #pragma unroll
-        for (int j = 0; j < EPT; j++) out[j] = select_<U>(zero, in[j], b[j]);
+        for (int j = 0; j < EPT; j++)
+            out[j] = MASKED ? select_m<U>(zero, in[j], b[j], g, i0 + j, PRG_DRELU_WORDS) : select_<U>(zero, in[j], b[j]);
         store3v<EPT>(y, n, i0, out);
     }
 }
@@ -118,45 +122,55 @@ constexpr int RELU_EPT = 4;
 
 static bool aligned_to(const void *p, size_t a) { return p == nullptr || ((uintptr_t)p % a) == 0; }
 
-template <typename U, bool RELU>
-static int launch_relu(const u64 *x, size_t n, U *dres, u64 *y, hipStream_t s) {
-    if (n == 0) return OK;
+template <typename U, bool RELU, bool MASKED>
+static void launch_relu_as(const u64 *x, size_t n, U *dres, u64 *y, const PrgStream &g, hipStream_t s) {
     if (n % RELU_EPT == 0 && aligned_to(x, 16) && aligned_to(y, 16) &&
         aligned_to(dres, sizeof(U) == 1 ? RELU_EPT : 16)) {
-        hipLaunchKernelGGL((relu_wide_kernel<U, RELU, RELU_EPT>), dim3(ew_grid(n / RELU_EPT)), dim3(EW_BLOCK), 0, s,
-                           x, n, dres, y);
+        hipLaunchKernelGGL((relu_wide_kernel<U, RELU, RELU_EPT, MASKED>), dim3(ew_grid(n / RELU_EPT)),
+                           dim3(EW_BLOCK), 0, s, x, n, dres, y, g);
     } else {
-        hipLaunchKernelGGL((relu_kernel<U, RELU>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, y);
+        hipLaunchKernelGGL((relu_kernel<U, RELU, MASKED>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, n, dres, y, g);
     }
+}
+
+template <typename U, bool RELU>
+static int launch_relu(const u64 *x, size_t n, U *dres, u64 *y, const PrgStream &g, hipStream_t s) {
+    if (n == 0) return OK;
+    if (g.on) launch_relu_as<U, RELU, true>(x, n, dres, y, g, s);
+    else launch_relu_as<U, RELU, false>(x, n, dres, y, g, s);
     PIR_LAUNCH_CHECK();
     return OK;
 }
 
 template <typename U>
-int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s) {
-    return launch_relu<U, false>(x, n, dres, (u64 *)nullptr, s);
+int fused_drelu(const u64 *x, size_t n, U *dres, hipStream_t s, const PrgStream &g) {
+    return launch_relu<U, false>(x, n, dres, (u64 *)nullptr, g, s);
 }
 
 template <typename U>
-int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s) {
-    return launch_relu<U, true>(x, n, dres, y, s);
+int fused_relu(const u64 *x, size_t n, u64 *y, U *dres, hipStream_t s, const PrgStream &g) {
+    return launch_relu<U, true>(x, n, dres, y, g, s);
 }
 
 // ---------------------------------------------------------------------------
 // selectShare
 
-template <typename U>
+template <typename U, bool MASKED>
 __global__ __launch_bounds__(EW_BLOCK) void select_kernel(const u64 *__restrict__ x, const u64 *__restrict__ y,
-                                                          const U *__restrict__ b, u64 *__restrict__ z, size_t n) {
+                                                          const U *__restrict__ b, u64 *__restrict__ z, size_t n,
+                                                          PrgStream g) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
-    store3(z, n, i, select_<U>(load3(x, n, i), load3(y, n, i), load3(b, n, i)));
+    const Sh3<u64> xv = load3(x, n, i), yv = load3(y, n, i);
+    const Sh3<U> bv = load3(b, n, i);
+    store3(z, n, i, MASKED ? select_m<U>(xv, yv, bv, g, i, 0) : select_<U>(xv, yv, bv));
 }
 
 template <typename U>
-int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s) {
+int fused_select(const u64 *x, const u64 *y, const U *b, u64 *z, size_t n, hipStream_t s, const PrgStream &g) {
     if (n == 0) return OK;
-    hipLaunchKernelGGL((select_kernel<U>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, b, z, n);
+    if (g.on) hipLaunchKernelGGL((select_kernel<U, true>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, b, z, n, g);
+    else hipLaunchKernelGGL((select_kernel<U, false>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, b, z, n, g);
     PIR_LAUNCH_CHECK();
     return OK;
 }
@@ -223,11 +237,59 @@ __device__ __forceinline__ void tournament(Sh3<u64> (&e)[K], Sh3<U> (&d)[K]) {
     }
 }
 
+// The masked form of the tournament (rss_masked.h) for window `win`: compare m
+// (in round order) draws words 36m .. 36m + 35 (its dReLU, then its
+// selectShare); the AND of round t at pool position pos draws words
+// 36(K-1) + 2(tK + pos) and + 1.  1 - b becomes b ^ 1 in the x0 slot.
+template <typename U, int K>
+__device__ __forceinline__ void tournament_m(Sh3<u64> (&e)[K], Sh3<U> (&d)[K], const PrgStream &g, u64 win) {
+#pragma unroll
+    for (int j = 0; j < K; j++) d[j] = fill1_<U>();
+    u32 m = 0;
+    int t = 0;
+#pragma unroll
+    for (int stride = 2; stride <= K; stride *= 2, t++) {
+        const int cnt = K / stride;
+        Sh3<U> b[K / 2];
+#pragma unroll
+        for (int j = 0; j < cnt; j++, m++) {
+            const Sh3<u64> &ev = e[j * stride];
+            const Sh3<u64> &od = e[j * stride + stride / 2];
+            Sh3<u64> diff;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int s = 0; s < 2; s++) diff.s[p][s] = ev.s[p][s] - od.s[p][s];
+            b[j] = drelu_m<U>(diff, g, win, PRG_RELU_WORDS * m);
+            e[j * stride] = select_m<U>(od, ev, b[j], g, win, PRG_RELU_WORDS * m + PRG_DRELU_WORDS);
+        }
+#pragma unroll
+        for (int j = 0; j < cnt; j++) {
+            Sh3<U> neg = b[j];
+            neg.s[PARTY_A][0] ^= (U)1;
+            neg.s[PARTY_C][1] ^= (U)1;
+#pragma unroll
+            for (int tt = 0; tt < stride; tt++) {
+                const int pos = j * stride + tt;
+                rss_and_m<U>(d[pos], tt < stride / 2 ? b[j] : neg, g, win,
+                             PRG_RELU_WORDS * (K - 1) + 2 * (t * K + pos), (U)1);
+            }
+        }
+    }
+}
+
+template <typename U, int K, bool MASKED>
+__device__ __forceinline__ void tournament_any(Sh3<u64> (&e)[K], Sh3<U> (&d)[K], const PrgStream &g, u64 win) {
+    if (MASKED) tournament_m<U, K>(e, d, g, win);
+    else tournament<U, K>(e, d);
+}
+
 // Free-function form over an already gathered pools buffer [windows][K]; the
 // even positions are written back, as the reference's strided views do.
-template <typename U, int K>
+template <typename U, int K, bool MASKED>
 __global__ __launch_bounds__(EW_BLOCK) void maxpool_pools_kernel(u64 *__restrict__ pools, size_t windows,
-                                                                 u64 *__restrict__ result, U *__restrict__ dres) {
+                                                                 u64 *__restrict__ result, U *__restrict__ dres,
+                                                                 PrgStream g) {
     size_t w = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (w >= windows) return;
     const size_t n = windows * K;
@@ -235,7 +297,7 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_pools_kernel(u64 *__restrict
     Sh3<U> d[K];
 #pragma unroll
     for (int j = 0; j < K; j++) e[j] = load3(pools, n, w * K + j);
-    tournament<U, K>(e, d);
+    tournament_any<U, K, MASKED>(e, d, g, w);
     store3(result, windows, w, e[0]);
 #pragma unroll
     for (int j = 0; j < K; j += 2) store3(pools, n, w * K + j, e[j]);
@@ -243,20 +305,28 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_pools_kernel(u64 *__restrict
     for (int j = 0; j < K; j++) store3(dres, n, w * K + j, d[j]);
 }
 
-template <typename U>
-int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s) {
-    PIR_REQUIRE(k > 0 && n % k == 0, "maxpool: pools size must be a multiple of k");
-    size_t windows = n / k;
-    if (windows == 0) return OK;
+template <typename U, bool MASKED>
+static int launch_maxpool_pools(u64 *pools, size_t windows, u64 *result, U *dres, int k, const PrgStream &g,
+                                hipStream_t s) {
+    const dim3 gr(ew_grid(windows)), bl(EW_BLOCK);
     switch (k) {
-        case 2: hipLaunchKernelGGL((maxpool_pools_kernel<U, 2>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
-        case 4: hipLaunchKernelGGL((maxpool_pools_kernel<U, 4>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
-        case 8: hipLaunchKernelGGL((maxpool_pools_kernel<U, 8>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
-        case 16: hipLaunchKernelGGL((maxpool_pools_kernel<U, 16>), dim3(ew_grid(windows)), dim3(EW_BLOCK), 0, s, pools, windows, result, dres); break;
+        case 2: hipLaunchKernelGGL((maxpool_pools_kernel<U, 2, MASKED>), gr, bl, 0, s, pools, windows, result, dres, g); break;
+        case 4: hipLaunchKernelGGL((maxpool_pools_kernel<U, 4, MASKED>), gr, bl, 0, s, pools, windows, result, dres, g); break;
+        case 8: hipLaunchKernelGGL((maxpool_pools_kernel<U, 8, MASKED>), gr, bl, 0, s, pools, windows, result, dres, g); break;
+        case 16: hipLaunchKernelGGL((maxpool_pools_kernel<U, 16, MASKED>), gr, bl, 0, s, pools, windows, result, dres, g); break;
         default: set_error("maxpool: k must be 2, 4, 8 or 16"); return ERR_UNSUPPORTED;
     }
     PIR_LAUNCH_CHECK();
     return OK;
+}
+
+template <typename U>
+int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipStream_t s, const PrgStream &g) {
+    PIR_REQUIRE(k > 0 && n % k == 0, "maxpool: pools size must be a multiple of k");
+    size_t windows = n / k;
+    if (windows == 0) return OK;
+    return g.on ? launch_maxpool_pools<U, true>(pools, windows, result, dres, k, g, s)
+                : launch_maxpool_pools<U, false>(pools, windows, result, dres, k, g, s);
 }
 
 // Layer form (MaxpoolLayer.cu:52-131): gathers each f x f window straight
@@ -264,11 +334,14 @@ int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipSt
 // offset inner), pads to K with pad_value in every share (stride_pad; the RSS
 // layer uses the same pad value for both shares of every party), runs the
 // tournament and keeps the first f*f derivative entries (truncate_cols).
-template <typename U, int K>
+// MASKED: the pool's tournament draws from gp (element = window = output
+// index), the fused ReLU from gr (its own call, element = output index).
+template <typename U, int K, bool MASKED>
 __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
                                                                  u64 *__restrict__ act, U *__restrict__ mprime,
                                                                  u64 *__restrict__ relu_out,
-                                                                 u8 *__restrict__ relu_d) {
+                                                                 u8 *__restrict__ relu_d, PrgStream gp,
+                                                                 PrgStream gr) {
     const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
     size_t o = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (o >= outN) return;
@@ -293,15 +366,22 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
             for (int p = 0; p < 3; p++) e[j].s[p][0] = e[j].s[p][1] = g.pad_value;
         }
     }
-    tournament<U, K>(e, d);
+    tournament_any<U, K, MASKED>(e, d, gp, o);
     store3(act, outN, o, e[0]);
     if (relu_out) {  // the following ReLU layer on the same element
-        Sh3<u8> rb = drelu_<u8>(e[0]);
-        store3(relu_d, outN, o, rb);
-        Sh3<u64> zero;
+        Sh3<u8> rb;
+        Sh3<u64> ro;
+        if (MASKED) {
+            ro = relu_m<u8>(e[0], rb, gr, o);
+        } else {
+            rb = drelu_<u8>(e[0]);
+            Sh3<u64> zero;
 #pragma unroll
-        for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
-        store3(relu_out, outN, o, select_<u8>(zero, e[0], rb));
+            for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
+            ro = select_<u8>(zero, e[0], rb);
+        }
+        store3(relu_d, outN, o, rb);
+        store3(relu_out, outN, o, ro);
     }
     const size_t mpN = outN * f2;
     if (sizeof(U) == 1 && K == 4 && f2 == 4) {
@@ -327,11 +407,12 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
 // array, the pooled value and the fused ReLU's output one 16-byte store, the
 // ReLU's bit shares one 2-byte store and the eight derivative bytes of the two
 // windows one 8-byte store per share array.
-template <typename U>
+template <typename U, bool MASKED>
 __global__ __launch_bounds__(EW_BLOCK) void maxpool2x2_pair_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
                                                                    u64 *__restrict__ act, U *__restrict__ mprime,
                                                                    u64 *__restrict__ relu_out,
-                                                                   u8 *__restrict__ relu_d) {
+                                                                   u8 *__restrict__ relu_d, PrgStream gp,
+                                                                   PrgStream gr) {
     static_assert(sizeof(U) == 1, "byte derivative shares");
     const size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
     size_t o = ((size_t)blockIdx.x * EW_BLOCK + threadIdx.x) * 2;
@@ -358,18 +439,25 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool2x2_pair_kernel(const u64 *__
                 e1[j].s[p][sh] = t.y;
             }
     }
-    tournament<U, 4>(e0, d0);
-    tournament<U, 4>(e1, d1);
+    tournament_any<U, 4, MASKED>(e0, d0, gp, o);
+    tournament_any<U, 4, MASKED>(e1, d1, gp, o + 1);
     Sh3<u64> pooled[2] = {e0[0], e1[0]};
     store3v<2>(act, outN, o, pooled);
     if (relu_out) {
-        Sh3<u8> rb[2] = {drelu_<u8>(e0[0]), drelu_<u8>(e1[0])};
-        store3v<2>(relu_d, outN, o, rb);
+        Sh3<u8> rb[2];
         Sh3<u64> zero, ro[2];
 #pragma unroll
         for (int p = 0; p < 3; p++) zero.s[p][0] = zero.s[p][1] = 0;
-        ro[0] = select_<u8>(zero, e0[0], rb[0]);
-        ro[1] = select_<u8>(zero, e1[0], rb[1]);
+        if (MASKED) {
+            ro[0] = relu_m<u8>(e0[0], rb[0], gr, o);
+            ro[1] = relu_m<u8>(e1[0], rb[1], gr, o + 1);
+        } else {
+            rb[0] = drelu_<u8>(e0[0]);
+            rb[1] = drelu_<u8>(e1[0]);
+            ro[0] = select_<u8>(zero, e0[0], rb[0]);
+            ro[1] = select_<u8>(zero, e1[0], rb[1]);
+        }
+        store3v<2>(relu_d, outN, o, rb);
         store3v<2>(relu_out, outN, o, ro);
     }
     const size_t mpN = outN * 4;
@@ -387,32 +475,41 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool2x2_pair_kernel(const u64 *__
         }
 }
 
+template <typename U, bool MASKED>
+static int launch_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s, u64 *relu_out,
+                                u8 *relu_d, const PrgStream &gp, const PrgStream &gr) {
+    size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
+    int k = expanded_pool_size(g.f);
+    if constexpr (sizeof(U) == 1) {
+        if (g.f == 2 && g.C % 2 == 0 && aligned_to(x, 16) && aligned_to(act, 16) && aligned_to(relu_out, 16) &&
+            aligned_to(relu_d, 2) && aligned_to(mprime, 8)) {
+            hipLaunchKernelGGL((maxpool2x2_pair_kernel<U, MASKED>), dim3(ew_grid(outN / 2)), dim3(EW_BLOCK), 0, s, x,
+                               g, act, mprime, relu_out, relu_d, gp, gr);
+            PIR_LAUNCH_CHECK();
+            return OK;
+        }
+    }
+    switch (k) {
+        case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4, MASKED>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d, gp, gr); break;
+        case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16, MASKED>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d, gp, gr); break;
+        default: set_error("maxpool layer: only 2x2 and 3x3 pools are supported"); return ERR_UNSUPPORTED;
+    }
+    PIR_LAUNCH_CHECK();
+    return OK;
+}
+
 template <typename U>
 int fused_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U *mprime, hipStream_t s, u64 *relu_out,
-                        u8 *relu_d) {
+                        u8 *relu_d, const PrgStream &gp, const PrgStream &gr) {
     PIR_REQUIRE(!relu_out == !relu_d, "maxpool layer: a fused ReLU needs both outputs");
     PIR_REQUIRE(g.f >= 2 && g.stride >= 1 && g.H >= g.f && g.W >= g.f, "maxpool layer: invalid geometry");
     PIR_REQUIRE(g.OH == (g.H - g.f) / g.stride + 1 && g.OW == (g.W - g.f) / g.stride + 1,
                 "maxpool layer: inconsistent output size");
     size_t outN = (size_t)g.batch * g.OH * g.OW * g.C;
     if (outN == 0) return OK;
-    int k = expanded_pool_size(g.f);
-    if constexpr (sizeof(U) == 1) {
-        if (g.f == 2 && g.C % 2 == 0 && aligned_to(x, 16) && aligned_to(act, 16) && aligned_to(relu_out, 16) &&
-            aligned_to(relu_d, 2) && aligned_to(mprime, 8)) {
-            hipLaunchKernelGGL((maxpool2x2_pair_kernel<U>), dim3(ew_grid(outN / 2)), dim3(EW_BLOCK), 0, s, x, g,
-                               act, mprime, relu_out, relu_d);
-            PIR_LAUNCH_CHECK();
-            return OK;
-        }
-    }
-    switch (k) {
-        case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d); break;
-        case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d); break;
-        default: set_error("maxpool layer: only 2x2 and 3x3 pools are supported"); return ERR_UNSUPPORTED;
-    }
-    PIR_LAUNCH_CHECK();
-    return OK;
+    PIR_REQUIRE(gp.on == gr.on || !relu_out, "maxpool layer: pool and ReLU streams must agree on the mode");
+    return gp.on ? launch_maxpool_layer<U, true>(x, g, act, mprime, s, relu_out, relu_d, gp, gr)
+                 : launch_maxpool_layer<U, false>(x, g, act, mprime, s, relu_out, relu_d, gp, gr);
 }
 
 // AveragepoolLayer::forward (AveragepoolLayer.cu:49-96): averagepool_im2row,
@@ -690,12 +787,17 @@ int fused_reconstruct(const T *x, size_t n, T *out, hipStream_t s) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(EW_BLOCK) void reshare_kernel(const T *__restrict__ c, size_t n, T *__restrict__ out) {
+__global__ __launch_bounds__(EW_BLOCK) void reshare_kernel(const T *__restrict__ c, size_t n, T *__restrict__ out,
+                                                           PrgStream g) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     T z[3];
 #pragma unroll
     for (int p = 0; p < 3; p++) z[p] = c[(size_t)p * n + i];
+    if (g.on) {  // masked: the arithmetic zero sharing of words 0, 1
+        store3(out, n, i, reshare_m<T, false>(z, g, i, PRG_RESHARE, (T)~(T)0));
+        return;
+    }
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         out[(size_t)(2 * p) * n + i] = z[p];
@@ -703,34 +805,40 @@ __global__ __launch_bounds__(EW_BLOCK) void reshare_kernel(const T *__restrict__
     }
 }
 
-int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s) {
+int fused_reshare(const u64 *c, size_t n, u64 *out, hipStream_t s, const PrgStream &g) {
     if (n == 0) return OK;
-    hipLaunchKernelGGL((reshare_kernel<u64>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, c, n, out);
+    hipLaunchKernelGGL((reshare_kernel<u64>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, c, n, out, g);
     PIR_LAUNCH_CHECK();
     return OK;
 }
 
 template <typename T, bool AND>
-__global__ __launch_bounds__(EW_BLOCK) void rss_mul_kernel(T *__restrict__ x, const T *__restrict__ y, size_t n) {
+__global__ __launch_bounds__(EW_BLOCK) void rss_mul_kernel(T *__restrict__ x, const T *__restrict__ y, size_t n,
+                                                           PrgStream g) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     Sh3<T> a = load3(x, n, i), b = load3(y, n, i);
-    if (AND) rss_and_elem<T>(a, b);
-    else rss_mult_<T>(a, b);
+    if (g.on) {  // masked: full-width zero sharing of words 0, 1
+        if (AND) rss_and_m<T>(a, b, g, i, PRG_RESHARE, (T)~(T)0);
+        else rss_mult_m<T>(a, b, g, i, PRG_RESHARE);
+    } else {
+        if (AND) rss_and_elem<T>(a, b);
+        else rss_mult_<T>(a, b);
+    }
     store3(x, n, i, a);
 }
 
-int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s) {
+int fused_rss_mult(u64 *x, const u64 *y, size_t n, hipStream_t s, const PrgStream &g) {
     if (n == 0) return OK;
-    hipLaunchKernelGGL((rss_mul_kernel<u64, false>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, n);
+    hipLaunchKernelGGL((rss_mul_kernel<u64, false>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, n, g);
     PIR_LAUNCH_CHECK();
     return OK;
 }
 
 template <typename T>
-int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s) {
+int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s, const PrgStream &g) {
     if (n == 0) return OK;
-    hipLaunchKernelGGL((rss_mul_kernel<T, true>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, n);
+    hipLaunchKernelGGL((rss_mul_kernel<T, true>), dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, x, y, n, g);
     PIR_LAUNCH_CHECK();
     return OK;
 }
@@ -741,10 +849,14 @@ int fused_rss_and(T *x, const T *y, size_t n, hipStream_t s) {
 // (S)((double)(S)t / (double)(S)d).  `dv` (per-party local denominators [3][n])
 // overrides the scalar `d` when set.
 __global__ __launch_bounds__(EW_BLOCK) void divide_public_kernel(u64 *__restrict__ a, size_t n, u64 d,
-                                                                 const u64 *__restrict__ dv) {
+                                                                 const u64 *__restrict__ dv, PrgStream g, int k) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     Sh3<u64> v = load3(a, n, i);
+    if (g.on) {  // masked: d = 2^k, the random truncation pair of element i
+        store3(a, n, i, divide_public_pow2_m(v, k, g, i));
+        return;
+    }
     u64 dd[3];
 #pragma unroll
     for (int p = 0; p < 3; p++) dd[p] = dv ? dv[(size_t)p * n + i] : d;
@@ -753,18 +865,31 @@ __global__ __launch_bounds__(EW_BLOCK) void divide_public_kernel(u64 *__restrict
     store3(a, n, i, divide_public_local_(v, dd));
 }
 
-int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s) {
+int fused_divide_public(u64 *a, size_t n, u64 d, const u64 *dv, hipStream_t s, const PrgStream &g) {
+    int k = 0;
+    if (g.on) {
+        PIR_REQUIRE(!dv && d != 0 && (d & (d - 1)) == 0,
+                    "dividePublic: the masked precompute supports one public power-of-two denominator");
+        while (((u64)1 << k) != d) k++;
+        PIR_REQUIRE(k >= 1 && k <= 62, "dividePublic: the masked precompute needs d = 2^k, 1 <= k <= 62");
+    }
     if (n == 0) return OK;
-    hipLaunchKernelGGL(divide_public_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, d, dv);
+    hipLaunchKernelGGL(divide_public_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, d, dv, g, k);
     PIR_LAUNCH_CHECK();
     return OK;
 }
 
 // dividePublic3out3 over 3-of-3 shares a [3][n] -> out [3][2][n]
 __global__ __launch_bounds__(EW_BLOCK) void divide_3out3_kernel(const u64 *__restrict__ a, size_t n, int k,
-                                                                u64 *__restrict__ out) {
+                                                                u64 *__restrict__ out, PrgStream g) {
     size_t i = (size_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
+    if (g.on) {  // masked: the random truncation pair of element i
+        Sh3<u64> o;
+        divide_3out3_m(a[i] + a[n + i] + a[2 * n + i], k, g, i, o.s);
+        store3(out, n, i, o);
+        return;
+    }
     u64 t = a[i] + a[n + i] + a[2 * n + i] - ((u64)1 << k);
     u64 base = 1 + trunc_div_pow2(t, k);
     Sh3<u64> o;
@@ -775,10 +900,11 @@ __global__ __launch_bounds__(EW_BLOCK) void divide_3out3_kernel(const u64 *__res
     store3(out, n, i, o);
 }
 
-int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s) {
+int fused_divide_3out3(const u64 *a, size_t n, int k, u64 *out, hipStream_t s, const PrgStream &g) {
     PIR_REQUIRE(k >= 0 && k <= 62, "dividePublic3out3: denominator must be 2^k, 0 <= k <= 62");
+    PIR_REQUIRE(!g.on || k >= 1, "dividePublic3out3: the masked precompute needs k >= 1");
     if (n == 0) return OK;
-    hipLaunchKernelGGL(divide_3out3_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, k, out);
+    hipLaunchKernelGGL(divide_3out3_kernel, dim3(ew_grid(n)), dim3(EW_BLOCK), 0, s, a, n, k, out, g);
     PIR_LAUNCH_CHECK();
     return OK;
 }
@@ -908,18 +1034,19 @@ int fused_convex_comb(u64 *d, const u64 *c, const U *e, size_t n, hipStream_t s)
 // Explicit instantiations
 template int fused_convex_comb<u8>(u64 *, const u64 *, const u8 *, size_t, hipStream_t);
 template int fused_convex_comb<u64>(u64 *, const u64 *, const u64 *, size_t, hipStream_t);
-template int fused_drelu<u8>(const u64 *, size_t, u8 *, hipStream_t);
-template int fused_drelu<u64>(const u64 *, size_t, u64 *, hipStream_t);
-template int fused_relu<u8>(const u64 *, size_t, u64 *, u8 *, hipStream_t);
-template int fused_relu<u64>(const u64 *, size_t, u64 *, u64 *, hipStream_t);
-template int fused_select<u8>(const u64 *, const u64 *, const u8 *, u64 *, size_t, hipStream_t);
-template int fused_select<u64>(const u64 *, const u64 *, const u64 *, u64 *, size_t, hipStream_t);
-template int fused_maxpool_pools<u8>(u64 *, size_t, u64 *, u8 *, int, hipStream_t);
-template int fused_maxpool_pools<u64>(u64 *, size_t, u64 *, u64 *, int, hipStream_t);
-template int fused_maxpool_layer<u8>(const u64 *, const MaxpoolGeom &, u64 *, u8 *, hipStream_t, u64 *, u8 *);
+template int fused_drelu<u8>(const u64 *, size_t, u8 *, hipStream_t, const PrgStream &);
+template int fused_drelu<u64>(const u64 *, size_t, u64 *, hipStream_t, const PrgStream &);
+template int fused_relu<u8>(const u64 *, size_t, u64 *, u8 *, hipStream_t, const PrgStream &);
+template int fused_relu<u64>(const u64 *, size_t, u64 *, u64 *, hipStream_t, const PrgStream &);
+template int fused_select<u8>(const u64 *, const u64 *, const u8 *, u64 *, size_t, hipStream_t, const PrgStream &);
+template int fused_select<u64>(const u64 *, const u64 *, const u64 *, u64 *, size_t, hipStream_t, const PrgStream &);
+template int fused_maxpool_pools<u8>(u64 *, size_t, u64 *, u8 *, int, hipStream_t, const PrgStream &);
+template int fused_maxpool_pools<u64>(u64 *, size_t, u64 *, u64 *, int, hipStream_t, const PrgStream &);
+template int fused_maxpool_layer<u8>(const u64 *, const MaxpoolGeom &, u64 *, u8 *, hipStream_t, u64 *, u8 *,
+                                     const PrgStream &, const PrgStream &);
 template int fused_reconstruct<u64>(const u64 *, size_t, u64 *, hipStream_t);
 template int fused_reconstruct<u8>(const u8 *, size_t, u8 *, hipStream_t);
-template int fused_rss_and<u8>(u8 *, const u8 *, size_t, hipStream_t);
-template int fused_rss_and<u64>(u64 *, const u64 *, size_t, hipStream_t);
+template int fused_rss_and<u8>(u8 *, const u8 *, size_t, hipStream_t, const PrgStream &);
+template int fused_rss_and<u64>(u64 *, const u64 *, size_t, hipStream_t, const PrgStream &);
 
 }  // namespace pir

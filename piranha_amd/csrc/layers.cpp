@@ -53,7 +53,7 @@ void FCLayer::printLayer() {
 
 // matmul + dividePublic3out3 + elementVectorAdd of both bias shares (and the
 // next ReLU layer when relu_out is given), one limb-GEMM launch
-void FCLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
+void FCLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d, const PrgStream &g, const PrgStream &gr) {
     const int M = (int)conf.batchSize, N = (int)conf.outputDim, K = (int)conf.inputDim;
     if (!packed_valid) {
         // repacked in place: a captured graph keeps pointing at the same buffer
@@ -62,12 +62,15 @@ void FCLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
         packed_valid = true;
     }
     check(rss_limb_product(input, true, nullptr, M, N, K, (const int8_t *)packed.get(), ctx().frac_bits, &biases,
-                           activations, true, 0, relu_out, relu_d));
+                           activations, true, 0, relu_out, relu_d, g, gr));
 }
 
+// In the masked precompute mode the product draws from this layer's call and
+// the fused ReLU from the ReLU layer's (the next call number).
 bool FCLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
     if (!(fused_limb() && input.base() && relu.activations.base() && relu.reluPrime.base())) return false;
-    limbForward(input, relu.activations.base(), relu.reluPrime.base());
+    const PrgStream g = call_prg("FC forward"), gr = call_prg("ReLU forward");
+    limbForward(input, relu.activations.base(), relu.reluPrime.base(), g, gr);
     return true;
 }
 
@@ -75,7 +78,7 @@ void FCLayer::forward(const RSS64 &input) {
     const int M = (int)conf.batchSize, N = (int)conf.outputDim, K = (int)conf.inputDim;
     const int fp = ctx().frac_bits;
     if (fused_limb() && input.base()) {
-        limbForward(input, nullptr, nullptr);
+        limbForward(input, nullptr, nullptr, call_prg("FC forward"), PrgStream{});
         return;
     }
     activations.zero();
@@ -92,6 +95,7 @@ void FCLayer::forward(const RSS64 &input) {
 // GEMMs, the second with the subtraction in its epilogue, and the bias update
 // as one kernel.
 void FCLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
+    require_placeholder("layer backward (training)");
     const int B = (int)conf.batchSize, in = (int)conf.inputDim, out = (int)conf.outputDim;
     const int fp = ctx().frac_bits, lr = ctx().log_lr;
     ensureDeltas();
@@ -191,7 +195,7 @@ void CNNLayer::printLayer() {
 
 // implicit-GEMM convolution + dividePublic3out3 (and the next ReLU layer when
 // relu_out is given), one limb-GEMM launch
-void CNNLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
+void CNNLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d, const PrgStream &g, const PrgStream &gr) {
     const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
     const int K = f * f * Din;
     if (!packed_valid) {
@@ -202,12 +206,13 @@ void CNNLayer::limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d) {
     ConvShape cs{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, Din, f, (int)conf.stride,
                  (int)conf.padding, outH(), outW()};
     check(rss_limb_product(input, true, &cs, cs.batch * cs.OH * cs.OW, Dout, K, (const int8_t *)packed.get(),
-                           ctx().frac_bits, nullptr, activations, true, 0, relu_out, relu_d));
+                           ctx().frac_bits, nullptr, activations, true, 0, relu_out, relu_d, g, gr));
 }
 
 bool CNNLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
     if (!(fused_limb() && input.base() && relu.activations.base() && relu.reluPrime.base())) return false;
-    limbForward(input, relu.activations.base(), relu.reluPrime.base());
+    const PrgStream g = call_prg("CNN forward"), gr = call_prg("ReLU forward");
+    limbForward(input, relu.activations.base(), relu.reluPrime.base(), g, gr);
     return true;
 }
 
@@ -215,7 +220,7 @@ void CNNLayer::forward(const RSS64 &input) {
     const int fp = ctx().frac_bits;
     const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
     if (fused_limb() && input.base()) {
-        limbForward(input, nullptr, nullptr);
+        limbForward(input, nullptr, nullptr, call_prg("CNN forward"), PrgStream{});
         return;
     }
     activations.zero();
@@ -228,6 +233,7 @@ void CNNLayer::forward(const RSS64 &input) {
 // reference's bias update is commented out).  The fused form runs dgrad and
 // wgrad as limb GEMMs, the dividePublic and the subtraction in wgrad's epilogue.
 void CNNLayer::backward(const RSS64 &delta, const RSS64 &forwardInput) {
+    require_placeholder("layer backward (training)");
     const int f = (int)conf.filterSize, Din = (int)conf.inputFeatures, Dout = (int)conf.outputFeatures;
     const int B = (int)conf.batchSize, H = (int)conf.imageHeight, W = (int)conf.imageWidth;
     const int st = (int)conf.stride, pad = (int)conf.padding, fp = ctx().frac_bits, lr = ctx().log_lr;
@@ -296,6 +302,7 @@ void ReLULayer::forward(const RSS64 &input) {
 
 // ReLULayer::backward, ReLULayer.cu:91-123: deltas = selectShare(0, delta, reluPrime)
 void ReLULayer::backward(const RSS64 &delta, const RSS64 &) {
+    require_placeholder("layer backward (training)");
     ensureDeltas();
     if (ctx().mode == MODE_FUSED && delta.base()) {
         check(fused_relu_backward<u8>(delta.base(), reluPrime.base(), delta.size(), deltas.base(), ctx().stream));
@@ -328,8 +335,9 @@ bool MaxpoolLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
     const u64 pad_value = (u64)(s64)(-10LL * (1LL << ctx().frac_bits));
     MaxpoolGeom g{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, (int)conf.features,
                   (int)conf.poolSize, (int)conf.stride, outH(), outW(), pad_value};
+    const PrgStream gp = call_prg("maxpool forward"), gr = call_prg("ReLU forward");
     check(fused_maxpool_layer<u8>(input.base(), g, activations.base(), maxPrime.base(), ctx().stream,
-                                  relu.activations.base(), relu.reluPrime.base()));
+                                  relu.activations.base(), relu.reluPrime.base(), gp, gr));
     return true;
 }
 
@@ -342,7 +350,8 @@ void MaxpoolLayer::forward(const RSS64 &input) {
     if (ctx().mode == MODE_FUSED && input.base()) {
         MaxpoolGeom g{(int)conf.batchSize, (int)conf.imageHeight, (int)conf.imageWidth, (int)conf.features, f,
                       (int)conf.stride, outH(), outW(), pad_value};
-        check(fused_maxpool_layer<u8>(input.base(), g, activations.base(), maxPrime.base(), ctx().stream));
+        check(fused_maxpool_layer<u8>(input.base(), g, activations.base(), maxPrime.base(), ctx().stream, nullptr,
+                                      nullptr, call_prg("maxpool forward")));
         return;
     }
     activations.zero();
@@ -367,6 +376,7 @@ void MaxpoolLayer::forward(const RSS64 &input) {
 // MaxpoolLayer::backward, MaxpoolLayer.cu:133-180: vectorExpand of delta over
 // each pool, selectShare with maxPrime, maxpool_row2im into deltas.
 void MaxpoolLayer::backward(const RSS64 &delta, const RSS64 &) {
+    require_placeholder("layer backward (training)");
     ensureDeltas();
     const int f = (int)conf.poolSize;
     if (ctx().mode == MODE_FUSED && delta.base()) {
@@ -411,6 +421,7 @@ MaxpoolGeom AveragepoolLayer::geom() const {
 }
 
 bool AveragepoolLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
+    require_placeholder("Averagepool layer");
     if (!(ctx().mode == MODE_FUSED && input.base() && relu.activations.base() && relu.reluPrime.base()))
         return false;
     check(fused_avgpool_layer(input.base(), geom(), activations.base(), ctx().stream, relu.activations.base(),
@@ -420,6 +431,7 @@ bool AveragepoolLayer::forwardWithReLU(const RSS64 &input, ReLULayer &relu) {
 
 // AveragepoolLayer.cu:49-96
 void AveragepoolLayer::forward(const RSS64 &input) {
+    require_placeholder("Averagepool layer");
     const int f = (int)conf.poolSize;
     if (ctx().mode == MODE_FUSED && input.base()) {
         check(fused_avgpool_layer(input.base(), geom(), activations.base(), ctx().stream));
@@ -489,6 +501,7 @@ void LNLayer::InvSqrtStage::compute(const u64 *in, u64 *out, size_t n) const {
 
 // LNLayer.cu:66-162
 void LNLayer::forward(const RSS64 &input) {
+    require_placeholder("LN layer");
     const int fp = ctx().frac_bits;
     if (fp < 8) {  // epsilon = 1 << (FLOAT_PRECISION - 8), LNLayer.cu:119
         set_error("LN layer: needs at least 8 fractional bits");
@@ -690,6 +703,7 @@ void ResLayer::forwardBlock(size_t idx, Block &b, const RSS64 *input, bool isSho
 
 // ResLayer.cu:227-236
 void ResLayer::forward(const RSS64 &input) {
+    require_placeholder("Res layer");
     forwardBlock(0, blocks[0], &input, false);
     for (size_t i = 1; i < blocks.size(); i++) forwardBlock(i, blocks[i], blocks[i - 1].back()->getActivation(), false);
 }
@@ -819,8 +833,15 @@ void NeuralNetwork::forward(const std::vector<double> &data) {
     run();
 }
 
+// In the masked precompute mode layer i of a forward owns call number
+// base + i of the context's stream (a ReLU layer computed inside the layer
+// before it keeps its own number), and the forward advances the counter by
+// the layer count.
 void NeuralNetwork::run(const std::function<void(size_t)> &after_layer) {
+    Context &c = ctx();
+    const u32 base = c.prg_call;
     for (size_t i = 0; i < layers.size(); i++) {
+        c.prg_call = base + (u32)i;
         const RSS64 &in = i == 0 ? input : *layers[i - 1]->getActivation();
         ReLULayer *next = i + 1 < layers.size() ? dynamic_cast<ReLULayer *>(layers[i + 1].get()) : nullptr;
         if (next && next->inputSize() == layers[i]->getActivation()->size() && layers[i]->forwardWithReLU(in, *next)) {
@@ -831,6 +852,7 @@ void NeuralNetwork::run(const std::function<void(size_t)> &after_layer) {
         }
         if (after_layer) after_layer(i);
     }
+    c.prg_call = base + (u32)layers.size();
 }
 
 void NeuralNetwork::backward(const std::vector<double> &labels) {
@@ -840,6 +862,7 @@ void NeuralNetwork::backward(const std::vector<double> &labels) {
 }
 
 void NeuralNetwork::backwardShares(const RSS64 &labelShare, const std::function<void(size_t)> &after_step) {
+    require_placeholder("network backward (training)");
     if (layers.size() < 2) {
         set_error("network backward: needs at least two layers");
         throw PirError(ERR_UNSUPPORTED);

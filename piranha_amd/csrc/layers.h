@@ -138,7 +138,7 @@ class FCLayer : public Layer {
     RSS64 activations, weights, biases;
 
   private:
-    void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d);
+    void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d, const PrgStream &g, const PrgStream &gr);
     void limbBackward(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer *relu);
     void initialize(int seed);
     DevBuf packed;
@@ -166,7 +166,7 @@ class CNNLayer : public Layer {
     RSS64 weights, activations;
 
   private:
-    void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d);
+    void limbForward(const RSS64 &input, u64 *relu_out, u8 *relu_d, const PrgStream &g, const PrgStream &gr);
     void limbBackward(const RSS64 &delta, const RSS64 &forwardInput, ReLULayer *relu);
     void initialize(int seed);
     DevBuf packed;

@@ -11,6 +11,13 @@ namespace pir {
 static bool fused_mode() { return ctx().mode == MODE_FUSED; }
 static hipStream_t S() { return ctx().stream; }
 
+// A masked-mode call whose operands are not contiguous share tensors (strided
+// views) would take the op-by-op form, which has no masked variant.
+[[noreturn]] static void refuse_masked(const char *what) {
+    set_error(std::string(what) + ": the masked precompute mode needs contiguous share tensors (fused form)");
+    throw PirError(ERR_UNSUPPORTED);
+}
+
 static int log2_exact(u64 d) {
     if (d == 0 || (d & (d - 1)) != 0) {
         set_error("denominator must be a power of two");
@@ -46,10 +53,12 @@ void reconstruct(const RSS<T> &in, PartyData<T> &out) {
 // RSS.inl:1052-1074: out_p = (c_p, c_{next(p)}), no mask (placeholder precompute)
 template <typename T>
 void reshare(const PartyData<T> &c, RSS<T> &out) {
+    const PrgStream g = call_prg("reshare");
     if (sizeof(T) == 8 && fused_mode() && c.base() && out.base()) {
-        check(fused_reshare((const u64 *)c.base(), c.size(), (u64 *)out.base(), S()));
+        check(fused_reshare((const u64 *)c.base(), c.size(), (u64 *)out.base(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("reshare");
     PartyData<T> mask(c.size());
     mask += c;
     out.zero();
@@ -80,10 +89,12 @@ void reconstruct3out3(const PartyData<u64> &in, PartyData<u64> &out) {
 
 template <typename T>
 void rss_mult_impl(RSS<T> &x, const RSS<T> &y) {
+    const PrgStream g = call_prg("RSS *=");
     if (sizeof(T) == 8 && fused_mode() && x.base() && y.base()) {
-        check(fused_rss_mult((u64 *)x.base(), (const u64 *)y.base(), x.size(), S()));
+        check(fused_rss_mult((u64 *)x.base(), (const u64 *)y.base(), x.size(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("RSS *=");
     PartyData<T> summed(y.size());
     MultiLaunch s0(OP_ADD), s1(OP_ADD), m0(OP_MUL), m1(OP_MUL), a0(OP_ADD);
     for (int q = 0; q < 3; q++) {
@@ -104,10 +115,12 @@ void rss_mult_impl(RSS<T> &x, const RSS<T> &y) {
 
 template <typename T>
 void rss_and_impl(RSS<T> &x, const RSS<T> &y) {
+    const PrgStream g = call_prg("RSS &=");
     if (fused_mode() && x.base() && y.base()) {
-        check(fused_rss_and<T>(x.base(), y.base(), x.size(), S()));
+        check(fused_rss_and<T>(x.base(), y.base(), x.size(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("RSS &=");
     PartyData<T> summed(y.size());
     MultiLaunch s0(OP_XOR), s1(OP_XOR), m0(OP_AND), m1(OP_AND), a0(OP_XOR);
     for (int q = 0; q < 3; q++) {
@@ -131,10 +144,12 @@ void rss_and_impl(RSS<T> &x, const RSS<T> &y) {
 
 // RSS.inl:422-436
 void dividePublic(RSS64 &a, u64 d) {
+    const PrgStream g = call_prg("dividePublic");
     if (fused_mode() && a.base()) {
-        check(fused_divide_public(a.base(), a.size(), d, nullptr, S()));
+        check(fused_divide_public(a.base(), a.size(), d, nullptr, S(), g));
         return;
     }
+    if (g.on) refuse_masked("dividePublic");
     size_t n = a.size();
     RSS64 r(n), rPrime(n);
     rPrime.fill(d);
@@ -150,6 +165,7 @@ void dividePublic(RSS64 &a, u64 d) {
 
 // RSS.inl:438-455
 void dividePublic(RSS64 &a, PartyData<u64> &den) {
+    require_placeholder("dividePublic by per-party denominators");
     size_t n = a.size();
     if (fused_mode() && a.base() && den.base()) {
         check(fused_divide_public(a.base(), n, 0, den.base(), S()));
@@ -173,10 +189,12 @@ void dividePublic(RSS64 &a, PartyData<u64> &den) {
 // RSS.inl:1076-1094 (`a` is left modified as in the reference in plain mode)
 void dividePublic3out3(PartyData<u64> &a, u64 d, RSS64 &out) {
     int k = log2_exact(d);
+    const PrgStream g = call_prg("dividePublic3out3");
     if (fused_mode() && a.base() && out.base()) {
-        check(fused_divide_3out3(a.base(), a.size(), k, out.base(), S()));
+        check(fused_divide_3out3(a.base(), a.size(), k, out.base(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("dividePublic3out3");
     size_t n = a.size();
     RSS64 r(n), rPrime(n);
     rPrime.fill(d);
@@ -269,7 +287,7 @@ LimbA limb_wgrad_view(int batch, int H, int W, int f, int Din, int Dout, int str
 
 int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
                      int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split, u64 *relu_out,
-                     u8 *relu_d) {
+                     u8 *relu_d, const PrgStream &g, const PrgStream &gr) {
     LimbA a{};
     a.M = M;
     a.K = K;
@@ -293,6 +311,8 @@ int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int 
     e.bias = bias ? bias->base() : nullptr;
     e.relu_out = relu_out;
     e.relu_d = relu_d;
+    e.prg = g;
+    e.relu_prg = gr;
     return rss_limb_run(A, a, packed, N, e, split);
 }
 
@@ -311,19 +331,21 @@ void localMatMul(const RSS64 &a, const RSS64 &b, PartyData<u64> &c, int M, int N
 
 // RSS.inl:481-540
 void matmul(const RSS64 &a, const RSS64 &b, RSS64 &c, int M, int N, int K, bool ta, bool tb, bool tc, u64 truncation) {
+    const PrgStream g = call_prg("matmul");
     if (fused_mode() && a.base() && b.base() && c.base()) {
         if (ctx().algo == ALGO_LIMB) {
             DevBuf packed(packed_rss_weights_bytes(K, N));
             check(pack_rss_weights(b, K, N, tb, (int8_t *)packed.get()));
             check(rss_limb_product(a, ta, nullptr, M, N, K, (const int8_t *)packed.get(), (int)truncation, nullptr, c,
-                                   tc));
+                                   tc, 0, nullptr, nullptr, g));
             return;
         }
         PartyData<u64> raw((size_t)M * N);
         localMatMul(a, b, raw, M, N, K, ta, tb, tc);
-        check(fused_divide_3out3(raw.base(), raw.size(), (int)truncation, c.base(), S()));
+        check(fused_divide_3out3(raw.base(), raw.size(), (int)truncation, c.base(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("matmul");
     PartyData<u64> raw((size_t)M * N);
     localMatMul(a, b, raw, M, N, K, ta, tb, tc);
     dividePublic3out3(raw, (u64)1 << truncation, c);
@@ -412,6 +434,8 @@ void rss_conv_grad_limb(ConvOperator op, const RSS64 &A, const RSS64 &B, RSS64 &
 void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int batch, int H, int W, int f, int Din,
                  int Dout, int stride, int pad, int trunc) {
     int OH = (H + 2 * pad - f) / stride + 1, OW = (W + 2 * pad - f) / stride + 1;
+    if (op != kFprop) require_placeholder("convolution (dgrad / wgrad)");
+    const PrgStream g = call_prg("convolution");
     if (fused_mode() && A.base() && B.base() && C.base()) {
         if (ctx().algo == ALGO_LIMB) {
             if (op != kFprop) {
@@ -423,14 +447,15 @@ void convolution(const RSS64 &A, const RSS64 &B, RSS64 &C, ConvOperator op, int 
             check(pack_rss_weights(B, K, Dout, false, (int8_t *)packed.get()));
             ConvShape cs{batch, H, W, Din, f, stride, pad, OH, OW};
             check(rss_limb_product(A, true, &cs, batch * OH * OW, Dout, K, (const int8_t *)packed.get(), trunc,
-                                   nullptr, C, true));
+                                   nullptr, C, true, 0, nullptr, nullptr, g));
             return;
         }
         PartyData<u64> local(C.size());
         local_conv(op, A, B, local, batch, H, W, f, Din, Dout, stride, pad, OH, OW);
-        check(fused_divide_3out3(local.base(), local.size(), trunc, C.base(), S()));
+        check(fused_divide_3out3(local.base(), local.size(), trunc, C.base(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("convolution");
     PartyData<u64> local(C.size());
     local_conv(op, A, B, local, batch, H, W, f, Din, Dout, stride, pad, OH, OW);
     dividePublic3out3(local, (u64)1 << trunc, C);
@@ -471,10 +496,12 @@ void carryOut(RSS<T> &p, RSS<T> &g, int k, RSS<T> &out) {
 // RSS.inl:806-852
 template <typename U>
 void dReLU(const RSS64 &input, RSS<U> &result) {
+    const PrgStream prg = call_prg("dReLU");
     if (fused_mode() && input.base() && result.base()) {
-        check(fused_drelu<U>(input.base(), input.size(), result.base(), S()));
+        check(fused_drelu<U>(input.base(), input.size(), result.base(), S(), prg));
         return;
     }
+    if (prg.on) refuse_masked("dReLU");
     const int bw = 64;
     size_t n = input.size();
     RSS64 r(n);
@@ -516,10 +543,12 @@ void dReLU(const RSS64 &input, RSS<U> &result) {
 // RSS.inl:542-569
 template <typename U>
 void selectShare(const RSS64 &x, const RSS64 &y, const RSS<U> &b, RSS64 &z) {
+    const PrgStream g = call_prg("selectShare");
     if (fused_mode() && x.base() && y.base() && b.base() && z.base()) {
-        check(fused_select<U>(x.base(), y.base(), b.base(), z.base(), x.size(), S()));
+        check(fused_select<U>(x.base(), y.base(), b.base(), z.base(), x.size(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("selectShare");
     size_t n = x.size();
     RSS64 c(n);
     RSS<U> cbits(b.size());
@@ -540,10 +569,12 @@ void selectShare(const RSS64 &x, const RSS64 &y, const RSS<U> &b, RSS64 &z) {
 // RSS.inl:854-865
 template <typename U>
 void ReLU(const RSS64 &input, RSS64 &result, RSS<U> &dresult) {
+    const PrgStream g = call_prg("ReLU");
     if (fused_mode() && input.base() && result.base() && dresult.base()) {
-        check(fused_relu<U>(input.base(), input.size(), result.base(), dresult.base(), S()));
+        check(fused_relu<U>(input.base(), input.size(), result.base(), dresult.base(), S(), g));
         return;
     }
+    if (g.on) refuse_masked("ReLU");
     dReLU(input, dresult);
     RSS64 zeros(input.size());
     selectShare(zeros, input, dresult, result);
@@ -552,10 +583,12 @@ void ReLU(const RSS64 &input, RSS64 &result, RSS<U> &dresult) {
 // RSS.inl:867-1017 (selectShare writes into the even view of `input`)
 template <typename U>
 void maxpool(RSS64 &input, RSS64 &result, RSS<U> &dresult, int k) {
+    const PrgStream g = call_prg("maxpool");
     if (fused_mode() && input.base() && result.base() && dresult.base()) {
-        check(fused_maxpool_pools<U>(input.base(), input.size(), result.base(), dresult.base(), k, S()));
+        check(fused_maxpool_pools<U>(input.base(), input.size(), result.base(), dresult.base(), k, S(), g));
         return;
     }
+    if (g.on) refuse_masked("maxpool");
     dresult.fill(1);
     int stride = 2, offset = 1;
     RSS64 even = input.strided(0, stride), odd = input.strided(offset, stride);
@@ -596,6 +629,7 @@ void maxpool(RSS64 &input, RSS64 &result, RSS<U> &dresult, int k) {
 
 // RSS.inl:1211-1235
 void getPowers(const RSS64 &in, PartyData<u64> &pow) {
+    require_placeholder("getPowers / taylorSeries");
     if (fused_mode() && in.base() && pow.base()) {
         check(fused_get_powers(in.base(), in.size(), pow.base(), S()));
         return;
@@ -670,6 +704,7 @@ TaylorTables taylor_tables(double a0, double a1, double a2, int fn, int fp) {
 // RSS.inl:1237-1320.  The coefficient functors are applied by table lookup
 // (their argument, the opened power plus one, takes at most 33 distinct values).
 void taylorSeries(const RSS64 &in, RSS64 &out, double a0, double a1, double a2, int fn, int fp) {
+    require_placeholder("getPowers / taylorSeries");
     const TaylorTables t = taylor_tables(a0, a1, a2, fn, fp);
     const size_t n = out.size();
     if (fused_mode() && in.base() && out.base()) {

@@ -120,8 +120,10 @@ struct ConvShape {
 };
 // relu_out / relu_d (optional, contiguous [3][2][M*N]): the following ReLU
 // layer evaluated on the product's output in the same epilogue.
+// g / gr: the masked-mode streams of the product and of the fused ReLU (off in
+// the placeholder mode).
 int rss_limb_product(const RSS64 &A, bool ta, const ConvShape *conv, int M, int N, int K, const int8_t *packed,
                      int trunc_bits, const RSS64 *bias, RSS64 &out, bool tc, int split = 0, u64 *relu_out = nullptr,
-                     u8 *relu_d = nullptr);
+                     u8 *relu_d = nullptr, const PrgStream &g = PrgStream{}, const PrgStream &gr = PrgStream{});
 
 }  // namespace pir

@@ -12,6 +12,7 @@
 #pragma once
 
 #include "common.h"
+#include "prg.h"
 
 namespace pir {
 
@@ -96,6 +97,13 @@ struct Epilogue {
     // with bw_relu_d that ReLU layer's reluPrime (ReLULayer.cu:91-123).
     const u8 *bw_relu_d;
     u64 *bw_relu_out;
+    // Masked precompute mode (EPI_TRUNC): prg.on draws the truncation pair of
+    // output element idx (its index in out's layout) from prg, and the fused
+    // ReLU from relu_prg; the product is then written raw and the truncation
+    // runs in the reduction kernel (limb_gemm).  No div_bits, sub_out or
+    // fused ReLU backward in this mode.
+    PrgStream prg;
+    PrgStream relu_prg;
 };
 
 constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;
@@ -138,8 +146,9 @@ struct LimbPlan {
 // The plan for this shape: the autotuned entry when one exists, else the
 // heuristic (split_hint > 0 forces that split factor).
 LimbPlan limb_plan(const LimbA &a, int N, int split_hint = 0);
-// Workspace bytes a plan needs (split * M * N * 8 when split > 1).
-size_t limb_workspace_bytes(const LimbPlan &p, int M, int N);
+// Workspace bytes a plan needs (split * M * N * 8 when split > 1, or when the
+// epilogue is masked, which always reduces from the workspace).
+size_t limb_workspace_bytes(const LimbPlan &p, int M, int N, bool masked = false);
 
 // Runs the limb GEMM with `plan`.  plan.split > 1 partitions the K-steps
 // across workgroups and reduces the 64-bit partials (exact: addition mod 2^64

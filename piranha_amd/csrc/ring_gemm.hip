@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "rss_device.h"
+#include "rss_masked.h"
 #include "runtime.h"
 
 namespace pir {
@@ -115,6 +116,35 @@ __device__ __forceinline__ void trunc_store(const Epilogue &e, size_t MN, int N,
     }
     for (int j = 0; j < e.nout; j++) {
         u64 v = ((e.base_mask >> j) & 1u) ? base : (u64)0;
+        if (e.bias) v += e.bias[(size_t)j * N + n];
+        e.out[(size_t)j * MN + idx] = v;
+    }
+}
+
+// The masked form of trunc_store for the forward epilogue (no division,
+// subtraction or backward ReLU): the output shares are the random truncation
+// pair's r plus q in the x0 slot (dividePublic3out3 with r' = r * 2^k), the
+// bias shares, and the fused ReLU drawing from relu_prg.
+__device__ __forceinline__ void trunc_store_m(const Epilogue &e, size_t MN, int N, int m, int n, u64 raw) {
+    size_t idx = e.tc ? (size_t)m * N + n : (size_t)n * (MN / N) + m;
+    Sh3<u64> x;
+    divide_3out3_m(raw, e.trunc_bits, e.prg, idx, x.s);
+    if (e.relu_out) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            u64 v = x.s[j >> 1][j & 1];
+            if (e.bias) v += e.bias[(size_t)j * N + n];
+            e.out[(size_t)j * MN + idx] = v;
+            x.s[j >> 1][j & 1] = v;
+        }
+        Sh3<u8> b;
+        Sh3<u64> r = relu_m<u8>(x, b, e.relu_prg, idx);
+        store3(e.relu_d, MN, idx, b);
+        store3(e.relu_out, MN, idx, r);
+        return;
+    }
+    for (int j = 0; j < e.nout; j++) {
+        u64 v = j < 6 ? x.s[j >> 1][j & 1] : (u64)0;
         if (e.bias) v += e.bias[(size_t)j * N + n];
         e.out[(size_t)j * MN + idx] = v;
     }
@@ -555,6 +585,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     }
 }
 
+template <bool MASKED>
 __global__ __launch_bounds__(256) void split_reduce_kernel(const u64 *__restrict__ part, int split, int M, int N,
                                                            Epilogue epi) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,13 +593,21 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(const u64 *__restrict
     if (t >= MN) return;
     u64 raw = 0;
     for (int z = 0; z < split; z++) raw += part[(size_t)z * MN + t];
-    epilogue_store(epi, M, N, (int)(t / N), (int)(t % N), raw);
+    if (MASKED) trunc_store_m(epi, MN, N, (int)(t / N), (int)(t % N), raw);
+    else epilogue_store(epi, M, N, (int)(t / N), (int)(t % N), raw);
 }
 
 int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &epi, hipStream_t s) {
     size_t MN = (size_t)M * N;
     if (MN == 0) return OK;
-    hipLaunchKernelGGL(split_reduce_kernel, dim3(grid_for(MN, 256)), dim3(256), 0, s, raw, nsum, M, N, epi);
+    const bool masked = epi.mode == EPI_TRUNC && epi.prg.on;
+    PIR_REQUIRE(!masked || (epi.trunc_bits >= 1 && epi.nout <= 6 && !epi.div_bits && !epi.sub_out &&
+                            !epi.bw_relu_out && (!epi.relu_out || epi.relu_prg.on)),
+                "masked precompute: the product epilogue supports the forward truncation only");
+    if (masked)
+        hipLaunchKernelGGL(split_reduce_kernel<true>, dim3(grid_for(MN, 256)), dim3(256), 0, s, raw, nsum, M, N, epi);
+    else
+        hipLaunchKernelGGL(split_reduce_kernel<false>, dim3(grid_for(MN, 256)), dim3(256), 0, s, raw, nsum, M, N, epi);
     PIR_LAUNCH_CHECK();
     return OK;
 }
@@ -633,8 +672,8 @@ LimbPlan limb_plan(const LimbA &a, int N, int split_hint) {
     return p;
 }
 
-size_t limb_workspace_bytes(const LimbPlan &p, int M, int N) {
-    return p.split > 1 ? (size_t)p.split * M * N * sizeof(u64) : 0;
+size_t limb_workspace_bytes(const LimbPlan &p, int M, int N, bool masked) {
+    return (p.split > 1 || masked) ? (size_t)(p.split > 1 ? p.split : 1) * M * N * sizeof(u64) : 0;
 }
 
 int limb_tune_entries() {
@@ -714,11 +753,12 @@ int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &e
     const int nks = a.nseg * a.Kp / LIMB_BK;
     const int split = limb_effective_split(nks, plan.split);
     const int per = (nks + split - 1) / split;
-    PIR_REQUIRE(split == 1 || workspace, "limb_gemm: split-K needs a workspace");
+    const bool masked = epi.mode == EPI_TRUNC && epi.prg.on;
+    PIR_REQUIRE((split == 1 && !masked) || workspace, "limb_gemm: split-K or a masked epilogue needs a workspace");
     PIR_REQUIRE(!plan.wide || can_be_wide(N), "limb_gemm: the 32x128 tile needs an even number of 64-column tiles");
     const int wide = plan.wide;
     dim3 grid(wide ? (a.M + 31) / 32 : (a.M + 63) / 64, wide ? (N + 127) / 128 : (N + 63) / 64, split);
-    u64 *part = split > 1 ? workspace : nullptr;
+    u64 *part = (split > 1 || masked) ? workspace : nullptr;
     int amode;
     switch (a.conv) {
         case LIMB_FPROP: amode = (a.Cin % LIMB_BK) == 0 ? 3 : 2; break;
@@ -758,7 +798,7 @@ int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &e
         PIR_HIP(hipEventRecord(rec.stop, s));
         timer.recs.push_back(rec);
     }
-    if (split > 1) PIR_TRY(rss_trunc_epilogue(workspace, a.M, N, split, epi, s));
+    if (split > 1 || masked) PIR_TRY(rss_trunc_epilogue(workspace, a.M, N, split, epi, s));
     return OK;
 }
 
@@ -788,7 +828,7 @@ int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, 
         float best_ms = 1e30f;
         int rc = OK;
         for (auto &p : cand) {
-            DevBuf ws(limb_workspace_bytes(p, a.M, N));
+            DevBuf ws(limb_workspace_bytes(p, a.M, N, epi.mode == EPI_TRUNC && epi.prg.on));
             rc = limb_gemm(a, packedB, N, epi, p, (u64 *)ws.get(), s);  // warm-up
             float ms = 1e30f;
             for (int r = 0; r < 2 && rc == OK; r++) {
@@ -816,7 +856,7 @@ int limb_run(const LimbA &a, const int8_t *packedB, int N, const Epilogue &epi, 
         g_tuned[key] = best;
     }
     LimbPlan p = limb_plan(a, N, split_hint);
-    DevBuf ws(limb_workspace_bytes(p, a.M, N));
+    DevBuf ws(limb_workspace_bytes(p, a.M, N, epi.mode == EPI_TRUNC && epi.prg.on));
     return limb_gemm(a, packedB, N, epi, p, (u64 *)ws.get(), s);
 }
 

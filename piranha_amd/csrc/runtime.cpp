@@ -18,6 +18,23 @@ static thread_local Context *g_ctx = nullptr;
 Context &ctx() { return g_ctx ? *g_ctx : g_default_ctx; }
 void set_ctx(Context *c) { g_ctx = c; }
 
+PrgStream call_prg(const char *what) {
+    Context &c = ctx();
+    if (c.precompute != PRECOMPUTE_MASKED) return PrgStream{};
+    if (c.mode != MODE_FUSED) {
+        set_error(std::string(what) + ": the masked precompute mode runs in the fused form only");
+        throw PirError(ERR_UNSUPPORTED);
+    }
+    return PrgStream{c.prg_seed, c.prg_call++, 1};
+}
+
+void require_placeholder(const char *what) {
+    if (ctx().precompute == PRECOMPUTE_MASKED) {
+        set_error(std::string(what) + ": no masked precompute form (placeholder precompute only)");
+        throw PirError(ERR_UNSUPPORTED);
+    }
+}
+
 Allocator::~Allocator() {
     for (auto &kv : free_) (void)hipFree(kv.second);
     for (auto &kv : used_) (void)hipFree(kv.first);

@@ -13,10 +13,14 @@
 #include <vector>
 
 #include "common.h"
+#include "prg.h"
 
 namespace pir {
 
 enum ExecMode { MODE_FUSED = 0, MODE_PLAIN = 1 };
+// Correlated randomness: the reference's placeholder constants
+// (src/mpc/Precompute.h:73-134) or the masked mode's seeded Philox stream.
+enum Precompute { PRECOMPUTE_PLACEHOLDER = 0, PRECOMPUTE_MASKED = 1 };
 enum GemmAlgo { ALGO_PLAIN = 0, ALGO_LIMB = 1 };
 
 // Freed blocks are cached per stream and handed out again only to requests on
@@ -65,7 +69,18 @@ struct Context {
     int frac_bits = 26;  // FLOAT_PRECISION (globals.h:19-21), a runtime value here
     int log_lr = 5;      // log_learning_rate (main.cu:48), weight updates divide by 2^log_lr
     int error = 0;
+    int precompute = PRECOMPUTE_PLACEHOLDER;
+    u64 prg_seed = 0;
+    u32 prg_call = 0;  // call number of the next masked protocol call
 };
+
+// The word stream of one protocol call that uses correlated randomness: in the
+// masked precompute mode the next call number of the current context (the
+// counter advances; the fused form only, anything else is refused with
+// `what` in the message); a stream that is off in the placeholder mode.
+PrgStream call_prg(const char *what);
+// Refuses the masked precompute mode for a function that has no masked form.
+void require_placeholder(const char *what);
 
 // The context used by the C++ protocol API (the reference uses globals:
 // partyNum, PrecomputeObject, piranha_config; main.cu:23-40).

@@ -257,6 +257,32 @@ def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True, m
             "precompute": "masked (Philox4x32-10)" if masked else "placeholder (the reference's constants)"}
 
 
+def bench_uneven(P, C, steps, warmup, autotune=True, profile=False):
+    """64 concurrent AlexNet-CIFAR queries with per-query batches drawn from
+    1..32 (986 images), FP20: the queries concatenated along the batch axis
+    into one forward, replayed from its hipGraph."""
+    qb = C.uneven_query_batches()
+    total = int(sum(qb))
+    layers = C.ALEXNET_CIFAR
+    n = make_net(P, C, layers, total, 20, P.MODE_FUSED, autotune=autotune)
+    tt = time_graph(P, n, steps, warmup)
+    out = {"images_per_s": total / tt, "ms_per_forward": tt * 1e3, "queries": len(qb), "images": total,
+           "grouping": "queries concatenated along the batch axis into one graph"}
+    log(f"alexnet uneven: {total} images, {total / tt:.1f} images/s ({tt * 1e3:.3f} ms)")
+    if profile:
+        layer_ms = n.profile_layers()
+        prof = n.profile_gemm()
+        log("per-layer ms: " + ", ".join(f"{l[0]}{i}:{m:.3f}" for i, (l, m) in enumerate(zip(layers, layer_ms))))
+        out["layer_ms"] = [float(m) for m in layer_ms]
+        out["limb_gemm"] = {f"limb_gemm_kernel<{kernel_args(P, v)}>": {
+            "ms": ms, "launches": k, "TOP/s": ops / ms / 1e9, "frac_of_peak": ops / ms / 1e9 / INT8_PEAK_TOPS}
+            for v, (ms, ops, k) in sorted(prof.items())}
+        log("limb GEMM: " + ", ".join(f"{k} {v['launches']}x {v['ms']:.3f} ms {v['TOP/s']:.0f} TOP/s"
+                                      for k, v in out["limb_gemm"].items()))
+    del n
+    return out
+
+
 def bench_training(P, C, layers, batch, fp, steps, warmup, autotune=True):
     """One training step (NeuralNetwork::forward + backward: reveal-softmax
     delta, every layer's backward and SGD update), fused limb-MFMA form,
@@ -429,6 +455,8 @@ def main():
     ap.add_argument("--train-only", action="store_true", help="VGG16 training step only (for profiling runs)")
     ap.add_argument("--random-sharings-only", action="store_true",
                     help="the VGG16 forward on uniformly random sharings only (for profiling runs)")
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="the uneven-batch AlexNet workload only, with its per-layer and GEMM profile")
     ap.add_argument("--masked-only", action="store_true",
                     help="the VGG16 forward in the masked precompute mode only (for profiling runs)")
     args = ap.parse_args()
@@ -449,6 +477,11 @@ def main():
                             autotune=not args.no_autotune)
         log(f"VGG16 training: {tr['ms_per_step']:.2f} ms/step, {tr['images_per_s']:.1f} images/s")
         print(json.dumps({"training": tr}), flush=True)
+        return
+
+    if args.uneven_only:
+        un = bench_uneven(P, C, args.steps, args.warmup, autotune=not args.no_autotune, profile=True)
+        print(json.dumps({"alexnet_uneven_64q_fp20": un}), flush=True)
         return
 
     if args.random_sharings_only or args.masked_only:
@@ -546,15 +579,8 @@ def main():
             other[name] = {"images_per_s": b / tt, "ms_per_forward": tt * 1e3}
             log(f"{name}: {b / tt:.1f} images/s")
             del n
-        qb = C.uneven_query_batches()
-        total = int(sum(qb))
-        n = make_net(P, C, C.ALEXNET_CIFAR, total, 20, P.MODE_FUSED)
-        tt = time_graph(P, n, max(5, args.steps // 2), 2)
-        other["alexnet_uneven_64q_fp20"] = {"images_per_s": total / tt, "ms_per_forward": tt * 1e3,
-                                            "queries": 64, "images": total,
-                                            "grouping": "queries concatenated along the batch axis into one graph"}
-        log(f"alexnet uneven: {total} images, {total / tt:.1f} images/s")
-        del n
+        other["alexnet_uneven_64q_fp20"] = bench_uneven(P, C, max(5, args.steps // 2), 20,
+                                                        autotune=not args.no_autotune)
         rs = bench_random_shares(P, C, layers, batch, fp, args.steps, 20, autotune=not args.no_autotune)
         other["vgg16_b64_fp26_random_sharings"] = rs
         log(f"VGG16 on random sharings: {rs['images_per_s']:.1f} images/s")

@@ -104,6 +104,10 @@ struct Epilogue {
     // fused ReLU backward in this mode.
     PrgStream prg;
     PrgStream relu_prg;
+    // Set by limb_gemm, not by callers: the limb kernel may store two adjacent
+    // columns per lane (row-major output or split-K partials, N even, the
+    // 64-bit arrays 16-byte and the bit arrays 2-byte aligned, no sub_out).
+    int pairs;
 };
 
 constexpr int LIMB_BM = 64, LIMB_BN = 64, LIMB_BK = 32;

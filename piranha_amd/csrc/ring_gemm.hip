@@ -70,12 +70,20 @@ __device__ __forceinline__ void transpose8x8(const u64 w[8], u64 out[8]) {
     out[7] = (u64)c3 | ((u64)d3 << 32);
 }
 
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// The placeholder truncation of one raw product (see Epilogue): 1 + q, then
+// the optional second division.
+__device__ __forceinline__ u64 trunc_base(const Epilogue &e, u64 raw) {
+    const int k = e.trunc_bits;
+    u64 base = (u64)1 + trunc_div_pow2(raw - ((u64)1 << k), k);
+    if (e.div_bits) base = (u64)1 + trunc_div_pow2(base - ((u64)1 << e.div_bits), e.div_bits);
+    return base;
+}
+
 // Truncation + bias epilogue for one output element (see Epilogue).
 __device__ __forceinline__ void trunc_store(const Epilogue &e, size_t MN, int N, int m, int n, u64 raw) {
-    const int k = e.trunc_bits;
-    u64 q = trunc_div_pow2(raw - ((u64)1 << k), k);
-    u64 base = (u64)1 + q;
-    if (e.div_bits) base = (u64)1 + trunc_div_pow2(base - ((u64)1 << e.div_bits), e.div_bits);
+    const u64 base = trunc_base(e, raw);
     size_t idx = e.tc ? (size_t)m * N + n : (size_t)n * (MN / N) + m;
     if (e.sub_out) {
         for (int j = 0; j < e.nout; j++)
@@ -157,6 +165,74 @@ __device__ __forceinline__ void epilogue_store(const Epilogue &e, int M, int N, 
     } else {
         trunc_store(e, (size_t)M * N, N, m, n, raw);
     }
+}
+
+// trunc_store for the two adjacent row-major outputs (m, n) and (m, n + 1)
+// with 16-byte stores of the 64-bit arrays and 2-byte stores of the bit
+// arrays.  Valid when Epilogue::pairs is set (tc, no sub_out, N and n even,
+// arrays aligned).
+__device__ __forceinline__ void trunc_store2(const Epilogue &e, size_t MN, int N, int m, int n, u64 raw0,
+                                             u64 raw1) {
+    const u64 base0 = trunc_base(e, raw0), base1 = trunc_base(e, raw1);
+    const size_t idx = (size_t)m * N + n;
+    if (e.relu_out || e.bw_relu_out) {
+        Sh3<u64> x0, x1, zero;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const bool on = (e.base_mask >> j) & 1u;
+            u64 v0 = on ? base0 : (u64)0, v1 = on ? base1 : (u64)0;
+            if (e.bias) {
+                const u64x2 b = *(const u64x2 *)(e.bias + (size_t)j * N + n);
+                v0 += b.x;
+                v1 += b.y;
+            }
+            *(u64x2 *)(e.out + (size_t)j * MN + idx) = (u64x2){v0, v1};
+            x0.s[j >> 1][j & 1] = v0;
+            x1.s[j >> 1][j & 1] = v1;
+            zero.s[j >> 1][j & 1] = 0;
+        }
+        Sh3<u8> b0, b1;
+        u64 *dst;
+        if (e.relu_out) {
+            // the following ReLU layer on the same elements: dReLU + selectShare
+            b0 = drelu_<u8>(x0);
+            b1 = drelu_<u8>(x1);
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+                *(unsigned short *)(e.relu_d + (size_t)j * MN + idx) =
+                    (unsigned short)(b0.s[j >> 1][j & 1] | ((unsigned)b1.s[j >> 1][j & 1] << 8));
+            dst = e.relu_out;
+        } else {
+            // the preceding ReLU layer's backward on the same elements
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const unsigned d = *(const unsigned short *)(e.bw_relu_d + (size_t)j * MN + idx);
+                b0.s[j >> 1][j & 1] = (u8)d;
+                b1.s[j >> 1][j & 1] = (u8)(d >> 8);
+            }
+            dst = e.bw_relu_out;
+        }
+        const Sh3<u64> r0 = select_<u8>(zero, x0, b0), r1 = select_<u8>(zero, x1, b1);
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+            *(u64x2 *)(dst + (size_t)j * MN + idx) = (u64x2){r0.s[j >> 1][j & 1], r1.s[j >> 1][j & 1]};
+        return;
+    }
+    for (int j = 0; j < e.nout; j++) {
+        const bool on = (e.base_mask >> j) & 1u;
+        u64 v0 = on ? base0 : (u64)0, v1 = on ? base1 : (u64)0;
+        if (e.bias) {
+            const u64x2 b = *(const u64x2 *)(e.bias + (size_t)j * N + n);
+            v0 += b.x;
+            v1 += b.y;
+        }
+        *(u64x2 *)(e.out + (size_t)j * MN + idx) = (u64x2){v0, v1};
+    }
+}
+
+__device__ __forceinline__ void epilogue_store2(const Epilogue &e, int M, int N, int m, int n, u64 raw0, u64 raw1) {
+    if (e.mode == EPI_RAW) *(u64x2 *)(e.C + (size_t)m * N + n) = (u64x2){raw0, raw1};
+    else trunc_store2(e, (size_t)M * N, N, m, n, raw0, raw1);
 }
 
 // ---------------------------------------------------------------------------
@@ -253,7 +329,7 @@ constexpr int LDS_ROW = 32;
 // masked to zero), so all of them are in flight together.  The K-step position
 // (segment, step within segment) is uniform across the workgroup.
 //   AMODE 0: dense row-major A, 1: dense column-major A,
-//   AMODE 2: implicit im2col, any Cin (per-element decode),
+//   AMODE 2: implicit im2col, any Cin (one decode per K-step, then a step along the filter row),
 //   AMODE 3: implicit im2col with Cin % 32 == 0 (one (r, s) per K-step, decoded
 //            once per workgroup and advanced incrementally),
 //   AMODE 4: dgrad gather of dy, any channel count (per-element decode),
@@ -270,12 +346,19 @@ struct ALoader {
     int cb, ih0, iw0;     // fprop: image and top-left input coordinate of this row's pixel;
                           // dgrad: image and (h + ph, w + pw) of this row's pixel
     int r, s, c0;         // AMODE 3/5: k = kk*32 = (r*fw + s)*Cin + c0; AMODE 6: the row's (r, s, c)
+    // AMODE 2: k = rr * fw*Cin + j reads element pix + rr * W*Cin + j of the
+    // segment (j = s*Cin + c runs along one input row), inside the image iff
+    // rr in [rlo, rhi) and j in [jlo, jhi)
+    long long pix;
+    int rlo, rhi, jlo, jhi;
 
     __device__ ALoader(const LimbA &a_, int am_, int akq_) : a(a_), am(am_), akq(akq_) {
         kps = a.Kp / LIMB_BK;
         row_ok = am < a.M;
         cb = ih0 = iw0 = 0;
         r = s = c0 = 0;
+        pix = 0;
+        rlo = rhi = jlo = jhi = 0;
         const int m = row_ok ? am : 0;
         if (AMODE == 2 || AMODE == 3) {
             int ohw = a.OH * a.OW;
@@ -284,6 +367,13 @@ struct ALoader {
             int oh = rem / a.OW, ow = rem - (rem / a.OW) * a.OW;
             ih0 = oh * a.stride - a.ph;
             iw0 = ow * a.stride - a.pw;
+            if (AMODE == 2) {
+                pix = (((long long)cb * a.H + ih0) * a.W + iw0) * a.Cin;
+                rlo = max(0, -ih0);
+                rhi = row_ok ? min(a.fh, a.H - ih0) : 0;  // empty range: the row reads nothing
+                jlo = max(0, -iw0) * a.Cin;
+                jhi = min(a.fw, a.W - iw0) * a.Cin;
+            }
         } else if (AMODE == 4 || AMODE == 5) {
             int hw = a.H * a.W;
             cb = m / hw;
@@ -363,7 +453,23 @@ struct ALoader {
             const u64 msk = ok ? ~0ull : 0ull;
 #pragma unroll
             for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
-        } else if (AMODE == 2 || AMODE == 4) {
+        } else if (AMODE == 2) {
+            // decode the thread's first k once, then step along the filter row
+            const int fwc = a.fw * a.Cin, wc = a.W * a.Cin;
+            int rr = (int)a.fd_fwc.div((u32)kb);
+            int j = kb - rr * fwc;
+            int d = rr * wc + j;
+#pragma unroll
+            for (int e = 0; e < KPT; e++) {
+                // rr < rhi <= fh also keeps k < K (the segment's last k is fh*fwc - 1)
+                const bool ok = rr >= rlo && rr < rhi && j >= jlo && j < jhi;
+                w[e] = src[ok ? pix + d : 0] & (ok ? ~0ull : 0ull);
+                const bool wrap = ++j == fwc;
+                j = wrap ? 0 : j;
+                rr += wrap ? 1 : 0;
+                d += wrap ? 1 + wc - fwc : 1;
+            }
+        } else if (AMODE == 4) {
             const int fwc = a.fw * a.Cin;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
@@ -372,17 +478,9 @@ struct ALoader {
                 int rem = k - rr * fwc;
                 int ss = (int)a.fd_cin.div((u32)rem);
                 int cc = rem - ss * a.Cin;
-                bool ok;
-                size_t idx;
-                if (AMODE == 2) {
-                    int ih = ih0 + rr, iw = iw0 + ss;
-                    ok = row_ok && k < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
-                    idx = ok ? (((size_t)cb * a.H + ih) * a.W + iw) * a.Cin + cc : 0;
-                } else {
-                    int pp, qq;
-                    ok = row_ok && k < a.K && dgrad_src(rr, ss, pp, qq);
-                    idx = ok ? (((size_t)cb * a.OH + pp) * a.OW + qq) * a.Cin + cc : 0;
-                }
+                int pp, qq;
+                const bool ok = row_ok && k < a.K && dgrad_src(rr, ss, pp, qq);
+                const size_t idx = ok ? (((size_t)cb * a.OH + pp) * a.OW + qq) * a.Cin + cc : 0;
                 w[j] = src[idx] & (ok ? ~0ull : 0ull);
             }
         } else if (AMODE == 6) {
@@ -474,7 +572,10 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     constexpr int PLANE = TM * LDS_ROW;
     constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
     constexpr bool PRE_A = AMODE >= 2 && AMODE <= 5;  // measured: faster for fprop / dgrad, slower otherwise
-    __shared__ __attribute__((aligned(16))) int8_t As2[2][8 * PLANE];
+    constexpr int TS = TN + 2;                         // row stride (u64) of the staged output tile
+    constexpr int A_BYTES = 2 * 8 * PLANE, T_BYTES = TM * TS * 8;
+    __shared__ __attribute__((aligned(16))) int8_t smem[A_BYTES > T_BYTES ? A_BYTES : T_BYTES];
+    int8_t(*As2)[8 * PLANE] = (int8_t(*)[8 * PLANE])smem;
     int cur = 0;
 
     // XCD-aware tile order: the dispatcher deals consecutive workgroup ids
@@ -570,18 +671,42 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         __syncthreads();
     }
 
-    // ---- epilogue: C = sum_s acc_s << 8s; 32x32 MFMA layout, lane -> column, register -> row
-    const int n = n0 + (lane & 31);
-    if (n >= N) return;
+    // ---- epilogue: C = sum_s acc_s << 8s (32x32 MFMA layout: lane -> column,
+    // register -> row) is staged in LDS, whose A buffers every wave has left
+    // behind the loop's last barrier; a rolled loop then stores the tile with
+    // consecutive lanes on consecutive elements of the output's contiguous
+    // dimension, two adjacent columns (16 bytes) per lane where the output
+    // allows.  One copy of the epilogue's code serves the whole tile.
+    u64 *T = (u64 *)smem;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (m >= M) continue;
         u64 raw = 0;
 #pragma unroll
         for (int q = 0; q < 8; q++) raw += ((u64)(s64)acc[q][r]) << (8 * q);
-        if (part) part[((size_t)bz * M + m) * N + n] = raw;
-        else epilogue_store(epi, M, N, m, n, raw);
+        T[(wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * TS + wn * 32 + (lane & 31)] = raw;
+    }
+    __syncthreads();
+    const int nt0 = by * TN;  // first column of the workgroup tile
+    if (epi.pairs) {
+        constexpr int HALF = TN / 2;
+        for (int p = tid; p < TM * HALF; p += 256) {
+            const int ml = p / HALF, nl = 2 * (p - ml * HALF);
+            const int m = m0 + ml, n = nt0 + nl;
+            if (m >= M || n >= N) continue;
+            const u64x2 v = *(const u64x2 *)(T + ml * TS + nl);
+            if (part) *(u64x2 *)(part + ((size_t)bz * M + m) * N + n) = v;
+            else epilogue_store2(epi, M, N, m, n, v.x, v.y);
+        }
+    } else {
+        const bool rows = part || epi.tc;  // the output is row-major
+        for (int q = tid; q < TM * TN; q += 256) {
+            const int ml = rows ? q / TN : q % TM, nl = rows ? q % TN : q / TM;
+            const int m = m0 + ml, n = nt0 + nl;
+            if (m >= M || n >= N) continue;
+            const u64 v = T[ml * TS + nl];
+            if (part) part[((size_t)bz * M + m) * N + n] = v;
+            else epilogue_store(epi, M, N, m, n, v);
+        }
     }
 }
 
@@ -767,6 +892,18 @@ int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &e
         default: amode = a.row_major ? 0 : 1; break;
     }
     const int variant = amode + (wide ? LIMB_VARIANT_WIDE : 0);
+    Epilogue ek = epi;
+    {
+        auto al = [](const void *q, uintptr_t b) { return ((uintptr_t)q % b) == 0; };
+        bool ok = N % 2 == 0;
+        if (part) ok = ok && al(part, 16);
+        else if (epi.mode == EPI_RAW) ok = ok && epi.tc && al(epi.C, 16);
+        else
+            ok = ok && epi.tc && !epi.sub_out && al(epi.out, 16) && (!epi.bias || al(epi.bias, 16)) &&
+                 (!epi.relu_out || (al(epi.relu_out, 16) && al(epi.relu_d, 2))) &&
+                 (!epi.bw_relu_out || (al(epi.bw_relu_out, 16) && al(epi.bw_relu_d, 2)));
+        ek.pairs = ok ? 1 : 0;
+    }
     GemmTimer &timer = ctx().timer;
     GemmTimer::Rec rec{nullptr, nullptr, 2.0 * 36 * a.M * (double)N * a.nseg * a.K, variant};
     if (timer.on) {
@@ -775,7 +912,7 @@ int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &e
         PIR_HIP(hipEventRecord(rec.start, s));
     }
 #define PIR_LIMB_LAUNCH(AM, WM)                                                                                \
-    hipLaunchKernelGGL((limb_gemm_kernel<AM, WM>), grid, dim3(256), 0, s, a, packedB, N, nks, per, epi, part)
+    hipLaunchKernelGGL((limb_gemm_kernel<AM, WM>), grid, dim3(256), 0, s, a, packedB, N, nks, per, ek, part)
     switch (variant) {
         case 0: PIR_LIMB_LAUNCH(0, 2); break;
         case 1: PIR_LIMB_LAUNCH(1, 2); break;

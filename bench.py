@@ -47,9 +47,9 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "limb_gemm_conv_vgg16_b64.md")
 
 
-def kernel_args(P, v):
-    """Template arguments of limb_gemm_kernel for a profile_gemm variant."""
-    return f"{v % P.LIMB_WIDE}, {1 if v >= P.LIMB_WIDE else 2}"
+def kernel_name(P, v):
+    """The kernel of a profile_gemm variant: limb_gemm_kernel<form, tile>."""
+    return f"limb_gemm_kernel<{v % P.LIMB_WIDE}, {1 if v >= P.LIMB_WIDE else 2}>"
 
 
 def read_traffic(kname):
@@ -178,10 +178,13 @@ def counter_calibration(P):
 def measured_peaks(P):
     """Measured peaks beside the datasheet figures (include/pir.h
     pir_measure_peak): HBM copy / read / write streams over 1 GiB buffers, the
-    int8 MFMA instruction rate, the 64-bit multiply-add rate of the VALUs."""
+    int8 MFMA instruction rate on low- and on full-entropy operands (the latter
+    is the power-limited rate that digit planes of random shares see), the
+    64-bit multiply-add rate of the VALUs."""
     out = {}
     for key, which, unit in (("hbm_copy", P.PEAK_HBM_COPY, "GB/s"), ("hbm_read", P.PEAK_HBM_READ, "GB/s"),
                              ("hbm_write", P.PEAK_HBM_WRITE, "GB/s"), ("mfma_i8", P.PEAK_MFMA_I8, "TOP/s"),
+                             ("mfma_i8_full_entropy", P.PEAK_MFMA_I8_RANDOM, "TOP/s"),
                              ("valu_u64_muladd", P.PEAK_VALU_U64, "G/s")):
         best = 0.0
         for _ in range(3):
@@ -190,6 +193,7 @@ def measured_peaks(P):
         out[key] = {"value": best, "unit": unit}
     out["hbm_copy"]["datasheet"] = HBM_PEAK_GBS
     out["mfma_i8"]["datasheet"] = INT8_PEAK_TOPS
+    out["mfma_i8_full_entropy"]["datasheet"] = INT8_PEAK_TOPS
     log("measured peaks: " + ", ".join(f"{k} {v['value']:.0f} {v['unit']}" for k, v in out.items()))
     return out
 
@@ -251,8 +255,15 @@ def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True, m
         net.autotune()
     net.capture()
     t = time_graph(P, net, steps, warmup)
+    prof = net.profile_gemm()
+    again = net.profile_gemm()
+    prof = {v: (min(prof[v][0], again[v][0]),) + prof[v][1:] for v in prof}
+    dom = max(prof, key=lambda v: prof[v][0])
+    k_ms, k_ops, k_n = prof[dom]
     del net
     return {"images_per_s": batch / t, "ms_per_forward": t * 1e3,
+            "dominant_kernel": {"kernel": kernel_name(P, dom), "launches": k_n,
+                                "avg_launch_ms": k_ms / k_n, "achieved_tops": k_ops / (k_ms * 1e-3) / 1e12},
             "sharing": "uniformly random consistent RSS sharings of the same weights and inputs, dealt on the GPU",
             "precompute": "masked (Philox4x32-10)" if masked else "placeholder (the reference's constants)"}
 
@@ -274,7 +285,7 @@ def bench_uneven(P, C, steps, warmup, autotune=True, profile=False):
         prof = n.profile_gemm()
         log("per-layer ms: " + ", ".join(f"{l[0]}{i}:{m:.3f}" for i, (l, m) in enumerate(zip(layers, layer_ms))))
         out["layer_ms"] = [float(m) for m in layer_ms]
-        out["limb_gemm"] = {f"limb_gemm_kernel<{kernel_args(P, v)}>": {
+        out["limb_gemm"] = {kernel_name(P, v): {
             "ms": ms, "launches": k, "TOP/s": ops / ms / 1e9, "frac_of_peak": ops / ms / 1e9 / INT8_PEAK_TOPS}
             for v, (ms, ops, k) in sorted(prof.items())}
         log("limb GEMM: " + ", ".join(f"{k} {v['launches']}x {v['ms']:.3f} ms {v['TOP/s']:.0f} TOP/s"
@@ -299,7 +310,7 @@ def bench_training(P, C, layers, batch, fp, steps, warmup, autotune=True):
     net.capture_train()
     t = time_graph_fn(net.replay_train, net, steps, warmup)
     prof = net.profile_train_gemm()
-    kern = {f"limb_gemm_kernel<{kernel_args(P, v)}>": {"ms": ms, "launches": n, "TOP/s": ops / ms / 1e9}
+    kern = {kernel_name(P, v): {"ms": ms, "launches": n, "TOP/s": ops / ms / 1e9}
             for v, (ms, ops, n) in sorted(prof.items())}
     fwd_ms, sm_ms, bwd_ms = net.profile_train_layers()
     log("training step per layer (ms) forward: " + ", ".join(f"{l[0]}{i}:{m:.3f}" for i, (l, m) in
@@ -510,10 +521,10 @@ def main():
     k_ms, k_ops, k_n = prof[dom]
     avg_ms = k_ms / k_n
     achieved = k_ops / k_n / (avg_ms * 1e-3) / 1e12
-    log("limb GEMM kernels: " + ", ".join(f"<{kernel_args(P, v)}> {prof[v][2]} launches {prof[v][0]:.3f} ms "
+    log("limb GEMM kernels: " + ", ".join(f"{kernel_name(P, v)} {prof[v][2]} launches {prof[v][0]:.3f} ms "
                                           f"{prof[v][1] / prof[v][0] / 1e9:.0f} TOP/s" for v in sorted(prof)))
     log("per-layer ms: " + ", ".join(f"{layers[i][0]}{i}:{m:.3f}" for i, m in enumerate(layer_ms)))
-    kname = f"limb_gemm_kernel<{kernel_args(P, dom)}>"
+    kname = kernel_name(P, dom)
     roofline = {"bound": "mfma", "kernel": f"{kname} (3-party implicit-GEMM conv, int8 limbs)",
                 "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                 "traffic": read_traffic(kname), "avg_launch_ms": avg_ms, "launches_per_forward": k_n,
@@ -582,6 +593,10 @@ def main():
         other["alexnet_uneven_64q_fp20"] = bench_uneven(P, C, max(5, args.steps // 2), 20,
                                                         autotune=not args.no_autotune)
         rs = bench_random_shares(P, C, layers, batch, fp, args.steps, 20, autotune=not args.no_autotune)
+        dk = rs["dominant_kernel"]
+        dk["frac_of_datasheet"] = dk["achieved_tops"] / INT8_PEAK_TOPS
+        dk["frac_of_measured_peak"] = dk["achieved_tops"] / peaks["mfma_i8"]["value"]
+        dk["frac_of_measured_full_entropy_peak"] = dk["achieved_tops"] / peaks["mfma_i8_full_entropy"]["value"]
         other["vgg16_b64_fp26_random_sharings"] = rs
         log(f"VGG16 on random sharings: {rs['images_per_s']:.1f} images/s")
         rm = bench_random_shares(P, C, layers, batch, fp, args.steps, 20, autotune=not args.no_autotune, masked=True)

@@ -463,11 +463,14 @@ int pir_autotune_load(const char *path);
  *   PIR_PEAK_HBM_READ   GB/s of a read-only stream over `bytes` (writes nothing)
  *   PIR_PEAK_HBM_WRITE  GB/s of a write-only fill of `bytes`
  *   PIR_PEAK_MFMA_I8    int8 TOP/s of back-to-back v_mfma_i32_32x32x32_i8 (bytes ignored)
+ *   PIR_PEAK_MFMA_I8_RANDOM  the same on full-entropy operand bytes (the MFMA array draws more
+ *                       power on them and sustains a lower rate; the digit planes of uniformly
+ *                       random shares are such operands)
  *   PIR_PEAK_VALU_U64   G 64-bit multiply-adds per second on the vector ALUs (bytes ignored)
  * rate = the measured rate, ms = average launch time over `reps` launches after
  * one warm-up.  Allocates and frees its buffers; synchronises the host. */
 enum { PIR_PEAK_HBM_COPY = 0, PIR_PEAK_HBM_READ = 1, PIR_PEAK_HBM_WRITE = 2, PIR_PEAK_MFMA_I8 = 3,
-       PIR_PEAK_VALU_U64 = 4 };
+       PIR_PEAK_VALU_U64 = 4, PIR_PEAK_MFMA_I8_RANDOM = 5 };
 int pir_measure_peak(int which, size_t bytes, int reps, double *rate, double *ms, void *stream);
 
 #ifdef __cplusplus

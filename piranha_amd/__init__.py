@@ -256,7 +256,7 @@ def to_dev(a, stream=None):
     return Buffer.from_numpy(a, stream)
 
 
-PEAK_HBM_COPY, PEAK_HBM_READ, PEAK_HBM_WRITE, PEAK_MFMA_I8, PEAK_VALU_U64 = range(5)
+PEAK_HBM_COPY, PEAK_HBM_READ, PEAK_HBM_WRITE, PEAK_MFMA_I8, PEAK_VALU_U64, PEAK_MFMA_I8_RANDOM = range(6)
 
 
 def measure_peak(which, nbytes=1 << 30, reps=10):

@@ -47,9 +47,35 @@ __global__ __launch_bounds__(PEAK_BLOCK) void peak_write_kernel(u64x2 *__restric
 // accumulator uses of one register block are 8 instructions apart) and stores
 // one lane's accumulator sum at the end.
 constexpr int MFMA_CHAINS = 8;
+// RND = 0: operands with few set bits per byte (lane-derived small values);
+// RND = 1: full-entropy operand bytes, each of the 8 chains with its own pair.
+// The difference is the MFMA array's data-dependent power draw: full-entropy
+// operands, such as the digit planes of uniformly random shares, run at a
+// lower sustained rate.
+__device__ __forceinline__ unsigned peak_hash(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+template <int RND>
 __global__ __launch_bounds__(PEAK_BLOCK) void peak_mfma_kernel(int iters, int *__restrict__ out) {
     const int lane = threadIdx.x & 63;
-    v4i a = {lane, lane * 3 + 1, lane ^ 5, 7}, b = {lane + 2, lane * 5, 11, lane ^ 9};
+    v4i a[MFMA_CHAINS], b[MFMA_CHAINS];
+#pragma unroll
+    for (int c = 0; c < MFMA_CHAINS; c++) {
+        const unsigned h = ((unsigned)(blockIdx.x * PEAK_BLOCK + threadIdx.x) * MFMA_CHAINS + c) * 8u;
+        if (RND) {
+            a[c] = (v4i){(int)peak_hash(h), (int)peak_hash(h + 1), (int)peak_hash(h + 2), (int)peak_hash(h + 3)};
+            b[c] = (v4i){(int)peak_hash(h + 4), (int)peak_hash(h + 5), (int)peak_hash(h + 6), (int)peak_hash(h + 7)};
+        } else {
+            a[c] = (v4i){lane, lane * 3 + 1, lane ^ 5, 7};
+            b[c] = (v4i){lane + 2, lane * 5, 11, lane ^ 9};
+        }
+    }
     v16i acc[MFMA_CHAINS];
 #pragma unroll
     for (int c = 0; c < MFMA_CHAINS; c++)
@@ -57,7 +83,7 @@ __global__ __launch_bounds__(PEAK_BLOCK) void peak_mfma_kernel(int iters, int *_
         for (int r = 0; r < 16; r++) acc[c][r] = 0;
     for (int it = 0; it < iters; it++) {
 #pragma unroll
-        for (int c = 0; c < MFMA_CHAINS; c++) acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[c], 0, 0, 0);
+        for (int c = 0; c < MFMA_CHAINS; c++) acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[c], b[c], acc[c], 0, 0, 0);
     }
     int s = 0;
 #pragma unroll
@@ -91,7 +117,8 @@ constexpr unsigned COMPUTE_GRID = 256 * 8;
 int measure(int which, size_t bytes, int reps, double *rate, double *ms_out, hipStream_t s) {
     PIR_REQUIRE(reps >= 1 && rate && ms_out, "pir_measure_peak: reps >= 1 and result pointers needed");
     const bool hbm = which == PIR_PEAK_HBM_COPY || which == PIR_PEAK_HBM_READ || which == PIR_PEAK_HBM_WRITE;
-    PIR_REQUIRE(hbm || which == PIR_PEAK_MFMA_I8 || which == PIR_PEAK_VALU_U64, "pir_measure_peak: unknown peak");
+    const bool mfma = which == PIR_PEAK_MFMA_I8 || which == PIR_PEAK_MFMA_I8_RANDOM;
+    PIR_REQUIRE(hbm || mfma || which == PIR_PEAK_VALU_U64, "pir_measure_peak: unknown peak");
     PIR_REQUIRE(!hbm || (bytes >= 16 && bytes % 16 == 0 && bytes / 16 / PEAK_BLOCK < (1ull << 31)),
                 "pir_measure_peak: bytes must be a positive multiple of 16 below 8 TiB");
     void *a = nullptr, *b = nullptr;
@@ -132,7 +159,12 @@ int measure(int which, size_t bytes, int reps, double *rate, double *ms_out, hip
                                    0xfedcba9876543210ull);
                 break;
             case PIR_PEAK_MFMA_I8:
-                hipLaunchKernelGGL(peak_mfma_kernel, dim3(COMPUTE_GRID), dim3(PEAK_BLOCK), 0, s, mfma_iters, (int *)b);
+                hipLaunchKernelGGL(peak_mfma_kernel<0>, dim3(COMPUTE_GRID), dim3(PEAK_BLOCK), 0, s, mfma_iters,
+                                   (int *)b);
+                break;
+            case PIR_PEAK_MFMA_I8_RANDOM:
+                hipLaunchKernelGGL(peak_mfma_kernel<1>, dim3(COMPUTE_GRID), dim3(PEAK_BLOCK), 0, s, mfma_iters,
+                                   (int *)b);
                 break;
             default:
                 hipLaunchKernelGGL(peak_valu_kernel, dim3(COMPUTE_GRID), dim3(PEAK_BLOCK), 0, s, valu_iters,
@@ -160,6 +192,7 @@ int measure(int which, size_t bytes, int reps, double *rate, double *ms_out, hip
         case PIR_PEAK_HBM_READ:
         case PIR_PEAK_HBM_WRITE: r = (double)bytes / sec / 1e9; break;
         case PIR_PEAK_MFMA_I8:  // TOP/s: 2 * 32 * 32 * 32 ops per instruction
+        case PIR_PEAK_MFMA_I8_RANDOM:
             r = (double)COMPUTE_GRID * (PEAK_BLOCK / 64) * mfma_iters * MFMA_CHAINS * 65536.0 / sec / 1e12;
             break;
         default:  // G 64-bit multiply-adds per second

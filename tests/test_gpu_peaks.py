@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
     (P.PEAK_HBM_READ, 800.0, 16000.0),
     (P.PEAK_HBM_WRITE, 800.0, 16000.0),
     (P.PEAK_MFMA_I8, 500.0, 10000.0),      # TOP/s, datasheet 5000 dense
+    (P.PEAK_MFMA_I8_RANDOM, 500.0, 10000.0),
     (P.PEAK_VALU_U64, 100.0, 40000.0),     # G 64-bit multiply-adds / s
 ])
 def test_peak_in_range(which, lo, hi):

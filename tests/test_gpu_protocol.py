@@ -159,7 +159,10 @@ def test_matmul(rng, mode, kind, algo, ta, tb, tc):
 
 @pytest.mark.parametrize("algo", [P.ALGO_PLAIN, P.ALGO_LIMB])
 @pytest.mark.parametrize("b,H,W,f,Din,Dout,stride,pad", [(2, 3, 3, 3, 1, 1, 1, 1), (2, 9, 9, 3, 2, 3, 2, 1),
-                                                         (3, 8, 8, 3, 16, 20, 1, 1)])
+                                                         (3, 8, 8, 3, 16, 20, 1, 1),
+                                                         # Cin = 3 with paired 16-byte stores: 1024 rows x 64
+                                                         # columns; 90 rows (a partial tile) x 96 columns
+                                                         (4, 16, 16, 3, 3, 64, 1, 1), (3, 11, 9, 3, 3, 96, 2, 1)])
 def test_convolution(rng, mode, kind, algo, b, H, W, f, Din, Dout, stride, pad):
     a, w = make(rng, kind, b * H * W * Din), make(rng, kind, Dout * f * f * Din)
     OH = (H + 2 * pad - f) // stride + 1

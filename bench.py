@@ -268,10 +268,11 @@ def bench_random_shares(P, C, layers, batch, fp, steps, warmup, autotune=True, m
             "precompute": "masked (Philox4x32-10)" if masked else "placeholder (the reference's constants)"}
 
 
-def bench_uneven(P, C, steps, warmup, autotune=True, profile=False):
+def bench_uneven(P, C, steps, warmup, autotune=True, profile=False, per_query=False):
     """64 concurrent AlexNet-CIFAR queries with per-query batches drawn from
     1..32 (986 images), FP20: the queries concatenated along the batch axis
-    into one forward, replayed from its hipGraph."""
+    into one forward, replayed from its hipGraph.  per_query: also time the
+    ungrouped alternative, one forward (and graph) per query back to back."""
     qb = C.uneven_query_batches()
     total = int(sum(qb))
     layers = C.ALEXNET_CIFAR
@@ -291,6 +292,18 @@ def bench_uneven(P, C, steps, warmup, autotune=True, profile=False):
         log("limb GEMM: " + ", ".join(f"{k} {v['launches']}x {v['ms']:.3f} ms {v['TOP/s']:.0f} TOP/s"
                                       for k, v in out["limb_gemm"].items()))
     del n
+    if per_query:
+        # the ungrouped alternative: one forward (its own graph) per query, run back to back
+        t_of = {}
+        for b in sorted(set(qb)):
+            nb = make_net(P, C, layers, b, 20, P.MODE_FUSED, autotune=autotune)
+            t_of[b] = time_graph(P, nb, 5, 3)
+            del nb
+        seq = sum(t_of[b] for b in qb)
+        out["one_forward_per_query"] = {"images_per_s": total / seq, "ms_total": seq * 1e3,
+                                        "ms_by_batch": {str(b): t * 1e3 for b, t in t_of.items()}}
+        log(f"one forward per query, back to back: {seq * 1e3:.3f} ms, {total / seq:.1f} images/s "
+            f"(concatenated: {tt / seq:.3f} of that time)")
     return out
 
 
@@ -468,6 +481,8 @@ def main():
                     help="the VGG16 forward on uniformly random sharings only (for profiling runs)")
     ap.add_argument("--uneven-only", action="store_true",
                     help="the uneven-batch AlexNet workload only, with its per-layer and GEMM profile")
+    ap.add_argument("--per-query", action="store_true",
+                    help="with --uneven-only: also time one forward per query, back to back")
     ap.add_argument("--masked-only", action="store_true",
                     help="the VGG16 forward in the masked precompute mode only (for profiling runs)")
     args = ap.parse_args()
@@ -491,7 +506,8 @@ def main():
         return
 
     if args.uneven_only:
-        un = bench_uneven(P, C, args.steps, args.warmup, autotune=not args.no_autotune, profile=True)
+        un = bench_uneven(P, C, args.steps, args.warmup, autotune=not args.no_autotune, profile=True,
+                          per_query=args.per_query)
         print(json.dumps({"alexnet_uneven_64q_fp20": un}), flush=True)
         return
 

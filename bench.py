@@ -476,6 +476,9 @@ def main():
                     help="also run the read-only and write-only kernels of known size (counter calibration)")
     ap.add_argument("--quick", action="store_true", help="headline only (for profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="heuristic GEMM plans only")
+    ap.add_argument("--tune-file", default=None,
+                    help="GEMM plan table: loaded before the run when the file exists, saved after the headline "
+                         "network is tuned (profiling runs that must use one plan per shape)")
     ap.add_argument("--train-only", action="store_true", help="VGG16 training step only (for profiling runs)")
     ap.add_argument("--random-sharings-only", action="store_true",
                     help="the VGG16 forward on uniformly random sharings only (for profiling runs)")
@@ -497,18 +500,27 @@ def main():
 
     if args.calib:
         counter_calibration(P)
+    if args.tune_file and os.path.exists(args.tune_file):
+        P.autotune_load(args.tune_file)
+        log(f"GEMM plans loaded from {args.tune_file}: {P.autotune_entries()} shapes")
+
+    def save_plans():
+        if args.tune_file and not os.path.exists(args.tune_file):
+            P.autotune_save(args.tune_file)
 
     if args.train_only:
         tr = bench_training(P, C, C.VGG16_CIFAR, args.batch, 26, args.steps, args.warmup,
                             autotune=not args.no_autotune)
         log(f"VGG16 training: {tr['ms_per_step']:.2f} ms/step, {tr['images_per_s']:.1f} images/s")
         print(json.dumps({"training": tr}), flush=True)
+        save_plans()
         return
 
     if args.uneven_only:
         un = bench_uneven(P, C, args.steps, args.warmup, autotune=not args.no_autotune, profile=True,
                           per_query=args.per_query)
         print(json.dumps({"alexnet_uneven_64q_fp20": un}), flush=True)
+        save_plans()
         return
 
     if args.random_sharings_only or args.masked_only:
@@ -516,12 +528,14 @@ def main():
                                  autotune=not args.no_autotune, masked=args.masked_only)
         log(f"VGG16 on random sharings: {rs['ms_per_forward']:.3f} ms/forward, {rs['images_per_s']:.1f} images/s")
         print(json.dumps({"random_sharings": rs}), flush=True)
+        save_plans()
         return
 
     layers, batch, fp = C.VGG16_CIFAR, args.batch, 26
     log(f"VGG16-CIFAR b{batch}: building (fused limb-MFMA, hipGraph)")
     net = make_net(P, C, layers, batch, fp, P.MODE_FUSED, autotune=not args.no_autotune)
     log(f"autotuned GEMM shapes: {P.autotune_entries()}")
+    save_plans()
     t = time_graph(P, net, args.steps, args.warmup)
     images_s = batch / t
     log(f"VGG16 fused: {t * 1e3:.3f} ms/forward, {images_s:.1f} images/s")

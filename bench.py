@@ -120,13 +120,14 @@ def hbm_roofline(P, layers, batch, layer_ms):
     """Achieved HBM bandwidth of the fused elementwise / comparison kernels.
 
     relu_kernel: the standalone fused ReLU (pir_rss_relu) on the first VGG16
-    activation's size, 102 B per element for the three parties (two 8-byte
-    input and output shares each, two 1-byte derivative shares).  In the
-    network the ReLU layers run inside the GEMM epilogue or the maxpool kernel
-    before them.  maxpool: the maxpool layers (with the fused ReLU that follows
-    each of them) from per-layer HIP-event times, 264 B per 2x2 window (4 values
-    x 2 shares x 8 B read, 2 x 8 B written, 4 x 2 derivative bytes, three
-    parties) plus 54 B for the fused ReLU's output."""
+    activation's size, 94 B per element: 5 of the 6 input share arrays (party
+    B's second share never reaches the output), 6 output arrays and 6
+    derivative bytes.  In the network the ReLU layers run inside the GEMM
+    epilogue or the maxpool kernel before them.  maxpool: the maxpool layers
+    (with the fused ReLU that follows each of them) from per-layer HIP-event
+    times; a k x k window reads 6k^2 - (k^2 - 1) words (the selects never read
+    the later operands' party-B second share) and writes 6 output arrays and
+    6k^2 derivative bytes, plus 54 B for the fused ReLU's output."""
     out = {}
     L = P.lib()
     n = batch * 32 * 32 * 64
@@ -141,7 +142,7 @@ def hbm_roofline(P, layers, batch, layer_ms):
         P.check(L.pir_rss_relu(x.ptr, n, y.ptr, d.ptr, 1, None))
     P.synchronize()
     ms = (time.perf_counter() - t0) / reps * 1e3
-    gbs = 102.0 * n / (ms * 1e-3) / 1e9
+    gbs = 94.0 * n / (ms * 1e-3) / 1e9
     out["relu_kernel"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                           "elements": n, "ms": ms}
     del x, y, d
@@ -152,7 +153,7 @@ def hbm_roofline(P, layers, batch, layer_ms):
         H, W, C, pool, s = l[1:]
         windows = batch * ((H - pool) // s + 1) * ((W - pool) // s + 1) * C
         fused = i + 1 < len(layers) and layers[i + 1][0] == "relu"
-        byts += windows * (264.0 + (54.0 if fused else 0.0))
+        byts += windows * (8.0 * (5 * pool * pool + 1) + 48.0 + 6.0 * pool * pool + (54.0 if fused else 0.0))
         mp_ms += float(layer_ms[i])
     if mp_ms > 0:
         gbs = byts / (mp_ms * 1e-3) / 1e9

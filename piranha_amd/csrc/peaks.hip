@@ -4,7 +4,7 @@
 // 64-bit integer multiply-adds on the vector ALUs (the plain GEMM's
 // operation).  They are the measured denominators reported beside the
 // datasheet peaks.  The read-only and write-only kernels, of known size, also
-// calibrate the FETCH_SIZE / WRITE_SIZE hardware counters.
+// calibrate the hardware counters of HBM read and write requests.
 #include "../../include/pir.h"
 
 #include "common.h"
@@ -93,21 +93,27 @@ __global__ __launch_bounds__(PEAK_BLOCK) void peak_mfma_kernel(int iters, int *_
     out[(size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x] = s;
 }
 
-// `iters` rounds of 8 independent 64-bit multiply-adds x = x * m + c per lane
-// (the plain ring GEMM's mod-2^64 multiply-accumulate).
+// `iters` rounds of 8 independent 64-bit multiply-accumulates acc += a * b
+// per lane, the plain ring GEMM's mod-2^64 inner-product step (a per lane and
+// accumulator, b stepped by m each round so it is not a loop constant).
 constexpr int VALU_CHAINS = 8;
 __global__ __launch_bounds__(PEAK_BLOCK) void peak_valu_kernel(int iters, u64 m, u64 *__restrict__ out) {
     const u64 t = (size_t)blockIdx.x * PEAK_BLOCK + threadIdx.x;
-    u64 x[VALU_CHAINS];
+    u64 a[VALU_CHAINS], acc[VALU_CHAINS];
 #pragma unroll
-    for (int c = 0; c < VALU_CHAINS; c++) x[c] = t * 0x9e3779b97f4a7c15ull + (u64)c;
+    for (int c = 0; c < VALU_CHAINS; c++) {
+        a[c] = t * 0x9e3779b97f4a7c15ull + (u64)c;
+        acc[c] = 0;
+    }
+    u64 b = t ^ m;
     for (int it = 0; it < iters; it++) {
 #pragma unroll
-        for (int c = 0; c < VALU_CHAINS; c++) x[c] = x[c] * m + (u64)(c + 1);
+        for (int c = 0; c < VALU_CHAINS; c++) acc[c] += a[c] * b;
+        b += m;
     }
     u64 s = 0;
 #pragma unroll
-    for (int c = 0; c < VALU_CHAINS; c++) s ^= x[c];
+    for (int c = 0; c < VALU_CHAINS; c++) s ^= acc[c];
     out[t] = s;
 }
 

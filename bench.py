@@ -54,7 +54,7 @@ def kernel_name(P, v):
 
 def read_traffic(kname):
     """HBM bytes per launch of kernel `kname` (e.g. "limb_gemm_kernel<3, 1>") in
-    this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    this workload, from the committed rocprofv3 request-size counter summary
     (a line `traffic_bytes_per_launch[<kname>] = N`), or None when it is absent."""
     try:
         with open(TRAFFIC_FILE) as f:

@@ -710,16 +710,60 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     }
 }
 
-template <bool MASKED>
+// Sum of the split-K partials of output element(s) t (PAIRS: t and t + 1 as one
+// 16-byte load per partial); the partial loads are issued four at a time, so a
+// thread waits on memory once per four partials, not once per partial.
+template <bool PAIRS>
+__device__ __forceinline__ u64x2 sum_partials(const u64 *__restrict__ part, int split, size_t MN, size_t t) {
+    u64x2 acc = (u64x2){0, 0};
+    int z = 0;
+    for (; z + 4 <= split; z += 4) {
+        u64x2 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const u64 *p = part + (size_t)(z + q) * MN + t;
+            v[q] = PAIRS ? *(const u64x2 *)p : (u64x2){*p, 0};
+        }
+        acc += (v[0] + v[1]) + (v[2] + v[3]);
+    }
+    for (; z < split; z++) {
+        const u64 *p = part + (size_t)z * MN + t;
+        acc += PAIRS ? *(const u64x2 *)p : (u64x2){*p, 0};
+    }
+    return acc;
+}
+
+// The split-K reduction with the product's epilogue; PAIRS: each thread
+// takes two adjacent row-major outputs (Epilogue::pairs conditions, partials
+// 16-byte aligned).
+template <bool MASKED, bool PAIRS>
 __global__ __launch_bounds__(256) void split_reduce_kernel(const u64 *__restrict__ part, int split, int M, int N,
                                                            Epilogue epi) {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t MN = (size_t)M * N;
+    const size_t MN = (size_t)M * N;
+    const size_t t = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * (PAIRS ? 2 : 1);
     if (t >= MN) return;
-    u64 raw = 0;
-    for (int z = 0; z < split; z++) raw += part[(size_t)z * MN + t];
-    if (MASKED) trunc_store_m(epi, MN, N, (int)(t / N), (int)(t % N), raw);
-    else epilogue_store(epi, M, N, (int)(t / N), (int)(t % N), raw);
+    const u64x2 raw = sum_partials<PAIRS>(part, split, MN, t);
+    const int m = (int)(t / N), n = (int)(t % N);
+    if (MASKED) {
+        trunc_store_m(epi, MN, N, m, n, raw.x);
+        if (PAIRS) trunc_store_m(epi, MN, N, m, n + 1, raw.y);
+    } else if (PAIRS) {
+        epilogue_store2(epi, M, N, m, n, raw.x, raw.y);
+    } else {
+        epilogue_store(epi, M, N, m, n, raw.x);
+    }
+}
+
+// Whether an epilogue may store two adjacent columns per lane: row-major
+// output, N even, no subtraction, the 64-bit arrays 16-byte and the bit
+// arrays 2-byte aligned.
+static bool epilogue_pairs_ok(const Epilogue &e, int N) {
+    auto al = [](const void *q, uintptr_t b) { return ((uintptr_t)q % b) == 0; };
+    if (N % 2 != 0 || !e.tc) return false;
+    if (e.mode == EPI_RAW) return al(e.C, 16);
+    return !e.sub_out && al(e.out, 16) && (!e.bias || al(e.bias, 16)) &&
+           (!e.relu_out || (al(e.relu_out, 16) && al(e.relu_d, 2))) &&
+           (!e.bw_relu_out || (al(e.bw_relu_out, 16) && al(e.bw_relu_d, 2)));
 }
 
 int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &epi, hipStream_t s) {
@@ -729,10 +773,14 @@ int rss_trunc_epilogue(const u64 *raw, int M, int N, int nsum, const Epilogue &e
     PIR_REQUIRE(!masked || (epi.trunc_bits >= 1 && epi.nout <= 6 && !epi.div_bits && !epi.sub_out &&
                             !epi.bw_relu_out && (!epi.relu_out || epi.relu_prg.on)),
                 "masked precompute: the product epilogue supports the forward truncation only");
-    if (masked)
-        hipLaunchKernelGGL(split_reduce_kernel<true>, dim3(grid_for(MN, 256)), dim3(256), 0, s, raw, nsum, M, N, epi);
-    else
-        hipLaunchKernelGGL(split_reduce_kernel<false>, dim3(grid_for(MN, 256)), dim3(256), 0, s, raw, nsum, M, N, epi);
+    // pairs: the partials are row-major M x N, so a pair never crosses a row when N is even
+    const bool pairs = N % 2 == 0 && ((uintptr_t)raw % 16) == 0 && (masked || epilogue_pairs_ok(epi, N));
+    const size_t threads = pairs ? MN / 2 : MN;
+    const dim3 grid(grid_for(threads, 256));
+    if (masked && pairs) hipLaunchKernelGGL((split_reduce_kernel<true, true>), grid, dim3(256), 0, s, raw, nsum, M, N, epi);
+    else if (masked) hipLaunchKernelGGL((split_reduce_kernel<true, false>), grid, dim3(256), 0, s, raw, nsum, M, N, epi);
+    else if (pairs) hipLaunchKernelGGL((split_reduce_kernel<false, true>), grid, dim3(256), 0, s, raw, nsum, M, N, epi);
+    else hipLaunchKernelGGL((split_reduce_kernel<false, false>), grid, dim3(256), 0, s, raw, nsum, M, N, epi);
     PIR_LAUNCH_CHECK();
     return OK;
 }
@@ -893,17 +941,7 @@ int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &e
     }
     const int variant = amode + (wide ? LIMB_VARIANT_WIDE : 0);
     Epilogue ek = epi;
-    {
-        auto al = [](const void *q, uintptr_t b) { return ((uintptr_t)q % b) == 0; };
-        bool ok = N % 2 == 0;
-        if (part) ok = ok && al(part, 16);
-        else if (epi.mode == EPI_RAW) ok = ok && epi.tc && al(epi.C, 16);
-        else
-            ok = ok && epi.tc && !epi.sub_out && al(epi.out, 16) && (!epi.bias || al(epi.bias, 16)) &&
-                 (!epi.relu_out || (al(epi.relu_out, 16) && al(epi.relu_d, 2))) &&
-                 (!epi.bw_relu_out || (al(epi.bw_relu_out, 16) && al(epi.bw_relu_d, 2)));
-        ek.pairs = ok ? 1 : 0;
-    }
+    ek.pairs = (part ? N % 2 == 0 && ((uintptr_t)part % 16) == 0 : epilogue_pairs_ok(epi, N)) ? 1 : 0;
     GemmTimer &timer = ctx().timer;
     GemmTimer::Rec rec{nullptr, nullptr, 2.0 * 36 * a.M * (double)N * a.nseg * a.K, variant};
     if (timer.on) {

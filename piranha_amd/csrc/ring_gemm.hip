@@ -591,6 +591,9 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / (4 / WM), wn = wave % (4 / WM);
     const int m0 = bx * TM, n0 = by * TN + wn * 32;
+    // a wave whose 32 columns lie past N (N not a multiple of the tile width)
+    // still stages A and meets the barriers, but loads no B and issues no MFMA
+    const bool live = n0 < N;
     const int M = a.M;
     const int ks_begin = bz * steps_per_split;
     const int ks_end = min(nks, ks_begin + steps_per_split);
@@ -609,8 +612,9 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         L.init(ks_begin);
         L.load(w);
         const int8_t *bp = bw + (size_t)ks_begin * STEP;
+        if (live)
 #pragma unroll
-        for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+            for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
         stage_a<KPT>(As2[0], PLANE, arow, akq, w);
     }
     __syncthreads();
@@ -627,8 +631,9 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         v4i ap[8];
         if (PRE_A) {
             const int8_t *As = As2[cur];
+            if (live)
 #pragma unroll
-            for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+                for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
             __builtin_amdgcn_sched_barrier(0);
         }
         // prefetch the next K-step into registers (the last step re-reads itself)
@@ -636,11 +641,12 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         if (more) L.advance();
         L.load(w);
         const int8_t *bp = bw + (size_t)(more ? ks + 1 : ks) * STEP;
+        if (live)
 #pragma unroll
-        for (int j = 0; j < 8; j++) bn[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+            for (int j = 0; j < 8; j++) bn[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
 
         // acc[i+j] += A_i . B_j for i + j < 8
-        if (PRE_A) {
+        if (live && PRE_A) {
             __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int i = 0; i < 8; i++)
@@ -648,7 +654,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
                 for (int j = 0; j < 8 - i; j++)
                     acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bv[j], acc[i + j], 0, 0, 0);
             __builtin_amdgcn_s_setprio(0);
-        } else {  // A planes streamed from LDS one at a time
+        } else if (live) {  // A planes streamed from LDS one at a time
             const int8_t *As = As2[cur];
             v4i av = *(const v4i *)(As + a_off);
             __builtin_amdgcn_s_setprio(1);

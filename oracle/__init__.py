@@ -103,6 +103,7 @@ _SIGS = {
     "ref_ln_forward": [_u64p] * 6 + [_sz, _i, _i],
     "ref_ln_backward": [_u64p] * 6 + [_sz, _i, _i, _i],
     "ref_inv_sqrt_fixed": [_u64, _i],
+    "ref_calc_fn": [_i, _u64, _i],
     "ref_transpose_u64": [_u64p, _u64p, _i, _i],
     "ref_get_powers": [_u64p, _sz, _u64p],
     "ref_taylor": [_u64p, _sz, _u64p, _i, _i],
@@ -157,7 +158,7 @@ _SIGS.update({
 })
 
 _RET = {"ref_expanded_pool_size": _i, "ref_maxpool_im2row": _sz, "ref_averagepool_im2row": _sz,
-        "ref_inv_sqrt_fixed": _u64,
+        "ref_inv_sqrt_fixed": _u64, "ref_calc_fn": _u64,
         "ref_scalar_divide": _u64, "ref_version": _i}
 
 _lib = None
@@ -592,6 +593,12 @@ def taylor(x, fn, fp):
     out = np.zeros_like(x)
     lib().ref_taylor(_p(x), x.shape[-1], _p(out), fn, fp)
     return out
+
+
+def calc_fn(fn, x, fp):
+    """One entry of taylorSeries's coefficient table, calc_fn (functors.h:22-31):
+    fn 0 sqrt, 1 inverse, 2 sigmoid of the float 2^(x - fp), times 1 << x."""
+    return lib().ref_calc_fn(fn, int(x), fp)
 
 
 def inv_sqrt_fixed(x, fp):

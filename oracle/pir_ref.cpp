@@ -27,7 +27,13 @@
 //   * share invariants of the placeholder precompute (x1 = x2 = 0 except the
 //     party-B setPublic quirk).
 // Functions with no such check would be marked "parity unpinned" here; every
-// exported function below is covered by at least one of those checks.
+// exported function below is covered by at least one of those checks.  One
+// partial gap: the sigmoid coefficient table (calc_fn) calls the C library's
+// float exp where the reference calls its GPU's expf.  The documented 2-ulp
+// bound of expf fixes every entry but six (x = 13 at FP 20; x = 15, 19, 24,
+// 26, 27 at FP 26), where it leaves two adjacent values open: sigmoid parity
+// is unpinned for inputs whose table argument is one of those
+// (tests/test_oracle_sigmoid_pin.py).
 
 #include <cstdint>
 #include <cstddef>
@@ -2985,6 +2991,9 @@ void ref_ln_backward(const u64 *delta, u64 *xhat, const u64 *inv_sigma, u64 *gam
     ln_backward(wrapRs((u64 *)delta, n), wrapRs(xhat, n), wrapRs((u64 *)inv_sigma, (size_t)num_batches),
                 wrapRs(gamma, g), wrapRs(beta, g), wrapRs(deltas, n), num_batches, fp, lr);
 }
+// One entry of taylorSeries's coefficient table (calc_fn, functors.h:22-31).
+u64 ref_calc_fn(int fn, u64 x, int fp) { return calc_fn(fn, x, fp); }
+
 u64 ref_inv_sqrt_fixed(u64 x, int fp) { return inv_sqrt_fixed(x, fp); }
 void ref_get_powers(const u64 *in, size_t n, u64 *pow) {
     u64 *pp[3] = {pow, pow + n, pow + 2 * n};

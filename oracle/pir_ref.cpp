@@ -2444,8 +2444,13 @@ void convolution_m(Rs<u64> A, Rs<u64> B, Rs<u64> C, int batch, int H, int W, int
     divide_public_3out3_m(local.v(), trunc, C, g);
 }
 
-// dReLU (RSS.inl:806-852), masked: value i draws words j0 .. j0 + 28 of
-// element elem[i]; the 64 bits of value i are uint8 shares as in the reference.
+// Words per element of the masked dReLU and of a ReLU (dReLU + selectShare).
+constexpr uint32_t M_DRELU_WORDS = 10, M_RELU_WORDS = 18;
+
+// dReLU (RSS.inl:806-852), masked: value i draws words j0 .. j0 + 9 of
+// element elem[i] (0, 1: the g-ANDs' zero sharing, 2, 3: the p-ANDs', 4: R,
+// 5, 6: the shares of R - 1, 7, 8: the shares of -R); the 64 bits of value i
+// are uint8 shares as in the reference.
 template <typename U>
 void drelu_m(Rs<u64> input, Rs<U> result, const MPrg &g, const std::vector<u64> &elem, uint32_t j0) {
     const int bw = 64;
@@ -2454,13 +2459,13 @@ void drelu_m(Rs<u64> input, Rs<U> result, const MPrg &g, const std::vector<u64> 
     RsBuf<U> rbits(n * bw);
     for (size_t i = 0; i < n; i++) {
         u64 e = elem[i];
-        u64 R = mword(g, e, j0), s1 = mword(g, e, j0 + 1), s2 = mword(g, e, j0 + 2);
+        u64 R = mword(g, e, j0 + 4), s1 = mword(g, e, j0 + 5), s2 = mword(g, e, j0 + 6);
         u64 s[3] = {R - 1 - s1 - s2, s1, s2};
         for (int q = 0; q < 3; q++) {
             r.d[q][0][i] = s[q];
             r.d[q][1][i] = s[nextParty(q)];
         }
-        u64 b1 = mword(g, e, j0 + 3), b2 = mword(g, e, j0 + 4);
+        u64 b1 = mword(g, e, j0 + 7), b2 = mword(g, e, j0 + 8);
         u64 b[3] = {((u64)0 - R) ^ b1 ^ b2, b1, b2};
         for (int q = 0; q < 3; q++)
             for (int t = 0; t < bw; t++) {
@@ -2492,22 +2497,24 @@ void drelu_m(Rs<u64> input, Rs<U> result, const MPrg &g, const std::vector<u64> 
     RsBuf<U> pp(n * bw);
     r_op_r(pp.v(), rb, ADD);
     r_op_l(pp.v(), ab, XOR);
-    // carryOut (RSS.inl:1112-1209) with the level-L masks: bit (P % 64) of the
-    // words of value P / 64, P the view entry's position in the bit array
+    // carryOut (RSS.inl:1112-1209) with the level-L masks: bit (P % 64) + shift
+    // of the words of value P / 64, P the view entry's position in the bit
+    // array; the g-ANDs take words j0, j0 + 1 unshifted, the p-ANDs words
+    // j0 + 2, j0 + 3 shifted by 2^L - 1 (both bit sets disjoint across levels)
     Rs<U> p = pp.v(), gv = gg.v();
     int stride = 2, offset = 1, L = 0, k = bw;
     Rs<U> pEven = stridedRs(p, 0, stride), pOdd = stridedRs(p, offset, stride);
     Rs<U> gEven = stridedRs(gv, 0, stride), gOdd = stridedRs(gv, offset, stride);
-    auto zs = [&](const Rs<U> &view, uint32_t j) {
+    auto zs = [&](const Rs<U> &view, uint32_t j, int shift) {
         ZeroSharing<U> z{g, j0 + j, true, nullptr, nullptr};
         const size_t off = view.s[0][0].off, st = view.s[0][0].stride;
         z.elem = [&elem, off, st, bw](size_t kk) { return elem[(off + kk * st) / bw]; };
-        z.bit = [off, st, bw](size_t kk) { return (int)((off + kk * st) % bw); };
+        z.bit = [off, st, bw, shift](size_t kk) { return (int)((off + kk * st) % bw) + shift; };
         return z;
     };
     while (k > 1) {
-        r_and_m(gEven, pOdd, zs(gEven, 5 + 4 * L));
-        r_and_m(pEven, pOdd, zs(pEven, 7 + 4 * L));
+        r_and_m(gEven, pOdd, zs(gEven, 0, 0));
+        r_and_m(pEven, pOdd, zs(pEven, 2, (1 << L) - 1));
         r_op_r(gOdd, gEven, XOR);
         gEven = stridedRs(gv, offset, stride * 2);
         offset += stride;
@@ -2533,7 +2540,9 @@ void drelu_m(Rs<u64> input, Rs<U> result, const MPrg &g, const std::vector<u64> 
             }
 }
 
-// selectShare (RSS.inl:542-569), masked: value i draws words j0 .. j0 + 6 of elem[i]
+// selectShare (RSS.inl:542-569), masked: value i draws words j0 .. j0 + 7 of
+// elem[i] (0, 1: the product's reshare, 2: the bit c, 3, 4: its binary shares,
+// 5, 6: its arithmetic shares)
 template <typename U>
 void select_share_m(Rs<u64> x, Rs<u64> y, Rs<U> b, Rs<u64> z, const MPrg &g, const std::vector<u64> &elem,
                     uint32_t j0) {
@@ -2542,9 +2551,9 @@ void select_share_m(Rs<u64> x, Rs<u64> y, Rs<U> b, Rs<u64> z, const MPrg &g, con
     RsBuf<U> cbits(n);
     for (size_t i = 0; i < n; i++) {
         u64 e = elem[i];
-        U cb = (U)(mword(g, e, j0) & 1), cb1 = (U)(mword(g, e, j0 + 1) & 1), cb2 = (U)(mword(g, e, j0 + 2) & 1);
+        U cb = (U)(mword(g, e, j0 + 2) & 1), cb1 = (U)(mword(g, e, j0 + 3) & 1), cb2 = (U)(mword(g, e, j0 + 4) & 1);
         U cbs[3] = {(U)(cb ^ cb1 ^ cb2), cb1, cb2};
-        u64 c1 = mword(g, e, j0 + 3), c2 = mword(g, e, j0 + 4);
+        u64 c1 = mword(g, e, j0 + 5), c2 = mword(g, e, j0 + 6);
         u64 cs[3] = {(u64)cb - c1 - c2, c1, c2};
         for (int q = 0; q < 3; q++) {
             cbits.d[q][0][i] = cbs[q];
@@ -2563,7 +2572,7 @@ void select_share_m(Rs<u64> x, Rs<u64> y, Rs<U> b, Rs<u64> z, const MPrg &g, con
     RsBuf<u64> result(n);
     r_op_r(result.v(), y, ADD);
     r_op_r(result.v(), x, SUB);
-    ZeroSharing<u64> zsh{g, j0 + 5, false, [&elem](size_t k) { return elem[k]; }, nullptr};
+    ZeroSharing<u64> zsh{g, j0, false, [&elem](size_t k) { return elem[k]; }, nullptr};
     r_mult_m(result.v(), d.v(), zsh);
     r_op_r(result.v(), x, ADD);
     r_zero(z);
@@ -2576,12 +2585,13 @@ void relu_m(Rs<u64> input, Rs<u64> result, Rs<U> dresult, const MPrg &g) {
     for (size_t i = 0; i < elem.size(); i++) elem[i] = i;
     drelu_m(input, dresult, g, elem, 0);
     RsBuf<u64> zeros(input.size());
-    select_share_m(zeros.v(), input, dresult, result, g, elem, 29);
+    select_share_m(zeros.v(), input, dresult, result, g, elem, M_DRELU_WORDS);
 }
 
 // maxpool (RSS.inl:867-1017), masked: compare m of a window (in round order)
-// draws words 36m .. 36m + 35, the AND of round t at pool position pos
-// 36(k-1) + 2(tk + pos) and + 1; 1 - b becomes b ^ 1 in the x0 slot.
+// draws words 18m .. 18m + 17 (dReLU, then selectShare), the AND of round t at
+// pool position pos 18(k-1) + 2(tk + pos) and + 1; 1 - b becomes b ^ 1 in the
+// x0 slot.
 template <typename U>
 void maxpool_m(Rs<u64> input, Rs<u64> result, Rs<U> dresult, int k, const MPrg &g) {
     const size_t n = input.size();
@@ -2622,8 +2632,8 @@ void maxpool_m(Rs<u64> input, Rs<u64> result, Rs<U> dresult, int k, const MPrg &
                     }
             }
             const uint32_t m = m_base + j;
-            drelu_m(dsub.v(), bsub.v(), g, el, 36 * m);
-            select_share_m(od.v(), ev.v(), bsub.v(), out.v(), g, el, 36 * m + 29);
+            drelu_m(dsub.v(), bsub.v(), g, el, M_RELU_WORDS * m);
+            select_share_m(od.v(), ev.v(), bsub.v(), out.v(), g, el, M_RELU_WORDS * m + M_DRELU_WORDS);
             for (size_t a = 0; a < idx.size(); a++)
                 for (int q = 0; q < 3; q++)
                     for (int s = 0; s < 2; s++) {
@@ -2651,14 +2661,14 @@ void maxpool_m(Rs<u64> input, Rs<u64> result, Rs<U> dresult, int k, const MPrg &
             for (int s = 0; s < 2; s++)
                 expand_compare(b.v().s[q][s], negated.v().s[q][s], expandedB.v().s[q][s]);
         const int tr = t;
-        // dresult &= expandedB with the zero sharing of words 36(k-1) + 2(tk + pos)
+        // dresult &= expandedB with the zero sharing of words 18(k-1) + 2(tk + pos)
         LocBuf<U> cz(n);
         for (int q = 0; q < 3; q++)
             for (size_t P = 0; P < n; P++)
                 cz.d[q][P] = (U)((dresult.s[q][0][P] & (U)(expandedB.d[q][0][P] ^ expandedB.d[q][1][P])) ^
                                  (dresult.s[q][1][P] & expandedB.d[q][0][P]));
         for (size_t P = 0; P < n; P++) {
-            uint32_t j = 36 * (uint32_t)(K - 1) + 2 * (uint32_t)(tr * K + (int)(P % K));
+            uint32_t j = M_RELU_WORDS * (uint32_t)(K - 1) + 2 * (uint32_t)(tr * K + (int)(P % K));
             u64 e = P / K;
             U a[3];
             a[0] = (U)(mword(g, e, j) & 1);

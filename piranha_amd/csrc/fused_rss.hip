@@ -238,9 +238,9 @@ __device__ __forceinline__ void tournament(Sh3<u64> (&e)[K], Sh3<U> (&d)[K]) {
 }
 
 // The masked form of the tournament (rss_masked.h) for window `win`: compare m
-// (in round order) draws words 36m .. 36m + 35 (its dReLU, then its
+// (in round order) draws words 18m .. 18m + 17 (its dReLU, then its
 // selectShare); the AND of round t at pool position pos draws words
-// 36(K-1) + 2(tK + pos) and + 1.  1 - b becomes b ^ 1 in the x0 slot.
+// 18(K-1) + 2(tK + pos) and + 1.  1 - b becomes b ^ 1 in the x0 slot.
 template <typename U, int K>
 __device__ __forceinline__ void tournament_m(Sh3<u64> (&e)[K], Sh3<U> (&d)[K], const PrgStream &g, u64 win) {
 #pragma unroll

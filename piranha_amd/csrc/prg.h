@@ -68,12 +68,14 @@ __host__ __device__ __forceinline__ u64 prg_word(const PrgStream &g, u64 e, u32 
 }
 
 // Word indices of the masked protocols for one element (all relative to the
-// element's base index within its call):
+// element's base index within its call).  Blocks of the dReLU / selectShare /
+// ReLU / maxpool layouts start at even words, so each Philox block drawn
+// yields two used words:
 constexpr u32 PRG_RESHARE = 0;       // 2 words: zero-sharing alpha_0, alpha_1 (alpha_2 closes the sum)
 constexpr u32 PRG_DIV = 0;           // 5 words: R, r_1, r_2, r'_a, r'_b of the truncation pair
-constexpr u32 PRG_DRELU = 0;         // 29 words: R, 2 arithmetic and 2 binary share words, 24 carry-tree masks
-constexpr u32 PRG_DRELU_WORDS = 29;
-constexpr u32 PRG_SELECT_WORDS = 7;  // bit, 2 binary and 2 arithmetic share words, 2 reshare masks
+constexpr u32 PRG_DRELU = 0;         // 10 words: 2 + 2 carry-tree masks, R, 2 arithmetic, 2 binary share words, 1 spare
+constexpr u32 PRG_DRELU_WORDS = 10;
+constexpr u32 PRG_SELECT_WORDS = 8;  // 2 reshare masks, bit, 2 binary and 2 arithmetic share words, 1 spare
 constexpr u32 PRG_RELU_WORDS = PRG_DRELU_WORDS + PRG_SELECT_WORDS;
 
 // Host launchers (prg.hip).  prg_philox: out[4i..4i+3] = Philox4x32-10 of

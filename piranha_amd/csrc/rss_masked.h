@@ -17,11 +17,14 @@
 // Word layout per element (prg.h): reshare / mult / AND: words 0, 1 (zero
 // sharings alpha_0, alpha_1, alpha_2 = -alpha_0 - alpha_1 or alpha_0 ^ alpha_1);
 // truncation: 0 = R (sign-extended to |R * 2^k| < 2^61), 1, 2 = r_1, r_2 with
-// r_0 = R - r_1 - r_2, 3, 4 = r'_1, r'_2 of the op-by-op form; dReLU: 0 = R,
-// 1, 2 = arithmetic shares of R - 1, 3, 4 = binary shares of -R, 5 + 4L ..
-// 8 + 4L = the zero sharings of the two ANDs of carry level L; selectShare:
-// 0 = the random bit c, 1, 2 = its binary shares, 3, 4 = its arithmetic
-// shares, 5, 6 = the product's reshare.
+// r_0 = R - r_1 - r_2, 3, 4 = r'_1, r'_2 of the op-by-op form; dReLU: 0, 1 =
+// the zero sharing of the carry tree's g-ANDs, 2, 3 = that of its p-ANDs (the
+// bit positions each AND writes are its own: level L's g-AND takes the words'
+// bits at its result positions, which no other level uses, and level L's
+// p-AND the bits 2^L - 1 above its result positions, again disjoint across
+// levels), 4 = R, 5, 6 = arithmetic shares of R - 1, 7, 8 = binary shares of
+// -R; selectShare: 0, 1 = the product's reshare, 2 = the random bit c, 3, 4 =
+// its binary shares, 5, 6 = its arithmetic shares.
 #pragma once
 
 #include "prg.h"
@@ -78,12 +81,10 @@ __device__ __forceinline__ void rss_and_m(Sh3<T> &x, const Sh3<T> &y, const PrgS
 }
 
 // One AND of the carry tree on bit-packed words restricted to `mask` (see
-// rss_and_masked), with the binary zero sharing of words j, j + 1 on the
+// rss_and_masked), with the binary zero sharing (w0, w1, w0 ^ w1) on the
 // positions the view covers.
 __device__ __forceinline__ void rss_and_masked_m(u64 xa[3], u64 xb[3], const u64 ya[3], const u64 yb[3], u64 mask,
-                                                 const PrgStream &g, u64 e, u32 j) {
-    u64 w0, w1;
-    prg_two(g, e, j, w0, w1);
+                                                 u64 w0, u64 w1) {
     const u64 a[3] = {w0, w1, w0 ^ w1};
     u64 z[3];
 #pragma unroll
@@ -97,13 +98,15 @@ __device__ __forceinline__ void rss_and_masked_m(u64 xa[3], u64 xb[3], const u64
 
 // dReLU (RSS.inl:806-852) with a random mask: r = arithmetic sharing of R - 1,
 // rbits = binary sharing of -R, so a = open(input + r) + 1 = x + R and the
-// carry circuit adds a + (-R) = x; words j .. j + 28.
+// carry circuit adds a + (-R) = x; words j .. j + 9 (j even).
 template <typename U>
 __device__ __forceinline__ Sh3<U> drelu_m(const Sh3<u64> &in, const PrgStream &g, u64 e, u32 j) {
-    u64 R, s1, s2, b1, b2;
-    prg_two(g, e, j + 0, R, s1);
-    prg_two(g, e, j + 2, s2, b1);
-    b2 = prg_word(g, e, j + 4);
+    u64 g0, g1, p0, p1, R, s1, s2, b1, b2;
+    prg_two(g, e, j + 0, g0, g1);
+    prg_two(g, e, j + 2, p0, p1);
+    prg_two(g, e, j + 4, R, s1);
+    prg_two(g, e, j + 6, s2, b1);
+    b2 = prg_word(g, e, j + 8);
     const u64 s[3] = {R - 1 - s1 - s2, s1, s2};
     u64 a[3];
 #pragma unroll
@@ -148,13 +151,13 @@ __device__ __forceinline__ Sh3<U> drelu_m(const Sh3<u64> &in, const PrgStream &g
             ya[p] = pa[p] >> 1;
             yb[p] = pb[p] >> 1;
         }
-        rss_and_masked_m(ga, gb, ya, yb, mge, g, e, j + 5 + 4 * L);
+        rss_and_masked_m(ga, gb, ya, yb, mge, g0, g1);
 #pragma unroll
         for (int p = 0; p < 3; p++) {
             ya[p] = pa[p] >> h;
             yb[p] = pb[p] >> h;
         }
-        rss_and_masked_m(pa, pb, ya, yb, mpe, g, e, j + 7 + 4 * L);
+        rss_and_masked_m(pa, pb, ya, yb, mpe, p0 >> (h - 1), p1 >> (h - 1));
 #pragma unroll
         for (int p = 0; p < 3; p++) {
             ga[p] ^= (ga[p] & mge) << h;
@@ -173,14 +176,14 @@ __device__ __forceinline__ Sh3<U> drelu_m(const Sh3<u64> &in, const PrgStream &g
 }
 
 // selectShare (RSS.inl:542-569) with a random bit c (binary shares cbits,
-// arithmetic shares c) and b ^ cbits opened by XOR; words j .. j + 6.
+// arithmetic shares c) and b ^ cbits opened by XOR; words j .. j + 7 (j even).
 template <typename U>
 __device__ __forceinline__ Sh3<u64> select_m(const Sh3<u64> &x, const Sh3<u64> &y, const Sh3<U> &b,
                                              const PrgStream &g, u64 e, u32 j) {
     u64 wc, wb1, wb2, wc1, wc2;
-    prg_two(g, e, j + 0, wc, wb1);
-    prg_two(g, e, j + 2, wb2, wc1);
-    wc2 = prg_word(g, e, j + 4);
+    prg_two(g, e, j + 2, wc, wb1);
+    prg_two(g, e, j + 4, wb2, wc1);
+    wc2 = prg_word(g, e, j + 6);
     const U cb = (U)(wc & 1), cb1 = (U)(wb1 & 1), cb2 = (U)(wb2 & 1);
     const U cbits[3] = {(U)(cb ^ cb1 ^ cb2), cb1, cb2};
     const u64 c[3] = {(u64)cb - wc1 - wc2, wc1, wc2};
@@ -200,7 +203,7 @@ __device__ __forceinline__ Sh3<u64> select_m(const Sh3<u64> &x, const Sh3<u64> &
         r.s[p][0] = y.s[p][0] - x.s[p][0];
         r.s[p][1] = y.s[p][1] - x.s[p][1];
     }
-    rss_mult_m(r, d, g, e, j + 5);
+    rss_mult_m(r, d, g, e, j + 0);
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         r.s[p][0] += x.s[p][0];
@@ -209,7 +212,7 @@ __device__ __forceinline__ Sh3<u64> select_m(const Sh3<u64> &x, const Sh3<u64> &
     return r;
 }
 
-// ReLU (RSS.inl:854-865): dReLU on words 0..28, selectShare(0, x, b) on 29..35.
+// ReLU (RSS.inl:854-865): dReLU on words 0..9, selectShare(0, x, b) on 10..17.
 template <typename U>
 __device__ __forceinline__ Sh3<u64> relu_m(const Sh3<u64> &in, Sh3<U> &b, const PrgStream &g, u64 e) {
     b = drelu_m<U>(in, g, e, 0);

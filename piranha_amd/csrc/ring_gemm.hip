@@ -485,33 +485,38 @@ struct ALoader {
             }
         } else if (AMODE == 6) {
             // k = (n, p, q): decode the first of the thread's KPT pixels, then
-            // step along the output row (a new row only when q wraps)
+            // step along the output row with additions only: the element
+            // offset is rowoff (row ih of image n, channel c) + iw * Cin, and a
+            // new output row or image moves rowoff by a fixed step
             const int ohw = a.OH * a.OW;
             int n = (int)a.fd_ohw.div((u32)kb);
             int rem = kb - n * ohw;
             int p = (int)a.fd_ow.div((u32)rem);
             int q = rem - p * a.OW;
             int ih = p * a.stride - a.ph + r, iw = q * a.stride - a.pw + s;
-            // rowp addresses row ih of image n only when both exist (n < batch
-            // <=> n * OH * OW < K); otherwise it stays at element c0 of the tensor
-            bool hok = ih >= 0 && ih < a.H && n * ohw < a.K;
-            const u64 *rowp = src + (hok ? (((size_t)n * a.H + ih) * a.W) * a.Cin : 0) + c0;
+            long long rowoff = (((long long)n * a.H + ih) * a.W) * a.Cin + c0;  // used only when in range
+            int off = iw * a.Cin;
+            const int col_step = a.stride * a.Cin;
+            const long long row_step = (long long)a.stride * a.W * a.Cin;
+            const long long img_step = (long long)a.H * a.W * a.Cin - (long long)a.OH * row_step;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
-                bool ok = row_ok && kb + j < a.K && hok && iw >= 0 && iw < a.W;
-                w[j] = rowp[ok ? (size_t)iw * a.Cin : 0] & (ok ? ~0ull : 0ull);
+                // kb + j < K keeps n < batch
+                const bool ok = row_ok && kb + j < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
+                w[j] = src[ok ? rowoff + off : 0] & (ok ? ~0ull : 0ull);
                 iw += a.stride;
+                off += col_step;
                 if (++q == a.OW) {
                     q = 0;
                     iw = s - a.pw;
+                    off = iw * a.Cin;
                     ih += a.stride;
+                    rowoff += row_step;
                     if (++p == a.OH) {
                         p = 0;
-                        ++n;
                         ih = r - a.ph;
+                        rowoff += img_step;
                     }
-                    hok = ih >= 0 && ih < a.H && n * ohw < a.K;
-                    rowp = src + (hok ? (((size_t)n * a.H + ih) * a.W) * a.Cin : 0) + c0;
                 }
             }
         } else if (AMODE == 0 && (a.K % KPT) == 0) {

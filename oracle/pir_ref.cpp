@@ -414,20 +414,43 @@ void divide_public_3out3(Loc<T> a, T d, Rs<T> out) {
 // gpu::gemm: transpose_x == true means row-major storage of that operand
 // (gemm.cuh:67-98 maps it to cutlass RowMajor with ld = K / N / N).
 void gemm(int M, int N, int K, const u64 *A, bool ta, const u64 *B, bool tb, u64 *C, bool tc) {
-    std::vector<u64> At((size_t)M * K), Bt((size_t)N * K);
-    for (int m = 0; m < M; m++)
-        for (int k = 0; k < K; k++) At[(size_t)m * K + k] = ta ? A[(size_t)m * K + k] : A[(size_t)k * M + m];
-    for (int n = 0; n < N; n++)
-        for (int k = 0; k < K; k++) Bt[(size_t)n * K + k] = tb ? B[(size_t)k * N + n] : B[(size_t)n * K + k];
-#pragma omp parallel for schedule(static)
-    for (int m = 0; m < M; m++) {
-        const u64 *a = &At[(size_t)m * K];
-        for (int n = 0; n < N; n++) {
-            const u64 *b = &Bt[(size_t)n * K];
-            u64 acc = 0;
-            for (int k = 0; k < K; k++) acc += a[k] * b[k];
-            if (tc) C[(size_t)m * N + n] = acc;
-            else C[(size_t)n * M + m] = acc;
+    // rows of A and columns of B as contiguous K-vectors (copied only when the
+    // operand is stored the other way; the copies run in blocks over all cores)
+    std::vector<u64> At, Bt;
+    const u64 *Ar = A, *Bc = B;
+    const int TB = 64;
+    if (!ta) {
+        At.resize((size_t)M * K);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int m0 = 0; m0 < M; m0 += TB)
+            for (int k0 = 0; k0 < K; k0 += TB)
+                for (int k = k0; k < std::min(K, k0 + TB); k++)
+                    for (int m = m0; m < std::min(M, m0 + TB); m++) At[(size_t)m * K + k] = A[(size_t)k * M + m];
+        Ar = At.data();
+    }
+    if (tb) {
+        Bt.resize((size_t)N * K);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int n0 = 0; n0 < N; n0 += TB)
+            for (int k0 = 0; k0 < K; k0 += TB)
+                for (int k = k0; k < std::min(K, k0 + TB); k++)
+                    for (int n = n0; n < std::min(N, n0 + TB); n++) Bt[(size_t)n * K + k] = B[(size_t)k * N + n];
+        Bc = Bt.data();
+    }
+    // a block of NB columns stays in cache while every row of A passes over it
+    const int NB = 32;
+#pragma omp parallel for schedule(dynamic)
+    for (int n0 = 0; n0 < N; n0 += NB) {
+        const int n1 = std::min(N, n0 + NB);
+        for (int m = 0; m < M; m++) {
+            const u64 *a = &Ar[(size_t)m * K];
+            for (int n = n0; n < n1; n++) {
+                const u64 *b = &Bc[(size_t)n * K];
+                u64 acc = 0;
+                for (int k = 0; k < K; k++) acc += a[k] * b[k];
+                if (tc) C[(size_t)m * N + n] = acc;
+                else C[(size_t)n * M + m] = acc;
+            }
         }
     }
 }

@@ -435,35 +435,41 @@ struct ALoader {
         }
         return ok && p < a.OH && q < a.OW;
     }
-    __device__ void load(u64 w[KPT]) const {
+    // Loads the thread's KPT words of the current K-step into w (raw: an
+    // out-of-range element reads a valid clamped address) and sets bit e of
+    // ok for each element inside the operand.  The mask is applied when the
+    // words are staged, after the MFMAs, so no wait on these loads precedes them.
+    __device__ void load(u64 w[KPT], u32 &ok_bits) const {
         const int kb = kk * LIMB_BK + akq * KPT;
         if (AMODE == 3) {
             int ih = ih0 + r, iw = iw0 + s;
             bool ok = row_ok && kb < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
             const u64 *p = src + (((size_t)cb * a.H + (ok ? ih : 0)) * a.W + (ok ? iw : 0)) * a.Cin +
                            (ok ? c0 + akq * KPT : 0);
-            const u64 msk = ok ? ~0ull : 0ull;
+            ok_bits = ok ? (1u << KPT) - 1 : 0u;
 #pragma unroll
-            for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
+            for (int j = 0; j < KPT; j++) w[j] = p[j];
         } else if (AMODE == 5) {
             int pp, qq;
             bool ok = dgrad_src(r, s, pp, qq) && row_ok && kb < a.K;
             const u64 *p = src + (((size_t)cb * a.OH + (ok ? pp : 0)) * a.OW + (ok ? qq : 0)) * a.Cin +
                            (ok ? c0 + akq * KPT : 0);
-            const u64 msk = ok ? ~0ull : 0ull;
+            ok_bits = ok ? (1u << KPT) - 1 : 0u;
 #pragma unroll
-            for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
+            for (int j = 0; j < KPT; j++) w[j] = p[j];
         } else if (AMODE == 2) {
             // decode the thread's first k once, then step along the filter row
             const int fwc = a.fw * a.Cin, wc = a.W * a.Cin;
             int rr = (int)a.fd_fwc.div((u32)kb);
             int j = kb - rr * fwc;
             int d = rr * wc + j;
+            ok_bits = 0;
 #pragma unroll
             for (int e = 0; e < KPT; e++) {
                 // rr < rhi <= fh also keeps k < K (the segment's last k is fh*fwc - 1)
                 const bool ok = rr >= rlo && rr < rhi && j >= jlo && j < jhi;
-                w[e] = src[ok ? pix + d : 0] & (ok ? ~0ull : 0ull);
+                w[e] = src[ok ? pix + d : 0];
+                ok_bits |= ok ? 1u << e : 0u;
                 const bool wrap = ++j == fwc;
                 j = wrap ? 0 : j;
                 rr += wrap ? 1 : 0;
@@ -471,6 +477,7 @@ struct ALoader {
             }
         } else if (AMODE == 4) {
             const int fwc = a.fw * a.Cin;
+            ok_bits = 0;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
                 int k = kb + j;
@@ -481,7 +488,8 @@ struct ALoader {
                 int pp, qq;
                 const bool ok = row_ok && k < a.K && dgrad_src(rr, ss, pp, qq);
                 const size_t idx = ok ? (((size_t)cb * a.OH + pp) * a.OW + qq) * a.Cin + cc : 0;
-                w[j] = src[idx] & (ok ? ~0ull : 0ull);
+                w[j] = src[idx];
+                ok_bits |= ok ? 1u << j : 0u;
             }
         } else if (AMODE == 6) {
             // k = (n, p, q): decode the first of the thread's KPT pixels, then
@@ -499,11 +507,13 @@ struct ALoader {
             const int col_step = a.stride * a.Cin;
             const long long row_step = (long long)a.stride * a.W * a.Cin;
             const long long img_step = (long long)a.H * a.W * a.Cin - (long long)a.OH * row_step;
+            ok_bits = 0;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
                 // kb + j < K keeps n < batch
                 const bool ok = row_ok && kb + j < a.K && ih >= 0 && ih < a.H && iw >= 0 && iw < a.W;
-                w[j] = src[ok ? rowoff + off : 0] & (ok ? ~0ull : 0ull);
+                w[j] = src[ok ? rowoff + off : 0];
+                ok_bits |= ok ? 1u << j : 0u;
                 iw += a.stride;
                 off += col_step;
                 if (++q == a.OW) {
@@ -523,30 +533,33 @@ struct ALoader {
             // a thread's KPT consecutive k are all inside K or all outside: one contiguous run
             bool ok = row_ok && kb < a.K;
             const u64 *p = src + (size_t)(row_ok ? am : 0) * a.lda + (ok ? kb : 0);
-            const u64 msk = ok ? ~0ull : 0ull;
+            ok_bits = ok ? (1u << KPT) - 1 : 0u;
 #pragma unroll
-            for (int j = 0; j < KPT; j++) w[j] = p[j] & msk;
+            for (int j = 0; j < KPT; j++) w[j] = p[j];
         } else {
             const int m = row_ok ? am : 0;
+            ok_bits = 0;
 #pragma unroll
             for (int j = 0; j < KPT; j++) {
                 int k = kb + j;
                 bool ok = row_ok && k < a.K;
                 int kc = ok ? k : 0;
                 size_t idx = AMODE == 0 ? (size_t)m * a.lda + kc : (size_t)kc * a.lda + m;
-                w[j] = src[idx] & (ok ? ~0ull : 0ull);
+                w[j] = src[idx];
+                ok_bits |= ok ? 1u << j : 0u;
             }
         }
     }
 };
 
-// Recode KPT consecutive k of one row and store limb plane i's bytes for them
-// at As + i * plane + arow * LDS_ROW + akq * KPT.
+// Recode KPT consecutive k of one row (element j zero unless bit j of ok_bits
+// is set) and store limb plane i's bytes for them at
+// As + i * plane + arow * LDS_ROW + akq * KPT.
 template <int KPT>
-__device__ __forceinline__ void stage_a(int8_t *As, int plane, int arow, int akq, const u64 w[KPT]) {
+__device__ __forceinline__ void stage_a(int8_t *As, int plane, int arow, int akq, const u64 w[KPT], u32 ok_bits) {
     u64 d[KPT];
 #pragma unroll
-    for (int j = 0; j < KPT; j++) d[j] = recode_digits(w[j]);
+    for (int j = 0; j < KPT; j++) d[j] = recode_digits(((ok_bits >> j) & 1u) ? w[j] : 0ull);
     int8_t *dst = As + arow * LDS_ROW + akq * KPT;
     if (KPT == 8) {
         u64 limbs[8];
@@ -612,15 +625,16 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     const int8_t *bw = Bp + (size_t)(n0 / LIMB_BN) * nks * STEP + ((n0 % LIMB_BN) + (lane & 31)) * LIMB_BK +
                        (lane >> 5) * 16;
     u64 w[KPT];
+    u32 wok = 0;
     v4i bv[8], bn[8];
     if (ks_begin < ks_end) {
         L.init(ks_begin);
-        L.load(w);
+        L.load(w, wok);
         const int8_t *bp = bw + (size_t)ks_begin * STEP;
         if (live)
 #pragma unroll
             for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
-        stage_a<KPT>(As2[0], PLANE, arow, akq, w);
+        stage_a<KPT>(As2[0], PLANE, arow, akq, w, wok);
     }
     __syncthreads();
 
@@ -632,19 +646,19 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
         // fprop / dgrad forms: all eight A limb planes of this K-step are read
         // from LDS before the next step's global loads issue, so the MFMA run
         // below waits on no LDS read (the dense and wgrad forms stream them one
-        // plane ahead, which measures faster for them)
+        // plane ahead, which measures faster for them).  Every wave reads them:
+        // a wave past N would otherwise need its registers zeroed on each step.
         v4i ap[8];
         if (PRE_A) {
             const int8_t *As = As2[cur];
-            if (live)
 #pragma unroll
-                for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+            for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
             __builtin_amdgcn_sched_barrier(0);
         }
         // prefetch the next K-step into registers (the last step re-reads itself)
         const bool more = ks + 1 < ks_end;
         if (more) L.advance();
-        L.load(w);
+        L.load(w, wok);
         const int8_t *bp = bw + (size_t)(more ? ks + 1 : ks) * STEP;
         if (live)
 #pragma unroll
@@ -675,7 +689,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
             __builtin_amdgcn_s_setprio(0);
         }
         // the other buffer was last read in the previous step, before the barrier
-        stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, w);
+        stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, w, wok);
         cur ^= 1;
 #pragma unroll
         for (int j = 0; j < 8; j++) bv[j] = bn[j];

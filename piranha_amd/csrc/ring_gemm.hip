@@ -590,6 +590,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     constexpr int PLANE = TM * LDS_ROW;
     constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
     constexpr bool PRE_A = AMODE >= 2 && AMODE <= 5;  // measured: faster for fprop / dgrad, slower otherwise
+    constexpr bool PIPE = (AMODE == 2 || AMODE == 3 || AMODE == 5) && WM == 1;
     constexpr int TS = TN + 2;                         // row stride (u64) of the staged output tile
     constexpr int A_BYTES = 2 * 8 * PLANE, T_BYTES = TM * TS * 8;
     __shared__ __attribute__((aligned(16))) int8_t smem[A_BYTES > T_BYTES ? A_BYTES : T_BYTES];
@@ -606,7 +607,9 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     }
     const int bx = bid % gridDim.x, by = (bid / gridDim.x) % gridDim.y, bz = bid / (gridDim.x * gridDim.y);
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform across the wave: read it into a scalar, so the
+    // per-wave quantities below (and the branches on them) are scalar
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave / (4 / WM), wn = wave % (4 / WM);
     const int m0 = bx * TM, n0 = by * TN + wn * 32;
     // a wave whose 32 columns lie past N (N not a multiple of the tile width)
@@ -624,6 +627,85 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     // this lane's 16 bytes of each B limb plane: column n0 + lane % 32, k half lane / 32
     const int8_t *bw = Bp + (size_t)(n0 / LIMB_BN) * nks * STEP + ((n0 % LIMB_BN) + (lane & 31)) * LIMB_BK +
                        (lane >> 5) * 16;
+    const int a_off = (wm * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 16;
+    v16i acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+    if constexpr (PIPE) {
+        // Software-pipelined K-loop of the fprop / dgrad forms on the 32x128 tile:
+        // A is loaded two K-steps ahead, so the words staged during a step
+        // were loaded one step earlier and need no wait, and the staging of
+        // A(ks + 1) is interleaved with the step's own 36 MFMAs: each wave
+        // keeps its matrix pipe busy while it stages.  The MFMAs run B-plane by
+        // B-plane, and plane j of B(ks + 1) is loaded into plane j's registers
+        // as soon as its last MFMA of the step has issued, so B needs one
+        // register set.  Unrolled by two so the two A register sets trade roles
+        // without copies (a copy would wait on loads still in flight).
+        u64 wa[KPT], wb[KPT];
+        u32 oka = 0, okb = 0;
+        v4i bv[8];
+        if (ks_begin < ks_end) {
+            L.init(ks_begin);
+            L.load(wa, oka);
+            const int8_t *bp = bw + (size_t)ks_begin * STEP;
+            if (live)
+#pragma unroll
+                for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+            stage_a<KPT>(As2[0], PLANE, arow, akq, wa, oka);
+            if (ks_begin + 1 < ks_end) {
+                L.advance();
+                L.load(wa, oka);  // A(ks_begin + 1)
+            }
+        }
+        __syncthreads();
+        // one K-step: MFMAs on A(ks) (LDS buffer cur) and B(ks); loads A(ks + 2)
+        // into wl and B(ks + 1) into the B registers; stages A(ks + 1), held in
+        // ws, into the other buffer
+        auto step = [&](int ks, u64(&ws)[KPT], u32 &oks, u64(&wl)[KPT], u32 &okl) __attribute__((always_inline)) {
+            v4i ap[8];
+            const int8_t *As = As2[cur];
+#pragma unroll
+            for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+            const bool more = ks + 1 < ks_end;
+            if (ks + 2 < ks_end) L.advance();
+            L.load(wl, okl);  // past the end: re-reads the last step
+            const int8_t *bp = bw + (size_t)(more ? ks + 1 : ks) * STEP;
+            __builtin_amdgcn_sched_barrier(0);
+            if (live) {
+                __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+#pragma unroll
+                    for (int i = 0; i < 8 - j; i++)
+                        acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bv[j], acc[i + j], 0, 0, 0);
+                    bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+                }
+                stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, ws, oks);
+                // one or two staging instructions between consecutive MFMAs,
+                // the LDS writes towards the end of the run
+#pragma unroll
+                for (int g = 0; g < 28; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+                }
+#pragma unroll
+                for (int g = 0; g < 8; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+                }
+                __builtin_amdgcn_s_setprio(0);
+            } else {
+                stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, ws, oks);
+            }
+            cur ^= 1;
+            __syncthreads();
+        };
+        for (int ks = ks_begin; ks < ks_end; ks += 2) {
+            step(ks, wa, oka, wb, okb);
+            if (ks + 1 < ks_end) step(ks + 1, wb, okb, wa, oka);
+        }
+    } else {
     u64 w[KPT];
     u32 wok = 0;
     v4i bv[8], bn[8];
@@ -638,10 +720,6 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     }
     __syncthreads();
 
-    const int a_off = (wm * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 16;
-    v16i acc[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int ks = ks_begin; ks < ks_end; ++ks) {
         // fprop / dgrad forms: all eight A limb planes of this K-step are read
         // from LDS before the next step's global loads issue, so the MFMA run
@@ -694,6 +772,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
 #pragma unroll
         for (int j = 0; j < 8; j++) bv[j] = bn[j];
         __syncthreads();
+    }
     }
 
     // ---- epilogue: C = sum_s acc_s << 8s (32x32 MFMA layout: lane -> column,

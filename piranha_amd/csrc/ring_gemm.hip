@@ -589,8 +589,10 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     constexpr int KG = LIMB_BK / KPT;  // threads per row for staging
     constexpr int PLANE = TM * LDS_ROW;
     constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
-    constexpr bool PRE_A = AMODE >= 2 && AMODE <= 5;  // measured: faster for fprop / dgrad, slower otherwise
-    constexpr bool PIPE = (AMODE == 2 || AMODE == 3 || AMODE == 5) && WM == 1;
+    // 64x64 tile: the fprop / dgrad forms read all A planes ahead of the MFMAs
+    // (measured faster for them), the dense and wgrad forms stream them
+    constexpr bool PRE_A = AMODE >= 2 && AMODE <= 5;
+    constexpr bool PIPE = WM == 1;  // the 64x64 tile has no registers to spare for it
     constexpr int TS = TN + 2;                         // row stride (u64) of the staged output tile
     constexpr int A_BYTES = 2 * 8 * PLANE, T_BYTES = TM * TS * 8;
     __shared__ __attribute__((aligned(16))) int8_t smem[A_BYTES > T_BYTES ? A_BYTES : T_BYTES];
@@ -633,11 +635,15 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     if constexpr (PIPE) {
-        // Software-pipelined K-loop of the fprop / dgrad forms on the 32x128 tile:
+        // Software-pipelined K-loop of the 32x128 tile (every operand form):
         // A is loaded two K-steps ahead, so the words staged during a step
         // were loaded one step earlier and need no wait, and the staging of
         // A(ks + 1) is interleaved with the step's own 36 MFMAs: each wave
-        // keeps its matrix pipe busy while it stages.  The MFMAs run B-plane by
+        // keeps its matrix pipe busy while it stages (measured: two staging
+        // instructions per MFMA, the LDS writes over the last eight MFMAs,
+        // beats one per MFMA, three per MFMA and the compiler's own order by
+        // 2-3 %).  The eight A planes of the step are read right after the
+        // barrier, before any MFMA.  The MFMAs run B-plane by
         // B-plane, and plane j of B(ks + 1) is loaded into plane j's registers
         // as soon as its last MFMA of the step has issued, so B needs one
         // register set.  Unrolled by two so the two A register sets trade roles

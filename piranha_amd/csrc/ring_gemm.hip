@@ -589,10 +589,12 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     constexpr int KG = LIMB_BK / KPT;  // threads per row for staging
     constexpr int PLANE = TM * LDS_ROW;
     constexpr int STEP = 8 * LIMB_BN * LIMB_BK;  // packed B bytes per n-tile per K-step
-    // 64x64 tile: the fprop / dgrad forms read all A planes ahead of the MFMAs
-    // (measured faster for them), the dense and wgrad forms stream them
-    constexpr bool PRE_A = AMODE >= 2 && AMODE <= 5;
-    constexpr bool PIPE = WM == 1;  // the 64x64 tile has no registers to spare for it
+    // software-pipelined K-loops: the 32x128 tile loads A two K-steps ahead;
+    // the 64x64 tile (twice the A words per thread) one step ahead in the
+    // fprop / dgrad forms, while its dense and wgrad forms keep the plain loop
+    // (they stream the A planes from LDS, which measures faster for them)
+    constexpr bool PIPE = WM == 1;
+    constexpr bool PIPE1 = WM == 2 && AMODE >= 2 && AMODE <= 5;
     constexpr int TS = TN + 2;                         // row stride (u64) of the staged output tile
     constexpr int A_BYTES = 2 * 8 * PLANE, T_BYTES = TM * TS * 8;
     __shared__ __attribute__((aligned(16))) int8_t smem[A_BYTES > T_BYTES ? A_BYTES : T_BYTES];
@@ -711,6 +713,68 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
             step(ks, wa, oka, wb, okb);
             if (ks + 1 < ks_end) step(ks + 1, wb, okb, wa, oka);
         }
+    } else if constexpr (PIPE1) {
+        // 64x64 tile, fprop / dgrad forms: A(ks + 1) is loaded at the start
+        // of step ks into the one register set there is room for (8 words per
+        // thread; two sets spill) and staged interleaved with the step's last
+        // 30 MFMAs, four staging instructions per MFMA, once the loads have
+        // had the first six MFMAs' time to land; B is reloaded plane by plane
+        // as in the 32x128 loop.  Measured against staging after the MFMA run:
+        // VGG16's second conv 385 -> 370 us, its first conv 104 -> 92 us (a
+        // lead of 2, 4, 8 or 10 MFMAs and 3-5 instructions per MFMA were
+        // within 1 % of this, but for a lead of 2).
+        u64 w[KPT];
+        u32 wok = 0;
+        v4i bv[8];
+        if (ks_begin < ks_end) {
+            L.init(ks_begin);
+            L.load(w, wok);
+            const int8_t *bp = bw + (size_t)ks_begin * STEP;
+            if (live)
+#pragma unroll
+                for (int j = 0; j < 8; j++) bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+            stage_a<KPT>(As2[0], PLANE, arow, akq, w, wok);
+        }
+        __syncthreads();
+        for (int ks = ks_begin; ks < ks_end; ++ks) {
+            v4i ap[8];
+            const int8_t *As = As2[cur];
+#pragma unroll
+            for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+            const bool more = ks + 1 < ks_end;
+            if (more) L.advance();
+            L.load(w, wok);  // past the end: re-reads the last step
+            const int8_t *bp = bw + (size_t)(more ? ks + 1 : ks) * STEP;
+            __builtin_amdgcn_sched_barrier(0);
+            if (live) {
+                __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+#pragma unroll
+                    for (int i = 0; i < 8 - j; i++)
+                        acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bv[j], acc[i + j], 0, 0, 0);
+                    bv[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
+                }
+                stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, w, wok);
+#pragma unroll
+                for (int g = 0; g < 6; g++) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+#pragma unroll
+                for (int g = 0; g < 22; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // VALU
+                }
+#pragma unroll
+                for (int g = 0; g < 8; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+                }
+                __builtin_amdgcn_s_setprio(0);
+            } else {
+                stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, w, wok);
+            }
+            cur ^= 1;
+            __syncthreads();
+        }
     } else {
     u64 w[KPT];
     u32 wok = 0;
@@ -726,19 +790,11 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     }
     __syncthreads();
 
+    // 64x64 tile, dense and wgrad forms: A(ks + 1) and B(ks + 1) are loaded
+    // during step ks, the A planes are streamed from LDS one plane ahead of
+    // their MFMAs (measured faster for these forms than reading all eight
+    // first), and A(ks + 1) is staged after the MFMA run.
     for (int ks = ks_begin; ks < ks_end; ++ks) {
-        // fprop / dgrad forms: all eight A limb planes of this K-step are read
-        // from LDS before the next step's global loads issue, so the MFMA run
-        // below waits on no LDS read (the dense and wgrad forms stream them one
-        // plane ahead, which measures faster for them).  Every wave reads them:
-        // a wave past N would otherwise need its registers zeroed on each step.
-        v4i ap[8];
-        if (PRE_A) {
-            const int8_t *As = As2[cur];
-#pragma unroll
-            for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
-            __builtin_amdgcn_sched_barrier(0);
-        }
         // prefetch the next K-step into registers (the last step re-reads itself)
         const bool more = ks + 1 < ks_end;
         if (more) L.advance();
@@ -749,15 +805,7 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
             for (int j = 0; j < 8; j++) bn[j] = *(const v4i *)(bp + j * LIMB_BN * LIMB_BK);
 
         // acc[i+j] += A_i . B_j for i + j < 8
-        if (live && PRE_A) {
-            __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-            for (int i = 0; i < 8; i++)
-#pragma unroll
-                for (int j = 0; j < 8 - i; j++)
-                    acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bv[j], acc[i + j], 0, 0, 0);
-            __builtin_amdgcn_s_setprio(0);
-        } else if (live) {  // A planes streamed from LDS one at a time
+        if (live) {
             const int8_t *As = As2[cur];
             v4i av = *(const v4i *)(As + a_off);
             __builtin_amdgcn_s_setprio(1);

@@ -582,9 +582,9 @@ __device__ __forceinline__ void stage_a(int8_t *As, int plane, int arow, int akq
 // partials are summed by split_reduce_kernel.  With <= 256 registers per lane
 // (128 accumulators + operands) two workgroups share a CU, so one's staging
 // overlaps the other's MFMAs.
-template <int AMODE, int WM>
-__global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t *__restrict__ Bp, int N, int nks,
-                                                           int steps_per_split, Epilogue epi, u64 *__restrict__ part) {
+template <int AMODE, int WM, bool HELP>
+__device__ __forceinline__ void limb_gemm_body(const LimbA &a, const int8_t *__restrict__ Bp, int N, int nks,
+                                               int steps_per_split, const Epilogue &epi, u64 *__restrict__ part) {
     constexpr int TM = 32 * WM, TN = 128 / WM, KPT = 4 * WM;
     constexpr int KG = LIMB_BK / KPT;  // threads per row for staging
     constexpr int PLANE = TM * LDS_ROW;
@@ -636,7 +636,134 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = (v16i){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-    if constexpr (PIPE) {
+    static_assert(!HELP || WM == 1, "the helper form is a 32x128 tile form");
+    if constexpr (HELP) {
+        // 32x128 tile with three column blocks (64 < N <= 96): waves 0-2 sum
+        // the limb products of s = 0..6 for their own block (28 MFMAs a
+        // K-step), and wave 3, which has no columns of its own, sums s = 7 for
+        // all three blocks (3 x 8 MFMAs), so every SIMD's matrix pipe takes a
+        // share instead of one SIMD idling.  acc[s] holds s in waves 0-2,
+        // acc[c] block c's s = 7 in wave 3.  Wave 3 keeps all three blocks' B
+        // planes, each reloaded for the next K-step as soon as its last MFMA
+        // has issued, so no load is waited on within a step.  A is staged by
+        // every wave, loaded two K-steps ahead and staged interleaved with the
+        // MFMAs, as in the loop below (four or five staging instructions per
+        // MFMA here: two or three measured 2-3 % slower).  Measured on the
+        // AlexNet-CIFAR first conv over 986 images (N = 96): 673 -> 610 us.
+        // Splitting s = 0 and 7 off to the fourth wave instead, which then
+        // needed block 2's B planes within the step they were loaded in, made
+        // it wait on the step's A loads too and ran 790 us.
+        u64 wa[KPT], wb[KPT];
+        u32 oka = 0, okb = 0;
+        if (ks_begin < ks_end) {
+            L.init(ks_begin);
+            L.load(wa, oka);
+            stage_a<KPT>(As2[0], PLANE, arow, akq, wa, oka);
+            if (ks_begin + 1 < ks_end) {
+                L.advance();
+                L.load(wa, oka);  // A(ks_begin + 1)
+            }
+        }
+        constexpr int PL = LIMB_BN * LIMB_BK;  // bytes per packed B plane
+        if (wave < 3) {
+            v4i bv[7];  // plane 7 only meets s = 7
+            if (ks_begin < ks_end) {
+                const int8_t *bp = bw + (size_t)ks_begin * STEP;
+#pragma unroll
+                for (int j = 0; j < 7; j++) bv[j] = *(const v4i *)(bp + j * PL);
+            }
+            __syncthreads();
+            auto step = [&](int ks, u64(&ws)[KPT], u32 &oks, u64(&wl)[KPT], u32 &okl) __attribute__((always_inline)) {
+                v4i ap[7];
+                const int8_t *As = As2[cur];
+#pragma unroll
+                for (int i = 0; i < 7; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+                const bool more = ks + 1 < ks_end;
+                if (ks + 2 < ks_end) L.advance();
+                L.load(wl, okl);
+                const int8_t *bp = bw + (size_t)(more ? ks + 1 : ks) * STEP;
+                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+                for (int j = 0; j < 7; j++) {
+#pragma unroll
+                    for (int i = 0; i + j <= 6; i++)
+                        acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bv[j], acc[i + j], 0, 0, 0);
+                    bv[j] = *(const v4i *)(bp + j * PL);
+                }
+                stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, ws, oks);
+#pragma unroll
+                for (int g = 0; g < 20; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // VALU
+                }
+#pragma unroll
+                for (int g = 0; g < 8; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+                }
+                __builtin_amdgcn_s_setprio(0);
+                cur ^= 1;
+                __syncthreads();
+            };
+            for (int ks = ks_begin; ks < ks_end; ks += 2) {
+                step(ks, wa, oka, wb, okb);
+                if (ks + 1 < ks_end) step(ks + 1, wb, okb, wa, oka);
+            }
+        } else {
+            // this lane's bytes of block c's B planes (block c = columns 32c..32c+31)
+            const int8_t *bwc[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                bwc[c] = Bp + (size_t)(c / 2) * nks * STEP + ((c % 2) * 32 + (lane & 31)) * LIMB_BK + (lane >> 5) * 16;
+            v4i bb[3][8];
+            if (ks_begin < ks_end) {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++) bb[c][j] = *(const v4i *)(bwc[c] + (size_t)ks_begin * STEP + j * PL);
+            }
+            __syncthreads();
+            auto step = [&](int ks, u64(&ws)[KPT], u32 &oks, u64(&wl)[KPT], u32 &okl) __attribute__((always_inline)) {
+                v4i ap[8];
+                const int8_t *As = As2[cur];
+#pragma unroll
+                for (int i = 0; i < 8; i++) ap[i] = *(const v4i *)(As + i * PLANE + a_off);
+                const bool more = ks + 1 < ks_end;
+                if (ks + 2 < ks_end) L.advance();
+                L.load(wl, okl);
+                const size_t nxt = (size_t)(more ? ks + 1 : ks) * STEP;
+                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_s_setprio(1);
+                // the three blocks' s = 7 chains interleaved
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap[i], bb[c][7 - i], acc[c], 0, 0, 0);
+                        bb[c][7 - i] = *(const v4i *)(bwc[c] + nxt + (7 - i) * PL);
+                    }
+                stage_a<KPT>(As2[cur ^ 1], PLANE, arow, akq, ws, oks);
+#pragma unroll
+                for (int g = 0; g < 16; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // VALU
+                }
+#pragma unroll
+                for (int g = 0; g < 8; g++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+                }
+                __builtin_amdgcn_s_setprio(0);
+                cur ^= 1;
+                __syncthreads();
+            };
+            for (int ks = ks_begin; ks < ks_end; ks += 2) {
+                step(ks, wa, oka, wb, okb);
+                if (ks + 1 < ks_end) step(ks + 1, wb, okb, wa, oka);
+            }
+        }
+    } else if constexpr (PIPE) {
         // Software-pipelined K-loop of the 32x128 tile (every operand form):
         // A is loaded two K-steps ahead, so the words staged during a step
         // were loaded one step earlier and need no wait, and the staging of
@@ -836,14 +963,28 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
     // dimension, two adjacent columns (16 bytes) per lane where the output
     // allows.  One copy of the epilogue's code serves the whole tile.
     u64 *T = (u64 *)smem;
+    if (!HELP || wave < 3) {
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        u64 raw = 0;
+        for (int r = 0; r < 16; r++) {
+            u64 raw = 0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) raw += ((u64)(s64)acc[q][r]) << (8 * q);
-        T[(wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * TS + wn * 32 + (lane & 31)] = raw;
+            for (int q = 0; q < (HELP ? 7 : 8); q++) raw += ((u64)(s64)acc[q][r]) << (8 * q);
+            T[(wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * TS + wn * 32 + (lane & 31)] = raw;
+        }
     }
     __syncthreads();
+    if constexpr (HELP) {
+        // wave 3 adds its s = 7 sums into the three blocks
+        if (wave == 3) {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    T[(8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * TS + c * 32 + (lane & 31)] +=
+                        (u64)(s64)acc[c][r] << 56;
+        }
+        __syncthreads();
+    }
     const int nt0 = by * TN;  // first column of the workgroup tile
     if (epi.pairs) {
         constexpr int HALF = TN / 2;
@@ -866,6 +1007,21 @@ __global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t
             else epilogue_store(epi, M, N, m, n, v);
         }
     }
+}
+
+template <int AMODE, int WM>
+__global__ __launch_bounds__(256, 2) void limb_gemm_kernel(LimbA a, const int8_t *__restrict__ Bp, int N, int nks,
+                                                           int steps_per_split, Epilogue epi, u64 *__restrict__ part) {
+    limb_gemm_body<AMODE, WM, false>(a, Bp, N, nks, steps_per_split, epi, part);
+}
+
+// The 32x128 tile with three column blocks (64 < N <= 96), fprop forms: the
+// fourth wave takes a share of the other three's MFMAs.
+template <int AMODE>
+__global__ __launch_bounds__(256, 2) void limb_gemm_help_kernel(LimbA a, const int8_t *__restrict__ Bp, int N,
+                                                                int nks, int steps_per_split, Epilogue epi,
+                                                                u64 *__restrict__ part) {
+    limb_gemm_body<AMODE, 1, true>(a, Bp, N, nks, steps_per_split, epi, part);
 }
 
 // Sum of the split-K partials of output element(s) t (PAIRS: t and t + 1 as one
@@ -1109,7 +1265,13 @@ int limb_gemm(const LimbA &a_in, const int8_t *packedB, int N, const Epilogue &e
     }
 #define PIR_LIMB_LAUNCH(AM, WM)                                                                                \
     hipLaunchKernelGGL((limb_gemm_kernel<AM, WM>), grid, dim3(256), 0, s, a, packedB, N, nks, per, ek, part)
-    switch (variant) {
+    // three column blocks in the 32x128 tile: the fourth wave helps (fprop forms)
+    const bool help = wide && (N + 31) / 32 == 3 && (amode == 2 || amode == 3);
+    if (help && amode == 2)
+        hipLaunchKernelGGL((limb_gemm_help_kernel<2>), grid, dim3(256), 0, s, a, packedB, N, nks, per, ek, part);
+    else if (help)
+        hipLaunchKernelGGL((limb_gemm_help_kernel<3>), grid, dim3(256), 0, s, a, packedB, N, nks, per, ek, part);
+    else switch (variant) {
         case 0: PIR_LIMB_LAUNCH(0, 2); break;
         case 1: PIR_LIMB_LAUNCH(1, 2); break;
         case 2: PIR_LIMB_LAUNCH(2, 2); break;

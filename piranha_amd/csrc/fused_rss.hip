@@ -336,7 +336,12 @@ int fused_maxpool_pools(u64 *pools, size_t n, u64 *result, U *dres, int k, hipSt
 // tournament and keeps the first f*f derivative entries (truncate_cols).
 // MASKED: the pool's tournament draws from gp (element = window = output
 // index), the fused ReLU from gr (its own call, element = output index).
-template <typename U, int K, bool MASKED>
+// F: the window side when fixed at compile time (0: g.f at run time); with it
+// the padded positions are known, so the compiler folds the compares of two
+// pad values (their difference is zero in every share) and drops the
+// derivative entries that truncate_cols discards: a 3x3 window padded to 16
+// runs 11 of the tournament's 15 compares.
+template <typename U, int K, bool MASKED, int F = 0>
 __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__restrict__ x, MaxpoolGeom g,
                                                                  u64 *__restrict__ act, U *__restrict__ mprime,
                                                                  u64 *__restrict__ relu_out,
@@ -354,11 +359,11 @@ __global__ __launch_bounds__(EW_BLOCK) void maxpool_layer_kernel(const u64 *__re
     int b = (int)(r / g.OH);
     Sh3<u64> e[K];
     Sh3<U> d[K];
-    const int f2 = g.f * g.f;
+    const int f = F ? F : g.f, f2 = f * f;
 #pragma unroll
     for (int j = 0; j < K; j++) {
         if (j < f2) {
-            int dh = j / g.f, dw = j - dh * g.f;
+            int dh = j / f, dw = j - dh * f;
             int ih = oh * g.stride + dh, iw = ow * g.stride + dw;
             e[j] = load3(x, inN, (((size_t)b * g.H + ih) * g.W + iw) * g.C + c);
         } else {
@@ -489,10 +494,19 @@ static int launch_maxpool_layer(const u64 *x, const MaxpoolGeom &g, u64 *act, U 
             return OK;
         }
     }
-    switch (k) {
-        case 4: hipLaunchKernelGGL((maxpool_layer_kernel<U, 4, MASKED>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d, gp, gr); break;
-        case 16: hipLaunchKernelGGL((maxpool_layer_kernel<U, 16, MASKED>), dim3(ew_grid(outN)), dim3(EW_BLOCK), 0, s, x, g, act, mprime, relu_out, relu_d, gp, gr); break;
-        default: set_error("maxpool layer: only 2x2 and 3x3 pools are supported"); return ERR_UNSUPPORTED;
+    const dim3 grid(ew_grid(outN));
+    if (k == 4)
+        hipLaunchKernelGGL((maxpool_layer_kernel<U, 4, MASKED, 2>), grid, dim3(EW_BLOCK), 0, s, x, g, act, mprime,
+                           relu_out, relu_d, gp, gr);
+    else if (k == 16 && g.f == 3)
+        hipLaunchKernelGGL((maxpool_layer_kernel<U, 16, MASKED, 3>), grid, dim3(EW_BLOCK), 0, s, x, g, act, mprime,
+                           relu_out, relu_d, gp, gr);
+    else if (k == 16)
+        hipLaunchKernelGGL((maxpool_layer_kernel<U, 16, MASKED>), grid, dim3(EW_BLOCK), 0, s, x, g, act, mprime,
+                           relu_out, relu_d, gp, gr);
+    else {
+        set_error("maxpool layer: only 2x2, 3x3 and 4x4 pools are supported");
+        return ERR_UNSUPPORTED;
     }
     PIR_LAUNCH_CHECK();
     return OK;

@@ -609,7 +609,25 @@ __device__ __forceinline__ void limb_gemm_body(const LimbA &a, const int8_t *__r
         const int total = gridDim.x * gridDim.y * gridDim.z, per_xcd = total / 8;
         if (bid < per_xcd * 8) bid = (bid % 8) * per_xcd + bid / 8;
     }
-    const int bx = bid % gridDim.x, by = (bid / gridDim.x) % gridDim.y, bz = bid / (gridDim.x * gridDim.y);
+    // Dense operands (no filter-tap reuse of A in L2) take the tiles in groups
+    // of 8 row tiles, M fastest within a group: an XCD's 64 resident
+    // workgroups then cover 8 row x 8 column tiles, not 64 x 1, which cuts the
+    // distinct A rows per K-step eightfold for eight times the B slices (which
+    // the column-sharing waves of several XCDs fetch anyway).
+    // Measured on the 3-term ring product: n = 8192 +10 %, 4096 +3 %, 2048
+    // unchanged (groups of 2, 4, 6, 16 and 32 were slower).
+    constexpr int DENSE_GM = 8;
+    const int tiles2d = gridDim.x * gridDim.y, bz = bid / tiles2d, t2 = bid - bz * tiles2d;
+    int bx, by;
+    if (AMODE <= 1) {
+        const int per_group = DENSE_GM * gridDim.y, g = t2 / per_group, first = g * DENSE_GM;
+        const int gm = min(DENSE_GM, (int)gridDim.x - first), r = t2 - g * per_group;
+        bx = first + r % gm;
+        by = r / gm;
+    } else {
+        bx = t2 % gridDim.x;
+        by = t2 / gridDim.x;
+    }
 
     // the wave index is uniform across the wave: read it into a scalar, so the
     // per-wave quantities below (and the branches on them) are scalar

@@ -6,7 +6,8 @@ bit for bit: non-square filters, padding on one axis only, strides 1-4, Cin
 plan, including splits whose workgroups start in the middle of a segment.
 Output channel counts of 65..96 take the 32x128 tile's helper form (three
 column blocks, the fourth wave summing their s = 7 limb products), for both
-im2col loaders (Cin % 32 != 0 and Cin % 32 == 0)."""
+im2col loaders (Cin % 32 != 0 and Cin % 32 == 0); counts up to 32 leave the
+64x64 tile's second column of waves idle, for both loaders too."""
 import os
 
 import numpy as np
@@ -34,6 +35,7 @@ GEOMS = [
     (2, 5, 5, 4, 10, 3, 3, 1, 1, 2),       # 3x3 output: a tile spans more than two images
     (2, 8, 8, 64, 96, 3, 3, 1, 1, 1),      # Cin % 32 == 0, three column blocks
     (3, 6, 7, 32, 72, 3, 2, 1, 0, 1),      # Cin % 32 == 0, N = 72: the third block partly past N
+    (2, 8, 8, 32, 16, 3, 3, 1, 1, 1),      # Cin % 32 == 0, one column block (N <= 32)
 ]
 
 
@@ -81,7 +83,8 @@ def force_plan(tmp_path, wide, split):
 
 # b, H, W, f, Din, Dout, stride, pad (square, as pir_rss_convolution takes them)
 RSS_GEOMS = [(3, 32, 32, 3, 3, 64, 1, 1), (2, 32, 32, 11, 3, 96, 4, 9), (2, 12, 12, 5, 20, 50, 1, 0),
-             (2, 10, 10, 3, 5, 40, 2, 2), (2, 8, 8, 3, 32, 80, 1, 1)]
+             (2, 10, 10, 3, 5, 40, 2, 2), (2, 8, 8, 3, 32, 80, 1, 1), (2, 6, 6, 3, 32, 24, 1, 1),
+             (3, 12, 12, 5, 1, 20, 1, 0)]
 
 
 @pytest.mark.parametrize("g", RSS_GEOMS)

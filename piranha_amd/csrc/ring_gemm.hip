@@ -503,6 +503,21 @@ struct ALoader {
             int q = rem - p * a.OW;
             int ih = p * a.stride - a.ph + r, iw = q * a.stride - a.pw + s;
             long long rowoff = (((long long)n * a.H + ih) * a.W) * a.Cin + c0;  // used only when in range
+            if (a.OW % KPT == 0) {
+                // the thread's KPT pixels lie in one output row (kb is a
+                // multiple of KPT, and so are OW and the segment's K): one
+                // row check, then only the column moves
+                const bool rin = row_ok && kb < a.K && ih >= 0 && ih < a.H;
+                ok_bits = 0;
+#pragma unroll
+                for (int j = 0; j < KPT; j++) {
+                    const int iwj = iw + j * a.stride;
+                    const bool ok = rin && iwj >= 0 && iwj < a.W;
+                    w[j] = src[ok ? rowoff + (long long)iwj * a.Cin : 0];
+                    ok_bits |= ok ? 1u << j : 0u;
+                }
+                return;
+            }
             int off = iw * a.Cin;
             const int col_step = a.stride * a.Cin;
             const long long row_step = (long long)a.stride * a.W * a.Cin;

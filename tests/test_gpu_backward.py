@@ -204,7 +204,7 @@ def test_relu_backward(rng, net_mode):
 
 @pytest.mark.parametrize("net_mode", MODES, ids=["fused", "plain"])
 @pytest.mark.parametrize("batch,H,W,Cc,pool,s", [(2, 2, 2, 2, 2, 1), (3, 9, 9, 2, 3, 2), (2, 8, 6, 3, 2, 2),
-                                                 (2, 16, 16, 8, 2, 2)])
+                                                 (2, 16, 16, 8, 2, 2), (2, 8, 8, 2, 4, 4), (1, 9, 9, 3, 3, 3)])
 def test_maxpool_backward(rng, net_mode, batch, H, W, Cc, pool, s):
     fp = 13
     net = P.Network([(P.LAYER_MAXPOOL, H, W, Cc, pool, s)], batch, fp, seed=1, mode=net_mode)

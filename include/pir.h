@@ -426,7 +426,9 @@ int pir_net_profile_layers(pir_net *net, float *ms, int count);
  * ops[v] = summed algorithmic int8 ops (2*36*M*N*nseg*K), launches[v] = launch
  * count.  v = A-operand form (PIR_LIMB_DENSE_ROW .. PIR_LIMB_WGRAD), plus
  * PIR_LIMB_WIDE for the 32x128 workgroup tile (else 64x64); the kernel is
- * limb_gemm_kernel<form, wide ? 1 : 2>.  Arrays hold PIR_LIMB_VARIANTS entries. */
+ * limb_gemm_kernel<form, wide ? 1 : 2>, or limb_gemm_help_kernel<form> for an
+ * fprop product with 64 < N <= 96 on the 32x128 tile (counted with that
+ * variant).  Arrays hold PIR_LIMB_VARIANTS entries. */
 enum { PIR_LIMB_DENSE_ROW = 0, PIR_LIMB_DENSE_COL, PIR_LIMB_CONV, PIR_LIMB_CONV_C32, PIR_LIMB_DGRAD,
        PIR_LIMB_DGRAD_C32, PIR_LIMB_WGRAD, PIR_LIMB_WIDE = 8, PIR_LIMB_VARIANTS = 16 };
 int pir_net_profile_gemm(pir_net *net, double *ms, double *ops, int *launches);

@@ -489,6 +489,8 @@ def main():
                     help="with --uneven-only: also time one forward per query, back to back")
     ap.add_argument("--masked-only", action="store_true",
                     help="the VGG16 forward in the masked precompute mode only (for profiling runs)")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="the ring-GEMM sweep (--gemm-sizes) only (for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -522,6 +524,11 @@ def main():
                           per_query=args.per_query)
         print(json.dumps({"alexnet_uneven_64q_fp20": un}), flush=True)
         save_plans()
+        return
+
+    if args.gemm_only:
+        sizes = [int(v) for v in args.gemm_sizes.split(",") if v]
+        print(json.dumps({"ring_gemm": bench_ring_gemm(P, sizes, reps=2)}), flush=True)
         return
 
     if args.random_sharings_only or args.masked_only:
